@@ -1,0 +1,183 @@
+"""Benchmark: grid points per second at standard error <= 1e-3 on one MI355X.
+
+Workload (one "step" = one full solve: classify, walk, merge):
+  3D screened Poisson on the unit icosphere at subdivision 4 (2562 vertices,
+  5120 triangles), alpha = 10, constant source f = 1 sampled uniformly in the
+  ball, g = 0, 32^3 grid over [-1, 1]^3, 4096 walks per inside point,
+  epsilon = 1e-4, seed 7.
+Forms: the FP32 persistent walker (the value) and the FP64 plain walker.
+
+Grid pts/s at SE <= 1e-3 = #inside / sum_p n_p * t_walk,p with
+n_p = ceil(var_p / 1e-6) from the FP64 run (shared by every form) and
+t_walk,p = (T / total steps) * mean_steps_p of the form being measured.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "walk-on-spheres_amd"))
+
+METRIC = "grid pts/s at SE≤1e-3, FP32 persistent vs FP64 plain; adjoint walks/s"
+WORKLOAD = ("3D screened Poisson, unit icosphere s4 (2562 v / 5120 t), alpha=10, f=1 uniform "
+            "ball sampling, g=0, 32^3 grid, N=4096 walks/point, eps=1e-4, seed 7")
+HBM_PEAK_GBS = 8000.0
+SE_TARGET = 1e-3
+
+
+def pts_per_s_at_se(n_inside, var_ref, msteps, seconds, total_steps):
+    n_p = np.maximum(1.0, np.ceil(var_ref / (SE_TARGET ** 2)))
+    t_step = seconds / max(1, total_steps)
+    return n_inside / float(np.sum(n_p * t_step * msteps))
+
+
+def cpu_baseline(V, F, eq, grid, inside_idx, var_ref, eps, seed, budget_s=12.0):
+    """The reference core (oracle/_ref) on the host cores, on a bounded sample
+    of the inside points; falls back to the C oracle if it is absent."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    cores = min(os.cpu_count() or 1, 16)
+    pts_all = pyoracle.grid_points(pyoracle.Oracle().mesh(V, F).bbox(), *grid, 3)
+    try:
+        lib = pyoracle.Reference()
+        kind = "reference"
+    except FileNotFoundError:
+        lib = pyoracle.Oracle()
+        kind = "port"
+    mesh = lib.mesh(V, F)
+    # calibrate on a small sample, then size the sample to ~budget_s
+    stride = max(1, len(inside_idx) // 256)
+    pick = inside_idx[::stride]
+    walks = 64
+
+    def run(pick, walks):
+        t0 = time.perf_counter()
+        if kind == "reference":
+            r = mesh.solve_points(eq, pts_all[pick], walks, eps, seed=seed, n_threads=cores,
+                                  seed_index=pick)
+        else:
+            r = mesh.solve_points(eq, pts_all[pick], walks, eps, seed=seed, rng_mode=1,
+                                  n_threads=cores, seed_index=pick)
+        dt = time.perf_counter() - t0
+        steps = float(np.nansum(r["mean_steps"]) * walks)
+        return dt, steps, r
+
+    dt, steps, r = run(pick, walks)
+    scale = max(1.0, budget_s / max(dt, 1e-3))
+    walks = int(min(4096, walks * scale))
+    dt, steps, r = run(pick, walks)
+    t_step = dt / steps
+    ms = r["mean_steps"]
+    # cost model over the sampled points, scaled to the full inside set
+    n_p = np.maximum(1.0, np.ceil(var_ref[np.searchsorted(inside_idx, pick)] / SE_TARGET ** 2))
+    pts_s = len(pick) / float(np.sum(n_p * t_step * ms))
+    return dict(value=pts_s, unit="grid pts/s", cores=cores, kind=kind,
+                steps_per_s=steps / dt, walks_per_s=len(pick) * walks / dt,
+                sample=f"{len(pick)} of {len(inside_idx)} inside points (every {stride}th), "
+                       f"{walks} walks each, {dt:.1f} s on {cores} threads")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--walks", type=int, default=4096)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(the single-GPU kernel at full shape is the measured unit).", file=sys.stderr)
+        sys.exit(2)
+
+    from wos_amd import Context, meshgen, problems
+    from wos_amd.capi import WOS_FP32_PERSISTENT, WOS_FP64_PLAIN
+
+    V, F, _ = meshgen.icosphere(4)
+    eq = problems.sphere_screened_constant(alpha=10.0, f=1.0)
+    grid = (32, 32, 32)
+    eps, seed, nw = 1e-4, 7, args.walks
+    ctx = Context(0)
+    ctx.set_mesh(V, F)
+
+    p32 = Context.params(nw, eps, seed, precision=WOS_FP32_PERSISTENT)
+    p64 = Context.params(nw, eps, seed, precision=WOS_FP64_PLAIN)
+
+    for _ in range(args.warmup):
+        ctx.solve_grid(*grid, eq, p32)
+    walk_k = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        r32 = ctx.solve_grid(*grid, eq, p32)
+        walk_k.append(r32["summary"]["walk_kernel_seconds"])
+    t32 = (time.perf_counter() - t0) / args.steps
+    s32 = r32["summary"]
+
+    t0 = time.perf_counter()
+    r64 = ctx.solve_grid(*grid, eq, p64)
+    t64 = time.perf_counter() - t0
+    s64 = r64["summary"]
+
+    ins = r64["location"] == 1
+    assert np.array_equal(r64["location"], r32["location"])
+    var_ref = r64["variance"][ins]
+    v32 = pts_per_s_at_se(ins.sum(), var_ref, r32["mean_steps"][ins], t32, s32["total_steps"])
+    v64 = pts_per_s_at_se(ins.sum(), var_ref, r64["mean_steps"][ins], t64, s64["total_steps"])
+
+    counts_path = os.path.join(ROOT, "profiles", "algorithmic_counts.json")
+    with open(counts_path) as f:
+        counts = json.load(f)["icosphere4_screened_32cubed"]
+    kern_s = float(np.mean(walk_k))
+    achieved = s32["total_steps"] * counts["B_step_f32"] / kern_s / 1e9
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "walk_fp32_icosphere4_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+
+    d = np.abs(r32["mean"][ins] - r64["mean"][ins])
+    line = {
+        "metric": METRIC,
+        "value": v32,
+        "unit": "grid pts/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": t32 * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": WORKLOAD, "inside_points": int(ins.sum()), "walks_per_point": nw,
+                   "accumulation": "f64 (Welford, terminal projection, boundary value)",
+                   "parallelism": "single GPU"},
+        "fp64_plain": {"value": v64, "ms_per_solve": t64 * 1e3,
+                       "walk_kernel_ms": s64["walk_kernel_seconds"] * 1e3,
+                       "steps_per_s": s64["total_steps"] / t64},
+        "fp32_over_fp64": v32 / v64,
+        "walks_per_s": s32["total_walks"] / t32,
+        "steps_per_s": s32["total_steps"] / t32,
+        "fp32_vs_fp64_mean_abs_diff": {"max": float(d.max()), "mean": float(d.mean())},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "k_walk_fp32<3,true,true>",
+                     "kernel_ms": kern_s * 1e3,
+                     "bytes_per_step": counts["B_step_f32"]},
+    }
+    if not args.no_cpu:
+        inside_idx = np.nonzero(ins)[0]
+        line["cpu_baseline"] = cpu_baseline(V, F, eq, grid, inside_idx, var_ref, eps, seed)
+        line["vs_cpu"] = v32 / line["cpu_baseline"]["value"]
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,179 @@
+/*
+ * wos_hip.h -- C ABI of the MI355X walk-on-spheres engine (libwos_hip.so).
+ *
+ * Plain C types only: host pointers, sizes and POD descriptors.  Every entry
+ * point returns an int status (WOS_OK on success) and never throws; the text
+ * of the last failure on the calling thread is available from
+ * wos_last_error().  A context owns one GPU's copies of the mesh and BVH and
+ * its scratch buffers; calls on one context are serialised by the caller.
+ *
+ * Reference interfaces replaced (paths relative to the reference repository):
+ *   - the per-point sweep   run<N,Eq>            include/wos/runner.hpp:22-308
+ *   - the estimator         wos_run/wos_sample   include/wos/wos.hpp:54-191
+ *   - the classifier        classify_point       src/inside.cpp:196-260
+ *   - the closest point     bvh_npq              src/bvh.cpp:295-387
+ *   - the BVH build         build_bvh            src/bvh.cpp:234-253
+ *   - the equation functors src/equations/{laplace,poisson,screened_poisson}.cpp
+ *   - the plugin table      struct Equation      src/equations/equations.hpp:7-31
+ */
+#ifndef WOS_HIP_H
+#define WOS_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes (reference: int 0 ok / 1 error, runner.hpp:29-64) ---- */
+enum wos_status {
+    WOS_OK = 0,
+    WOS_ERR_INPUT = 1,   /* invalid argument or size                        */
+    WOS_ERR_MESH = 2,    /* mesh missing, malformed or degenerate            */
+    WOS_ERR_DEVICE = 3,  /* HIP runtime failure                              */
+    WOS_ERR_IO = 4,      /* file create / write failure                      */
+    WOS_ERR_STATE = 5    /* call order (e.g. solve before a mesh was set)    */
+};
+
+/* ---- equation descriptor ------------------------------------------------
+ * A POD restatement of the reference's compile-time Eq functors
+ * (src/equations/laplace.cpp:12-43, poisson.cpp:15-118,
+ * screened_poisson.cpp:40-141) plus the data the benchmark problems need. */
+enum wos_equation_kind {
+    WOS_EQ_LAPLACE = 0,          /* Delta u = 0                                  */
+    WOS_EQ_POISSON = 1,          /* Delta u = -f                                 */
+    WOS_EQ_SCREENED = 2          /* Delta u - alpha^2 u = -f                     */
+};
+
+/* values equal the reference SourceMode enum (include/wos/source_mode.hpp:5-9) */
+enum wos_source_mode {
+    WOS_SOURCE_UNIFORM = 0,      /* uniform ball point x Green function          */
+    WOS_SOURCE_GREEN = 1         /* Green-density radius, mass x f               */
+};
+
+enum wos_source_kind {
+    WOS_SRC_NONE = 0,            /* no source term, no draws (Laplace, screened 3D) */
+    WOS_SRC_DISK_SQ = 1,         /* f = value if |x-c|^2 <= R^2 (poisson.cpp:27-41); 2D prefilter */
+    WOS_SRC_DISK_SQRT = 2,       /* f = value if sqrt(|x-c|^2) <= R (screened_poisson.cpp:48-61); 2D prefilter */
+    WOS_SRC_CONSTANT = 3,        /* f = value everywhere                          */
+    WOS_SRC_GAUSSIAN = 4,        /* f = value * exp(-|x-c|^2 / R^2)               */
+    WOS_SRC_ZERO = 5             /* f = 0 but sampled (reference Poisson3D, poisson.cpp:79-82) */
+};
+
+enum wos_boundary_kind {
+    WOS_BND_X2Y2_ID0 = 0,        /* 2D: x^2-y^2 on boundary id 0, else 0 (laplace.cpp:16-18) */
+    WOS_BND_Y5 = 1,              /* 3D: y5^0 + 4 y5^-3 harmonic (laplace.cpp:30-36)          */
+    WOS_BND_ZERO = 2,            /* g = 0 (poisson.cpp:43-48)                                */
+    WOS_BND_ID_CONST = 3,        /* g = bnd_values[boundary id] (2D)                         */
+    WOS_BND_VERTEX = 4,          /* g = barycentric / linear blend of per-vertex values      */
+    WOS_BND_SIN3X_COS2Y = 5      /* g = sin(3x) cos(2y)                                      */
+};
+
+#define WOS_MAX_BND_VALUES 16
+
+typedef struct wos_eq_desc {
+    int32_t kind;                /* enum wos_equation_kind                     */
+    int32_t source_mode;         /* enum wos_source_mode                       */
+    int32_t source_kind;         /* enum wos_source_kind                       */
+    int32_t boundary_kind;       /* enum wos_boundary_kind                     */
+    double alpha;                /* screening coefficient (WOS_EQ_SCREENED)    */
+    double src_value;            /* disk value / constant / Gaussian amplitude */
+    double src_center[3];        /* disk centre / Gaussian centre              */
+    double src_radius;           /* disk radius / Gaussian width sigma         */
+    double bnd_values[WOS_MAX_BND_VALUES];  /* WOS_BND_ID_CONST               */
+    int32_t n_bnd_values;
+    int32_t reserved0;
+    const double *vertex_values; /* WOS_BND_VERTEX: one value per mesh vertex (host) */
+    int64_t n_vertex_values;
+} wos_eq_desc;
+
+/* ---- execution parameters ---------------------------------------------- */
+enum wos_precision {
+    WOS_FP64_PLAIN = 0,          /* one lane per walk, FP64 geometry           */
+    WOS_FP32_PERSISTENT = 1      /* persistent lane-refilling walker, FP32 geometry, FP64 stats */
+};
+
+typedef struct wos_run_params {
+    int32_t n_walks;             /* walks per inside point (reference --walks)     */
+    int32_t max_steps;           /* reference --max-steps (wos.hpp:58)             */
+    int32_t max_ray_attempts;    /* reference --max-ray-attempts (inside.cpp:228)  */
+    int32_t precision;           /* enum wos_precision                             */
+    double epsilon;              /* stopping distance (already screened-clamped)   */
+    uint64_t seed;               /* global seed (runner.hpp:163-168)               */
+    double target_se;            /* > 0: adaptive walks per point until SE <= target */
+    int32_t max_walks;           /* adaptive cap per point                         */
+    int32_t tune;                /* 0: default launch shape, 1: autotuned shape    */
+} wos_run_params;
+
+/* Per-point outputs, caller-allocated arrays of n entries (any may be NULL
+ * except mean).  Matches the reference's per-point result policy
+ * (runner.hpp:195-235): Outside/Indeterminate -> NaN, Boundary -> g, 0, 0, 0. */
+typedef struct wos_point_stats {
+    double *mean;
+    double *variance;
+    double *standard_error;
+    double *mean_steps;
+    uint8_t *location;           /* 0 outside, 1 inside, 2 boundary, 3 indeterminate */
+    int32_t *max_steps_hits;
+    int32_t *walks;              /* walks taken per point (adaptive mode)          */
+    int32_t *ambiguous_ray_retries;
+} wos_point_stats;
+
+typedef struct wos_run_summary {
+    int64_t inside_count;
+    int64_t max_steps_hits;
+    int64_t ambiguous_ray_retries;
+    int64_t indeterminate_points;
+    double sum_variance;         /* runner.hpp:226-228 sums over inside points     */
+    double sum_standard_error;
+    double sum_mean_steps;
+    double wos_seconds;          /* device time of classify + walks + merge        */
+    double walk_kernel_seconds;  /* device time of the walker kernel(s) alone      */
+    int64_t total_walks;
+    int64_t total_steps;
+    double grid_bounds[6];       /* xmin,xmax,ymin,ymax,zmin,zmax (mesh bbox)      */
+    double boundary_tolerance;   /* 1e-10 * max(1, extent) (runner.hpp:86-88)      */
+    int32_t launch_blocks;
+    int32_t launch_threads;
+} wos_run_summary;
+
+typedef struct wos_ctx wos_ctx;
+
+/* Text of the last error reported on this thread ("" if none). */
+const char *wos_last_error(void);
+
+/* Library version string. */
+const char *wos_version(void);
+
+/* Create a context on HIP device `device`.  Returns NULL on failure. */
+wos_ctx *wos_ctx_create(int device);
+void wos_ctx_destroy(wos_ctx *ctx);
+
+/* Set the domain: vertices (n_verts x dim doubles, row-major), primitives
+ * (n_prims x dim vertex indices: segments in 2D, triangles in 3D) and, in
+ * 2D, the boundary id of every segment (the reference's `l` record index,
+ * src/mesh.cpp:156-212; NULL means all zero).  Builds the BVH on the host
+ * with the reference split rule (src/bvh.cpp:174-253) and uploads FP64 and
+ * FP32 device layouts.  Replaces load_mesh+build_bvh+bcast (runner.hpp:28-80). */
+int wos_ctx_set_mesh(wos_ctx *ctx, int dim, const double *verts, int64_t n_verts,
+                     const int32_t *prims, int64_t n_prims,
+                     const int32_t *boundary_ids);
+
+/* Sweep an Nx x Ny x Nz grid over the mesh bounding box (Nz forced to 1 in
+ * 2D), classify every point, walk every inside point and write row-major
+ * (i, j, k) fields.  Replaces the body of run<N,Eq> (runner.hpp:82-238). */
+int wos_solve_grid(wos_ctx *ctx, int nx, int ny, int nz, const wos_eq_desc *eq,
+                   const wos_run_params *params, wos_point_stats *out,
+                   wos_run_summary *summary);
+
+/* Same estimator on an explicit point list (n x dim doubles).  Point i uses
+ * the seed splitmix64(seed ^ splitmix64(i)).  Replaces the direct
+ * wos::wos_run calls of the welding apps (level1_stationary_heat.cpp:257). */
+int wos_solve_points(wos_ctx *ctx, const double *points, int64_t n,
+                     const wos_eq_desc *eq, const wos_run_params *params,
+                     wos_point_stats *out, wos_run_summary *summary);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* WOS_HIP_H */

@@ -1,0 +1,176 @@
+"""GPU parity: the HIP library through its C ABI against the C oracle.
+
+The FP64 plain walker uses the same counter streams as the oracle's counter
+mode and restates its arithmetic, so per-point statistics agree to rounding
+(a walk can only differ where OCML and glibc transcendentals differ by an
+ulp and flip a termination test).  Location codes are compared bit-exactly.
+The FP32 persistent walker draws the same uniforms (top 24 bits), so its
+per-point means track the FP64 ones closely (common random numbers).
+"""
+import numpy as np
+import pytest
+
+from wos_amd import meshgen as M
+from wos_amd import problems as Pr
+from wos_amd.capi import WOS_FP32_PERSISTENT, WOS_FP64_PLAIN
+from wos_amd.engine import Context
+
+pytestmark = pytest.mark.gpu
+
+THREADS = 16
+
+
+def _run_both(ctx, om, mesh, eq, grid, n_walks, eps, seed, precision=WOS_FP64_PLAIN,
+              max_steps=1000):
+    V, F, B = mesh
+    ctx.set_mesh(V, F, B)
+    g = ctx.solve_grid(*grid, eq, Context.params(n_walks, eps, seed, max_steps=max_steps,
+                                                 precision=precision))
+    o = om.solve_grid(eq, *grid, n_walks, eps, max_steps=max_steps, seed=seed, rng_mode=1,
+                      n_threads=THREADS)
+    return g, o
+
+
+def _compare_fp64(g, o, label, exact_frac=0.97):
+    assert np.array_equal(g["location"], o["location"]), label + ": location codes differ"
+    ins = g["location"] == 1
+    assert ins.any()
+    dm = np.abs(g["mean"][ins] - o["mean"][ins])
+    scale = np.maximum(1.0, np.abs(o["mean"][ins]))
+    close = dm <= 1e-11 * scale
+    se = np.maximum(o["standard_error"][ins], 1e-300)
+    print(f"{label}: inside={ins.sum()} exact-to-1e-11={close.mean():.4f} "
+          f"max|dmean|={dm.max():.3e} max dmean/se={np.max(dm / se):.3f}")
+    assert close.mean() >= exact_frac, label
+    assert np.all(dm <= 3.0 * se + 1e-11 * scale), label
+    b = g["location"] == 2
+    assert np.array_equal(g["mean"][b], o["mean"][b]), label + ": boundary values differ"
+    out = (g["location"] == 0) | (g["location"] == 3)
+    assert np.all(np.isnan(g["mean"][out]))
+
+
+def test_annulus_fp64_matches_oracle(gpu_ctx, oracle):
+    mesh = M.annulus()
+    om = oracle.mesh(*mesh)
+    eq = Pr.annulus_laplace()
+    g, o = _run_both(gpu_ctx, om, mesh, eq, (64, 64, 1), 256, 1e-3, 12345)
+    _compare_fp64(g, o, "annulus fp64")
+    ins = g["location"] == 1
+    assert np.mean(g["mean_steps"][ins] == o["mean_steps"][ins]) >= 0.97
+    s = g["summary"]
+    assert s["inside_count"] == ins.sum()
+    assert s["total_walks"] == ins.sum() * 256
+
+
+def test_annulus_fp32_tracks_fp64_and_analytic(gpu_ctx):
+    V, F, B = M.annulus()
+    gpu_ctx.set_mesh(V, F, B)
+    eq = Pr.annulus_laplace()
+    n = 1024
+    a = gpu_ctx.solve_grid(64, 64, 1, eq, Context.params(n, 1e-3, 12345, precision=WOS_FP64_PLAIN))
+    b = gpu_ctx.solve_grid(64, 64, 1, eq, Context.params(n, 1e-3, 12345,
+                                                         precision=WOS_FP32_PERSISTENT))
+    assert np.array_equal(a["location"], b["location"])
+    ins = a["location"] == 1
+    d = np.abs(a["mean"][ins] - b["mean"][ins])
+    comb = np.sqrt(a["standard_error"][ins] ** 2 + b["standard_error"][ins] ** 2)
+    print(f"fp32 vs fp64: max|d|={d.max():.3e} mean|d|={d.mean():.3e} max d/se={np.max(d / comb):.3f}")
+    assert np.all(d <= 3.0 * comb)
+    # exact solution u = ln r / ln 0.25; bias budget: epsilon shell and the 128-gon
+    xs = np.linspace(-1, 1, 64)
+    X, Y = np.meshgrid(xs, xs, indexing="ij")
+    r = np.sqrt(X ** 2 + Y ** 2).ravel()
+    exact = np.log(r[ins]) / np.log(0.25)
+    err = np.abs(b["mean"][ins] - exact)
+    print(f"fp32 vs exact: max err={err.max():.3e} mean err={err.mean():.3e}")
+    assert np.all(err <= 4.0 * b["standard_error"][ins] + 6e-3)
+
+
+def test_icosphere_codes_and_fp64(gpu_ctx, oracle):
+    mesh = M.icosphere(3)
+    om = oracle.mesh(*mesh)
+    eq = Pr.sphere_screened_constant()
+    g, o = _run_both(gpu_ctx, om, mesh, eq, (16, 16, 16), 64, 1e-4, 7)
+    assert np.array_equal(g["ambiguous_ray_retries"], o["ambiguous"])
+    _compare_fp64(g, o, "icosphere screened fp64")
+
+
+CASES_2D = [
+    ("laplace2d", lambda: M.annulus(64), lambda: Pr.laplace_builtin(2)),
+    ("poisson2d-uniform", lambda: M.circle(128), lambda: Pr.poisson_builtin(2, 0)),
+    ("poisson2d-green", lambda: M.circle(128), lambda: Pr.poisson_builtin(2, 1)),
+    ("screened2d-uniform", lambda: M.circle(128), lambda: Pr.screened_builtin(2, 5.0, 0)),
+    ("screened2d-green", lambda: M.circle(128), lambda: Pr.screened_builtin(2, 5.0, 1)),
+    ("screened2d-green-strong", lambda: M.circle(96), lambda: Pr.screened_builtin(2, 40.0, 1)),
+    ("gaussian2d-green", lambda: M.star(512, 0.3, 5), lambda: Pr.star_poisson_gaussian(1)),
+]
+CASES_3D = [
+    ("laplace3d", lambda: M.icosphere(2), lambda: Pr.laplace_builtin(3)),
+    ("poisson3d-zero", lambda: M.icosphere(2), lambda: Pr.poisson_builtin(3, 0)),
+    ("screened3d", lambda: M.icosphere(2), lambda: Pr.screened_builtin(3, 5.0)),
+    ("screened3d-green", lambda: M.icosphere(2), lambda: Pr.sphere_screened_constant(10.0, 1.0, 1)),
+    ("screened3d-green-strong", lambda: M.icosphere(2), lambda: Pr.sphere_screened_constant(60.0, 1.0, 1)),
+    ("poisson3d-gauss-green", lambda: M.icosphere(2),
+     lambda: Pr.plate_gaussian(alpha=1e-9, q=1.0, sigma=0.5, x0=(0.2, 0.0, 0.0), source_mode=1)),
+]
+
+
+@pytest.mark.parametrize("name,mesh_fn,eq_fn", CASES_2D + CASES_3D, ids=[c[0] for c in CASES_2D + CASES_3D])
+def test_equations_fp64_vs_oracle_and_fp32(gpu_ctx, oracle, name, mesh_fn, eq_fn):
+    mesh = mesh_fn()
+    om = oracle.mesh(*mesh)
+    eq = eq_fn()
+    dim = mesh[1].shape[1]
+    grid = (14, 14, 1) if dim == 2 else (8, 8, 8)
+    eps = 1e-3 if dim == 2 else 1e-3
+    g, o = _run_both(gpu_ctx, om, mesh, eq, grid, 48, eps, 99)
+    _compare_fp64(g, o, name, exact_frac=0.9)
+    f = gpu_ctx.solve_grid(*grid, eq, Context.params(48, eps, 99, precision=WOS_FP32_PERSISTENT))
+    assert np.array_equal(f["location"], g["location"])
+    ins = g["location"] == 1
+    d = np.abs(f["mean"][ins] - g["mean"][ins])
+    comb = np.sqrt(f["standard_error"][ins] ** 2 + g["standard_error"][ins] ** 2) + 1e-12
+    print(f"{name} fp32-fp64: max|d|={d.max():.3e} max d/se={np.max(d / comb):.3f}")
+    assert np.all(d <= 4.0 * comb + 1e-6)
+
+
+def test_vertex_values_torus(gpu_ctx, oracle):
+    V, F, _ = M.torus(1.0, 0.35, 40, 20)
+    vals = np.random.default_rng(3).standard_normal(len(V))
+    eq = Pr.laplace_vertex_values(vals)
+    om = oracle.mesh(V, F)
+    g, o = _run_both(gpu_ctx, om, (V, F, None), eq, (10, 10, 4), 32, 1e-3, 3)
+    _compare_fp64(g, o, "torus vertex values", exact_frac=0.9)
+
+
+def test_point_list_edge_cases(gpu_ctx, oracle):
+    V, F, B = M.annulus(32)
+    gpu_ctx.set_mesh(V, F, B)
+    eq = Pr.annulus_laplace()
+    pts = np.array([[0.5, 0.1], [5.0, 5.0], [0.0, 0.0], list(V[0]), list(V[40]), [0.6, -0.2]])
+    for prec in (WOS_FP64_PLAIN, WOS_FP32_PERSISTENT):
+        r = gpu_ctx.solve_points(pts, eq, Context.params(1, 1e-3, 5, precision=prec))
+        assert list(r["location"]) == [1, 0, 0, 2, 2, 1]
+        assert r["variance"][0] == 0.0 and r["standard_error"][0] == 0.0
+        assert r["mean"][3] == 1.0 and r["mean"][4] == 0.0
+        r = gpu_ctx.solve_points(pts, eq, Context.params(16, 1e-3, 5, max_steps=1, precision=prec))
+        # every walk takes exactly one step; it counts as a max-steps hit
+        # unless that step landed inside the epsilon shell
+        assert r["mean_steps"][0] == 1.0 and r["max_steps_hits"][0] >= 10
+    om = oracle.mesh(V, F, B)
+    o = om.solve_points(eq, pts, 64, 1e-3, seed=5, rng_mode=1, n_threads=1)
+    r = gpu_ctx.solve_points(pts, eq, Context.params(64, 1e-3, 5))
+    assert np.array_equal(r["location"], o["location"])
+    ins = r["location"] == 1
+    assert np.allclose(r["mean"][ins], o["mean"][ins], rtol=0, atol=1e-12)
+
+
+def test_fp32_bitwise_reproducible(gpu_ctx):
+    V, F, _ = M.icosphere(3)
+    gpu_ctx.set_mesh(V, F)
+    eq = Pr.sphere_screened_constant()
+    p = Context.params(128, 1e-4, 7, precision=WOS_FP32_PERSISTENT)
+    a = gpu_ctx.solve_grid(12, 12, 12, eq, p)
+    b = gpu_ctx.solve_grid(12, 12, 12, eq, p)
+    for k in ("mean", "variance", "mean_steps", "location"):
+        assert np.array_equal(a[k], b[k], equal_nan=True), k

@@ -1,0 +1,711 @@
+// wos_device.hpp -- device functions of the walk-on-spheres kernels (gfx950).
+//
+// FP64 functions restate the reference operation for operation and are
+// compiled with -ffp-contract=off, so the classifier's location codes and the
+// FP64 walker's per-walk values match the reference / oracle arithmetic.
+// FP32 functions are the redesigned fast path (child-box BVH, conservative
+// radii); statistics stay FP64.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cmath>
+#include <cstdint>
+
+#include "../../include/wos_hip.h"
+#include "bvh_build.hpp"
+#include "wos_common.hpp"
+
+namespace wosx {
+
+// --------------------------------------------------------------------------
+// kernel-side views
+// --------------------------------------------------------------------------
+struct MeshView {
+    int dim;
+    int n_prims;
+    int n_nodes;
+    int depth;
+    // FP64, reference order
+    const NodeD2 *nd2;
+    const NodeD3 *nd3;
+    const SegD *segd;
+    const TriD *trid;
+    const int32_t *slot_prim;  // leaf slot -> primitive id
+    const int32_t *prim_bid;   // primitive -> boundary id (2D)
+    const int32_t *prims;      // primitive -> vertex ids (dim per primitive)
+    const double *verts;       // vertex coordinates (dim per vertex)
+    // FP32, child-box layout
+    const NodeF2 *nf2;
+    const NodeF3 *nf3;
+    const SegF *segf;
+    const float *seg_inv_vv;
+    const TriF *trif;
+    int32_t root_ref;
+    float shrink_abs;          // absolute conservative margin of FP32 radii
+};
+
+struct EqView {
+    int kind, source_mode, source_kind, boundary_kind;
+    double alpha, src_value, cx, cy, cz, src_radius;
+    double bnd[WOS_MAX_BND_VALUES];
+    int n_bnd;
+    const double *vertex_values;
+};
+
+// start of a walk, produced by the classifier for every point
+struct StartInfo {
+    double x, y, z;       // the point
+    double r;             // distance to the boundary
+    double nx, ny, nz;    // nearest boundary point
+    double g;             // g at the nearest point (Boundary value, zero-step walks)
+    double s, t;          // nearest primitive parameters
+    int32_t prim, bid;
+    uint32_t key0, key1;  // walk-stream key
+};
+
+// nearest-point query result (FP64)
+struct NpqD {
+    double d2;
+    double px, py, pz;
+    double s, t;
+    int32_t prim;
+};
+
+// --------------------------------------------------------------------------
+// FP64 primitives (src/npq.cpp:8-29, :52-189)
+// --------------------------------------------------------------------------
+__device__ __forceinline__ double npq_seg_d(const SegD &g, double px, double py, double &nx,
+                                            double &ny, double &par) {
+    const double vx = g.s1x - g.s0x, vy = g.s1y - g.s0y;
+    const double wx = px - g.s0x, wy = py - g.s0y;
+    const double vv = vx * vx + vy * vy;
+    const double wv = wx * vx + wy * vy;
+    par = (vv > 0.0) ? wv / vv : 0.0;
+    if (par < 0.0) par = 0.0;
+    if (par > 1.0) par = 1.0;
+    nx = g.s0x + par * vx;
+    ny = g.s0y + par * vy;
+    const double dx = px - nx, dy = py - ny;
+    return dx * dx + dy * dy;
+}
+
+__device__ __forceinline__ double eberly_d(const TriD &T, double px, double py, double pz,
+                                           double &nx, double &ny, double &nz, double &so,
+                                           double &to) {
+    const double a = T.a, b = T.b, c = T.c, det = T.det, denom = T.denom;
+    const double bpx = T.v0x - px, bpy = T.v0y - py, bpz = T.v0z - pz;
+    const double d = T.e1x * bpx + T.e1y * bpy + T.e1z * bpz;
+    const double e = T.e2x * bpx + T.e2y * bpy + T.e2z * bpz;
+    double s = b * e - c * d;
+    double t = b * d - a * e;
+    if (s + t <= det) {
+        if (s < 0) {
+            if (t < 0) {
+                if (d < 0) {
+                    t = 0;
+                    if (-d >= a) s = 1; else s = -d / a;
+                } else {
+                    s = 0;
+                    if (e >= 0) t = 0; else if (-e >= c) t = 1; else t = -e / c;
+                }
+            } else {
+                s = 0;
+                if (e >= 0) t = 0; else if (-e >= c) t = 1; else t = -e / c;
+            }
+        } else if (t < 0) {
+            t = 0;
+            if (d >= 0) s = 0; else if (-d >= a) s = 1; else s = -d / a;
+        } else {
+            s /= det;
+            t /= det;
+        }
+    } else {
+        if (s < 0) {
+            const double bd = b + d, ce = c + e;
+            if (ce > bd) {
+                const double numer = ce - bd;
+                if (numer >= denom) s = 1; else s = numer / denom;
+                t = 1 - s;
+            } else {
+                s = 0;
+                if (ce <= 0) t = 1; else if (e >= 0) t = 0; else t = -e / c;
+            }
+        } else if (t < 0) {
+            const double be = b + e, ad = a + d;
+            if (ad > be) {
+                const double numer = ad - be;
+                if (numer >= denom) t = 1; else t = numer / denom;
+                s = 1 - t;
+            } else {
+                t = 0;
+                if (ad <= 0) s = 1; else if (d >= 0) s = 0; else s = -d / a;
+            }
+        } else {
+            const double numer = (c + e) - (b + d);
+            if (numer <= 0) s = 0;
+            else if (numer >= denom) s = 1;
+            else s = numer / denom;
+            t = 1 - s;
+        }
+    }
+    nx = T.v0x + s * T.e1x + t * T.e2x;
+    ny = T.v0y + s * T.e1y + t * T.e2y;
+    nz = T.v0z + s * T.e1z + t * T.e2z;
+    const double dx = nx - px, dy = ny - py, dz = nz - pz;
+    so = s;
+    to = t;
+    return dx * dx + dy * dy + dz * dz;
+}
+
+// src/bvh.cpp:111-121 (dmax of include/wos/fastmath.hpp:11)
+__device__ __forceinline__ double dmaxd(double a, double b) { return a > b ? a : b; }
+__device__ __forceinline__ double box_d2_2(const NodeD2 &n, double px, double py) {
+    const double dx = dmaxd(0.0, dmaxd(n.lox - px, px - n.hix));
+    const double dy = dmaxd(0.0, dmaxd(n.loy - py, py - n.hiy));
+    return dx * dx + dy * dy;
+}
+__device__ __forceinline__ double box_d2_3(const NodeD3 &n, double px, double py, double pz) {
+    const double dx = dmaxd(0.0, dmaxd(n.lox - px, px - n.hix));
+    const double dy = dmaxd(0.0, dmaxd(n.loy - py, py - n.hiy));
+    const double dz = dmaxd(0.0, dmaxd(n.loz - pz, pz - n.hiz));
+    return dx * dx + dy * dy + dz * dz;
+}
+
+// Closest point, reference traversal (src/bvh.cpp:295-387): pop, cull by
+// the popped box, test leaf primitives with strict <, push the farther
+// child first.  `stk` is this lane's LDS stack (stride = block size).
+__device__ __forceinline__ NpqD npq_ref_2d(const MeshView &m, double px, double py, int *stk,
+                                           int stride) {
+    NpqD r;
+    r.d2 = INFINITY; r.px = r.py = r.pz = 0.0; r.s = r.t = 0.0; r.prim = -1;
+    int sp = 0;
+    stk[0] = 0;
+    sp = 1;
+    while (sp > 0) {
+        const int curr = stk[(--sp) * stride];
+        const NodeD2 n = m.nd2[curr];
+        if (box_d2_2(n, px, py) >= r.d2) continue;
+        if (n.count > 0) {
+            for (int i = 0; i < n.count; ++i) {
+                const int slot = n.idx + i;
+                double nx, ny, par;
+                const double d = npq_seg_d(m.segd[slot], px, py, nx, ny, par);
+                if (d < r.d2) {
+                    r.d2 = d; r.px = nx; r.py = ny; r.s = par; r.prim = m.slot_prim[slot];
+                }
+            }
+        } else {
+            const int l = curr + 1, rr = n.idx;
+            const double ld = box_d2_2(m.nd2[l], px, py);
+            const double rd = box_d2_2(m.nd2[rr], px, py);
+            if (ld <= rd) { stk[(sp++) * stride] = rr; stk[(sp++) * stride] = l; }
+            else { stk[(sp++) * stride] = l; stk[(sp++) * stride] = rr; }
+        }
+    }
+    return r;
+}
+
+__device__ __forceinline__ NpqD npq_ref_3d(const MeshView &m, double px, double py, double pz,
+                                           int *stk, int stride) {
+    NpqD r;
+    r.d2 = INFINITY; r.px = r.py = r.pz = 0.0; r.s = r.t = 0.0; r.prim = -1;
+    int sp = 1;
+    stk[0] = 0;
+    while (sp > 0) {
+        const int curr = stk[(--sp) * stride];
+        const NodeD3 n = m.nd3[curr];
+        if (box_d2_3(n, px, py, pz) >= r.d2) continue;
+        if (n.count > 0) {
+            for (int i = 0; i < n.count; ++i) {
+                const int slot = n.idx + i;
+                double nx, ny, nz, s, t;
+                const double d = eberly_d(m.trid[slot], px, py, pz, nx, ny, nz, s, t);
+                if (d < r.d2) {
+                    r.d2 = d; r.px = nx; r.py = ny; r.pz = nz; r.s = s; r.t = t;
+                    r.prim = m.slot_prim[slot];
+                }
+            }
+        } else {
+            const int l = curr + 1, rr = n.idx;
+            const double ld = box_d2_3(m.nd3[l], px, py, pz);
+            const double rd = box_d2_3(m.nd3[rr], px, py, pz);
+            if (ld <= rd) { stk[(sp++) * stride] = rr; stk[(sp++) * stride] = l; }
+            else { stk[(sp++) * stride] = l; stk[(sp++) * stride] = rr; }
+        }
+    }
+    return r;
+}
+
+// --------------------------------------------------------------------------
+// FP64 classification (src/inside.cpp)
+// --------------------------------------------------------------------------
+// src/inside.cpp:117-137 over every segment (order independent)
+__device__ __forceinline__ int winding_number_d(const MeshView &m, double px, double py) {
+    int wn = 0;
+    for (int s = 0; s < m.n_prims; ++s) {
+        const SegD g = m.segd[s];
+        const double area = (g.s1x - g.s0x) * (py - g.s0y) - (px - g.s0x) * (g.s1y - g.s0y);
+        if (g.s0y <= py) {
+            if (g.s1y > py && area > 0) wn++;
+        } else {
+            if (g.s1y <= py && area < 0) wn--;
+        }
+    }
+    return wn;
+}
+
+#define WOS_INTERSECTION_EPSILON 1e-9  // src/inside.cpp:14
+
+// src/inside.cpp:140-182 on the cached (v0, edge1, edge2): 0 miss, 1 hit, 2 ambiguous
+__device__ __forceinline__ int moller_trumbore_d(const TriD &T, double px, double py, double pz,
+                                                 double rx, double ry, double rz) {
+    const double e1x = T.e1x, e1y = T.e1y, e1z = T.e1z;
+    const double e2x = T.e2x, e2y = T.e2y, e2z = T.e2z;
+    const double sx = px - T.v0x, sy = py - T.v0y, sz = pz - T.v0z;
+    const double qx = ry * e2z - rz * e2y, qy = rz * e2x - rx * e2z, qz = rx * e2y - ry * e2x;
+    const double det = e1x * qx + e1y * qy + e1z * qz;
+    const double nx = e1y * e2z - e1z * e2y, ny = e1z * e2x - e1x * e2z, nz = e1x * e2y - e1y * e2x;
+    const double normal_length = sqrt(nx * nx + ny * ny + nz * nz);
+    const double det_tolerance = WOS_INTERSECTION_EPSILON * normal_length;
+    if (fabs(det) <= det_tolerance) {
+        if (normal_length == 0.0) return 0;
+        double edge_scale = 1.0;
+        const double l1 = sqrt(e1x * e1x + e1y * e1y + e1z * e1z);
+        const double l2 = sqrt(e2x * e2x + e2y * e2y + e2z * e2z);
+        if (l1 > edge_scale) edge_scale = l1;
+        if (l2 > edge_scale) edge_scale = l2;
+        const double plane_distance = fabs(nx * sx + ny * sy + nz * sz) / normal_length;
+        return plane_distance <= WOS_INTERSECTION_EPSILON * edge_scale ? 2 : 0;
+    }
+    const double u = (sx * qx + sy * qy + sz * qz) / det;
+    if (u < -WOS_INTERSECTION_EPSILON || u - 1.0 > WOS_INTERSECTION_EPSILON) return 0;
+    const double cx = sy * e1z - sz * e1y, cy = sz * e1x - sx * e1z, cz = sx * e1y - sy * e1x;
+    const double v = (rx * cx + ry * cy + rz * cz) / det;
+    if (v < -WOS_INTERSECTION_EPSILON || u + v - 1.0 > WOS_INTERSECTION_EPSILON) return 0;
+    const double t = (e2x * cx + e2y * cy + e2z * cz) / det;
+    if (t <= WOS_INTERSECTION_EPSILON) return 0;
+    const double w = 1.0 - u - v;
+    if (u <= WOS_INTERSECTION_EPSILON || v <= WOS_INTERSECTION_EPSILON ||
+        w <= WOS_INTERSECTION_EPSILON)
+        return 2;
+    return 1;
+}
+
+// src/inside.cpp:27-51
+__device__ __forceinline__ bool update_ray_interval(double origin, double direction, double box_min,
+                                                    double box_max, double &t_min, double &t_max) {
+    const double ext = fabs(box_max - box_min);
+    const double extent_scale = ext > 1.0 ? ext : 1.0;
+    double coord = 1.0;
+    if (fabs(origin) > coord) coord = fabs(origin);
+    if (fabs(box_min) > coord) coord = fabs(box_min);
+    if (fabs(box_max) > coord) coord = fabs(box_max);
+    const double padding = WOS_INTERSECTION_EPSILON * extent_scale + 16.0 * DBL_EPSILON * coord;
+    box_min -= padding;
+    box_max += padding;
+    if (direction == 0.0) return origin >= box_min && origin <= box_max;
+    double t0 = (box_min - origin) / direction;
+    double t1 = (box_max - origin) / direction;
+    if (t0 > t1) { const double tmp = t0; t0 = t1; t1 = tmp; }
+    if (t0 > t_min) t_min = t0;
+    if (t1 < t_max) t_max = t1;
+    return t_min <= t_max;
+}
+
+__device__ __forceinline__ bool ray_hits_box(const NodeD3 &n, double px, double py, double pz,
+                                             double rx, double ry, double rz) {
+    double tmin = 0.0, tmax = INFINITY;
+    return update_ray_interval(px, rx, n.lox, n.hix, tmin, tmax) &&
+           update_ray_interval(py, ry, n.loy, n.hiy, tmin, tmax) &&
+           update_ray_interval(pz, rz, n.loz, n.hiz, tmin, tmax);
+}
+
+// src/inside.cpp:78-111; returns hit count, sets ambiguous
+__device__ __forceinline__ int ray_cast_d(const MeshView &m, double px, double py, double pz,
+                                          double rx, double ry, double rz, bool &ambiguous,
+                                          int *stk, int stride) {
+    int hits = 0, sp = 1;
+    stk[0] = 0;
+    ambiguous = false;
+    while (sp > 0) {
+        const int ni = stk[(--sp) * stride];
+        const NodeD3 n = m.nd3[ni];
+        if (!ray_hits_box(n, px, py, pz, rx, ry, rz)) continue;
+        if (n.count > 0) {
+            for (int i = 0; i < n.count; ++i) {
+                const int h = moller_trumbore_d(m.trid[n.idx + i], px, py, pz, rx, ry, rz);
+                if (h == 2) { ambiguous = true; return hits; }
+                if (h == 1) ++hits;
+            }
+        } else {
+            stk[(sp++) * stride] = n.idx;
+            stk[(sp++) * stride] = ni + 1;
+        }
+    }
+    return hits;
+}
+
+// --------------------------------------------------------------------------
+// special functions (FP64: the oracle's methods; FP32: same, fewer terms)
+// --------------------------------------------------------------------------
+template <typename T> struct Lim;
+template <> struct Lim<double> { static constexpr double eps = DBL_EPSILON; };
+template <> struct Lim<float> { static constexpr float eps = FLT_EPSILON; };
+
+template <typename T>
+__device__ __forceinline__ T asymptotic_sum(T x, bool alternating) {
+    T sum = 1, term = 1, previous = (T)INFINITY;
+    for (int k = 1; k <= 24; ++k) {
+        const T odd = (T)(2 * k - 1);
+        T factor = odd * odd / ((T)8 * x * (T)k);
+        if (alternating) factor = -factor;
+        term *= factor;
+        const T mag = fabs(term);
+        if (mag > previous) break;
+        sum += term;
+        if (mag <= Lim<T>::eps * fabs(sum)) break;
+        previous = mag;
+    }
+    return sum;
+}
+
+// e^-x I0(x): power series below 50 (A&S 9.6.12), Hankel series above
+// (include/wos/bessel.hpp:40-48)
+template <typename T>
+__device__ __forceinline__ T i0_scaled(T x) {
+    x = fabs(x);
+    if (x < (T)50) {
+        const T q = (T)0.25 * x * x;
+        T term = 1, sum = 1;
+        for (int k = 1; k < 500; ++k) {
+            term *= q / ((T)k * (T)k);
+            sum += term;
+            if (term <= (T)1e-17 * sum) break;
+        }
+        return exp(-x) * sum;
+    }
+    return asymptotic_sum<T>(x, false) / sqrt((T)2 * (T)kPi * x);
+}
+
+// e^x K0(x): A&S 9.6.13 series for x <= 2, trapezoidal rule on
+// int_0^inf exp(-2x sinh^2(t/2)) dt (A&S 9.6.24) below 50, Hankel above
+template <typename T>
+__device__ __forceinline__ T k0_scaled(T x) {
+    if (x < (T)0) return (T)NAN;
+    if (x == (T)0) return (T)INFINITY;
+    if (x <= (T)2) {
+        const T q = (T)0.25 * x * x;
+        T term = 1, hk = 0, sum = 0, i0 = 1;
+        for (int k = 1; k < 200; ++k) {
+            term *= q / ((T)k * (T)k);
+            hk += (T)1 / (T)k;
+            i0 += term;
+            sum += term * hk;
+            if (term <= (T)1e-18) break;
+        }
+        const T k0 = -(log((T)0.5 * x) + (T)0.57721566490153286060651209) * i0 + sum;
+        return exp(x) * k0;
+    }
+    if (x < (T)50) {
+        const T h = (T)0.125;
+        T sum = (T)0.5;
+        for (int i = 1; i < 4000; ++i) {
+            const T sh = sinh((T)0.5 * h * (T)i);
+            const T v = exp((T)-2 * x * sh * sh);
+            sum += v;
+            if (v < (T)1e-18) break;
+        }
+        return h * sum;
+    }
+    return sqrt((T)kPi / ((T)2 * x)) * asymptotic_sum<T>(x, true);
+}
+
+template <typename T> __device__ __forceinline__ T log_i0(T x) {
+    x = fabs(x);
+    return x + log(i0_scaled<T>(x));
+}
+template <typename T> __device__ __forceinline__ T log_k0(T x) {
+    if (x < (T)0) return (T)NAN;
+    if (x == (T)0) return (T)INFINITY;
+    return -x + log(k0_scaled<T>(x));
+}
+
+// include/wos/poisson.hpp:12-38
+template <typename T> __device__ __forceinline__ T poisson_green_2d(T R, T r) {
+    if (!(r > (T)0) || r > R || R <= (T)0) return (T)NAN;
+    if (r == R) return (T)0;
+    return log(R / r) / ((T)2 * (T)kPi);
+}
+template <typename T> __device__ __forceinline__ T poisson_green_3d(T R, T r) {
+    if (!(r > (T)0) || r > R || R <= (T)0) return (T)NAN;
+    if (r == R) return (T)0;
+    return (R - r) / ((T)4 * (T)kPi * r * R);
+}
+// include/wos/screened.hpp:14-91
+template <typename T> __device__ __forceinline__ T screened_weight_2d(T alpha, T R) {
+    const T x = alpha * R;
+    if (x == (T)0) return (T)1;
+    if (x < (T)1e-3) {
+        const T x2 = x * x;
+        return (T)1 - x2 / (T)4 + (T)3 * x2 * x2 / (T)64;
+    }
+    return exp(-log_i0<T>(x));
+}
+template <typename T> __device__ __forceinline__ T screened_green_2d(T alpha, T R, T r) {
+    if (!(r > (T)0) || r > R || R <= (T)0) return (T)NAN;
+    if (r == R) return (T)0;
+    const T x = alpha * r, X = alpha * R;
+    if (X < (T)1e-5) return poisson_green_2d<T>(R, r);
+    const T log_a = log_k0<T>(x);
+    const T log_b = log_i0<T>(x) + log_k0<T>(X) - log_i0<T>(X);
+    T delta = log_b - log_a;
+    if (delta > (T)0) delta = (T)0;
+    return exp(log_a) * (-expm1(delta)) / ((T)2 * (T)kPi);
+}
+template <typename T> __device__ __forceinline__ T screened_mass_2d(T alpha, T R) {
+    const T x = alpha * R;
+    if (x < (T)1e-3) {
+        const T x2 = x * x;
+        return R * R * ((T)1 / (T)4 - (T)3 * x2 / (T)64 + (T)19 * x2 * x2 / (T)2304);
+    }
+    return -expm1(-log_i0<T>(x)) / (alpha * alpha);
+}
+template <typename T> __device__ __forceinline__ T screened_weight_3d(T alpha, T R) {
+    const T x = alpha * R;
+    if (x == (T)0) return (T)1;
+    if (x < (T)1e-3) {
+        const T x2 = x * x;
+        return (T)1 - x2 / (T)6 + (T)7 * x2 * x2 / (T)360;
+    }
+    return (T)2 * x * exp(-x) / (-expm1((T)-2 * x));
+}
+template <typename T> __device__ __forceinline__ T screened_green_3d(T alpha, T R, T r) {
+    if (!(r > (T)0) || r > R || R <= (T)0) return (T)NAN;
+    if (r == R) return (T)0;
+    const T x = alpha * r, X = alpha * R;
+    if (X < (T)1e-5) return poisson_green_3d<T>(R, r);
+    const T ratio = exp(-x) * (-expm1((T)-2 * (X - x))) / (-expm1((T)-2 * X));
+    return ratio / ((T)4 * (T)kPi * r);
+}
+template <typename T> __device__ __forceinline__ T screened_mass_3d(T alpha, T R) {
+    const T x = alpha * R;
+    if (x < (T)1e-3) {
+        const T x2 = x * x;
+        return R * R * ((T)1 / (T)6 - (T)7 * x2 / (T)360 + (T)31 * x2 * x2 / (T)15120);
+    }
+    return ((T)1 - screened_weight_3d<T>(alpha, R)) / (alpha * alpha);
+}
+
+// --------------------------------------------------------------------------
+// equation functors (src/equations/*.cpp) driven by the descriptor
+// --------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ T eq_source(const EqView &e, T x, T y, T z, int dim) {
+    const T dx = x - (T)e.cx, dy = y - (T)e.cy;
+    T acc = dx * dx + dy * dy;
+    if (dim == 3) { const T dz = z - (T)e.cz; acc = acc + dz * dz; }
+    switch (e.source_kind) {
+    case WOS_SRC_DISK_SQ: return acc <= (T)e.src_radius * (T)e.src_radius ? (T)e.src_value : (T)0;
+    case WOS_SRC_DISK_SQRT: return sqrt(acc) <= (T)e.src_radius ? (T)e.src_value : (T)0;
+    case WOS_SRC_CONSTANT: return (T)e.src_value;
+    case WOS_SRC_GAUSSIAN: return (T)e.src_value * exp(-acc / ((T)e.src_radius * (T)e.src_radius));
+    default: return (T)0;
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ bool eq_source_may_intersect(const EqView &e, T cx, T cy, T R) {
+    if (e.source_kind != WOS_SRC_DISK_SQ && e.source_kind != WOS_SRC_DISK_SQRT) return true;
+    const T dx = cx - (T)e.cx, dy = cy - (T)e.cy;
+    const T reach = R + (T)e.src_radius;
+    return dx * dx + dy * dy <= reach * reach;
+}
+
+// g at a nearest point (FP64 always)
+__device__ __forceinline__ double eq_boundary(const EqView &e, const MeshView &m, double x,
+                                              double y, double z, int prim, double s, double t) {
+    switch (e.boundary_kind) {
+    case WOS_BND_X2Y2_ID0: {
+        const int bid = prim >= 0 ? m.prim_bid[prim] : -1;
+        return bid == 0 ? x * x - y * y : 0.0;
+    }
+    case WOS_BND_Y5: {
+        const double r_sq = x * x + y * y;
+        const double y50 = z * (8 * z * z * z * z - 40 * z * z * r_sq + 15 * r_sq * r_sq);
+        const double y5m3 = (8 * z * z - r_sq) * y * (3 * x * x - y * y);
+        return y50 + 4.0 * y5m3;
+    }
+    case WOS_BND_ID_CONST: {
+        const int bid = prim >= 0 ? m.prim_bid[prim] : -1;
+        return (bid >= 0 && bid < e.n_bnd) ? e.bnd[bid] : 0.0;
+    }
+    case WOS_BND_VERTEX: {
+        if (prim < 0) return 0.0;
+        const int32_t *pr = m.prims + (size_t)prim * m.dim;
+        const double *vv = e.vertex_values;
+        if (m.dim == 2) return (1.0 - s) * vv[pr[0]] + s * vv[pr[1]];
+        return (1.0 - s - t) * vv[pr[0]] + s * vv[pr[1]] + t * vv[pr[2]];
+    }
+    case WOS_BND_SIN3X_COS2Y: return sin(3.0 * x) * cos(2.0 * y);
+    default: return 0.0;
+    }
+}
+
+// --------------------------------------------------------------------------
+// FP32 child-box traversal
+// --------------------------------------------------------------------------
+__device__ __forceinline__ float fmaxf3(float a, float b, float c) { return fmaxf(fmaxf(a, b), c); }
+
+// squared distance point -> box, FP32
+__device__ __forceinline__ float box_d2f(float lox, float loy, float loz, float hix, float hiy,
+                                         float hiz, float px, float py, float pz) {
+    const float dx = fmaxf3(0.0f, lox - px, px - hix);
+    const float dy = fmaxf3(0.0f, loy - py, py - hiy);
+    const float dz = fmaxf3(0.0f, loz - pz, pz - hiz);
+    return __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+}
+__device__ __forceinline__ float box_d2f2(float lox, float loy, float hix, float hiy, float px,
+                                          float py) {
+    const float dx = fmaxf3(0.0f, lox - px, px - hix);
+    const float dy = fmaxf3(0.0f, loy - py, py - hiy);
+    return __builtin_fmaf(dx, dx, dy * dy);
+}
+
+__device__ __forceinline__ float rcpf(float x) { return __builtin_amdgcn_rcpf(x); }
+
+// FP32 Eberly (same regions as npq.cpp:52-189), squared distance only
+__device__ __forceinline__ float eberly_f(const TriF &T, float px, float py, float pz) {
+    const float a = T.a, b = T.b, c = T.c, det = T.det, denom = T.denom;
+    const float bpx = T.v0x - px, bpy = T.v0y - py, bpz = T.v0z - pz;
+    const float d = __builtin_fmaf(T.e1x, bpx, __builtin_fmaf(T.e1y, bpy, T.e1z * bpz));
+    const float e = __builtin_fmaf(T.e2x, bpx, __builtin_fmaf(T.e2y, bpy, T.e2z * bpz));
+    float s = __builtin_fmaf(b, e, -c * d);
+    float t = __builtin_fmaf(b, d, -a * e);
+    if (s + t <= det) {
+        if (s < 0.0f) {
+            if (t < 0.0f) {
+                if (d < 0.0f) {
+                    t = 0.0f;
+                    s = (-d >= a) ? 1.0f : -d * rcpf(a);
+                } else {
+                    s = 0.0f;
+                    t = (e >= 0.0f) ? 0.0f : ((-e >= c) ? 1.0f : -e * rcpf(c));
+                }
+            } else {
+                s = 0.0f;
+                t = (e >= 0.0f) ? 0.0f : ((-e >= c) ? 1.0f : -e * rcpf(c));
+            }
+        } else if (t < 0.0f) {
+            t = 0.0f;
+            s = (d >= 0.0f) ? 0.0f : ((-d >= a) ? 1.0f : -d * rcpf(a));
+        } else {
+            s *= T.inv_det;
+            t *= T.inv_det;
+        }
+    } else {
+        if (s < 0.0f) {
+            const float bd = b + d, ce = c + e;
+            if (ce > bd) {
+                const float numer = ce - bd;
+                s = (numer >= denom) ? 1.0f : numer * rcpf(denom);
+                t = 1.0f - s;
+            } else {
+                s = 0.0f;
+                t = (ce <= 0.0f) ? 1.0f : ((e >= 0.0f) ? 0.0f : -e * rcpf(c));
+            }
+        } else if (t < 0.0f) {
+            const float be = b + e, ad = a + d;
+            if (ad > be) {
+                const float numer = ad - be;
+                t = (numer >= denom) ? 1.0f : numer * rcpf(denom);
+                s = 1.0f - t;
+            } else {
+                t = 0.0f;
+                s = (ad <= 0.0f) ? 1.0f : ((d >= 0.0f) ? 0.0f : -d * rcpf(a));
+            }
+        } else {
+            const float numer = (c + e) - (b + d);
+            s = (numer <= 0.0f) ? 0.0f : ((numer >= denom) ? 1.0f : numer * rcpf(denom));
+            t = 1.0f - s;
+        }
+    }
+    // nearest - p = (v0 - p) + s e1 + t e2
+    const float dx = __builtin_fmaf(t, T.e2x, __builtin_fmaf(s, T.e1x, bpx));
+    const float dy = __builtin_fmaf(t, T.e2y, __builtin_fmaf(s, T.e1y, bpy));
+    const float dz = __builtin_fmaf(t, T.e2z, __builtin_fmaf(s, T.e1z, bpz));
+    return __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+}
+
+__device__ __forceinline__ float seg_d2f(const SegF &g, float inv_vv, float px, float py) {
+    const float wx = px - g.s0x, wy = py - g.s0y;
+    float par = __builtin_fmaf(wx, g.vx, wy * g.vy) * inv_vv;
+    par = fminf(fmaxf(par, 0.0f), 1.0f);
+    const float dx = __builtin_fmaf(-par, g.vx, wx), dy = __builtin_fmaf(-par, g.vy, wy);
+    return __builtin_fmaf(dx, dx, dy * dy);
+}
+
+struct StackEntry { int32_t ref; float d2; };
+
+// Closest point FP32: near child first, far child with its box distance on
+// the LDS stack (popped entries whose distance is no longer below the best
+// are dropped without a node load).  Returns the squared distance and the
+// best leaf slot.
+template <int DIM>
+__device__ __forceinline__ float npq_f(const MeshView &m, float px, float py, float pz,
+                                       int &best_slot, StackEntry *stk, int stride) {
+    float best = INFINITY;
+    best_slot = -1;
+    int sp = 0;
+    int32_t ref = m.root_ref;
+    for (;;) {
+        if (ref >= 0) {
+            float dl, dr;
+            int32_t cl, cr;
+            if constexpr (DIM == 3) {
+                const NodeF3 n = m.nf3[ref];
+                dl = box_d2f(n.l_lox, n.l_loy, n.l_loz, n.l_hix, n.l_hiy, n.l_hiz, px, py, pz);
+                dr = box_d2f(n.r_lox, n.r_loy, n.r_loz, n.r_hix, n.r_hiy, n.r_hiz, px, py, pz);
+                cl = n.left; cr = n.right;
+            } else {
+                const NodeF2 n = m.nf2[ref];
+                dl = box_d2f2(n.l_lox, n.l_loy, n.l_hix, n.l_hiy, px, py);
+                dr = box_d2f2(n.r_lox, n.r_loy, n.r_hix, n.r_hiy, px, py);
+                cl = n.left; cr = n.right;
+            }
+            const bool hl = dl < best, hr = dr < best;
+            if (hl && hr) {
+                const bool left_first = dl <= dr;
+                const int32_t near = left_first ? cl : cr;
+                StackEntry far;
+                far.ref = left_first ? cr : cl;
+                far.d2 = left_first ? dr : dl;
+                stk[(sp++) * stride] = far;
+                ref = near;
+                continue;
+            }
+            if (hl) { ref = cl; continue; }
+            if (hr) { ref = cr; continue; }
+        } else {
+            const int32_t code = -ref - 1;
+            const int first = code >> 3, count = code & 7;
+            for (int i = 0; i < count; ++i) {
+                const int slot = first + i;
+                float d;
+                if constexpr (DIM == 3) d = eberly_f(m.trif[slot], px, py, pz);
+                else d = seg_d2f(m.segf[slot], m.seg_inv_vv[slot], px, py);
+                if (d < best) { best = d; best_slot = slot; }
+            }
+        }
+        // pop
+        bool found = false;
+        while (sp > 0) {
+            const StackEntry e = stk[(--sp) * stride];
+            if (e.d2 < best) { ref = e.ref; found = true; break; }
+        }
+        if (!found) break;
+    }
+    return best;
+}
+
+}  // namespace wosx

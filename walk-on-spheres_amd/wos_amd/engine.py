@@ -1,0 +1,109 @@
+"""Python host mirror of the C ABI: a context per GPU, a mesh, grid and
+point-list solves.  Everything runs in libwos_hip.so; this module only
+marshals numpy arrays.
+
+Reference counterparts: run<N,Eq> (include/wos/runner.hpp:22-308) for
+`solve_grid`, wos::wos_run as the welding apps call it
+(apps/welding/src/level1_stationary_heat.cpp:257) for `solve_points`.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import capi
+from .capi import (EqDesc, PointStats, RunParams, RunSummary, WOS_FP32_PERSISTENT,
+                   WOS_FP64_PLAIN, ptr)
+
+
+def screened_epsilon(epsilon: float, alpha: float) -> float:
+    """Stopping distance the reference uses for screened_poisson:
+    min(epsilon, 0.05 / alpha) (src/equations/screened_poisson.cpp:20-38)."""
+    return min(epsilon, 0.05 / alpha)
+
+
+class Context:
+    """One GPU's engine state (wos_ctx)."""
+
+    def __init__(self, device: int = 0):
+        self.lib = capi.load()
+        h = self.lib.wos_ctx_create(device)
+        if not h:
+            raise RuntimeError("wos_ctx_create failed: " + self.lib.wos_last_error().decode())
+        self.h = C.c_void_p(h)
+        self.dim = None
+        self.n_verts = 0
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.wos_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_mesh(self, verts, prims, boundary_ids=None):
+        v = np.ascontiguousarray(verts, np.float64)
+        p = np.ascontiguousarray(prims, np.int32)
+        b = None if boundary_ids is None else np.ascontiguousarray(boundary_ids, np.int32)
+        dim = p.shape[1]
+        if v.shape[1] != dim:
+            v = np.ascontiguousarray(v[:, :dim])
+        rc = self.lib.wos_ctx_set_mesh(self.h, dim, ptr(v, C.c_double), len(v),
+                                       ptr(p, C.c_int32), len(p), ptr(b, C.c_int32))
+        capi.check(self.lib, rc)
+        self.dim = dim
+        self.n_verts = len(v)
+
+    @staticmethod
+    def params(n_walks, epsilon, seed, max_steps=1000, max_ray_attempts=12,
+               precision=WOS_FP64_PLAIN, target_se=0.0, max_walks=0, tune=0):
+        p = RunParams()
+        p.n_walks = n_walks
+        p.max_steps = max_steps
+        p.max_ray_attempts = max_ray_attempts
+        p.precision = precision
+        p.epsilon = epsilon
+        p.seed = seed
+        p.target_se = target_se
+        p.max_walks = max_walks
+        p.tune = tune
+        return p
+
+    def _solve(self, fn, n, eq: EqDesc, prm: RunParams, *args):
+        out = dict(mean=np.zeros(n), variance=np.zeros(n), standard_error=np.zeros(n),
+                   mean_steps=np.zeros(n), location=np.zeros(n, np.uint8),
+                   max_steps_hits=np.zeros(n, np.int32), walks=np.zeros(n, np.int32),
+                   ambiguous_ray_retries=np.zeros(n, np.int32))
+        st = PointStats()
+        st.mean = ptr(out["mean"], C.c_double)
+        st.variance = ptr(out["variance"], C.c_double)
+        st.standard_error = ptr(out["standard_error"], C.c_double)
+        st.mean_steps = ptr(out["mean_steps"], C.c_double)
+        st.location = ptr(out["location"], C.c_uint8)
+        st.max_steps_hits = ptr(out["max_steps_hits"], C.c_int32)
+        st.walks = ptr(out["walks"], C.c_int32)
+        st.ambiguous_ray_retries = ptr(out["ambiguous_ray_retries"], C.c_int32)
+        summ = RunSummary()
+        rc = fn(self.h, *args, C.byref(eq), C.byref(prm), C.byref(st), C.byref(summ))
+        capi.check(self.lib, rc)
+        out["summary"] = {k: getattr(summ, k) for k, _ in RunSummary._fields_}
+        out["summary"]["grid_bounds"] = list(summ.grid_bounds)
+        return out
+
+    def solve_grid(self, nx, ny, nz, eq: EqDesc, prm: RunParams):
+        if self.dim == 2:
+            nz = 1
+        return self._solve(self.lib.wos_solve_grid, nx * ny * nz, eq, prm, nx, ny, nz)
+
+    def solve_points(self, points, eq: EqDesc, prm: RunParams):
+        pts = np.ascontiguousarray(points, np.float64)
+        return self._solve(self.lib.wos_solve_points, len(pts), eq, prm,
+                           ptr(pts, C.c_double), len(pts))
+
+
+__all__ = ["Context", "screened_epsilon", "WOS_FP64_PLAIN", "WOS_FP32_PERSISTENT"]
