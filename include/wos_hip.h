@@ -135,7 +135,18 @@ typedef struct wos_run_summary {
     double boundary_tolerance;   /* 1e-10 * max(1, extent) (runner.hpp:86-88)      */
     int32_t launch_blocks;
     int32_t launch_threads;
+    int32_t lds_cached_nodes;    /* BVH nodes staged in LDS by the FP32 walker     */
+    int32_t rounds;              /* walker launches (adaptive mode: > 1)           */
 } wos_run_summary;
+
+/* Launch shape of the FP32 persistent walker (set by the autotuner). */
+typedef struct wos_tuning {
+    int32_t block_threads;       /* 64, 128, 192 or 256 walkers per block          */
+    int32_t lds_nodes;           /* BVH nodes cached in LDS per block; -1 = all that
+                                    fit without lowering occupancy                  */
+    int32_t chunk;               /* walks a wavefront claims per refill            */
+    int32_t blocks_per_cu;       /* persistent blocks per CU; 0 = occupancy limit  */
+} wos_tuning;
 
 typedef struct wos_ctx wos_ctx;
 
@@ -158,6 +169,20 @@ void wos_ctx_destroy(wos_ctx *ctx);
 int wos_ctx_set_mesh(wos_ctx *ctx, int dim, const double *verts, int64_t n_verts,
                      const int32_t *prims, int64_t n_prims,
                      const int32_t *boundary_ids);
+
+/* Host-only BVH build with the reference split rule, exported for
+ * verification (replaces build_bvh, src/bvh.cpp:234-253; no device work).
+ * Returns the node count (or -1 on error).  With non-NULL arrays it fills
+ * per node: count (0 = internal), idx (leaf: first slot; internal: right
+ * child), lo/hi (3 doubles each), and slots[n_prims] (leaf slot -> primitive). */
+int64_t wos_bvh_build_host(int dim, const double *verts, int64_t n_verts,
+                           const int32_t *prims, int64_t n_prims, int32_t *count,
+                           int32_t *idx, double *lo, double *hi, int32_t *slots);
+
+/* Launch shape of the FP32 walker.  No reference counterpart (the reference
+ * has one CPU code path); used by the autotuner. */
+int wos_ctx_set_tuning(wos_ctx *ctx, const wos_tuning *tuning);
+int wos_ctx_get_tuning(const wos_ctx *ctx, wos_tuning *tuning);
 
 /* Sweep an Nx x Ny x Nz grid over the mesh bounding box (Nz forced to 1 in
  * 2D), classify every point, walk every inside point and write row-major

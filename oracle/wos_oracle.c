@@ -196,7 +196,9 @@ double orc_k0_scaled(double x) {
         return exp(x) * k0;
     }
     if (x < 50.0) {
-        const double h = 0.125;
+        /* the integrand is about exp(-x t^2 / 2): keep the step below
+         * 0.5 / sqrt(x) so the trapezoid's aliasing error stays < e^-70 */
+        const double h = fmin(0.125, 0.5 / sqrt(x));
         double sum = 0.5;  /* t = 0 end point, weight 1/2 */
         for (int i = 1; i < 4000; ++i) {
             const double sh = sinh(0.5 * h * (double)i);

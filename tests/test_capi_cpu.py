@@ -5,6 +5,7 @@ import ctypes as C
 import os
 import re
 
+import numpy as np
 import pytest
 
 from wos_amd import capi
@@ -55,3 +56,46 @@ def test_null_context_is_an_input_error():
     rc = lib.wos_ctx_set_mesh(None, 2, None, 0, None, 0, None)
     assert rc == capi.WOS_ERR_INPUT
     assert b"NULL" in lib.wos_last_error()
+
+
+def _host_bvh(lib, V, F):
+    V = np.ascontiguousarray(V, np.float64)
+    F = np.ascontiguousarray(F, np.int32)
+    dim = F.shape[1]
+    n = lib.wos_bvh_build_host(dim, capi.ptr(V, C.c_double), len(V), capi.ptr(F, C.c_int32),
+                               len(F), None, None, None, None, None)
+    assert n > 0
+    cnt = np.zeros(n, np.int32)
+    idx = np.zeros(n, np.int32)
+    lo = np.zeros((n, 3))
+    hi = np.zeros((n, 3))
+    slots = np.zeros(len(F), np.int32)
+    lib.wos_bvh_build_host(dim, capi.ptr(V, C.c_double), len(V), capi.ptr(F, C.c_int32), len(F),
+                           capi.ptr(cnt, C.c_int32), capi.ptr(idx, C.c_int32),
+                           capi.ptr(lo, C.c_double), capi.ptr(hi, C.c_double),
+                           capi.ptr(slots, C.c_int32))
+    return cnt, idx, lo, hi, slots
+
+
+def test_host_bvh_matches_reference_golden():
+    """The product's BVH builder reproduces the reference node array and its
+    leaf-slot order (same split rule, std::nth_element with the same order)."""
+    lib = capi.load()
+    g = np.load(os.path.join(ROOT, "tests", "golden", "bvh.npz"), allow_pickle=False)
+    names = sorted({k.split("__")[0] for k in g.files})
+    for name in names:
+        cnt, idx, lo, hi, slots = _host_bvh(lib, g[f"{name}__mesh_verts"], g[f"{name}__mesh_prims"])
+        assert np.array_equal(cnt, g[f"{name}__prim_count"]), name
+        assert np.array_equal(idx, g[f"{name}__idx"]), name
+        assert np.array_equal(lo, g[f"{name}__bmin"]), name
+        assert np.array_equal(hi, g[f"{name}__bmax"]), name
+        assert np.array_equal(slots, g[f"{name}__slots"]), name
+
+
+def test_host_bvh_rejects_bad_mesh():
+    lib = capi.load()
+    V = np.zeros((3, 2))
+    F = np.array([[0, 7]], np.int32)
+    assert lib.wos_bvh_build_host(2, capi.ptr(V, C.c_double), 3, capi.ptr(F, C.c_int32), 1,
+                                  None, None, None, None, None) == -1
+    assert b"out of range" in lib.wos_last_error()

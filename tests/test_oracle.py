@@ -1,0 +1,310 @@
+"""Pins of the C oracle (CPU only).
+
+1. Golden vectors written by tests/golden/make_golden.py from the reference's
+   own compiled core: bit-for-bit equality (screened 2D walks, whose Bessel
+   functions come from libstdc++ in the reference, to 1e-12 relative).
+2. When oracle/_ref is built (this container), a live comparison on fresh
+   random inputs.
+3. Mathematics the reference does not pin: Philox known-answer vectors
+   (Random123), scipy's i0e/k0e, Green-function masses by quadrature,
+   brute-force closest points, analytic inside tests and analytic solutions.
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+from scipy import integrate, special
+
+import pyoracle
+from wos_amd import meshgen, problems
+from wos_amd.capi import WOS_BND_X2Y2_ID0, WOS_EQ_LAPLACE, make_eq
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def load(name):
+    return np.load(os.path.join(GOLD, name + ".npz"), allow_pickle=False)
+
+
+def gold_mesh(g, name):
+    V = g[f"{name}__mesh_verts"]
+    F = g[f"{name}__mesh_prims"]
+    B = g[f"{name}__mesh_bids"] if f"{name}__mesh_bids" in g.files else None
+    return V, F, B
+
+
+def leaf_sets(prim_count, idx, slots):
+    return [sorted(slots[idx[i]:idx[i] + prim_count[i]]) for i in np.nonzero(prim_count > 0)[0]]
+
+
+# ------------------------------------------------------------------ rng
+def test_rng_golden(oracle):
+    g = load("rng")
+    for x, y in zip(g["splitmix_in"], g["splitmix_out"]):
+        assert oracle.splitmix64(int(x)) == int(y)
+    for s, draws in zip(g["prng_seeds"], g["prng_draws"]):
+        assert np.array_equal(oracle.xoshiro(int(s), len(draws)), draws)
+
+
+def test_philox_known_answers(oracle):
+    # Random123 kat_vectors, philox4x32 with 10 rounds
+    kat = [
+        ([0, 0, 0, 0], [0, 0], [0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8]),
+        ([0xFFFFFFFF] * 4, [0xFFFFFFFF] * 2, [0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD]),
+        ([0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344], [0xA4093822, 0x299F31D0],
+         [0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1]),
+    ]
+    for ctr, key, out in kat:
+        assert list(oracle.philox(ctr, key)) == out
+
+
+def test_philox_stream_is_uniform(oracle):
+    # top-53-bit uniforms of 4096 counters: Kolmogorov-Smirnov against U(0,1)
+    from scipy import stats
+    u = []
+    for w in range(2048):
+        o = oracle.philox([w, 3, 0, 0], [0x1234, 0x5678])
+        u.append(((int(o[0]) | int(o[1]) << 32) >> 11) / 2.0 ** 53)
+        u.append(((int(o[2]) | int(o[3]) << 32) >> 11) / 2.0 ** 53)
+    assert stats.kstest(u, "uniform").pvalue > 1e-3
+
+
+# ------------------------------------------------------------------ geometry
+def test_bvh_matches_reference_nodes(oracle):
+    g = load("bvh")
+    names = sorted({k.split("__")[0] for k in g.files})
+    assert len(names) >= 6
+    for name in names:
+        V, F, B = gold_mesh(g, name)
+        e = oracle.mesh(V, F, B).export()
+        for k in ("prim_count", "idx", "bmin", "bmax"):
+            assert np.array_equal(e[k], g[f"{name}__{k}"]), (name, k)
+        # the same primitive set in every leaf; the order inside a leaf is an
+        # nth_element detail that only matters at exact distance ties
+        assert leaf_sets(e["prim_count"], e["idx"], e["prims"]) == \
+            leaf_sets(g[f"{name}__prim_count"], g[f"{name}__idx"], g[f"{name}__slots"]), name
+
+
+def test_npq_and_winding_golden(oracle):
+    g = load("npq")
+    meshes = {"annulus32": meshgen.annulus(32), "star96": meshgen.star(96, 0.3, 5),
+              "ico1": meshgen.icosphere(1), "torus": meshgen.torus(1.0, 0.35, 24, 12)}
+    for name, (V, F, B) in meshes.items():
+        om = oracle.mesh(V, F, B)
+        pts = g[f"{name}__points"]
+        for i, p in enumerate(pts):
+            r = om.npq(p)
+            assert r["distance"] == g[f"{name}__distance"][i], (name, i)
+            # equal distance; at a shared edge or vertex the tied primitives'
+            # nearest points differ by rounding and the leaf order picks one
+            assert np.allclose(r["point"], g[f"{name}__nearest"][i], rtol=0, atol=1e-15), (name, i)
+            assert om.npq(p, brute=True)["distance"] == r["distance"]
+            if F.shape[1] == 2:
+                assert r["bid"] == g[f"{name}__bid"][i]
+                assert om.winding(p) == g[f"{name}__winding"][i]
+
+
+def test_primitives_golden(oracle):
+    g = load("primitives")
+    tri = g["mt_tri"]
+    for p, r, out in zip(g["mt_origin"], g["mt_ray"], g["mt_out"]):
+        assert oracle.moller_trumbore(p, r, *tri) == out
+    assert set(g["mt_out"]) == {0, 1, 2}
+    for p, d2 in zip(g["seg_pts"], g["seg_d2"]):
+        assert oracle.npq_seg(g["seg_s0"], g["seg_s1"], p)[0] == d2
+    t = g["tri"]
+    for p, out in zip(g["tri_pts"], g["tri_out"]):
+        d, n, _ = oracle.eberly(p, *t)
+        assert d == out[0] and np.array_equal(n, out[1:])
+
+
+def test_classify_golden(oracle):
+    g = load("classify")
+    meshes = {"annulus32": meshgen.annulus(32), "star96": meshgen.star(96, 0.3, 5),
+              "ico2": meshgen.icosphere(2), "torus": meshgen.torus(1.0, 0.35, 24, 12)}
+    for name, (V, F, B) in meshes.items():
+        om = oracle.mesh(V, F, B)
+        grid = tuple(int(x) for x in g[f"{name}__grid"])
+        eq = problems.laplace_builtin(F.shape[1])
+        r = om.solve_grid(eq, *grid, 1, 1e-3, seed=2024, rng_mode=0, n_threads=8)
+        assert np.array_equal(r["location"], g[f"{name}__location"]), name
+        assert np.array_equal(r["ambiguous"], g[f"{name}__ambiguous"]), name
+
+
+# ------------------------------------------------------------------ estimator
+def _golden_cases():
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("make_golden", os.path.join(GOLD, "make_golden.py"))
+    mg = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mg)
+    return mg.eq_cases(), mg.meshes()
+
+
+def test_walks_golden_reference_streams(oracle):
+    g = load("walks")
+    cases, meshes = _golden_cases()
+    for name, mname, eqf, p0, n, eps in cases:
+        V, F, B = meshes[mname]
+        om = oracle.mesh(V, F, B)
+        ps = int(g[f"{name}__point_seed"][0])
+        r = om.walks(eqf(), np.array(p0, float), n, eps, 1000, 0, ps)
+        ref_v, ref_s = g[f"{name}__values"], g[f"{name}__steps"]
+        if name.startswith("screened2d"):
+            # the reference evaluates I0/K0 with libstdc++; the oracle with its
+            # own series (pinned against scipy below)
+            assert np.array_equal(r["steps"], ref_s), name
+            assert np.allclose(r["values"], ref_v, rtol=1e-12, atol=1e-15), name
+        else:
+            assert np.array_equal(r["values"], ref_v), name
+            assert np.array_equal(r["steps"], ref_s), name
+
+
+def test_sweeps_golden(oracle):
+    g = load("sweeps")
+    cases = {"annulus_laplace": (meshgen.annulus(32), problems.laplace_builtin(2), (12, 12, 1), 32, 1e-3),
+             "circle_poisson_green": (meshgen.circle(48), problems.poisson_builtin(2, 1), (10, 10, 1), 32, 1e-3),
+             "ico_screened": (meshgen.icosphere(2), problems.screened_builtin(3, 5.0), (6, 6, 6), 16, 1e-3)}
+    for name, ((V, F, B), eq, grid, n, eps) in cases.items():
+        r = oracle.mesh(V, F, B).solve_grid(eq, *grid, n, eps, seed=31337, rng_mode=0, n_threads=4)
+        for k in ("mean", "variance", "standard_error", "mean_steps", "location", "hits"):
+            assert np.array_equal(r[k], g[f"{name}__{k}"], equal_nan=True), (name, k)
+
+
+def test_welford_statistics_match_numpy(oracle):
+    V, F, B = meshgen.circle(64)
+    om = oracle.mesh(V, F, B)
+    eq = problems.poisson_builtin(2, 1)
+    for mode in (0, 1):
+        r = om.walks(eq, [0.05, 0.02], 257, 1e-3, 1000, mode, 12345)
+        v = r["values"]
+        assert abs(r["mean"] - v.mean()) <= 1e-15 * max(1, abs(v.mean())) * 10
+        assert math.isclose(r["variance"], v.var(ddof=1), rel_tol=1e-12)
+        assert math.isclose(r["standard_error"], math.sqrt(v.var(ddof=1) / 257), rel_tol=1e-12)
+        assert r["mean_steps"] == r["steps"].mean()
+
+
+# ------------------------------------------------------------------ math pins
+@pytest.mark.parametrize("x", [1e-8, 1e-3, 0.1, 0.5, 1.0, 1.999, 2.0, 2.001, 3.7, 10.0,
+                               25.0, 49.9, 50.0, 80.0, 500.0])
+def test_bessel_scaled_against_scipy(oracle, x):
+    assert math.isclose(oracle.L.orc_i0_scaled(x), special.i0e(x), rel_tol=5e-14)
+    assert math.isclose(oracle.L.orc_k0_scaled(x), special.k0e(x), rel_tol=5e-14)
+
+
+def test_green_functions(oracle):
+    L = oracle.L
+    R, r = 0.7, 0.2
+    assert math.isclose(L.orc_poisson_green_2d(R, r), math.log(R / r) / (2 * math.pi), rel_tol=1e-15)
+    assert math.isclose(L.orc_poisson_green_3d(R, r), (R - r) / (4 * math.pi * r * R), rel_tol=1e-15)
+    assert L.orc_poisson_green_2d(R, R) == 0.0 and math.isnan(L.orc_poisson_green_2d(R, 1.1 * R))
+    for alpha in (0.5, 5.0, 40.0):
+        x = alpha * R
+        assert math.isclose(L.orc_screened_weight_3d(alpha, R), x / math.sinh(x), rel_tol=1e-13)
+        assert math.isclose(L.orc_screened_weight_2d(alpha, R), 1 / special.i0(x), rel_tol=1e-13)
+        m2 = integrate.quad(lambda s: 2 * math.pi * s * L.orc_screened_green_2d(alpha, R, s), 0, R,
+                            limit=200, points=[1e-6])[0]
+        m3 = integrate.quad(lambda s: 4 * math.pi * s * s * L.orc_screened_green_3d(alpha, R, s), 0, R,
+                            limit=200)[0]
+        assert math.isclose(L.orc_screened_mass_2d(alpha, R), m2, rel_tol=1e-7)
+        assert math.isclose(L.orc_screened_mass_3d(alpha, R), m3, rel_tol=1e-9)
+        # screened Green functions solve (Delta - alpha^2) G = 0 away from 0:
+        # 3D radial G = C sinh(alpha (R - r)) / r
+        c = L.orc_screened_green_3d(alpha, R, 0.3 * R) * 0.3 * R / math.sinh(alpha * 0.7 * R)
+        assert math.isclose(L.orc_screened_green_3d(alpha, R, 0.6 * R) * 0.6 * R,
+                            c * math.sinh(alpha * 0.4 * R), rel_tol=1e-10)
+    # Poisson masses: R^2/4 and R^2/6
+    assert math.isclose(integrate.quad(lambda s: 2 * math.pi * s * L.orc_poisson_green_2d(R, s), 0, R)[0],
+                        R * R / 4, rel_tol=1e-9)
+
+
+def test_closest_point_brute_force(oracle):
+    rng = np.random.default_rng(5)
+    t = rng.uniform(-1, 1, (3, 3))
+    # dense barycentric sampling of the triangle
+    a, b = np.meshgrid(np.linspace(0, 1, 301), np.linspace(0, 1, 301))
+    keep = a + b <= 1
+    S = t[0] + np.outer(a[keep], t[1] - t[0]) + np.outer(b[keep], t[2] - t[0])
+    for _ in range(50):
+        p = rng.uniform(-2, 2, 3)
+        d2, n, _ = oracle.eberly(p, *t)
+        dense = np.min(np.sum((S - p) ** 2, axis=1))
+        assert d2 <= dense + 1e-12 and d2 >= dense - 2e-2 * math.sqrt(dense) - 1e-12
+        assert math.isclose(np.sum((n - p) ** 2), d2, rel_tol=1e-12, abs_tol=1e-14)
+    s0, s1 = np.array([0.3, -0.2]), np.array([-0.4, 0.9])
+    lin = s0 + np.outer(np.linspace(0, 1, 20001), s1 - s0)
+    for _ in range(50):
+        p = rng.uniform(-2, 2, 2)
+        d2, n, par = oracle.npq_seg(s0, s1, p)
+        assert d2 <= np.min(np.sum((lin - p) ** 2, axis=1)) + 1e-15
+        assert 0 <= par <= 1
+
+
+def test_winding_and_ray_vote_analytic(oracle):
+    a, k = 0.3, 5
+    V, F, B = meshgen.star(720, a, k)
+    om = oracle.mesh(V, F, B)
+    rng = np.random.default_rng(2)
+    for p in rng.uniform(-1.4, 1.4, (400, 2)):
+        r, th = np.hypot(*p), math.atan2(p[1], p[0])
+        rb = 1 + a * math.cos(k * th)
+        if abs(r - rb) > 0.01:
+            assert (om.winding(p) != 0) == (r < rb)
+    V, F, _ = meshgen.icosphere(3)
+    om = oracle.mesh(V, F)
+    for i, p in enumerate(rng.uniform(-1.1, 1.1, (200, 3))):
+        r = np.linalg.norm(p)
+        if abs(r - 0.99) > 0.02:
+            code, _ = om.classify(p, 1.0, 1e-10, 12, 1000 + i)
+            assert code == (1 if r < 0.99 else 0)
+
+
+def test_estimator_harmonic_and_screened(oracle):
+    # x^2 - y^2 is harmonic: Laplace on a one-loop circle (boundary id 0)
+    V, F, B = meshgen.circle(256)
+    om = oracle.mesh(V, F, B)
+    eq = make_eq(WOS_EQ_LAPLACE, boundary_kind=WOS_BND_X2Y2_ID0)
+    for p in ([0.3, 0.2], [-0.5, 0.4], [0.0, -0.7]):
+        r = om.walks(eq, p, 4000, 1e-4, 1000, 1, 77)
+        assert abs(r["mean"] - (p[0] ** 2 - p[1] ** 2)) <= 4 * r["standard_error"] + 2e-3
+    # Y5 harmonic on any closed surface: u = g
+    V, F, _ = meshgen.icosphere(3)
+    om = oracle.mesh(V, F)
+    eq = problems.laplace_builtin(3)
+    p = np.array([0.2, -0.3, 0.1])
+    r = om.walks(eq, p, 4000, 1e-4, 1000, 1, 78)
+    x, y, z = p
+    rs = x * x + y * y
+    g = z * (8 * z ** 4 - 40 * z * z * rs + 15 * rs * rs) + 4 * (8 * z * z - rs) * y * (3 * x * x - y * y)
+    assert abs(r["mean"] - g) <= 4 * r["standard_error"] + 2e-3
+    # screened Poisson, f = 1, g = 0 on the unit ball (Green sampling)
+    alpha = 10.0
+    eq = problems.sphere_screened_constant(alpha, 1.0, 1)
+    for p in ([0.0, 0.0, 0.0], [0.5, 0.1, -0.2]):
+        r = om.walks(eq, p, 4000, 1e-4, 1000, 1, 79)
+        rr = np.linalg.norm(p)
+        exact = (1 - (alpha / math.sinh(alpha) if rr == 0 else math.sinh(alpha * rr) / (rr * math.sinh(alpha)))) / alpha ** 2
+        assert abs(r["mean"] - exact) <= 4 * r["standard_error"] + 3e-4
+
+
+# ------------------------------------------------------------------ live reference
+def test_live_reference_random_meshes(oracle, reference):
+    rng = np.random.default_rng(99)
+    for trial in range(3):
+        n = int(rng.integers(40, 400))
+        V, F, B = meshgen.star(n, float(rng.uniform(0.1, 0.4)), int(rng.integers(3, 10)))
+        a, b = oracle.mesh(V, F, B).export(), reference.mesh(V, F, B).export()
+        for k in ("prim_count", "idx", "bmin", "bmax"):
+            assert np.array_equal(a[k], b[k])
+        V, F, _ = meshgen.torus(1.0, float(rng.uniform(0.2, 0.5)), int(rng.integers(8, 30)),
+                                int(rng.integers(6, 16)))
+        om, rm = oracle.mesh(V, F), reference.mesh(V, F)
+        a, b = om.export(), rm.export()
+        for k in ("prim_count", "idx", "bmin", "bmax"):
+            assert np.array_equal(a[k], b[k])
+        eq = problems.laplace_builtin(3)
+        pts = rng.uniform(-1.4, 1.4, (64, 3))
+        ro = om.solve_points(eq, pts, 8, 1e-3, seed=trial, rng_mode=0, n_threads=4)
+        rr = rm.solve_points(eq, pts, 8, 1e-3, seed=trial, n_threads=4)
+        for k in ("mean", "variance", "location", "ambiguous"):
+            assert np.array_equal(ro[k], rr[k], equal_nan=True), k

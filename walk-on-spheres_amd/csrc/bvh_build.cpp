@@ -177,8 +177,9 @@ DeviceLayouts make_layouts(const HostMesh &m, const HostBVH &b) {
         }
     }
 
-    // FP32 child-box layout: the internal nodes of the reference tree in the
-    // same DFS order, renumbered densely.
+    // FP32 child-box layout: the internal nodes of the reference tree in
+    // breadth-first order, so the top levels are one contiguous prefix that a
+    // block can stage in LDS.
     auto ref_of = [&](int32_t node, const std::vector<int32_t> &remap) -> int32_t {
         const RefNode &r = b.nodes[node];
         if (r.count > 0) return -(1 + ((r.idx << 3) | r.count));
@@ -186,8 +187,18 @@ DeviceLayouts make_layouts(const HostMesh &m, const HostBVH &b) {
     };
     std::vector<int32_t> remap(nn, -1);
     int32_t ni = 0;
-    for (size_t i = 0; i < nn; ++i)
-        if (b.nodes[i].count == 0) remap[i] = ni++;
+    {
+        std::vector<int32_t> queue;
+        queue.reserve(nn);
+        if (b.nodes[0].count == 0) queue.push_back(0);
+        for (size_t h = 0; h < queue.size(); ++h) {
+            const int32_t i = queue[h];
+            remap[i] = ni++;
+            const int32_t l = i + 1, r = b.nodes[i].idx;
+            if (b.nodes[l].count == 0) queue.push_back(l);
+            if (b.nodes[r].count == 0) queue.push_back(r);
+        }
+    }
     L.root_ref = ref_of(0, remap);
     if (dim == 2) L.nf2.resize(ni); else L.nf3.resize(ni);
     for (size_t i = 0; i < nn; ++i) {

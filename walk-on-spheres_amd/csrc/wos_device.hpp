@@ -61,6 +61,7 @@ struct StartInfo {
     double g;             // g at the nearest point (Boundary value, zero-step walks)
     double s, t;          // nearest primitive parameters
     int32_t prim, bid;
+    int32_t slot, pad;    // leaf slot of the nearest primitive
     uint32_t key0, key1;  // walk-stream key
 };
 
@@ -70,6 +71,7 @@ struct NpqD {
     double px, py, pz;
     double s, t;
     int32_t prim;
+    int32_t slot;
 };
 
 // --------------------------------------------------------------------------
@@ -178,7 +180,7 @@ __device__ __forceinline__ double box_d2_3(const NodeD3 &n, double px, double py
 __device__ __forceinline__ NpqD npq_ref_2d(const MeshView &m, double px, double py, int *stk,
                                            int stride) {
     NpqD r;
-    r.d2 = INFINITY; r.px = r.py = r.pz = 0.0; r.s = r.t = 0.0; r.prim = -1;
+    r.d2 = INFINITY; r.px = r.py = r.pz = 0.0; r.s = r.t = 0.0; r.prim = -1; r.slot = -1;
     int sp = 0;
     stk[0] = 0;
     sp = 1;
@@ -192,7 +194,7 @@ __device__ __forceinline__ NpqD npq_ref_2d(const MeshView &m, double px, double 
                 double nx, ny, par;
                 const double d = npq_seg_d(m.segd[slot], px, py, nx, ny, par);
                 if (d < r.d2) {
-                    r.d2 = d; r.px = nx; r.py = ny; r.s = par; r.prim = m.slot_prim[slot];
+                    r.d2 = d; r.px = nx; r.py = ny; r.s = par; r.prim = m.slot_prim[slot]; r.slot = slot;
                 }
             }
         } else {
@@ -209,7 +211,7 @@ __device__ __forceinline__ NpqD npq_ref_2d(const MeshView &m, double px, double 
 __device__ __forceinline__ NpqD npq_ref_3d(const MeshView &m, double px, double py, double pz,
                                            int *stk, int stride) {
     NpqD r;
-    r.d2 = INFINITY; r.px = r.py = r.pz = 0.0; r.s = r.t = 0.0; r.prim = -1;
+    r.d2 = INFINITY; r.px = r.py = r.pz = 0.0; r.s = r.t = 0.0; r.prim = -1; r.slot = -1;
     int sp = 1;
     stk[0] = 0;
     while (sp > 0) {
@@ -224,6 +226,7 @@ __device__ __forceinline__ NpqD npq_ref_3d(const MeshView &m, double px, double 
                 if (d < r.d2) {
                     r.d2 = d; r.px = nx; r.py = ny; r.pz = nz; r.s = s; r.t = t;
                     r.prim = m.slot_prim[slot];
+                    r.slot = slot;
                 }
             }
         } else {
@@ -408,7 +411,8 @@ __device__ __forceinline__ T k0_scaled(T x) {
         return exp(x) * k0;
     }
     if (x < (T)50) {
-        const T h = (T)0.125;
+        // step below 0.5/sqrt(x): the integrand is about exp(-x t^2 / 2)
+        const T h = fmin((T)0.125, (T)0.5 / sqrt(x));
         T sum = (T)0.5;
         for (int i = 1; i < 4000; ++i) {
             const T sh = sinh((T)0.5 * h * (T)i);
@@ -649,13 +653,24 @@ struct StackEntry { int32_t ref; float d2; };
 
 // Closest point FP32: near child first, far child with its box distance on
 // the LDS stack (popped entries whose distance is no longer below the best
-// are dropped without a node load).  Returns the squared distance and the
-// best leaf slot.
+// are dropped without a node load).  Internal nodes are in breadth-first
+// order; the first n_cached of them are read from the block's LDS copy.
+// Returns the squared distance and the best leaf slot.
+template <int DIM>
+__device__ __forceinline__ float leaf_d2f(const MeshView &m, int slot, float px, float py, float pz) {
+    if constexpr (DIM == 3) return eberly_f(m.trif[slot], px, py, pz);
+    else return seg_d2f(m.segf[slot], m.seg_inv_vv[slot], px, py);
+}
+
+// `best_slot` on entry: a primitive to test first (the previous step's
+// nearest, -1 for none).  Its distance bounds the search from the start, so
+// the traversal only opens boxes that can still hold a closer primitive; the
+// result is the same minimum as a search from infinity.
 template <int DIM>
 __device__ __forceinline__ float npq_f(const MeshView &m, float px, float py, float pz,
-                                       int &best_slot, StackEntry *stk, int stride) {
-    float best = INFINITY;
-    best_slot = -1;
+                                       int &best_slot, StackEntry *stk, int stride,
+                                       const void *cache, int n_cached) {
+    float best = best_slot >= 0 ? leaf_d2f<DIM>(m, best_slot, px, py, pz) : INFINITY;
     int sp = 0;
     int32_t ref = m.root_ref;
     for (;;) {
@@ -663,12 +678,12 @@ __device__ __forceinline__ float npq_f(const MeshView &m, float px, float py, fl
             float dl, dr;
             int32_t cl, cr;
             if constexpr (DIM == 3) {
-                const NodeF3 n = m.nf3[ref];
+                const NodeF3 n = ref < n_cached ? static_cast<const NodeF3 *>(cache)[ref] : m.nf3[ref];
                 dl = box_d2f(n.l_lox, n.l_loy, n.l_loz, n.l_hix, n.l_hiy, n.l_hiz, px, py, pz);
                 dr = box_d2f(n.r_lox, n.r_loy, n.r_loz, n.r_hix, n.r_hiy, n.r_hiz, px, py, pz);
                 cl = n.left; cr = n.right;
             } else {
-                const NodeF2 n = m.nf2[ref];
+                const NodeF2 n = ref < n_cached ? static_cast<const NodeF2 *>(cache)[ref] : m.nf2[ref];
                 dl = box_d2f2(n.l_lox, n.l_loy, n.l_hix, n.l_hiy, px, py);
                 dr = box_d2f2(n.r_lox, n.r_loy, n.r_hix, n.r_hiy, px, py);
                 cl = n.left; cr = n.right;
@@ -691,9 +706,7 @@ __device__ __forceinline__ float npq_f(const MeshView &m, float px, float py, fl
             const int first = code >> 3, count = code & 7;
             for (int i = 0; i < count; ++i) {
                 const int slot = first + i;
-                float d;
-                if constexpr (DIM == 3) d = eberly_f(m.trif[slot], px, py, pz);
-                else d = seg_d2f(m.segf[slot], m.seg_inv_vv[slot], px, py);
+                const float d = leaf_d2f<DIM>(m, slot, px, py, pz);
                 if (d < best) { best = d; best_slot = slot; }
             }
         }

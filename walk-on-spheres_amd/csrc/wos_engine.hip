@@ -262,6 +262,8 @@ __global__ __launch_bounds__(kBlock) void k_classify(MeshView m, EqView e, GridS
     S.nx = q.px; S.ny = q.py; S.nz = q.pz;
     S.g = gval; S.s = q.s; S.t = q.t;
     S.prim = q.prim;
+    S.slot = q.slot;
+    S.pad = 0;
     S.bid = (DIM == 2 && q.prim >= 0) ? m.prim_bid[q.prim] : -1;
     S.key0 = (uint32_t)key;
     S.key1 = (uint32_t)(key >> 32);
@@ -272,23 +274,54 @@ __global__ __launch_bounds__(kBlock) void k_classify(MeshView m, EqView e, GridS
 }
 
 // --------------------------------------------------------------------------
+// one round of walks: active point k runs walks [wbeg[k], wbeg[k] + count_k);
+// item i of the round is walk (i - off[k]) of the point with
+// off[k] <= i < off[k+1], and its result goes to staging slot i (point major).
+// off == nullptr: every point runs cw walks starting at w0.
+// --------------------------------------------------------------------------
+struct RoundView {
+    const StartInfo *starts;
+    const int32_t *inside;   // inside id -> point index
+    const int32_t *pid;      // active k -> inside id
+    const int32_t *wbeg;     // active k -> first walk index (off != nullptr)
+    const int64_t *off;      // exclusive prefix of counts, n + 1 entries, or nullptr
+    int n;
+    int cw;
+    int w0;
+    int64_t total;
+};
+
+__device__ __forceinline__ const StartInfo &round_item(const RoundView &r, int64_t i, uint32_t &walk) {
+    int k;
+    if (!r.off) {
+        k = (int)(i / r.cw);
+        walk = (uint32_t)(r.w0 + (int)(i - (int64_t)k * r.cw));
+    } else {
+        int lo = 0, hi = r.n - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (r.off[mid] <= i) lo = mid; else hi = mid - 1;
+        }
+        k = lo;
+        walk = (uint32_t)(r.wbeg[k] + (int)(i - r.off[k]));
+    }
+    return r.starts[r.inside[r.pid[k]]];
+}
+
+// --------------------------------------------------------------------------
 // kernel 2a: FP64 plain walker, one lane per (point, walk)
 // --------------------------------------------------------------------------
 template <int DIM, bool SRC, bool SCR>
-__global__ __launch_bounds__(kBlock) void k_walk_fp64(MeshView m, EqView e,
-                                                      const StartInfo *starts,
-                                                      const int32_t *plist, int n_pts, int cw,
-                                                      int w0, double eps, int max_steps,
-                                                      double *vals, int32_t *steps_out) {
+__global__ __launch_bounds__(kBlock) void k_walk_fp64(MeshView m, EqView e, RoundView rv,
+                                                      double eps, int max_steps, double *vals,
+                                                      int32_t *steps_out) {
     extern __shared__ int lds_i[];
     int *stk = lds_i + threadIdx.x;
     const int stride = blockDim.x;
     const int64_t item = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (item >= (int64_t)n_pts * cw) return;
-    const int p = (int)(item / cw);
-    const int wl = (int)(item % cw);
-    const uint32_t w = (uint32_t)(w0 + wl);
-    const StartInfo S = starts[plist[p]];
+    if (item >= rv.total) return;
+    uint32_t w;
+    const StartInfo S = round_item(rv, item, w);
     double px = S.x, py = S.y, pz = S.z, R = S.r;
     NpqD q;
     q.px = S.nx; q.py = S.ny; q.pz = S.nz; q.s = S.s; q.t = S.t; q.prim = S.prim;
@@ -321,20 +354,76 @@ __global__ __launch_bounds__(kBlock) void k_walk_fp64(MeshView m, EqView e,
     }
     const bool hit = R > eps;
     const double g = steps == 0 ? S.g : eq_boundary(e, m, q.px, q.py, q.pz, q.prim, q.s, q.t);
-    const size_t o = (size_t)wl * n_pts + p;
-    vals[o] = sample + weight * g;
-    steps_out[o] = steps | (hit ? (int32_t)0x80000000 : 0);
+    vals[item] = sample + weight * g;
+    steps_out[item] = steps | (hit ? (int32_t)0x80000000 : 0);
+}
+
+// FP32 source term.  Same draws as source_term (the walk's tag-1 counter
+// block), but the Green function takes the sample's distance from the
+// centre in closed form (R sqrt(u) in 2D, R cbrt(w) in 3D) rather than from
+// rounded coordinates, and a constant source skips the direction entirely:
+// only the radius draw is made (block 1 word 0 in 3D, block 0 word 1 in 2D).
+template <int DIM, int SMODE>
+__device__ __forceinline__ double source_term_f(const EqView &e, float px, float py, float pz,
+                                                float R, float weight, uint32_t w, uint32_t step,
+                                                uint32_t k0, uint32_t k1) {
+    if (SMODE == 2) {
+        CursorF src(w, step, 1, k0, k1);
+        return source_term<DIM, float>(e, px, py, pz, R, weight, src);
+    }
+    if (DIM == 2 && !eq_source_may_intersect<float>(e, px, py, R)) return 0.0;
+    const bool screened = e.kind == WOS_EQ_SCREENED;
+    const float alpha = (float)e.alpha;
+    float f, dist;
+    if (e.source_kind == WOS_SRC_CONSTANT) {
+        if (DIM == 2) {
+            const U64x2 r = philox(w, step, 0, 1, k0, k1);
+            dist = R * sqrtf(u01f_open(r.b));
+        } else {
+            const U64x2 r = philox(w, step, 1, 1, k0, k1);
+            dist = R * cbrtf(u01f_open(r.a));
+        }
+        f = (float)e.src_value;
+    } else {
+        CursorF rng(w, step, 1, k0, k1);
+        float sx, sy, sz = 0.0f;
+        if (DIM == 2) {
+            const float theta = rng.unit();
+            dist = R * sqrtf(rng.unit_open());
+            float sn, cs;
+            __sincosf(theta * (float)(2.0 * kPi), &sn, &cs);
+            sx = __builtin_fmaf(dist, cs, px);
+            sy = __builtin_fmaf(dist, sn, py);
+        } else {
+            const float u = rng.unit(), v = rng.unit(), ww = rng.unit_open();
+            const float z = 2.0f * u - 1.0f;
+            const float surf = sqrtf(fmaxf(0.0f, 1.0f - z * z));
+            dist = R * cbrtf(ww);
+            float sn, cs;
+            __sincosf(v * (float)(2.0 * kPi), &sn, &cs);
+            sx = __builtin_fmaf(dist * surf, cs, px);
+            sy = __builtin_fmaf(dist * surf, sn, py);
+            sz = __builtin_fmaf(dist, z, pz);
+        }
+        f = eq_source<float>(e, sx, sy, sz, DIM);
+    }
+    if (f == 0.0f || !(dist > 0.0f)) return 0.0;
+    float volume, G;
+    if (DIM == 2) {
+        volume = (float)kPi * R * R;
+        G = screened ? screened_green_2d<float>(alpha, R, dist) : poisson_green_2d<float>(R, dist);
+    } else {
+        volume = (float)(4.0 * kPi / 3.0) * R * R * R;
+        G = screened ? screened_green_3d<float>(alpha, R, dist) : poisson_green_3d<float>(R, dist);
+    }
+    return (double)(weight * volume * f * G);
 }
 
 // --------------------------------------------------------------------------
 // kernel 2b: FP32 persistent walker with ballot-driven lane refill
 // --------------------------------------------------------------------------
 struct PersistArgs {
-    const StartInfo *starts;
-    const int32_t *plist;
-    int n_pts;
-    int cw;
-    int w0;
+    RoundView rv;
     int max_steps;
     double eps;
     float eps_f;
@@ -342,14 +431,26 @@ struct PersistArgs {
     uint64_t total;
     unsigned long long *counter;
     int chunk;
+    int n_cached;        // internal BVH nodes staged in LDS (breadth-first prefix)
+    int cache_bytes;     // LDS bytes reserved for them (16 B aligned)
     double *vals;
     int32_t *steps_out;
 };
 
-template <int DIM, bool SRC, bool SCR>
+template <int DIM, int SMODE, bool SCR>
 __global__ __launch_bounds__(kBlock) void k_walk_fp32(MeshView m, EqView e, PersistArgs a) {
-    extern __shared__ StackEntry lds_s[];
-    StackEntry *stk = lds_s + threadIdx.x;
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    // stage the top of the tree once per block
+    {
+        const int4 *src = DIM == 3 ? reinterpret_cast<const int4 *>(m.nf3)
+                                   : reinterpret_cast<const int4 *>(m.nf2);
+        int4 *dst = reinterpret_cast<int4 *>(lds_raw);
+        const int n16 = a.n_cached * (int)(DIM == 3 ? sizeof(NodeF3) : sizeof(NodeF2)) / 16;
+        for (int i = threadIdx.x; i < n16; i += blockDim.x) dst[i] = src[i];
+        __syncthreads();
+    }
+    const void *cache = lds_raw;
+    StackEntry *stk = reinterpret_cast<StackEntry *>(lds_raw + a.cache_bytes) + threadIdx.x;
     const int stride = blockDim.x;
     const int lane = __lane_id();
 
@@ -374,12 +475,12 @@ __global__ __launch_bounds__(kBlock) void k_walk_fp32(MeshView m, EqView e, Pers
                 unsigned long long base = 0;
                 if (lane == 0) base = atomicAdd(a.counter, (unsigned long long)a.chunk);
                 base = __shfl(base, 0);
-                if (base >= a.total) {
+                if (base >= (unsigned long long)a.rv.total) {
                     exhausted = true;
                 } else {
                     pool_next = base;
                     pool_end = base + (uint64_t)a.chunk;
-                    if (pool_end > a.total) pool_end = a.total;
+                    if (pool_end > (uint64_t)a.rv.total) pool_end = (uint64_t)a.rv.total;
                 }
             }
             if (!exhausted) {
@@ -387,19 +488,16 @@ __global__ __launch_bounds__(kBlock) void k_walk_fp32(MeshView m, EqView e, Pers
                 const int rank = __builtin_amdgcn_mbcnt_hi(hi, __builtin_amdgcn_mbcnt_lo(lo, 0));
                 const uint64_t item = pool_next + (uint64_t)rank;
                 if (!active && item < pool_end) {
-                    const int p = (int)(item / (uint64_t)a.cw);
-                    const int wl = (int)(item % (uint64_t)a.cw);
-                    const StartInfo &S = a.starts[a.plist[p]];
+                    const StartInfo &S = round_item(a.rv, (int64_t)item, w);
                     px = (float)S.x; py = (float)S.y; pz = (float)S.z;
                     R = (float)S.r * a.shrink_rel - m.shrink_abs;
                     weight = 1.0f;
                     sample = 0.0;
                     steps = 0;
-                    slot = -1;
-                    w = (uint32_t)(a.w0 + wl);
+                    slot = S.slot;
                     k0 = S.key0;
                     k1 = S.key1;
-                    out_index = (size_t)wl * a.n_pts + p;
+                    out_index = (size_t)item;
                     active = true;
                     if (!(R > a.eps_f)) {
                         // zero-step walk: g at the start's nearest point
@@ -418,10 +516,8 @@ __global__ __launch_bounds__(kBlock) void k_walk_fp32(MeshView m, EqView e, Pers
         }
         if (active) {
         // ---- one walk step (include/wos/wos.hpp:68-127)
-        if (SRC && e.source_kind != WOS_SRC_ZERO) {
-            CursorF src(w, (uint32_t)steps, 1, k0, k1);
-            sample += source_term<DIM, float>(e, px, py, pz, R, weight, src);
-        }
+        if (SMODE != 0 && e.source_kind != WOS_SRC_ZERO)
+            sample += source_term_f<DIM, SMODE>(e, px, py, pz, R, weight, w, (uint32_t)steps, k0, k1);
         if (SCR) weight *= DIM == 2 ? screened_weight_2d<float>(alpha, R)
                                     : screened_weight_3d<float>(alpha, R);
         Cursor wk(w, (uint32_t)steps, 0, k0, k1);
@@ -442,7 +538,7 @@ __global__ __launch_bounds__(kBlock) void k_walk_fp32(MeshView m, EqView e, Pers
             pz = __builtin_fmaf(R, z, pz);
         }
         ++steps;
-        const float d2 = npq_f<DIM>(m, px, py, pz, slot, stk, stride);
+        const float d2 = npq_f<DIM>(m, px, py, pz, slot, stk, stride, cache, a.n_cached);
         R = sqrtf(d2) * a.shrink_rel - m.shrink_abs;
         if (!(R > a.eps_f) || steps >= a.max_steps) {
             // terminal value in FP64: nearest point on the nearest primitive
@@ -466,16 +562,19 @@ __global__ __launch_bounds__(kBlock) void k_walk_fp32(MeshView m, EqView e, Pers
 // --------------------------------------------------------------------------
 // kernel 3: sequential Welford per point (include/wos/wos.hpp:169-180)
 // --------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_welford(const double *vals, const int32_t *steps,
-                                                    int n_pts, int cw, double *mean, double *m2,
+__global__ __launch_bounds__(kBlock) void k_welford(RoundView rv, const double *vals,
+                                                    const int32_t *steps, double *mean, double *m2,
                                                     double *count, double *steps_sum,
                                                     int32_t *hits) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n_pts) return;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= rv.n) return;
+    const int p = rv.pid[k];
+    const int64_t base = rv.off ? rv.off[k] : (int64_t)k * rv.cw;
+    const int cnt = rv.off ? (int)(rv.off[k + 1] - rv.off[k]) : rv.cw;
     double mu = mean[p], M2 = m2[p], n = count[p], ts = steps_sum[p];
     int h = hits[p];
-    for (int wl = 0; wl < cw; ++wl) {
-        const size_t o = (size_t)wl * n_pts + p;
+    for (int j = 0; j < cnt; ++j) {
+        const size_t o = (size_t)(base + j);
         const double x = vals[o];
         const int32_t s = steps[o];
         n = n + 1.0;
@@ -541,9 +640,15 @@ struct wos_ctx {
     bool has_mesh = false;
     DevBuf nd2, nd3, segd, trid, slot_prim, prim_bid, prims, verts, nf2, nf3, segf, seg_inv, trif;
     DevBuf plist_in, loc, starts, amb, bvalue, inside, vals, steps, mean, m2, count, ssum, hits,
-        counter, vv;
+        counter, vv, pid_all, r_pid, r_wbeg, r_off;
+    int last_rounds = 0;
     MeshView view{};
     hipEvent_t ev[6]{};
+    wos_tuning tune{256, -1, 256, 0};
+    size_t lds_per_cu = 65536;
+    size_t lds_per_block = 65536;
+    int last_threads = 0;
+    int last_cached = 0;
 };
 
 namespace {
@@ -560,56 +665,79 @@ hipError_t launch_classify(wos_ctx *c, K kernel, const EqView &ev, const GridSpe
 }
 
 template <int DIM, bool SRC, bool SCR>
-hipError_t launch_walk(wos_ctx *c, const EqView &ev, int precision, int n_pts, int cw, int w0,
+hipError_t launch_walk(wos_ctx *c, const EqView &ev, int precision, const RoundView &rv,
                        double eps, int max_steps, int *blocks_out) {
     if (precision == WOS_FP64_PLAIN) {
-        const int64_t items = (int64_t)n_pts * cw;
-        const int blocks = (int)((items + kBlock - 1) / kBlock);
+        const int blocks = (int)((rv.total + kBlock - 1) / kBlock);
         const size_t lds = (size_t)kBlock * (c->bvh.depth + 2) * sizeof(int);
         hipLaunchKernelGGL((k_walk_fp64<DIM, SRC, SCR>), dim3(blocks), dim3(kBlock), lds,
-                           c->stream, c->view, ev, c->starts.as<StartInfo>(),
-                           c->inside.as<int32_t>(), n_pts, cw, w0, eps, max_steps,
-                           c->vals.as<double>(), c->steps.as<int32_t>());
+                           c->stream, c->view, ev, rv, eps, max_steps, c->vals.as<double>(),
+                           c->steps.as<int32_t>());
         *blocks_out = blocks;
         return hipGetLastError();
     }
-    const size_t lds = (size_t)kBlock * (c->bvh.depth + 1) * sizeof(StackEntry);
+    constexpr int SMODE = SRC ? 1 : 0;
+    const void *kfn = ev.source_mode == WOS_SOURCE_GREEN && SRC
+                          ? (const void *)k_walk_fp32<DIM, 2, SCR>
+                          : (const void *)k_walk_fp32<DIM, SMODE, SCR>;
+    const int threads = c->tune.block_threads;
+    const size_t stack_bytes = (size_t)threads * (c->bvh.depth + 1) * sizeof(StackEntry);
+    const int node_bytes = DIM == 3 ? (int)sizeof(NodeF3) : (int)sizeof(NodeF2);
+    const int n_internal = DIM == 3 ? (int)c->L.nf3.size() : (int)c->L.nf2.size();
+    // occupancy allowed by registers alone, then the LDS each block may use
+    // for the node cache without lowering it
     int per_cu = 0;
-    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        &per_cu, (const void *)k_walk_fp32<DIM, SRC, SCR>, kBlock, lds);
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, threads, stack_bytes);
     if (e != hipSuccess) return e;
     per_cu = std::max(1, per_cu);
-    const int blocks = c->n_cu * per_cu;
+    if (c->tune.blocks_per_cu > 0) per_cu = std::min(per_cu, c->tune.blocks_per_cu);
+    int n_cached = c->tune.lds_nodes;
+    if (n_cached < 0) {
+        const size_t per_block = std::min<size_t>(c->lds_per_cu / per_cu, c->lds_per_block);
+        n_cached = per_block > stack_bytes ? (int)((per_block - stack_bytes) / node_bytes) : 0;
+    }
+    n_cached = std::max(0, std::min(n_cached, n_internal));
+    const int cache_bytes = (n_cached * node_bytes + 15) / 16 * 16;
+    const size_t lds = stack_bytes + cache_bytes;
+    if (lds > c->lds_per_block) return hipErrorInvalidValue;
+    int per_cu2 = 0;
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu2, kfn, threads, lds);
+    if (e != hipSuccess) return e;
+    per_cu2 = std::max(1, std::min(per_cu2, per_cu));
+    const int blocks = c->n_cu * per_cu2;
     PersistArgs a;
-    a.starts = c->starts.as<StartInfo>();
-    a.plist = c->inside.as<int32_t>();
-    a.n_pts = n_pts;
-    a.cw = cw;
-    a.w0 = w0;
+    a.n_cached = n_cached;
+    a.cache_bytes = cache_bytes;
+    a.rv = rv;
     a.max_steps = max_steps;
     a.eps = eps;
     a.eps_f = (float)eps;
     a.shrink_rel = 1.0f - 1.0f / 131072.0f;
-    a.total = (uint64_t)n_pts * (uint64_t)cw;
     a.counter = c->counter.as<unsigned long long>();
-    a.chunk = 256;
+    a.chunk = c->tune.chunk;
     a.vals = c->vals.as<double>();
     a.steps_out = c->steps.as<int32_t>();
     e = hipMemsetAsync(c->counter.p, 0, sizeof(unsigned long long), c->stream);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((k_walk_fp32<DIM, SRC, SCR>), dim3(blocks), dim3(kBlock), lds, c->stream,
-                       c->view, ev, a);
+    if (ev.source_mode == WOS_SOURCE_GREEN && SRC)
+        hipLaunchKernelGGL((k_walk_fp32<DIM, 2, SCR>), dim3(blocks), dim3(threads), lds,
+                           c->stream, c->view, ev, a);
+    else
+        hipLaunchKernelGGL((k_walk_fp32<DIM, SMODE, SCR>), dim3(blocks), dim3(threads), lds,
+                           c->stream, c->view, ev, a);
     *blocks_out = blocks;
+    c->last_threads = threads;
+    c->last_cached = n_cached;
     return hipGetLastError();
 }
 
 template <int DIM>
 hipError_t dispatch_walk(wos_ctx *c, const EqView &ev, bool src, bool scr, int precision,
-                         int n_pts, int cw, int w0, double eps, int max_steps, int *blocks) {
-    if (src && scr) return launch_walk<DIM, true, true>(c, ev, precision, n_pts, cw, w0, eps, max_steps, blocks);
-    if (src) return launch_walk<DIM, true, false>(c, ev, precision, n_pts, cw, w0, eps, max_steps, blocks);
-    if (scr) return launch_walk<DIM, false, true>(c, ev, precision, n_pts, cw, w0, eps, max_steps, blocks);
-    return launch_walk<DIM, false, false>(c, ev, precision, n_pts, cw, w0, eps, max_steps, blocks);
+                         const RoundView &rv, double eps, int max_steps, int *blocks) {
+    if (src && scr) return launch_walk<DIM, true, true>(c, ev, precision, rv, eps, max_steps, blocks);
+    if (src) return launch_walk<DIM, true, false>(c, ev, precision, rv, eps, max_steps, blocks);
+    if (scr) return launch_walk<DIM, false, true>(c, ev, precision, rv, eps, max_steps, blocks);
+    return launch_walk<DIM, false, false>(c, ev, precision, rv, eps, max_steps, blocks);
 }
 
 int validate_eq(const wos_ctx *c, const wos_eq_desc *eq) {
@@ -656,7 +784,10 @@ int solve(wos_ctx *c, const GridSpec *grid, const double *points, int64_t n,
     if (prm->precision != WOS_FP64_PLAIN && prm->precision != WOS_FP32_PERSISTENT)
         return fail(WOS_ERR_INPUT, "unknown precision");
     if (n <= 0 || n > ((int64_t)1 << 31) - 1) return fail(WOS_ERR_INPUT, "point count out of range");
-    if (prm->target_se > 0.0) return fail(WOS_ERR_INPUT, "adaptive walks are not enabled in this build");
+    if (!(prm->target_se >= 0.0) || !std::isfinite(prm->target_se))
+        return fail(WOS_ERR_INPUT, "target_se must be finite and >= 0");
+    if (prm->target_se > 0.0 && prm->max_walks < prm->n_walks)
+        return fail(WOS_ERR_INPUT, "adaptive mode needs max_walks >= n_walks");
     const int dim = c->mesh.dim;
     HIPX(hipSetDevice(c->device));
 
@@ -731,6 +862,7 @@ int solve(wos_ctx *c, const GridSpec *grid, const double *points, int64_t n,
         if (loc[i] == 1) inside.push_back((int32_t)i);
     const int n_in = (int)inside.size();
 
+    c->last_rounds = 0;
     std::vector<double> h_mean(n_in), h_m2(n_in), h_cnt(n_in), h_ss(n_in);
     std::vector<int32_t> h_hits(n_in);
     double walk_s = 0.0, total_walk_s = 0.0;
@@ -748,24 +880,90 @@ int solve(wos_ctx *c, const GridSpec *grid, const double *points, int64_t n,
         HIPX(hipMemsetAsync(c->ssum.p, 0, n_in * sizeof(double), c->stream));
         HIPX(hipMemsetAsync(c->hits.p, 0, n_in * sizeof(int32_t), c->stream));
         HIPX(c->counter.reserve(sizeof(unsigned long long)));
-        // walks per round bounded by a 4 GiB staging buffer (12 B per walk)
-        const int64_t budget = (int64_t)4 << 30;
-        int64_t cw64 = std::max<int64_t>(1, budget / (12 * (int64_t)n_in));
-        const int cw_max = (int)std::min<int64_t>(cw64, prm->n_walks);
-        HIPX(c->vals.reserve((size_t)n_in * cw_max * sizeof(double)));
-        HIPX(c->steps.reserve((size_t)n_in * cw_max * sizeof(int32_t)));
         const bool src = eq->source_kind != WOS_SRC_NONE;
         const bool scr = eq->kind == WOS_EQ_SCREENED;
-        for (int w0 = 0; w0 < prm->n_walks; w0 += cw_max) {
-            const int cw = std::min(cw_max, prm->n_walks - w0);
+        const bool adaptive = prm->target_se > 0.0;
+        const int64_t cap = adaptive ? std::max<int64_t>(prm->max_walks, prm->n_walks) : prm->n_walks;
+        // staging: one FP64 value + one int32 per walk, at most 4 GiB a round
+        const int64_t budget = std::max<int64_t>((int64_t)(4LL << 30) / 12, 1);
+        std::vector<int64_t> done(n_in, 0), want(n_in, std::min<int64_t>(prm->n_walks, cap));
+        std::vector<int32_t> pid, wbeg;
+        std::vector<int64_t> off;
+        std::vector<double> a_cnt(n_in), a_m2(n_in);
+        std::vector<int32_t> all_ids(n_in);
+        for (int k = 0; k < n_in; ++k) all_ids[k] = k;
+        HIPX(upload(c->pid_all, all_ids, c->stream));
+        for (int round = 0; round < 4096; ++round) {
+            // ---- this round: uniform when every point needs the same walks
+            int64_t remaining = 0;
+            bool uniform = true;
+            for (int k = 0; k < n_in; ++k) {
+                remaining += want[k];
+                if (want[k] != want[0] || done[k] != done[0]) uniform = false;
+            }
+            if (remaining == 0) {
+                if (!adaptive) break;
+                // SE test on every point below the cap (wos_run statistics)
+                HIPX(hipMemcpyAsync(a_cnt.data(), c->count.p, n_in * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+                HIPX(hipMemcpyAsync(a_m2.data(), c->m2.p, n_in * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+                HIPX(hipStreamSynchronize(c->stream));
+                const double t2 = prm->target_se * prm->target_se;
+                for (int k = 0; k < n_in; ++k) {
+                    const double N = a_cnt[k];
+                    if (done[k] >= cap || N < 2) continue;
+                    const double var = a_m2[k] / (N - 1.0);
+                    if (std::sqrt(var / N) <= prm->target_se) continue;
+                    const double need = std::ceil(1.1 * var / t2) - N;
+                    want[k] = std::min<int64_t>(cap - done[k],
+                                                std::max<int64_t>((int64_t)need, prm->n_walks));
+                    remaining += want[k];
+                }
+                if (remaining == 0) break;
+                uniform = false;
+            }
+            RoundView rv{};
+            rv.starts = c->starts.as<StartInfo>();
+            rv.inside = c->inside.as<int32_t>();
+            if (uniform) {
+                const int64_t cw = std::min<int64_t>(want[0], std::max<int64_t>(1, budget / n_in));
+                rv.pid = c->pid_all.as<int32_t>();
+                rv.n = n_in;
+                rv.cw = (int)cw;
+                rv.w0 = (int)done[0];
+                rv.total = cw * n_in;
+                for (int k = 0; k < n_in; ++k) { done[k] += cw; want[k] -= cw; }
+            } else {
+                pid.clear(); wbeg.clear(); off.assign(1, 0);
+                int64_t tot = 0;
+                for (int k = 0; k < n_in && tot < budget; ++k) {
+                    if (want[k] == 0) continue;
+                    const int64_t cnt = std::min<int64_t>(want[k], budget - tot);
+                    pid.push_back(k);
+                    wbeg.push_back((int32_t)done[k]);
+                    tot += cnt;
+                    off.push_back(tot);
+                    done[k] += cnt;
+                    want[k] -= cnt;
+                }
+                HIPX(upload(c->r_pid, pid, c->stream));
+                HIPX(upload(c->r_wbeg, wbeg, c->stream));
+                HIPX(upload(c->r_off, off, c->stream));
+                rv.pid = c->r_pid.as<int32_t>();
+                rv.wbeg = c->r_wbeg.as<int32_t>();
+                rv.off = c->r_off.as<int64_t>();
+                rv.n = (int)pid.size();
+                rv.total = tot;
+            }
+            HIPX(c->vals.reserve((size_t)rv.total * sizeof(double)));
+            HIPX(c->steps.reserve((size_t)rv.total * sizeof(int32_t)));
             HIPX(hipEventRecord(c->ev[2], c->stream));
             hipError_t we = dim == 2
-                ? dispatch_walk<2>(c, ev, src, scr, prm->precision, n_in, cw, w0, prm->epsilon, prm->max_steps, &blocks)
-                : dispatch_walk<3>(c, ev, src, scr, prm->precision, n_in, cw, w0, prm->epsilon, prm->max_steps, &blocks);
+                ? dispatch_walk<2>(c, ev, src, scr, prm->precision, rv, prm->epsilon, prm->max_steps, &blocks)
+                : dispatch_walk<3>(c, ev, src, scr, prm->precision, rv, prm->epsilon, prm->max_steps, &blocks);
             HIPX(we);
             HIPX(hipEventRecord(c->ev[3], c->stream));
-            hipLaunchKernelGGL(k_welford, dim3((n_in + kBlock - 1) / kBlock), dim3(kBlock), 0,
-                               c->stream, c->vals.as<double>(), c->steps.as<int32_t>(), n_in, cw,
+            hipLaunchKernelGGL(k_welford, dim3((rv.n + kBlock - 1) / kBlock), dim3(kBlock), 0,
+                               c->stream, rv, c->vals.as<double>(), c->steps.as<int32_t>(),
                                c->mean.as<double>(), c->m2.as<double>(), c->count.as<double>(),
                                c->ssum.as<double>(), c->hits.as<int32_t>());
             HIPX(hipGetLastError());
@@ -773,6 +971,7 @@ int solve(wos_ctx *c, const GridSpec *grid, const double *points, int64_t n,
             HIPX(hipEventSynchronize(c->ev[4]));
             walk_s += elapsed(c->ev[2], c->ev[3]);
             total_walk_s += elapsed(c->ev[2], c->ev[4]);
+            ++c->last_rounds;
         }
         HIPX(hipMemcpyAsync(h_mean.data(), c->mean.p, n_in * sizeof(double), hipMemcpyDeviceToHost, c->stream));
         HIPX(hipMemcpyAsync(h_m2.data(), c->m2.p, n_in * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -822,12 +1021,15 @@ int solve(wos_ctx *c, const GridSpec *grid, const double *points, int64_t n,
     }
     S.wos_seconds = classify_s + total_walk_s;
     S.walk_kernel_seconds = walk_s;
-    S.total_walks = (int64_t)n_in * prm->n_walks;
+    S.total_walks = 0;
+    for (int k = 0; k < n_in; ++k) S.total_walks += (int64_t)h_cnt[k];
+    S.rounds = c->last_rounds;
     S.total_steps = total_steps;
     for (int k = 0; k < 6; ++k) S.grid_bounds[k] = gb[k];
     S.boundary_tolerance = tol;
     S.launch_blocks = blocks;
-    S.launch_threads = kBlock;
+    S.launch_threads = prm->precision == WOS_FP32_PERSISTENT ? c->last_threads : kBlock;
+    S.lds_cached_nodes = prm->precision == WOS_FP32_PERSISTENT ? c->last_cached : 0;
     if (summary) *summary = S;
     return WOS_OK;
 }
@@ -862,6 +1064,9 @@ wos_ctx *wos_ctx_create(int device) {
         return nullptr;
     }
     c->n_cu = prop.multiProcessorCount;
+    c->lds_per_block = prop.sharedMemPerBlock;
+    c->lds_per_cu = prop.maxSharedMemoryPerMultiProcessor > 0
+                        ? prop.maxSharedMemoryPerMultiProcessor : prop.sharedMemPerBlock;
     for (auto &e : c->ev)
         if (hipEventCreate(&e) != hipSuccess) {
             fail(WOS_ERR_DEVICE, "hipEventCreate failed");
@@ -937,6 +1142,57 @@ int wos_ctx_set_mesh(wos_ctx *c, int dim, const double *verts, int64_t n_verts,
     c->bvh = std::move(b);
     c->L = std::move(L);
     c->has_mesh = true;
+    return WOS_OK;
+}
+
+int64_t wos_bvh_build_host(int dim, const double *verts, int64_t n_verts, const int32_t *prims,
+                           int64_t n_prims, int32_t *count, int32_t *idx, double *lo, double *hi,
+                           int32_t *slots) {
+    if ((dim != 2 && dim != 3) || !verts || !prims || n_verts <= 0 || n_prims <= 0) {
+        fail(WOS_ERR_INPUT, "invalid mesh arrays");
+        return -1;
+    }
+    HostMesh m;
+    m.dim = dim;
+    m.verts.assign(verts, verts + n_verts * dim);
+    m.prims.assign(prims, prims + n_prims * dim);
+    for (int32_t i : m.prims)
+        if (i < 0 || i >= n_verts) {
+            fail(WOS_ERR_MESH, "vertex index out of range");
+            return -1;
+        }
+    std::string err;
+    HostBVH b = build_reference_bvh(m, &err);
+    if (b.nodes.empty()) {
+        fail(WOS_ERR_MESH, err);
+        return -1;
+    }
+    for (size_t i = 0; i < b.nodes.size(); ++i) {
+        if (count) count[i] = b.nodes[i].count;
+        if (idx) idx[i] = b.nodes[i].idx;
+        for (int k = 0; k < 3; ++k) {
+            if (lo) lo[3 * i + k] = b.nodes[i].lo[k];
+            if (hi) hi[3 * i + k] = b.nodes[i].hi[k];
+        }
+    }
+    if (slots) std::memcpy(slots, b.slots.data(), b.slots.size() * sizeof(int32_t));
+    return (int64_t)b.nodes.size();
+}
+
+int wos_ctx_set_tuning(wos_ctx *c, const wos_tuning *t) {
+    if (!c || !t) return fail(WOS_ERR_INPUT, "context or tuning is NULL");
+    if (t->block_threads < 64 || t->block_threads > kBlock || t->block_threads % 64)
+        return fail(WOS_ERR_INPUT, "block_threads must be 64, 128, 192 or 256");
+    if (t->chunk < 1 || t->chunk > (1 << 20)) return fail(WOS_ERR_INPUT, "chunk out of range");
+    if (t->lds_nodes < -1 || t->blocks_per_cu < 0)
+        return fail(WOS_ERR_INPUT, "lds_nodes must be >= -1 and blocks_per_cu >= 0");
+    c->tune = *t;
+    return WOS_OK;
+}
+
+int wos_ctx_get_tuning(const wos_ctx *c, wos_tuning *t) {
+    if (!c || !t) return fail(WOS_ERR_INPUT, "context or tuning is NULL");
+    *t = c->tune;
     return WOS_OK;
 }
 

@@ -95,6 +95,17 @@ class RunSummary(C.Structure):
         ("boundary_tolerance", C.c_double),
         ("launch_blocks", C.c_int32),
         ("launch_threads", C.c_int32),
+        ("lds_cached_nodes", C.c_int32),
+        ("rounds", C.c_int32),
+    ]
+
+
+class Tuning(C.Structure):
+    _fields_ = [
+        ("block_threads", C.c_int32),
+        ("lds_nodes", C.c_int32),
+        ("chunk", C.c_int32),
+        ("blocks_per_cu", C.c_int32),
     ]
 
 
@@ -158,6 +169,13 @@ def load(path: str | None = None):
     lib.wos_solve_points.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.c_int64,
                                      C.POINTER(EqDesc), C.POINTER(RunParams),
                                      C.POINTER(PointStats), C.POINTER(RunSummary)]
+    lib.wos_bvh_build_host.restype = C.c_int64
+    lib.wos_bvh_build_host.argtypes = [C.c_int, C.POINTER(C.c_double), C.c_int64,
+                                       C.POINTER(C.c_int32), C.c_int64, C.POINTER(C.c_int32),
+                                       C.POINTER(C.c_int32), C.POINTER(C.c_double),
+                                       C.POINTER(C.c_double), C.POINTER(C.c_int32)]
+    lib.wos_ctx_set_tuning.argtypes = [C.c_void_p, C.POINTER(Tuning)]
+    lib.wos_ctx_get_tuning.argtypes = [C.c_void_p, C.POINTER(Tuning)]
     if path is None:
         _LIB = lib
     return lib

@@ -59,6 +59,15 @@ class Context:
         self.dim = dim
         self.n_verts = len(v)
 
+    def set_tuning(self, block_threads=256, lds_nodes=-1, chunk=256, blocks_per_cu=0):
+        t = capi.Tuning(block_threads, lds_nodes, chunk, blocks_per_cu)
+        capi.check(self.lib, self.lib.wos_ctx_set_tuning(self.h, C.byref(t)))
+
+    def get_tuning(self):
+        t = capi.Tuning()
+        capi.check(self.lib, self.lib.wos_ctx_get_tuning(self.h, C.byref(t)))
+        return {k: getattr(t, k) for k, _ in capi.Tuning._fields_}
+
     @staticmethod
     def params(n_walks, epsilon, seed, max_steps=1000, max_ray_attempts=12,
                precision=WOS_FP64_PLAIN, target_se=0.0, max_walks=0, tune=0):
