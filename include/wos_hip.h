@@ -170,6 +170,25 @@ int wos_ctx_set_mesh(wos_ctx *ctx, int dim, const double *verts, int64_t n_verts
                      const int32_t *prims, int64_t n_prims,
                      const int32_t *boundary_ids);
 
+/* Several domains in one context (all of one dimension), for batched
+ * solves: domain d is (verts[d], n_verts[d], prims[d], n_prims[d],
+ * boundary_ids[d] or NULL).  Each gets its own reference-rule BVH; the
+ * device arrays are concatenated (vertex indices of per-vertex data become
+ * global, in domain order).  No reference counterpart: the reference runs one
+ * mesh per process. */
+int wos_ctx_set_domains(wos_ctx *ctx, int dim, int n_domains, const double *const *verts,
+                        const int64_t *n_verts, const int32_t *const *prims,
+                        const int64_t *n_prims, const int32_t *const *boundary_ids);
+
+/* One Nx x Ny x Nz grid over every domain's own bounding box, all walked in
+ * one launch (adaptive mode included).  Outputs are concatenated in domain
+ * order (n_domains x Nx*Ny*Nz entries); point seeds restart per domain, as
+ * in separate runs of the reference sweep.  domain_summaries: n_domains
+ * entries or NULL; summary: totals or NULL. */
+int wos_solve_grid_batch(wos_ctx *ctx, int nx, int ny, int nz, const wos_eq_desc *eq,
+                         const wos_run_params *params, wos_point_stats *out,
+                         wos_run_summary *domain_summaries, wos_run_summary *summary);
+
 /* Host-only BVH build with the reference split rule, exported for
  * verification (replaces build_bvh, src/bvh.cpp:234-253; no device work).
  * Returns the node count (or -1 on error).  With non-NULL arrays it fills
@@ -197,6 +216,37 @@ int wos_solve_grid(wos_ctx *ctx, int nx, int ny, int nz, const wos_eq_desc *eq,
 int wos_solve_points(wos_ctx *ctx, const double *points, int64_t n,
                      const wos_eq_desc *eq, const wos_run_params *params,
                      wos_point_stats *out, wos_run_summary *summary);
+
+/* ---- adjoint (no reference counterpart: the reference has no gradients) --
+ * Gradient of L with respect to the equation data, given dL/du for every
+ * point of the solve (u = the per-point mean of the forward estimator).  The
+ * walks are replayed from the counter streams (same seeds, same draws as the
+ * forward solve with the same parameters) and differentiated pathwise:
+ *   d value / d vertex data  = weight x blend coefficient at the exit
+ *   d value / d bnd_values   = weight at an exit on that boundary id
+ *   d value / d src_value, src_radius, src_center: derivative of each source term
+ *   d value / d alpha (3D screened, uniform source sampling or no source):
+ *       the weights' and Green functions' alpha-derivatives; NaN where the
+ *       sampling depends on alpha (Green source mode, 2D).
+ * grad_vertex: n_verts doubles (WOS_BND_VERTEX problems; zeros otherwise).
+ * grad_params: WOS_N_GRAD doubles indexed by the WOS_GRAD_* offsets.
+ * `out` receives the replayed forward statistics (out->mean required). */
+enum wos_grad_index {
+    WOS_GRAD_ALPHA = 0,
+    WOS_GRAD_SRC_VALUE = 1,
+    WOS_GRAD_SRC_RADIUS = 2,
+    WOS_GRAD_SRC_CENTER = 3,     /* 3 entries */
+    WOS_GRAD_BND_VALUES = 6,     /* WOS_MAX_BND_VALUES entries */
+    WOS_N_GRAD = 6 + WOS_MAX_BND_VALUES
+};
+
+int wos_adjoint_grid(wos_ctx *ctx, int nx, int ny, int nz, const wos_eq_desc *eq,
+                     const wos_run_params *params, const double *dl_du, double *grad_vertex,
+                     double *grad_params, wos_point_stats *out, wos_run_summary *summary);
+
+int wos_adjoint_points(wos_ctx *ctx, const double *points, int64_t n, const wos_eq_desc *eq,
+                       const wos_run_params *params, const double *dl_du, double *grad_vertex,
+                       double *grad_params, wos_point_stats *out, wos_run_summary *summary);
 
 #ifdef __cplusplus
 }

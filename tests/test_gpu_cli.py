@@ -1,0 +1,79 @@
+"""The `wos` tool end to end on the GPU: progress/summary lines, the HDF5
+layout of include/wos/hdf5_io.hpp:147-322 (read back with h5dump), and the
+stored fields equal to the library's own solve with the same seed."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from wos_amd import meshgen as M
+from wos_amd import problems as Pr
+from wos_amd.engine import Context
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+WOS = os.path.join(ROOT, "walk-on-spheres_amd", "bin", "wos")
+H5DUMP = "/opt/conda/bin/h5dump"
+
+
+def h5_array(path, name, dtype, tmp):
+    out = os.path.join(tmp, name.strip("/").replace("/", "_") + ".bin")
+    subprocess.run([H5DUMP, "-d", name, "-b", "LE", "-o", out, path], check=True,
+                   capture_output=True, timeout=60)
+    return np.fromfile(out, dtype=dtype)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_cli_writes_reference_layout(tmp_path, dim):
+    if not os.path.exists(H5DUMP):
+        pytest.skip("h5dump not available")
+    if dim == 2:
+        V, F, B = M.annulus(64)
+        args = ["laplace", "12", "10"]
+        grid = (12, 10, 1)
+        eq = Pr.laplace_builtin(2)
+    else:
+        V, F, B = M.icosphere(2)
+        args = ["screened_poisson", "6", "5", "4"]
+        grid = (6, 5, 4)
+        eq = Pr.screened_builtin(3, 5.0)
+    mesh = tmp_path / "m.obj"
+    M.write_obj(str(mesh), V, F, B)
+    out = tmp_path / "r.h5"
+    cmd = [WOS, *args, str(mesh), "--walks", "32", "--epsilon", "1e-3", "--seed", "4242",
+           "-o", str(out)]
+    p = subprocess.run(cmd, capture_output=True, text=True, timeout=300)
+    print(p.stdout[-2000:], p.stderr[-2000:])
+    assert p.returncode == 0
+    for line in ("Reading mesh from disk", "Finished BVH build in", "Parallelisation: 1 x 1 x 1",
+                 "Beginning walk-on-spheres...", "WoS inside-grid averages:",
+                 "Finished walk-on-spheres in", f"Finished writing to {out}."):
+        assert line in p.stdout
+    if dim == 3:
+        assert "3D ray-classification diagnostics:" in p.stdout
+    hdr = subprocess.run([H5DUMP, "-H", str(out)], capture_output=True, text=True, timeout=60).stdout
+    for ds in ("mean", "variance", "standard_error", "mean_steps", "location", "x", "y", "z"):
+        assert f'DATASET "{ds}"' in hdr
+    md = ["N_walks", "epsilon", "max_steps", "max_steps_hits", "seed"]
+    if dim == 3:
+        md += ["max_ray_attempts", "ambiguous_ray_retries", "indeterminate_points", "alpha",
+               "source_mode"]
+    for name in md:
+        assert f'DATASET "{name}"' in hdr
+    if dim == 2:
+        assert 'DATASET "alpha"' not in hdr  # laplace writes no source metadata
+    n = grid[0] * grid[1] * grid[2]
+    mean = h5_array(str(out), "/mean", "<f8", str(tmp_path))
+    loc = h5_array(str(out), "/location", "u1", str(tmp_path))
+    assert mean.size == n and loc.size == n
+    ctx = Context(0)
+    ctx.set_mesh(V, F, B)
+    r = ctx.solve_grid(*grid, eq, Context.params(32, 1e-3, 4242))
+    assert np.array_equal(loc, r["location"])
+    assert np.array_equal(mean, r["mean"], equal_nan=True)
+    x = h5_array(str(out), "/x", "<f8", str(tmp_path))
+    assert x[0] == r["summary"]["grid_bounds"][0] and x[-1] == r["summary"]["grid_bounds"][1]
+    assert h5_array(str(out), "/metadata/N_walks", "<i4", str(tmp_path))[0] == 32
+    assert h5_array(str(out), "/metadata/seed", "<u8", str(tmp_path))[0] == 4242

@@ -116,113 +116,144 @@ HostBVH build_reference_bvh(const HostMesh &m, std::string *err) {
     return std::move(b.out);
 }
 
-DeviceLayouts make_layouts(const HostMesh &m, const HostBVH &b) {
+DeviceLayouts make_layouts(const std::vector<const HostMesh *> &meshes,
+                           const std::vector<const HostBVH *> &bvhs) {
     DeviceLayouts L;
-    const int dim = m.dim;
-    const size_t nn = b.nodes.size(), ns = b.slots.size();
-    double scale = 1.0;
-    for (double v : m.verts) scale = std::max(scale, std::fabs(v));
-    L.coord_scale = scale;
-    auto V = [&](int32_t vi, int k) { return m.verts[(size_t)vi * dim + k]; };
-
+    const int dim = meshes[0]->dim;
+    const size_t nd = meshes.size();
+    // global offsets
+    L.domains.resize(nd);
+    int64_t vo = 0, po = 0;
+    int32_t no = 0;
+    size_t n_nodes = 0;
+    for (size_t d = 0; d < nd; ++d) {
+        DomainInfo &D = L.domains[d];
+        D.vert_off = vo; D.n_verts = meshes[d]->n_verts();
+        D.prim_off = po; D.n_prims = meshes[d]->n_prims();
+        D.node_off = no;
+        D.depth = bvhs[d]->depth;
+        L.depth = std::max(L.depth, D.depth);
+        vo += D.n_verts; po += D.n_prims;
+        no += (int32_t)bvhs[d]->nodes.size();
+        n_nodes += bvhs[d]->nodes.size();
+        // bounding box as the reference computes it (src/mesh.cpp:296-322)
+        const auto &V = meshes[d]->verts;
+        double lo[3] = {V[0], V[1], dim == 3 ? V[2] : 0.0};
+        double hi[3] = {lo[0], lo[1], lo[2]};
+        for (int64_t i = 1; i < D.n_verts; ++i)
+            for (int k = 0; k < dim; ++k) {
+                const double v = V[(size_t)i * dim + k];
+                if (v < lo[k]) lo[k] = v;
+                else if (v > hi[k]) hi[k] = v;
+            }
+        for (int k = 0; k < 3; ++k) { D.bounds[2 * k] = lo[k]; D.bounds[2 * k + 1] = hi[k]; }
+        for (double v : V) L.coord_scale = std::max(L.coord_scale, std::fabs(v));
+    }
+    const size_t ns = (size_t)po;
+    L.slot_prim.resize(ns);
     if (dim == 2) {
-        L.nd2.resize(nn);
-        for (size_t i = 0; i < nn; ++i) {
-            const RefNode &r = b.nodes[i];
-            L.nd2[i] = NodeD2{r.lo[0], r.lo[1], r.hi[0], r.hi[1], r.count, r.idx};
-        }
-        L.segd.resize(ns);
-        L.segf.resize(ns);
-        L.seg_inv_vv.resize(ns);
-        for (size_t s = 0; s < ns; ++s) {
-            const int32_t p = b.slots[s];
-            const int32_t a = m.prims[2 * p], c = m.prims[2 * p + 1];
-            L.segd[s] = SegD{V(a, 0), V(a, 1), V(c, 0), V(c, 1)};
-            const float ax = (float)V(a, 0), ay = (float)V(a, 1);
-            const float vx = (float)V(c, 0) - ax, vy = (float)V(c, 1) - ay;
-            L.segf[s] = SegF{ax, ay, vx, vy};
-            const float vv = vx * vx + vy * vy;
-            L.seg_inv_vv[s] = vv > 0.0f ? 1.0f / vv : 0.0f;
-        }
+        L.nd2.resize(n_nodes); L.segd.resize(ns); L.segf.resize(ns); L.seg_inv_vv.resize(ns);
     } else {
-        L.nd3.resize(nn);
-        for (size_t i = 0; i < nn; ++i) {
+        L.nd3.resize(n_nodes); L.trid.resize(ns); L.trif.resize(ns);
+    }
+    for (size_t d = 0; d < nd; ++d) {
+        const HostMesh &m = *meshes[d];
+        const HostBVH &b = *bvhs[d];
+        const DomainInfo &D = L.domains[d];
+        auto V = [&](int32_t vi, int k) { return m.verts[(size_t)vi * dim + k]; };
+        for (size_t i = 0; i < b.nodes.size(); ++i) {
             const RefNode &r = b.nodes[i];
-            L.nd3[i] = NodeD3{r.lo[0], r.lo[1], r.lo[2], r.hi[0], r.hi[1], r.hi[2], r.count, r.idx};
+            // leaf: first global slot; internal: global right child
+            const int32_t idx = r.count > 0 ? (int32_t)(r.idx + D.prim_off) : r.idx + D.node_off;
+            if (dim == 2) L.nd2[D.node_off + i] = NodeD2{r.lo[0], r.lo[1], r.hi[0], r.hi[1], r.count, idx};
+            else L.nd3[D.node_off + i] = NodeD3{r.lo[0], r.lo[1], r.lo[2], r.hi[0], r.hi[1], r.hi[2], r.count, idx};
         }
-        L.trid.resize(ns);
-        L.trif.resize(ns);
-        for (size_t s = 0; s < ns; ++s) {
+        for (size_t s = 0; s < b.slots.size(); ++s) {
+            const size_t gs = (size_t)D.prim_off + s;
             const int32_t p = b.slots[s];
-            const int32_t i0 = m.prims[3 * p], i1 = m.prims[3 * p + 1], i2 = m.prims[3 * p + 2];
-            TriD t{};
-            t.v0x = V(i0, 0); t.v0y = V(i0, 1); t.v0z = V(i0, 2);
-            t.e1x = V(i1, 0) - t.v0x; t.e1y = V(i1, 1) - t.v0y; t.e1z = V(i1, 2) - t.v0z;
-            t.e2x = V(i2, 0) - t.v0x; t.e2y = V(i2, 1) - t.v0y; t.e2z = V(i2, 2) - t.v0z;
-            // cache fill of src/bvh.cpp:199-203
-            t.a = t.e1x * t.e1x + t.e1y * t.e1y + t.e1z * t.e1z;
-            t.b = t.e1x * t.e2x + t.e1y * t.e2y + t.e1z * t.e2z;
-            t.c = t.e2x * t.e2x + t.e2y * t.e2y + t.e2z * t.e2z;
-            t.det = t.a * t.c - t.b * t.b;
-            t.denom = t.a - 2 * t.b + t.c;
-            L.trid[s] = t;
-            TriF f{};
-            f.v0x = (float)t.v0x; f.v0y = (float)t.v0y; f.v0z = (float)t.v0z;
-            f.e1x = (float)t.e1x; f.e1y = (float)t.e1y; f.e1z = (float)t.e1z;
-            f.e2x = (float)t.e2x; f.e2y = (float)t.e2y; f.e2z = (float)t.e2z;
-            f.a = (float)t.a; f.b = (float)t.b; f.c = (float)t.c;
-            f.det = (float)t.det; f.denom = (float)t.denom;
-            f.inv_det = t.det != 0.0 ? (float)(1.0 / t.det) : 0.0f;
-            L.trif[s] = f;
+            L.slot_prim[gs] = (int32_t)(p + D.prim_off);
+            if (dim == 2) {
+                const int32_t a = m.prims[2 * p], c = m.prims[2 * p + 1];
+                L.segd[gs] = SegD{V(a, 0), V(a, 1), V(c, 0), V(c, 1)};
+                const float ax = (float)V(a, 0), ay = (float)V(a, 1);
+                const float vx = (float)V(c, 0) - ax, vy = (float)V(c, 1) - ay;
+                L.segf[gs] = SegF{ax, ay, vx, vy};
+                const float vv = vx * vx + vy * vy;
+                L.seg_inv_vv[gs] = vv > 0.0f ? 1.0f / vv : 0.0f;
+            } else {
+                const int32_t i0 = m.prims[3 * p], i1 = m.prims[3 * p + 1], i2 = m.prims[3 * p + 2];
+                TriD t{};
+                t.v0x = V(i0, 0); t.v0y = V(i0, 1); t.v0z = V(i0, 2);
+                t.e1x = V(i1, 0) - t.v0x; t.e1y = V(i1, 1) - t.v0y; t.e1z = V(i1, 2) - t.v0z;
+                t.e2x = V(i2, 0) - t.v0x; t.e2y = V(i2, 1) - t.v0y; t.e2z = V(i2, 2) - t.v0z;
+                // cache fill of src/bvh.cpp:199-203
+                t.a = t.e1x * t.e1x + t.e1y * t.e1y + t.e1z * t.e1z;
+                t.b = t.e1x * t.e2x + t.e1y * t.e2y + t.e1z * t.e2z;
+                t.c = t.e2x * t.e2x + t.e2y * t.e2y + t.e2z * t.e2z;
+                t.det = t.a * t.c - t.b * t.b;
+                t.denom = t.a - 2 * t.b + t.c;
+                L.trid[gs] = t;
+                TriF f{};
+                f.v0x = (float)t.v0x; f.v0y = (float)t.v0y; f.v0z = (float)t.v0z;
+                f.e1x = (float)t.e1x; f.e1y = (float)t.e1y; f.e1z = (float)t.e1z;
+                f.e2x = (float)t.e2x; f.e2y = (float)t.e2y; f.e2z = (float)t.e2z;
+                f.a = (float)t.a; f.b = (float)t.b; f.c = (float)t.c;
+                f.det = (float)t.det; f.denom = (float)t.denom;
+                f.inv_det = t.det != 0.0 ? (float)(1.0 / t.det) : 0.0f;
+                L.trif[gs] = f;
+            }
         }
     }
 
-    // FP32 child-box layout: the internal nodes of the reference tree in
-    // breadth-first order, so the top levels are one contiguous prefix that a
-    // block can stage in LDS.
-    auto ref_of = [&](int32_t node, const std::vector<int32_t> &remap) -> int32_t {
-        const RefNode &r = b.nodes[node];
-        if (r.count > 0) return -(1 + ((r.idx << 3) | r.count));
-        return remap[node];
-    };
-    std::vector<int32_t> remap(nn, -1);
+    // FP32 child-box layout: internal nodes of the whole forest in
+    // breadth-first order (all roots first), global leaf slots.
+    std::vector<int32_t> remap(n_nodes, -1);
     int32_t ni = 0;
     {
         std::vector<int32_t> queue;
-        queue.reserve(nn);
-        if (b.nodes[0].count == 0) queue.push_back(0);
+        queue.reserve(n_nodes);
+        for (size_t d = 0; d < nd; ++d) {
+            const int32_t r = L.domains[d].node_off;
+            if ((dim == 2 ? L.nd2[r].count : L.nd3[r].count) == 0) queue.push_back(r);
+        }
         for (size_t h = 0; h < queue.size(); ++h) {
             const int32_t i = queue[h];
             remap[i] = ni++;
-            const int32_t l = i + 1, r = b.nodes[i].idx;
-            if (b.nodes[l].count == 0) queue.push_back(l);
-            if (b.nodes[r].count == 0) queue.push_back(r);
+            const int32_t l = i + 1, r = dim == 2 ? L.nd2[i].idx : L.nd3[i].idx;
+            if ((dim == 2 ? L.nd2[l].count : L.nd3[l].count) == 0) queue.push_back(l);
+            if ((dim == 2 ? L.nd2[r].count : L.nd3[r].count) == 0) queue.push_back(r);
         }
     }
-    L.root_ref = ref_of(0, remap);
+    auto count_of = [&](int32_t i) { return dim == 2 ? L.nd2[i].count : L.nd3[i].count; };
+    auto idx_of = [&](int32_t i) { return dim == 2 ? L.nd2[i].idx : L.nd3[i].idx; };
+    auto ref_of = [&](int32_t node) -> int32_t {
+        if (count_of(node) > 0) return -(1 + ((idx_of(node) << 3) | count_of(node)));
+        return remap[node];
+    };
+    for (size_t d = 0; d < nd; ++d) L.domains[d].root_ref = ref_of(L.domains[d].node_off);
     if (dim == 2) L.nf2.resize(ni); else L.nf3.resize(ni);
-    for (size_t i = 0; i < nn; ++i) {
-        const RefNode &r = b.nodes[i];
-        if (r.count > 0) continue;
-        const int32_t l = (int32_t)i + 1, rr = r.idx;
-        const RefNode &A = b.nodes[l], &B = b.nodes[rr];
+    for (size_t i = 0; i < n_nodes; ++i) {
+        if (count_of((int32_t)i) > 0) continue;
+        const int32_t l = (int32_t)i + 1, rr = idx_of((int32_t)i);
         if (dim == 2) {
+            const NodeD2 &A = L.nd2[l], &B = L.nd2[rr];
             NodeF2 n{};
-            n.l_lox = round_down(A.lo[0]); n.l_loy = round_down(A.lo[1]);
-            n.l_hix = round_up(A.hi[0]); n.l_hiy = round_up(A.hi[1]);
-            n.r_lox = round_down(B.lo[0]); n.r_loy = round_down(B.lo[1]);
-            n.r_hix = round_up(B.hi[0]); n.r_hiy = round_up(B.hi[1]);
-            n.left = ref_of(l, remap);
-            n.right = ref_of(rr, remap);
+            n.l_lox = round_down(A.lox); n.l_loy = round_down(A.loy);
+            n.l_hix = round_up(A.hix); n.l_hiy = round_up(A.hiy);
+            n.r_lox = round_down(B.lox); n.r_loy = round_down(B.loy);
+            n.r_hix = round_up(B.hix); n.r_hiy = round_up(B.hiy);
+            n.left = ref_of(l);
+            n.right = ref_of(rr);
             L.nf2[remap[i]] = n;
         } else {
+            const NodeD3 &A = L.nd3[l], &B = L.nd3[rr];
             NodeF3 n{};
-            n.l_lox = round_down(A.lo[0]); n.l_loy = round_down(A.lo[1]); n.l_loz = round_down(A.lo[2]);
-            n.l_hix = round_up(A.hi[0]); n.l_hiy = round_up(A.hi[1]); n.l_hiz = round_up(A.hi[2]);
-            n.r_lox = round_down(B.lo[0]); n.r_loy = round_down(B.lo[1]); n.r_loz = round_down(B.lo[2]);
-            n.r_hix = round_up(B.hi[0]); n.r_hiy = round_up(B.hi[1]); n.r_hiz = round_up(B.hi[2]);
-            n.left = ref_of(l, remap);
-            n.right = ref_of(rr, remap);
+            n.l_lox = round_down(A.lox); n.l_loy = round_down(A.loy); n.l_loz = round_down(A.loz);
+            n.l_hix = round_up(A.hix); n.l_hiy = round_up(A.hiy); n.l_hiz = round_up(A.hiz);
+            n.r_lox = round_down(B.lox); n.r_loy = round_down(B.loy); n.r_loz = round_down(B.loz);
+            n.r_hix = round_up(B.hix); n.r_hiy = round_up(B.hiy); n.r_hiz = round_up(B.hiz);
+            n.left = ref_of(l);
+            n.right = ref_of(rr);
             L.nf3[remap[i]] = n;
         }
     }

@@ -76,6 +76,16 @@ struct alignas(16) TriF {
 };
 struct alignas(16) SegF { float s0x, s0y, vx, vy; };
 
+// One domain of a forest: its slice of the concatenated arrays.
+struct DomainInfo {
+    int64_t vert_off = 0, n_verts = 0;
+    int64_t prim_off = 0, n_prims = 0;   // primitives == leaf slots
+    int32_t node_off = 0;                // FP64 root node (reference order)
+    int32_t root_ref = 0;                // FP32 root reference
+    int depth = 0;
+    double bounds[6] = {0, 0, 0, 0, 0, 0};  // bbox of the domain's vertices (mesh.cpp:296-322)
+};
+
 struct DeviceLayouts {
     std::vector<NodeD2> nd2;
     std::vector<NodeD3> nd3;
@@ -86,10 +96,17 @@ struct DeviceLayouts {
     std::vector<SegF> segf;
     std::vector<TriF> trif;
     std::vector<float> seg_inv_vv;  // 2D: 1 / |v|^2 per slot
-    int32_t root_ref = 0;      // FP32 root reference (node 0 or a leaf)
-    double coord_scale = 1.0;  // max |coordinate| of the mesh, >= 1
+    std::vector<int32_t> slot_prim; // global slot -> global primitive
+    std::vector<DomainInfo> domains;
+    int depth = 0;             // deepest domain
+    double coord_scale = 1.0;  // max |coordinate| over all domains, >= 1
 };
 
-DeviceLayouts make_layouts(const HostMesh &m, const HostBVH &b);
+// Concatenate domains (each with its own reference-rule BVH) into one set of
+// device arrays.  Vertex, primitive, slot and node indices become global; the
+// FP32 internal nodes are numbered breadth-first over the whole forest, so
+// the top levels of every domain form the prefix staged in LDS.
+DeviceLayouts make_layouts(const std::vector<const HostMesh *> &meshes,
+                           const std::vector<const HostBVH *> &bvhs);
 
 }  // namespace wosx

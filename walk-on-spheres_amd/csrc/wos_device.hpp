@@ -41,7 +41,6 @@ struct MeshView {
     const SegF *segf;
     const float *seg_inv_vv;
     const TriF *trif;
-    int32_t root_ref;
     float shrink_abs;          // absolute conservative margin of FP32 radii
 };
 
@@ -61,7 +60,10 @@ struct StartInfo {
     double g;             // g at the nearest point (Boundary value, zero-step walks)
     double s, t;          // nearest primitive parameters
     int32_t prim, bid;
-    int32_t slot, pad;    // leaf slot of the nearest primitive
+    int32_t slot;         // leaf slot of the nearest primitive
+    int32_t root64;       // the point's domain: FP64 root node
+    int32_t root32;       // the point's domain: FP32 root reference
+    int32_t pad;
     uint32_t key0, key1;  // walk-stream key
 };
 
@@ -177,13 +179,12 @@ __device__ __forceinline__ double box_d2_3(const NodeD3 &n, double px, double py
 // Closest point, reference traversal (src/bvh.cpp:295-387): pop, cull by
 // the popped box, test leaf primitives with strict <, push the farther
 // child first.  `stk` is this lane's LDS stack (stride = block size).
-__device__ __forceinline__ NpqD npq_ref_2d(const MeshView &m, double px, double py, int *stk,
-                                           int stride) {
+__device__ __forceinline__ NpqD npq_ref_2d(const MeshView &m, int root, double px, double py,
+                                           int *stk, int stride) {
     NpqD r;
     r.d2 = INFINITY; r.px = r.py = r.pz = 0.0; r.s = r.t = 0.0; r.prim = -1; r.slot = -1;
-    int sp = 0;
-    stk[0] = 0;
-    sp = 1;
+    int sp = 1;
+    stk[0] = root;
     while (sp > 0) {
         const int curr = stk[(--sp) * stride];
         const NodeD2 n = m.nd2[curr];
@@ -208,12 +209,12 @@ __device__ __forceinline__ NpqD npq_ref_2d(const MeshView &m, double px, double 
     return r;
 }
 
-__device__ __forceinline__ NpqD npq_ref_3d(const MeshView &m, double px, double py, double pz,
-                                           int *stk, int stride) {
+__device__ __forceinline__ NpqD npq_ref_3d(const MeshView &m, int root, double px, double py,
+                                           double pz, int *stk, int stride) {
     NpqD r;
     r.d2 = INFINITY; r.px = r.py = r.pz = 0.0; r.s = r.t = 0.0; r.prim = -1; r.slot = -1;
     int sp = 1;
-    stk[0] = 0;
+    stk[0] = root;
     while (sp > 0) {
         const int curr = stk[(--sp) * stride];
         const NodeD3 n = m.nd3[curr];
@@ -243,10 +244,11 @@ __device__ __forceinline__ NpqD npq_ref_3d(const MeshView &m, double px, double 
 // --------------------------------------------------------------------------
 // FP64 classification (src/inside.cpp)
 // --------------------------------------------------------------------------
-// src/inside.cpp:117-137 over every segment (order independent)
-__device__ __forceinline__ int winding_number_d(const MeshView &m, double px, double py) {
+// src/inside.cpp:117-137 over every segment of the domain (order independent)
+__device__ __forceinline__ int winding_number_d(const MeshView &m, int s0, int s1, double px,
+                                                double py) {
     int wn = 0;
-    for (int s = 0; s < m.n_prims; ++s) {
+    for (int s = s0; s < s1; ++s) {
         const SegD g = m.segd[s];
         const double area = (g.s1x - g.s0x) * (py - g.s0y) - (px - g.s0x) * (g.s1y - g.s0y);
         if (g.s0y <= py) {
@@ -325,11 +327,11 @@ __device__ __forceinline__ bool ray_hits_box(const NodeD3 &n, double px, double 
 }
 
 // src/inside.cpp:78-111; returns hit count, sets ambiguous
-__device__ __forceinline__ int ray_cast_d(const MeshView &m, double px, double py, double pz,
-                                          double rx, double ry, double rz, bool &ambiguous,
-                                          int *stk, int stride) {
+__device__ __forceinline__ int ray_cast_d(const MeshView &m, int root, double px, double py,
+                                          double pz, double rx, double ry, double rz,
+                                          bool &ambiguous, int *stk, int stride) {
     int hits = 0, sp = 1;
-    stk[0] = 0;
+    stk[0] = root;
     ambiguous = false;
     while (sp > 0) {
         const int ni = stk[(--sp) * stride];
@@ -502,6 +504,60 @@ template <typename T> __device__ __forceinline__ T screened_mass_3d(T alpha, T R
 }
 
 // --------------------------------------------------------------------------
+// alpha-derivatives of the 3D screened weight and Green function, as the
+// derivatives of the functions above (series branch included), in FP64
+// --------------------------------------------------------------------------
+// d/dalpha log w3(alpha R)
+__device__ __forceinline__ double dlog_w3_dalpha(double alpha, double R) {
+    const double x = alpha * R;
+    if (x == 0.0) return 0.0;
+    if (x < 1e-3) {
+        const double x2 = x * x;
+        return R * (-x / 3.0 + 7.0 * x2 * x / 90.0) / (1.0 - x2 / 6.0 + 7.0 * x2 * x2 / 360.0);
+    }
+    return R * (1.0 / x - 1.0 / tanh(x));
+}
+// 2 y / expm1(2 alpha y), with its limit 1/alpha at y = 0
+__device__ __forceinline__ double h_screen(double alpha, double y) {
+    const double z = 2.0 * alpha * y;
+    return z == 0.0 ? 1.0 / alpha : 2.0 * y / expm1(z);
+}
+// d/dalpha log G3(alpha; R, r)
+__device__ __forceinline__ double dlog_g3_dalpha(double alpha, double R, double r) {
+    if (alpha * R < 1e-5) return 0.0;
+    return -r + h_screen(alpha, R - r) - h_screen(alpha, R);
+}
+
+// Per-walk derivatives accumulated by the adjoint replay.
+struct WalkDer {
+    double d_alpha;           // d value / d alpha
+    double d_q;               // d value / d src_value
+    double d_sigma;           // d value / d src_radius
+    double d_c[3];            // d value / d src_center
+    double L;                 // running d log(weight) / d alpha
+};
+
+// source-term derivatives for a term S = base * f(y) (base = weight x
+// volume x G or weight x mass); dlogG = d log G / d alpha (0 if unused)
+__device__ __forceinline__ void add_source_der(WalkDer &D, const EqView &e, int dim, double base,
+                                               double S, double yx, double yy, double yz,
+                                               double dlogG) {
+    if (e.source_kind == WOS_SRC_GAUSSIAN) {
+        const double dx = yx - e.cx, dy = yy - e.cy, dz = dim == 3 ? yz - e.cz : 0.0;
+        const double s2 = e.src_radius * e.src_radius;
+        const double d2 = dx * dx + dy * dy + dz * dz;
+        D.d_q += base * exp(-d2 / s2);
+        D.d_sigma += S * 2.0 * d2 / (s2 * e.src_radius);
+        D.d_c[0] += S * 2.0 * dx / s2;
+        D.d_c[1] += S * 2.0 * dy / s2;
+        D.d_c[2] += S * 2.0 * dz / s2;
+    } else if (e.src_value != 0.0) {
+        D.d_q += S / e.src_value;
+    }
+    D.d_alpha += S * (D.L + dlogG);
+}
+
+// --------------------------------------------------------------------------
 // equation functors (src/equations/*.cpp) driven by the descriptor
 // --------------------------------------------------------------------------
 template <typename T>
@@ -667,12 +723,12 @@ __device__ __forceinline__ float leaf_d2f(const MeshView &m, int slot, float px,
 // the traversal only opens boxes that can still hold a closer primitive; the
 // result is the same minimum as a search from infinity.
 template <int DIM>
-__device__ __forceinline__ float npq_f(const MeshView &m, float px, float py, float pz,
-                                       int &best_slot, StackEntry *stk, int stride,
+__device__ __forceinline__ float npq_f(const MeshView &m, int32_t root, float px, float py,
+                                       float pz, int &best_slot, StackEntry *stk, int stride,
                                        const void *cache, int n_cached) {
     float best = best_slot >= 0 ? leaf_d2f<DIM>(m, best_slot, px, py, pz) : INFINITY;
     int sp = 0;
-    int32_t ref = m.root_ref;
+    int32_t ref = root;
     for (;;) {
         if (ref >= 0) {
             float dl, dr;

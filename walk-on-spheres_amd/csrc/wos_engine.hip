@@ -198,16 +198,26 @@ struct GridSpec {
     double b[6];
 };
 
+// the domain a classify launch works on
+struct DomainView {
+    int root64;        // FP64 root node
+    int32_t root32;    // FP32 root reference
+    int s0, s1;        // leaf-slot range (winding number)
+    double tol;        // boundary tolerance 1e-10 max(1, extent) (runner.hpp:86-88)
+    int64_t out_off;   // first output index of this domain's points
+};
+
 __device__ __forceinline__ double grid_coordinate(double lo, double hi, int index, int count) {
     return count > 1 ? lo + (hi - lo) * (double)index / (double)(count - 1) : lo;
 }
 
 template <int DIM>
 __global__ __launch_bounds__(kBlock) void k_classify(MeshView m, EqView e, GridSpec g,
-                                                     const double *plist, int64_t n, uint64_t seed,
-                                                     double tol, int max_ray_attempts,
-                                                     uint8_t *loc, StartInfo *starts,
-                                                     int32_t *amb, double *bvalue) {
+                                                     DomainView dom, const double *plist,
+                                                     int64_t n, uint64_t seed,
+                                                     int max_ray_attempts, uint8_t *loc,
+                                                     StartInfo *starts, int32_t *amb,
+                                                     double *bvalue) {
     extern __shared__ int lds_i[];
     int *stk = lds_i + threadIdx.x;
     const int stride = blockDim.x;
@@ -228,15 +238,15 @@ __global__ __launch_bounds__(kBlock) void k_classify(MeshView m, EqView e, GridS
     }
     const uint64_t ps = point_seed(seed, (uint64_t)i);
     NpqD q;
-    if (DIM == 2) q = npq_ref_2d(m, x, y, stk, stride);
-    else q = npq_ref_3d(m, x, y, z, stk, stride);
+    if (DIM == 2) q = npq_ref_2d(m, dom.root64, x, y, stk, stride);
+    else q = npq_ref_3d(m, dom.root64, x, y, z, stk, stride);
     const double r = sqrt(q.d2);
     int code;
     int retries = 0;
-    if (r <= tol) {
+    if (r <= dom.tol) {
         code = 2;
     } else if (DIM == 2) {
-        code = winding_number_d(m, x, y) != 0 ? 1 : 0;
+        code = winding_number_d(m, dom.s0, dom.s1, x, y) != 0 ? 1 : 0;
     } else {
         // src/inside.cpp:228-260 with the inside stream of runner.hpp:182-183
         Xoshiro rng(splitmix64(ps ^ kInsideStream));
@@ -247,8 +257,8 @@ __global__ __launch_bounds__(kBlock) void k_classify(MeshView m, EqView e, GridS
             const double phi = 2 * kPi * v;
             const double rr = sqrt(1 - zz * zz);
             bool ambiguous;
-            const int hits = ray_cast_d(m, x, y, z, rr * cos(phi), rr * sin(phi), zz, ambiguous,
-                                        stk, stride);
+            const int hits = ray_cast_d(m, dom.root64, x, y, z, rr * cos(phi), rr * sin(phi), zz,
+                                        ambiguous, stk, stride);
             if (ambiguous) { ++retries; continue; }
             votes += hits & 1;
             ++valid;
@@ -263,15 +273,47 @@ __global__ __launch_bounds__(kBlock) void k_classify(MeshView m, EqView e, GridS
     S.g = gval; S.s = q.s; S.t = q.t;
     S.prim = q.prim;
     S.slot = q.slot;
+    S.root64 = dom.root64;
+    S.root32 = dom.root32;
     S.pad = 0;
     S.bid = (DIM == 2 && q.prim >= 0) ? m.prim_bid[q.prim] : -1;
     S.key0 = (uint32_t)key;
     S.key1 = (uint32_t)(key >> 32);
-    starts[i] = S;
-    loc[i] = (uint8_t)code;
-    amb[i] = retries;
-    bvalue[i] = gval;
+    const int64_t o = dom.out_off + i;
+    starts[o] = S;
+    loc[o] = (uint8_t)code;
+    amb[o] = retries;
+    bvalue[o] = gval;
 }
+
+// --------------------------------------------------------------------------
+// adjoint staging: one record per walk of a round (point major, like vals)
+// --------------------------------------------------------------------------
+struct AdjOut {
+    double *scal;      // 6 per walk: d alpha, d src_value, d src_radius, d src_center[3]
+    double *vw;        // 3 per walk: weight x blend coefficient of each primitive vertex
+    int32_t *prim;     // terminal primitive per walk (-1: none)
+};
+
+// terminal contribution: d(weight g)/d alpha = weight g L; the boundary data
+// enter linearly through the blend (1-s, s) / (1-s-t, s, t) of the exit primitive
+__device__ __forceinline__ void store_adjoint(const AdjOut &a, int64_t item, WalkDer &D,
+                                              double weight, double g, int prim, double s,
+                                              double t) {
+    D.d_alpha += weight * g * D.L;
+    double *o = a.scal + (size_t)item * 6;
+    o[0] = D.d_alpha; o[1] = D.d_q; o[2] = D.d_sigma;
+    o[3] = D.d_c[0]; o[4] = D.d_c[1]; o[5] = D.d_c[2];
+    double *v = a.vw + (size_t)item * 3;
+    v[0] = weight * (1.0 - s - t);
+    v[1] = weight * s;
+    v[2] = weight * t;
+    a.prim[item] = prim;
+}
+
+template <int DIM, typename T, int SMODE>
+__device__ double source_term_adj(const EqView &e, T px, T py, T pz, T R, T weight, uint32_t w,
+                                  uint32_t step, uint32_t k0, uint32_t k1, WalkDer &D);
 
 // --------------------------------------------------------------------------
 // one round of walks: active point k runs walks [wbeg[k], wbeg[k] + count_k);
@@ -311,10 +353,10 @@ __device__ __forceinline__ const StartInfo &round_item(const RoundView &r, int64
 // --------------------------------------------------------------------------
 // kernel 2a: FP64 plain walker, one lane per (point, walk)
 // --------------------------------------------------------------------------
-template <int DIM, bool SRC, bool SCR>
+template <int DIM, bool SRC, bool SCR, bool ADJ>
 __global__ __launch_bounds__(kBlock) void k_walk_fp64(MeshView m, EqView e, RoundView rv,
                                                       double eps, int max_steps, double *vals,
-                                                      int32_t *steps_out) {
+                                                      int32_t *steps_out, AdjOut adj) {
     extern __shared__ int lds_i[];
     int *stk = lds_i + threadIdx.x;
     const int stride = blockDim.x;
@@ -328,13 +370,22 @@ __global__ __launch_bounds__(kBlock) void k_walk_fp64(MeshView m, EqView e, Roun
     double sample = 0.0, weight = 1.0;
     int steps = 0;
     const double alpha = e.alpha;
+    WalkDer D{};
     while (R > eps && steps < max_steps) {
         if (SRC && e.source_kind != WOS_SRC_ZERO) {
-            CursorD src(w, (uint32_t)steps, 1, S.key0, S.key1);
-            sample += source_term<DIM, double>(e, px, py, pz, R, weight, src);
+            if constexpr (ADJ) {
+                sample += source_term_adj<DIM, double, 1>(e, px, py, pz, R, weight, w,
+                                                          (uint32_t)steps, S.key0, S.key1, D);
+            } else {
+                CursorD src(w, (uint32_t)steps, 1, S.key0, S.key1);
+                sample += source_term<DIM, double>(e, px, py, pz, R, weight, src);
+            }
         }
-        if (SCR) weight *= DIM == 2 ? screened_weight_2d<double>(alpha, R)
-                                    : screened_weight_3d<double>(alpha, R);
+        if (SCR) {
+            if constexpr (ADJ && DIM == 3) D.L += dlog_w3_dalpha(alpha, R);
+            weight *= DIM == 2 ? screened_weight_2d<double>(alpha, R)
+                               : screened_weight_3d<double>(alpha, R);
+        }
         CursorD wk(w, (uint32_t)steps, 0, S.key0, S.key1);
         if (DIM == 2) {
             const double theta = wk.unit() * 2 * kPi;
@@ -349,13 +400,19 @@ __global__ __launch_bounds__(kBlock) void k_walk_fp64(MeshView m, EqView e, Roun
             pz = pz + R * z;
         }
         ++steps;
-        q = DIM == 2 ? npq_ref_2d(m, px, py, stk, stride) : npq_ref_3d(m, px, py, pz, stk, stride);
+        q = DIM == 2 ? npq_ref_2d(m, S.root64, px, py, stk, stride)
+                     : npq_ref_3d(m, S.root64, px, py, pz, stk, stride);
         R = sqrt(q.d2);
     }
     const bool hit = R > eps;
     const double g = steps == 0 ? S.g : eq_boundary(e, m, q.px, q.py, q.pz, q.prim, q.s, q.t);
     vals[item] = sample + weight * g;
     steps_out[item] = steps | (hit ? (int32_t)0x80000000 : 0);
+    if constexpr (ADJ) {
+        const int prim = steps == 0 ? S.prim : q.prim;
+        const double s = steps == 0 ? S.s : q.s, t = steps == 0 ? S.t : q.t;
+        store_adjoint(adj, item, D, weight, g, prim, s, DIM == 3 ? t : 0.0);
+    }
 }
 
 // FP32 source term.  Same draws as source_term (the walk's tag-1 counter
@@ -419,6 +476,127 @@ __device__ __forceinline__ double source_term_f(const EqView &e, float px, float
     return (double)(weight * volume * f * G);
 }
 
+// Source term plus its parameter derivatives (adjoint replay).  Same draws
+// and same value as source_term (T = double) / source_term_f (T = float);
+// the derivatives are formed in FP64.
+template <int DIM, typename T, int SMODE>
+__device__ double source_term_adj(const EqView &e, T px, T py, T pz, T R, T weight, uint32_t w,
+                                  uint32_t step, uint32_t k0, uint32_t k1, WalkDer &D) {
+    constexpr bool F32 = sizeof(T) == 4;
+    const bool screened = e.kind == WOS_EQ_SCREENED;
+    const T alpha = (T)e.alpha;
+    if (DIM == 2 && !eq_source_may_intersect<T>(e, px, py, R)) return 0.0;
+    T sx = px, sy = py, sz = pz, dist = (T)0;
+    if (SMODE == 2 || e.source_mode == WOS_SOURCE_GREEN) {
+        T radius;
+        if (F32) {
+            CursorF rng(w, step, 1, k0, k1);
+            if (DIM == 2) {
+                radius = screened ? (T)screened_radius_2d<float>((float)alpha, (float)R, rng)
+                                  : (T)poisson_radius_2d<float>((float)R, rng);
+                const T angle = (T)2 * (T)kPi * (T)rng.unit();
+                sx = px + radius * cos(angle);
+                sy = py + radius * sin(angle);
+            } else {
+                radius = screened ? (T)screened_radius_3d<float>((float)alpha, (float)R, rng)
+                                  : (T)poisson_radius_3d<float>((float)R, rng);
+                const T z = (T)2 * (T)rng.unit() - (T)1;
+                const T angle = (T)2 * (T)kPi * (T)rng.unit();
+                const T xy = sqrt((T)1 - z * z);
+                sx = px + radius * xy * cos(angle);
+                sy = py + radius * xy * sin(angle);
+                sz = pz + radius * z;
+            }
+        } else {
+            CursorD rng(w, step, 1, k0, k1);
+            if (DIM == 2) {
+                radius = screened ? (T)screened_radius_2d<double>((double)alpha, (double)R, rng)
+                                  : (T)poisson_radius_2d<double>((double)R, rng);
+                const T angle = (T)2 * (T)kPi * (T)rng.unit();
+                sx = px + radius * cos(angle);
+                sy = py + radius * sin(angle);
+            } else {
+                radius = screened ? (T)screened_radius_3d<double>((double)alpha, (double)R, rng)
+                                  : (T)poisson_radius_3d<double>((double)R, rng);
+                const T z = (T)2 * (T)rng.unit() - (T)1;
+                const T angle = (T)2 * (T)kPi * (T)rng.unit();
+                const T xy = sqrt((T)1 - z * z);
+                sx = px + radius * xy * cos(angle);
+                sy = py + radius * xy * sin(angle);
+                sz = pz + radius * z;
+            }
+        }
+        const T f = eq_source<T>(e, sx, sy, sz, DIM);
+        if (f == (T)0) return 0.0;
+        T mass;
+        if (DIM == 2) mass = screened ? screened_mass_2d<T>(alpha, R) : R * R / (T)4;
+        else mass = screened ? screened_mass_3d<T>(alpha, R) : R * R / (T)6;
+        const double S = (double)(weight * mass * f);
+        add_source_der(D, e, DIM, (double)(weight * mass), S, sx, sy, sz, 0.0);
+        return S;
+    }
+    // uniform ball point
+    if constexpr (F32) {
+        if (e.source_kind == WOS_SRC_CONSTANT) {
+            if (DIM == 2) dist = R * sqrtf(u01f_open(philox(w, step, 0, 1, k0, k1).b));
+            else dist = R * cbrtf(u01f_open(philox(w, step, 1, 1, k0, k1).a));
+        } else {
+            CursorF rng(w, step, 1, k0, k1);
+            if (DIM == 2) {
+                const float theta = rng.unit();
+                dist = R * sqrtf(rng.unit_open());
+                float sn, cs;
+                __sincosf(theta * (float)(2.0 * kPi), &sn, &cs);
+                sx = __builtin_fmaf(dist, cs, px);
+                sy = __builtin_fmaf(dist, sn, py);
+            } else {
+                const float u = rng.unit(), v = rng.unit(), ww = rng.unit_open();
+                const float z = 2.0f * u - 1.0f;
+                const float surf = sqrtf(fmaxf(0.0f, 1.0f - z * z));
+                dist = R * cbrtf(ww);
+                float sn, cs;
+                __sincosf(v * (float)(2.0 * kPi), &sn, &cs);
+                sx = __builtin_fmaf(dist * surf, cs, px);
+                sy = __builtin_fmaf(dist * surf, sn, py);
+                sz = __builtin_fmaf(dist, z, pz);
+            }
+        }
+    } else {
+        CursorD rng(w, step, 1, k0, k1);
+        if (DIM == 2) {
+            const T theta = (T)rng.unit() * (T)2 * (T)kPi;
+            const T r = R * sqrt((T)rng.unit_open());
+            sx = px + r * cos(theta);
+            sy = py + r * sin(theta);
+            const T dx = px - sx, dy = py - sy;
+            dist = sqrt(dx * dx + dy * dy);
+        } else {
+            const T u = (T)rng.unit(), v = (T)rng.unit(), ww = (T)rng.unit_open();
+            const T z = (T)2 * u - (T)1, phi = (T)2 * (T)kPi * v;
+            const T surf = sqrt((T)1 - z * z), vol = R * cbrt(ww);
+            sx = px + vol * surf * cos(phi);
+            sy = py + vol * surf * sin(phi);
+            sz = pz + vol * z;
+            const T dx = px - sx, dy = py - sy, dz = pz - sz;
+            dist = sqrt(dx * dx + dy * dy + dz * dz);
+        }
+    }
+    const T f = e.source_kind == WOS_SRC_CONSTANT ? (T)e.src_value : eq_source<T>(e, sx, sy, sz, DIM);
+    if (f == (T)0 || (F32 && !(dist > (T)0))) return 0.0;
+    T volume, G;
+    if (DIM == 2) {
+        volume = (T)kPi * R * R;
+        G = screened ? screened_green_2d<T>(alpha, R, dist) : poisson_green_2d<T>(R, dist);
+    } else {
+        volume = F32 ? (T)(4.0 * kPi / 3.0) * R * R * R : (T)4 * (T)kPi * (R * R * R) / (T)3;
+        G = screened ? screened_green_3d<T>(alpha, R, dist) : poisson_green_3d<T>(R, dist);
+    }
+    const double S = (double)(weight * volume * f * G);
+    const double dlogG = (DIM == 3 && screened) ? dlog_g3_dalpha(e.alpha, (double)R, (double)dist) : 0.0;
+    add_source_der(D, e, DIM, (double)(weight * volume * G), S, sx, sy, sz, dlogG);
+    return S;
+}
+
 // --------------------------------------------------------------------------
 // kernel 2b: FP32 persistent walker with ballot-driven lane refill
 // --------------------------------------------------------------------------
@@ -437,8 +615,9 @@ struct PersistArgs {
     int32_t *steps_out;
 };
 
-template <int DIM, int SMODE, bool SCR>
-__global__ __launch_bounds__(kBlock) void k_walk_fp32(MeshView m, EqView e, PersistArgs a) {
+template <int DIM, int SMODE, bool SCR, bool ADJ>
+__global__ __launch_bounds__(kBlock) void k_walk_fp32(MeshView m, EqView e, PersistArgs a,
+                                                      AdjOut adj) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     // stage the top of the tree once per block
     {
@@ -459,9 +638,11 @@ __global__ __launch_bounds__(kBlock) void k_walk_fp32(MeshView m, EqView e, Pers
     float px = 0, py = 0, pz = 0, R = 0, weight = 1.0f;
     double sample = 0.0;
     int steps = 0, slot = -1;
+    int32_t root = 0;
     uint32_t w = 0, k0 = 0, k1 = 0;
     size_t out_index = 0;
     const float alpha = (float)e.alpha;
+    WalkDer D{};
 
     // wave-uniform pool of work items
     uint64_t pool_next = 0, pool_end = 0;
@@ -495,14 +676,19 @@ __global__ __launch_bounds__(kBlock) void k_walk_fp32(MeshView m, EqView e, Pers
                     sample = 0.0;
                     steps = 0;
                     slot = S.slot;
+                    root = S.root32;
                     k0 = S.key0;
                     k1 = S.key1;
                     out_index = (size_t)item;
                     active = true;
+                    if constexpr (ADJ) D = WalkDer{};
                     if (!(R > a.eps_f)) {
                         // zero-step walk: g at the start's nearest point
                         a.vals[out_index] = S.g;
                         a.steps_out[out_index] = 0;
+                        if constexpr (ADJ)
+                            store_adjoint(adj, (int64_t)item, D, 1.0, S.g, S.prim, S.s,
+                                          DIM == 3 ? S.t : 0.0);
                         active = false;
                     }
                 }
@@ -516,10 +702,18 @@ __global__ __launch_bounds__(kBlock) void k_walk_fp32(MeshView m, EqView e, Pers
         }
         if (active) {
         // ---- one walk step (include/wos/wos.hpp:68-127)
-        if (SMODE != 0 && e.source_kind != WOS_SRC_ZERO)
-            sample += source_term_f<DIM, SMODE>(e, px, py, pz, R, weight, w, (uint32_t)steps, k0, k1);
-        if (SCR) weight *= DIM == 2 ? screened_weight_2d<float>(alpha, R)
-                                    : screened_weight_3d<float>(alpha, R);
+        if (SMODE != 0 && e.source_kind != WOS_SRC_ZERO) {
+            if constexpr (ADJ)
+                sample += source_term_adj<DIM, float, SMODE>(e, px, py, pz, R, weight, w,
+                                                             (uint32_t)steps, k0, k1, D);
+            else
+                sample += source_term_f<DIM, SMODE>(e, px, py, pz, R, weight, w, (uint32_t)steps, k0, k1);
+        }
+        if (SCR) {
+            if constexpr (ADJ && DIM == 3) D.L += dlog_w3_dalpha(e.alpha, (double)R);
+            weight *= DIM == 2 ? screened_weight_2d<float>(alpha, R)
+                               : screened_weight_3d<float>(alpha, R);
+        }
         Cursor wk(w, (uint32_t)steps, 0, k0, k1);
         if (DIM == 2) {
             const float u = wk.unitf();
@@ -538,21 +732,23 @@ __global__ __launch_bounds__(kBlock) void k_walk_fp32(MeshView m, EqView e, Pers
             pz = __builtin_fmaf(R, z, pz);
         }
         ++steps;
-        const float d2 = npq_f<DIM>(m, px, py, pz, slot, stk, stride, cache, a.n_cached);
+        const float d2 = npq_f<DIM>(m, root, px, py, pz, slot, stk, stride, cache, a.n_cached);
         R = sqrtf(d2) * a.shrink_rel - m.shrink_abs;
         if (!(R > a.eps_f) || steps >= a.max_steps) {
             // terminal value in FP64: nearest point on the nearest primitive
             const bool hit = R > a.eps_f;
-            double g = 0.0;
+            double g = 0.0, s = 0.0, t = 0.0;
+            int prim = -1;
             if (slot >= 0) {
-                const int prim = m.slot_prim[slot];
-                double nx, ny, nz = 0.0, s, t = 0.0;
+                prim = m.slot_prim[slot];
+                double nx, ny, nz = 0.0;
                 if (DIM == 2) npq_seg_d(m.segd[slot], (double)px, (double)py, nx, ny, s);
                 else eberly_d(m.trid[slot], (double)px, (double)py, (double)pz, nx, ny, nz, s, t);
                 g = eq_boundary(e, m, nx, ny, nz, prim, s, t);
             }
             a.vals[out_index] = sample + (double)weight * g;
             a.steps_out[out_index] = steps | (hit ? (int32_t)0x80000000 : 0);
+            if constexpr (ADJ) store_adjoint(adj, (int64_t)out_index, D, (double)weight, g, prim, s, t);
             active = false;
         }
         }  // active
@@ -586,6 +782,61 @@ __global__ __launch_bounds__(kBlock) void k_welford(RoundView rv, const double *
         h += (s < 0) ? 1 : 0;
     }
     mean[p] = mu; m2[p] = M2; count[p] = n; steps_sum[p] = ts; hits[p] = h;
+}
+
+// --------------------------------------------------------------------------
+// kernel 4 (adjoint): per point, sum the walks' parameter derivatives in walk
+// order; scatter coef x (weight x blend) to the exit primitive's vertices and
+// coef x weight to its boundary id.  coef = dL/du_p / N_p.
+// --------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_adj_reduce(RoundView rv, AdjOut adj, MeshView m,
+                                                       const double *coef, double *scal_sum,
+                                                       double *grad_v, double *grad_bnd) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= rv.n) return;
+    const int p = rv.pid[k];
+    const int64_t base = rv.off ? rv.off[k] : (int64_t)k * rv.cw;
+    const int cnt = rv.off ? (int)(rv.off[k + 1] - rv.off[k]) : rv.cw;
+    const double c = coef[p];
+    double s[6];
+    for (int i = 0; i < 6; ++i) s[i] = scal_sum[(size_t)p * 6 + i];
+    for (int j = 0; j < cnt; ++j) {
+        const size_t o = (size_t)(base + j);
+        for (int i = 0; i < 6; ++i) s[i] += adj.scal[o * 6 + i];
+        const int prim = adj.prim[o];
+        if (prim < 0 || c == 0.0) continue;
+        const double *vw = adj.vw + o * 3;
+        if (grad_v)
+            for (int v = 0; v < m.dim; ++v)
+                atomicAdd(&grad_v[m.prims[(size_t)prim * m.dim + v]], c * vw[v]);
+        if (grad_bnd) {
+            const int b = m.prim_bid[prim];
+            if (b >= 0 && b < WOS_MAX_BND_VALUES) atomicAdd(&grad_bnd[b], c * (vw[0] + vw[1] + vw[2]));
+        }
+    }
+    for (int i = 0; i < 6; ++i) scal_sum[(size_t)p * 6 + i] = s[i];
+}
+
+// boundary points: u = g(start's nearest point), linear in the boundary data
+__global__ __launch_bounds__(kBlock) void k_adj_boundary(MeshView m, const uint8_t *loc,
+                                                         const StartInfo *starts,
+                                                         const double *dl_du, int64_t n,
+                                                         double *grad_v, double *grad_bnd) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || loc[i] != 2) return;
+    const StartInfo &S = starts[i];
+    const double c = dl_du[i];
+    if (S.prim < 0 || c == 0.0) return;
+    if (grad_v) {
+        const double t = m.dim == 3 ? S.t : 0.0;
+        const double wv[3] = {1.0 - S.s - t, S.s, t};
+        for (int v = 0; v < m.dim; ++v)
+            atomicAdd(&grad_v[m.prims[(size_t)S.prim * m.dim + v]], c * wv[v]);
+    }
+    if (grad_bnd) {
+        const int b = m.prim_bid[S.prim];
+        if (b >= 0 && b < WOS_MAX_BND_VALUES) atomicAdd(&grad_bnd[b], c);
+    }
 }
 
 // --------------------------------------------------------------------------
@@ -635,12 +886,12 @@ struct wos_ctx {
     hipStream_t stream = nullptr;
     int n_cu = 0;
     HostMesh mesh;
-    HostBVH bvh;
     DeviceLayouts L;
     bool has_mesh = false;
     DevBuf nd2, nd3, segd, trid, slot_prim, prim_bid, prims, verts, nf2, nf3, segf, seg_inv, trif;
     DevBuf plist_in, loc, starts, amb, bvalue, inside, vals, steps, mean, m2, count, ssum, hits,
-        counter, vv, pid_all, r_pid, r_wbeg, r_off;
+        counter, vv, pid_all, r_pid, r_wbeg, r_off, coef, scal_sum, grad_v, grad_b, dl_du,
+        adj_scal, adj_vw, adj_prim;
     int last_rounds = 0;
     MeshView view{};
     hipEvent_t ev[6]{};
@@ -655,33 +906,34 @@ namespace {
 
 template <typename K>
 hipError_t launch_classify(wos_ctx *c, K kernel, const EqView &ev, const GridSpec &g,
-                           const double *plist, int64_t n, uint64_t seed, double tol, int mra) {
+                           const DomainView &dv, const double *plist, int64_t n, uint64_t seed,
+                           int mra) {
     const int blocks = (int)((n + kBlock - 1) / kBlock);
-    const size_t lds = (size_t)kBlock * (c->bvh.depth + 2) * sizeof(int);
-    hipLaunchKernelGGL(kernel, dim3(blocks), dim3(kBlock), lds, c->stream, c->view, ev, g, plist,
-                       n, seed, tol, mra, c->loc.as<uint8_t>(), c->starts.as<StartInfo>(),
+    const size_t lds = (size_t)kBlock * (c->L.depth + 2) * sizeof(int);
+    hipLaunchKernelGGL(kernel, dim3(blocks), dim3(kBlock), lds, c->stream, c->view, ev, g, dv,
+                       plist, n, seed, mra, c->loc.as<uint8_t>(), c->starts.as<StartInfo>(),
                        c->amb.as<int32_t>(), c->bvalue.as<double>());
     return hipGetLastError();
 }
 
-template <int DIM, bool SRC, bool SCR>
+template <int DIM, bool SRC, bool SCR, bool ADJ>
 hipError_t launch_walk(wos_ctx *c, const EqView &ev, int precision, const RoundView &rv,
-                       double eps, int max_steps, int *blocks_out) {
+                       double eps, int max_steps, const AdjOut &adj, int *blocks_out) {
     if (precision == WOS_FP64_PLAIN) {
         const int blocks = (int)((rv.total + kBlock - 1) / kBlock);
-        const size_t lds = (size_t)kBlock * (c->bvh.depth + 2) * sizeof(int);
-        hipLaunchKernelGGL((k_walk_fp64<DIM, SRC, SCR>), dim3(blocks), dim3(kBlock), lds,
+        const size_t lds = (size_t)kBlock * (c->L.depth + 2) * sizeof(int);
+        hipLaunchKernelGGL((k_walk_fp64<DIM, SRC, SCR, ADJ>), dim3(blocks), dim3(kBlock), lds,
                            c->stream, c->view, ev, rv, eps, max_steps, c->vals.as<double>(),
-                           c->steps.as<int32_t>());
+                           c->steps.as<int32_t>(), adj);
         *blocks_out = blocks;
         return hipGetLastError();
     }
     constexpr int SMODE = SRC ? 1 : 0;
-    const void *kfn = ev.source_mode == WOS_SOURCE_GREEN && SRC
-                          ? (const void *)k_walk_fp32<DIM, 2, SCR>
-                          : (const void *)k_walk_fp32<DIM, SMODE, SCR>;
+    const bool green = ev.source_mode == WOS_SOURCE_GREEN && SRC;
+    const void *kfn = green ? (const void *)k_walk_fp32<DIM, 2, SCR, ADJ>
+                            : (const void *)k_walk_fp32<DIM, SMODE, SCR, ADJ>;
     const int threads = c->tune.block_threads;
-    const size_t stack_bytes = (size_t)threads * (c->bvh.depth + 1) * sizeof(StackEntry);
+    const size_t stack_bytes = (size_t)threads * (c->L.depth + 1) * sizeof(StackEntry);
     const int node_bytes = DIM == 3 ? (int)sizeof(NodeF3) : (int)sizeof(NodeF2);
     const int n_internal = DIM == 3 ? (int)c->L.nf3.size() : (int)c->L.nf2.size();
     // occupancy allowed by registers alone, then the LDS each block may use
@@ -719,25 +971,36 @@ hipError_t launch_walk(wos_ctx *c, const EqView &ev, int precision, const RoundV
     a.steps_out = c->steps.as<int32_t>();
     e = hipMemsetAsync(c->counter.p, 0, sizeof(unsigned long long), c->stream);
     if (e != hipSuccess) return e;
-    if (ev.source_mode == WOS_SOURCE_GREEN && SRC)
-        hipLaunchKernelGGL((k_walk_fp32<DIM, 2, SCR>), dim3(blocks), dim3(threads), lds,
-                           c->stream, c->view, ev, a);
+    if (green)
+        hipLaunchKernelGGL((k_walk_fp32<DIM, 2, SCR, ADJ>), dim3(blocks), dim3(threads), lds,
+                           c->stream, c->view, ev, a, adj);
     else
-        hipLaunchKernelGGL((k_walk_fp32<DIM, SMODE, SCR>), dim3(blocks), dim3(threads), lds,
-                           c->stream, c->view, ev, a);
+        hipLaunchKernelGGL((k_walk_fp32<DIM, SMODE, SCR, ADJ>), dim3(blocks), dim3(threads), lds,
+                           c->stream, c->view, ev, a, adj);
     *blocks_out = blocks;
     c->last_threads = threads;
     c->last_cached = n_cached;
     return hipGetLastError();
 }
 
-template <int DIM>
-hipError_t dispatch_walk(wos_ctx *c, const EqView &ev, bool src, bool scr, int precision,
-                         const RoundView &rv, double eps, int max_steps, int *blocks) {
-    if (src && scr) return launch_walk<DIM, true, true>(c, ev, precision, rv, eps, max_steps, blocks);
-    if (src) return launch_walk<DIM, true, false>(c, ev, precision, rv, eps, max_steps, blocks);
-    if (scr) return launch_walk<DIM, false, true>(c, ev, precision, rv, eps, max_steps, blocks);
-    return launch_walk<DIM, false, false>(c, ev, precision, rv, eps, max_steps, blocks);
+template <int DIM, bool ADJ>
+hipError_t dispatch_walk_t(wos_ctx *c, const EqView &ev, bool src, bool scr, int precision,
+                           const RoundView &rv, double eps, int max_steps, const AdjOut &adj,
+                           int *blocks) {
+    if (src && scr) return launch_walk<DIM, true, true, ADJ>(c, ev, precision, rv, eps, max_steps, adj, blocks);
+    if (src) return launch_walk<DIM, true, false, ADJ>(c, ev, precision, rv, eps, max_steps, adj, blocks);
+    if (scr) return launch_walk<DIM, false, true, ADJ>(c, ev, precision, rv, eps, max_steps, adj, blocks);
+    return launch_walk<DIM, false, false, ADJ>(c, ev, precision, rv, eps, max_steps, adj, blocks);
+}
+
+hipError_t dispatch_walk(wos_ctx *c, int dim, bool adjoint, const EqView &ev, bool src, bool scr,
+                         int precision, const RoundView &rv, double eps, int max_steps,
+                         const AdjOut &adj, int *blocks) {
+    if (dim == 2)
+        return adjoint ? dispatch_walk_t<2, true>(c, ev, src, scr, precision, rv, eps, max_steps, adj, blocks)
+                       : dispatch_walk_t<2, false>(c, ev, src, scr, precision, rv, eps, max_steps, adj, blocks);
+    return adjoint ? dispatch_walk_t<3, true>(c, ev, src, scr, precision, rv, eps, max_steps, adj, blocks)
+                   : dispatch_walk_t<3, false>(c, ev, src, scr, precision, rv, eps, max_steps, adj, blocks);
 }
 
 int validate_eq(const wos_ctx *c, const wos_eq_desc *eq) {
@@ -766,13 +1029,33 @@ double elapsed(hipEvent_t a, hipEvent_t b) {
     return ms * 1e-3;
 }
 
-// shared body of wos_solve_grid / wos_solve_points
-int solve(wos_ctx *c, const GridSpec *grid, const double *points, int64_t n,
-          const wos_eq_desc *eq, const wos_run_params *prm, wos_point_stats *out,
-          wos_run_summary *summary) {
+// adjoint request: dL/du per point in, gradients out (any output may be NULL)
+struct AdjRequest {
+    const double *dl_du;
+    double *grad_vertex;
+    double *grad_params;
+};
+
+// shared body of wos_solve_grid / wos_solve_points / the adjoint entry points
+// one slice of a solve: a grid over a domain's bbox or a point list in it
+struct Part {
+    int domain;
+    const GridSpec *grid;
+    const double *points;
+    int64_t count;
+};
+
+int solve(wos_ctx *c, const std::vector<Part> &parts, const wos_eq_desc *eq,
+          const wos_run_params *prm, wos_point_stats *out, wos_run_summary *summary,
+          const AdjRequest *adjr = nullptr, wos_run_summary *part_summaries = nullptr) {
+    int64_t n = 0;
+    for (const Part &p : parts) n += p.count;
     if (!c) return fail(WOS_ERR_INPUT, "context is NULL");
     if (!c->has_mesh) return fail(WOS_ERR_STATE, "no mesh set on this context");
     if (!prm || !out || !out->mean) return fail(WOS_ERR_INPUT, "params and out->mean are required");
+    if (adjr && !adjr->dl_du) return fail(WOS_ERR_INPUT, "dl_du is required");
+    if (adjr && prm->target_se > 0.0)
+        return fail(WOS_ERR_INPUT, "the adjoint replays a fixed number of walks (target_se = 0)");
     int rc = validate_eq(c, eq);
     if (rc) return rc;
     if (prm->n_walks < 1) return fail(WOS_ERR_INPUT, "n_walks must be >= 1");
@@ -805,47 +1088,46 @@ int solve(wos_ctx *c, const GridSpec *grid, const double *points, int64_t n,
         ev.vertex_values = c->vv.as<double>();
     }
 
-    // bounding box and tolerance (runner.hpp:82-88)
-    double gb[6];
-    {
-        const auto &V = c->mesh.verts;
-        double lo[3] = {V[0], V[1], dim == 3 ? V[2] : 0.0};
-        double hi[3] = {lo[0], lo[1], lo[2]};
-        for (int64_t i = 1; i < c->mesh.n_verts(); ++i)
-            for (int k = 0; k < dim; ++k) {
-                const double v = V[(size_t)i * dim + k];
-                if (v < lo[k]) lo[k] = v;
-                else if (v > hi[k]) hi[k] = v;
-            }
-        gb[0] = lo[0]; gb[1] = hi[0]; gb[2] = lo[1]; gb[3] = hi[1]; gb[4] = lo[2]; gb[5] = hi[2];
-    }
-    const double scale = std::max({1.0, gb[1] - gb[0], gb[3] - gb[2], gb[5] - gb[4]});
-    const double tol = 1e-10 * scale;
-
-    GridSpec g{};
-    if (grid) {
-        g = *grid;
-        for (int k = 0; k < 6; ++k) g.b[k] = gb[k];
-    }
-
     HIPX(c->loc.reserve(n));
     HIPX(c->starts.reserve(n * sizeof(StartInfo)));
     HIPX(c->amb.reserve(n * sizeof(int32_t)));
     HIPX(c->bvalue.reserve(n * sizeof(double)));
-    const double *dpts = nullptr;
-    if (points) {
-        HIPX(c->plist_in.reserve(n * dim * sizeof(double)));
-        HIPX(hipMemcpyAsync(c->plist_in.p, points, n * dim * sizeof(double), hipMemcpyHostToDevice,
-                            c->stream));
-        dpts = c->plist_in.as<double>();
+    {
+        std::vector<double> pts;
+        for (const Part &p : parts)
+            if (p.points) pts.insert(pts.end(), p.points, p.points + p.count * dim);
+        if (!pts.empty()) HIPX(upload(c->plist_in, pts, c->stream));
     }
 
+    // per part: the domain's bbox grid and tolerance (runner.hpp:82-88), then
+    // classification of its points into the part's slice of the outputs
     HIPX(hipEventRecord(c->ev[0], c->stream));
-    hipError_t le = dim == 2 ? launch_classify(c, k_classify<2>, ev, g, dpts, n, prm->seed, tol,
-                                               prm->max_ray_attempts)
-                             : launch_classify(c, k_classify<3>, ev, g, dpts, n, prm->seed, tol,
-                                               prm->max_ray_attempts);
-    HIPX(le);
+    std::vector<double> part_tol(parts.size());
+    {
+        int64_t off = 0, poff = 0;
+        for (size_t pi = 0; pi < parts.size(); ++pi) {
+            const Part &p = parts[pi];
+            const DomainInfo &D = c->L.domains[p.domain];
+            const double *gb = D.bounds;
+            const double scale = std::max({1.0, gb[1] - gb[0], gb[3] - gb[2], gb[5] - gb[4]});
+            part_tol[pi] = 1e-10 * scale;
+            GridSpec g{};
+            if (p.grid) {
+                g = *p.grid;
+                for (int k = 0; k < 6; ++k) g.b[k] = gb[k];
+            }
+            DomainView dv{D.node_off, D.root_ref, (int)D.prim_off, (int)(D.prim_off + D.n_prims),
+                          part_tol[pi], off};
+            const double *dp = p.points ? c->plist_in.as<double>() + poff * dim : nullptr;
+            hipError_t le = dim == 2 ? launch_classify(c, k_classify<2>, ev, g, dv, dp, p.count,
+                                                       prm->seed, prm->max_ray_attempts)
+                                     : launch_classify(c, k_classify<3>, ev, g, dv, dp, p.count,
+                                                       prm->seed, prm->max_ray_attempts);
+            HIPX(le);
+            off += p.count;
+            if (p.points) poff += p.count;
+        }
+    }
     HIPX(hipEventRecord(c->ev[1], c->stream));
 
     std::vector<uint8_t> loc(n);
@@ -884,8 +1166,24 @@ int solve(wos_ctx *c, const GridSpec *grid, const double *points, int64_t n,
         const bool scr = eq->kind == WOS_EQ_SCREENED;
         const bool adaptive = prm->target_se > 0.0;
         const int64_t cap = adaptive ? std::max<int64_t>(prm->max_walks, prm->n_walks) : prm->n_walks;
-        // staging: one FP64 value + one int32 per walk, at most 4 GiB a round
-        const int64_t budget = std::max<int64_t>((int64_t)(4LL << 30) / 12, 1);
+        // staging: one FP64 value + one int32 per walk (plus the adjoint
+        // record: 9 doubles + one int32), at most 4 GiB a round
+        const int64_t walk_bytes = adjr ? 12 + 76 : 12;
+        const int64_t budget = std::max<int64_t>((int64_t)(4LL << 30) / walk_bytes, 1);
+        AdjOut adj{};
+        const bool want_v = adjr && adjr->grad_vertex && eq->boundary_kind == WOS_BND_VERTEX;
+        const bool want_b = adjr && adjr->grad_params && eq->boundary_kind == WOS_BND_ID_CONST;
+        if (adjr) {
+            std::vector<double> coef(n_in);
+            for (int k = 0; k < n_in; ++k) coef[k] = adjr->dl_du[inside[k]] / (double)prm->n_walks;
+            HIPX(upload(c->coef, coef, c->stream));
+            HIPX(c->scal_sum.reserve((size_t)n_in * 6 * sizeof(double)));
+            HIPX(hipMemsetAsync(c->scal_sum.p, 0, (size_t)n_in * 6 * sizeof(double), c->stream));
+            HIPX(c->grad_v.reserve(std::max<size_t>(1, c->mesh.n_verts()) * sizeof(double)));
+            HIPX(hipMemsetAsync(c->grad_v.p, 0, c->mesh.n_verts() * sizeof(double), c->stream));
+            HIPX(c->grad_b.reserve(WOS_MAX_BND_VALUES * sizeof(double)));
+            HIPX(hipMemsetAsync(c->grad_b.p, 0, WOS_MAX_BND_VALUES * sizeof(double), c->stream));
+        }
         std::vector<int64_t> done(n_in, 0), want(n_in, std::min<int64_t>(prm->n_walks, cap));
         std::vector<int32_t> pid, wbeg;
         std::vector<int64_t> off;
@@ -956,17 +1254,30 @@ int solve(wos_ctx *c, const GridSpec *grid, const double *points, int64_t n,
             }
             HIPX(c->vals.reserve((size_t)rv.total * sizeof(double)));
             HIPX(c->steps.reserve((size_t)rv.total * sizeof(int32_t)));
+            if (adjr) {
+                HIPX(c->adj_scal.reserve((size_t)rv.total * 6 * sizeof(double)));
+                HIPX(c->adj_vw.reserve((size_t)rv.total * 3 * sizeof(double)));
+                HIPX(c->adj_prim.reserve((size_t)rv.total * sizeof(int32_t)));
+                adj.scal = c->adj_scal.as<double>();
+                adj.vw = c->adj_vw.as<double>();
+                adj.prim = c->adj_prim.as<int32_t>();
+            }
             HIPX(hipEventRecord(c->ev[2], c->stream));
-            hipError_t we = dim == 2
-                ? dispatch_walk<2>(c, ev, src, scr, prm->precision, rv, prm->epsilon, prm->max_steps, &blocks)
-                : dispatch_walk<3>(c, ev, src, scr, prm->precision, rv, prm->epsilon, prm->max_steps, &blocks);
-            HIPX(we);
+            HIPX(dispatch_walk(c, dim, adjr != nullptr, ev, src, scr, prm->precision, rv,
+                               prm->epsilon, prm->max_steps, adj, &blocks));
             HIPX(hipEventRecord(c->ev[3], c->stream));
             hipLaunchKernelGGL(k_welford, dim3((rv.n + kBlock - 1) / kBlock), dim3(kBlock), 0,
                                c->stream, rv, c->vals.as<double>(), c->steps.as<int32_t>(),
                                c->mean.as<double>(), c->m2.as<double>(), c->count.as<double>(),
                                c->ssum.as<double>(), c->hits.as<int32_t>());
             HIPX(hipGetLastError());
+            if (adjr) {
+                hipLaunchKernelGGL(k_adj_reduce, dim3((rv.n + kBlock - 1) / kBlock), dim3(kBlock), 0,
+                                   c->stream, rv, adj, c->view, c->coef.as<double>(),
+                                   c->scal_sum.as<double>(), want_v ? c->grad_v.as<double>() : nullptr,
+                                   want_b ? c->grad_b.as<double>() : nullptr);
+                HIPX(hipGetLastError());
+            }
             HIPX(hipEventRecord(c->ev[4], c->stream));
             HIPX(hipEventSynchronize(c->ev[4]));
             walk_s += elapsed(c->ev[2], c->ev[3]);
@@ -980,14 +1291,79 @@ int solve(wos_ctx *c, const GridSpec *grid, const double *points, int64_t n,
         HIPX(hipMemcpyAsync(h_hits.data(), c->hits.p, n_in * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
         HIPX(hipStreamSynchronize(c->stream));
     }
+    if (adjr) {
+        // boundary points (u = g at the start's nearest point), then the
+        // parameter gradients summed over points in sweep order
+        const bool want_v = adjr->grad_vertex && eq->boundary_kind == WOS_BND_VERTEX;
+        const bool want_b = adjr->grad_params && eq->boundary_kind == WOS_BND_ID_CONST;
+        if (n_in == 0) {
+            HIPX(c->grad_v.reserve(std::max<size_t>(1, c->mesh.n_verts()) * sizeof(double)));
+            HIPX(hipMemsetAsync(c->grad_v.p, 0, c->mesh.n_verts() * sizeof(double), c->stream));
+            HIPX(c->grad_b.reserve(WOS_MAX_BND_VALUES * sizeof(double)));
+            HIPX(hipMemsetAsync(c->grad_b.p, 0, WOS_MAX_BND_VALUES * sizeof(double), c->stream));
+        }
+        std::vector<double> dl(adjr->dl_du, adjr->dl_du + n);
+        HIPX(upload(c->dl_du, dl, c->stream));
+        hipLaunchKernelGGL(k_adj_boundary, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                           c->stream, c->view, c->loc.as<uint8_t>(), c->starts.as<StartInfo>(),
+                           c->dl_du.as<double>(), n, want_v ? c->grad_v.as<double>() : nullptr,
+                           want_b ? c->grad_b.as<double>() : nullptr);
+        HIPX(hipGetLastError());
+        std::vector<double> scal((size_t)n_in * 6), gb(WOS_MAX_BND_VALUES, 0.0);
+        if (n_in > 0)
+            HIPX(hipMemcpyAsync(scal.data(), c->scal_sum.p, scal.size() * sizeof(double),
+                                hipMemcpyDeviceToHost, c->stream));
+        HIPX(hipMemcpyAsync(gb.data(), c->grad_b.p, WOS_MAX_BND_VALUES * sizeof(double),
+                            hipMemcpyDeviceToHost, c->stream));
+        if (adjr->grad_vertex) {
+            if (want_v)
+                HIPX(hipMemcpyAsync(adjr->grad_vertex, c->grad_v.p, c->mesh.n_verts() * sizeof(double),
+                                    hipMemcpyDeviceToHost, c->stream));
+            else
+                std::fill(adjr->grad_vertex, adjr->grad_vertex + c->mesh.n_verts(), 0.0);
+        }
+        HIPX(hipStreamSynchronize(c->stream));
+        if (adjr->grad_params) {
+            double g6[6] = {0, 0, 0, 0, 0, 0};
+            for (int k = 0; k < n_in; ++k) {
+                const double cf = adjr->dl_du[inside[k]] / (double)prm->n_walks;
+                for (int i = 0; i < 6; ++i) g6[i] += cf * scal[(size_t)k * 6 + i];
+            }
+            double *gp = adjr->grad_params;
+            // alpha: pathwise only where the walk's draws do not depend on it
+            const bool alpha_ok = eq->kind == WOS_EQ_SCREENED && dim == 3 &&
+                                  (eq->source_kind == WOS_SRC_NONE || eq->source_mode == WOS_SOURCE_UNIFORM);
+            gp[WOS_GRAD_ALPHA] = eq->kind != WOS_EQ_SCREENED ? 0.0 : (alpha_ok ? g6[0] : NAN);
+            gp[WOS_GRAD_SRC_VALUE] = g6[1];
+            gp[WOS_GRAD_SRC_RADIUS] = eq->source_kind == WOS_SRC_GAUSSIAN ? g6[2] : 0.0;
+            for (int i = 0; i < 3; ++i)
+                gp[WOS_GRAD_SRC_CENTER + i] = eq->source_kind == WOS_SRC_GAUSSIAN ? g6[3 + i] : 0.0;
+            for (int i = 0; i < WOS_MAX_BND_VALUES; ++i)
+                gp[WOS_GRAD_BND_VALUES + i] = want_b ? gb[i] : 0.0;
+        }
+    }
     const double classify_s = elapsed(c->ev[0], c->ev[1]);
 
     // per-point result policy (runner.hpp:195-235) and the summary sums in
-    // sweep order (runner.hpp:224-228)
+    // sweep order (runner.hpp:224-228), for the whole solve and per part
     wos_run_summary S{};
     int ii = 0;
-    int64_t total_steps = 0;
+    size_t pi = 0;
+    int64_t part_end = parts.empty() ? 0 : parts[0].count;
+    wos_run_summary P{};
+    auto close_part = [&](size_t k) {
+        const DomainInfo &D = c->L.domains[parts[k].domain];
+        for (int j = 0; j < 6; ++j) P.grid_bounds[j] = D.bounds[j];
+        P.boundary_tolerance = part_tol[k];
+        if (part_summaries) part_summaries[k] = P;
+        P = wos_run_summary{};
+    };
     for (int64_t i = 0; i < n; ++i) {
+        while (i >= part_end && pi + 1 < parts.size()) {
+            close_part(pi);
+            ++pi;
+            part_end += parts[pi].count;
+        }
         double mu = NAN, var = NAN, se = NAN, ms = NAN;
         int32_t hits = 0, walks = 0;
         if (loc[i] == 2) {
@@ -1000,16 +1376,20 @@ int solve(wos_ctx *c, const GridSpec *grid, const double *points, int64_t n,
             ms = N > 0 ? h_ss[ii] / N : 0.0;
             hits = h_hits[ii];
             walks = (int32_t)N;
-            total_steps += (int64_t)h_ss[ii];
             ++ii;
-            S.inside_count++;
-            S.max_steps_hits += hits;
-            S.sum_variance += var;
-            S.sum_standard_error += se;
-            S.sum_mean_steps += ms;
+            for (wos_run_summary *T : {&S, &P}) {
+                T->inside_count++;
+                T->max_steps_hits += hits;
+                T->sum_variance += var;
+                T->sum_standard_error += se;
+                T->sum_mean_steps += ms;
+                T->total_walks += walks;
+                T->total_steps += (int64_t)h_ss[ii - 1];
+            }
         }
-        if (loc[i] == 3) S.indeterminate_points++;
+        if (loc[i] == 3) { S.indeterminate_points++; P.indeterminate_points++; }
         S.ambiguous_ray_retries += amb[i];
+        P.ambiguous_ray_retries += amb[i];
         out->mean[i] = mu;
         if (out->variance) out->variance[i] = var;
         if (out->standard_error) out->standard_error[i] = se;
@@ -1019,14 +1399,13 @@ int solve(wos_ctx *c, const GridSpec *grid, const double *points, int64_t n,
         if (out->walks) out->walks[i] = walks;
         if (out->ambiguous_ray_retries) out->ambiguous_ray_retries[i] = amb[i];
     }
+    if (!parts.empty()) close_part(pi);
     S.wos_seconds = classify_s + total_walk_s;
     S.walk_kernel_seconds = walk_s;
-    S.total_walks = 0;
-    for (int k = 0; k < n_in; ++k) S.total_walks += (int64_t)h_cnt[k];
     S.rounds = c->last_rounds;
-    S.total_steps = total_steps;
-    for (int k = 0; k < 6; ++k) S.grid_bounds[k] = gb[k];
-    S.boundary_tolerance = tol;
+    const DomainInfo &D0 = c->L.domains[parts[0].domain];
+    for (int k = 0; k < 6; ++k) S.grid_bounds[k] = D0.bounds[k];
+    S.boundary_tolerance = part_tol[0];
     S.launch_blocks = blocks;
     S.launch_threads = prm->precision == WOS_FP32_PERSISTENT ? c->last_threads : kBlock;
     S.lds_cached_nodes = prm->precision == WOS_FP32_PERSISTENT ? c->last_cached : 0;
@@ -1084,27 +1463,48 @@ void wos_ctx_destroy(wos_ctx *c) {
     delete c;
 }
 
-int wos_ctx_set_mesh(wos_ctx *c, int dim, const double *verts, int64_t n_verts,
-                     const int32_t *prims, int64_t n_prims, const int32_t *boundary_ids) {
+int wos_ctx_set_domains(wos_ctx *c, int dim, int n_domains, const double *const *verts,
+                        const int64_t *n_verts, const int32_t *const *prims,
+                        const int64_t *n_prims, const int32_t *const *boundary_ids) {
     if (!c) return fail(WOS_ERR_INPUT, "context is NULL");
     if (dim != 2 && dim != 3) return fail(WOS_ERR_INPUT, "dim must be 2 or 3");
-    if (!verts || !prims || n_verts <= 0 || n_prims <= 0)
-        return fail(WOS_ERR_MESH, "mesh has no vertices or primitives");
-    HostMesh m;
-    m.dim = dim;
-    m.verts.assign(verts, verts + n_verts * dim);
-    m.prims.assign(prims, prims + n_prims * dim);
-    m.bids.assign(n_prims, 0);
-    if (boundary_ids && dim == 2) m.bids.assign(boundary_ids, boundary_ids + n_prims);
-    for (double v : m.verts)
-        if (!std::isfinite(v)) return fail(WOS_ERR_MESH, "vertex coordinates must be finite");
-    for (int32_t i : m.prims)
-        if (i < 0 || i >= n_verts) return fail(WOS_ERR_MESH, "vertex index out of range");
-    std::string err;
-    HostBVH b = build_reference_bvh(m, &err);
-    if (b.nodes.empty()) return fail(WOS_ERR_MESH, err);
-    if (b.depth > 60) return fail(WOS_ERR_MESH, "BVH deeper than the 60-level traversal stack");
-    DeviceLayouts L = make_layouts(m, b);
+    if (n_domains < 1 || !verts || !n_verts || !prims || !n_prims)
+        return fail(WOS_ERR_INPUT, "need at least one domain");
+    std::vector<HostMesh> meshes(n_domains);
+    std::vector<HostBVH> bvhs(n_domains);
+    HostMesh all;
+    all.dim = dim;
+    int64_t vo = 0, po = 0;
+    for (int d = 0; d < n_domains; ++d) {
+        if (!verts[d] || !prims[d] || n_verts[d] <= 0 || n_prims[d] <= 0)
+            return fail(WOS_ERR_MESH, "mesh has no vertices or primitives");
+        HostMesh &m = meshes[d];
+        m.dim = dim;
+        m.verts.assign(verts[d], verts[d] + n_verts[d] * dim);
+        m.prims.assign(prims[d], prims[d] + n_prims[d] * dim);
+        m.bids.assign(n_prims[d], 0);
+        if (boundary_ids && boundary_ids[d] && dim == 2)
+            m.bids.assign(boundary_ids[d], boundary_ids[d] + n_prims[d]);
+        for (double v : m.verts)
+            if (!std::isfinite(v)) return fail(WOS_ERR_MESH, "vertex coordinates must be finite");
+        for (int32_t i : m.prims)
+            if (i < 0 || i >= n_verts[d]) return fail(WOS_ERR_MESH, "vertex index out of range");
+        std::string err;
+        bvhs[d] = build_reference_bvh(m, &err);
+        if (bvhs[d].nodes.empty()) return fail(WOS_ERR_MESH, err);
+        if (bvhs[d].depth > 60) return fail(WOS_ERR_MESH, "BVH deeper than the 60-level traversal stack");
+        // the concatenated mesh: global vertex and primitive numbering
+        all.verts.insert(all.verts.end(), m.verts.begin(), m.verts.end());
+        for (int32_t i : m.prims) all.prims.push_back((int32_t)(i + vo));
+        all.bids.insert(all.bids.end(), m.bids.begin(), m.bids.end());
+        vo += n_verts[d];
+        po += n_prims[d];
+    }
+    if (po >= (1 << 27)) return fail(WOS_ERR_MESH, "more than 2^27 primitives in total");
+    std::vector<const HostMesh *> mp;
+    std::vector<const HostBVH *> bp;
+    for (int d = 0; d < n_domains; ++d) { mp.push_back(&meshes[d]); bp.push_back(&bvhs[d]); }
+    DeviceLayouts L = make_layouts(mp, bp);
     HIPX(hipSetDevice(c->device));
     hipStream_t s = c->stream;
     if (dim == 2) {
@@ -1119,30 +1519,36 @@ int wos_ctx_set_mesh(wos_ctx *c, int dim, const double *verts, int64_t n_verts,
         HIPX(upload(c->nf3, L.nf3, s));
         HIPX(upload(c->trif, L.trif, s));
     }
-    HIPX(upload(c->slot_prim, b.slots, s));
-    HIPX(upload(c->prim_bid, m.bids, s));
-    HIPX(upload(c->prims, m.prims, s));
-    HIPX(upload(c->verts, m.verts, s));
+    HIPX(upload(c->slot_prim, L.slot_prim, s));
+    HIPX(upload(c->prim_bid, all.bids, s));
+    HIPX(upload(c->prims, all.prims, s));
+    HIPX(upload(c->verts, all.verts, s));
     MeshView v{};
     v.dim = dim;
-    v.n_prims = (int)n_prims;
-    v.n_nodes = (int)b.nodes.size();
-    v.depth = b.depth;
+    v.n_prims = (int)po;
+    v.n_nodes = dim == 2 ? (int)L.nd2.size() : (int)L.nd3.size();
+    v.depth = L.depth;
     v.nd2 = c->nd2.as<NodeD2>(); v.nd3 = c->nd3.as<NodeD3>();
     v.segd = c->segd.as<SegD>(); v.trid = c->trid.as<TriD>();
     v.slot_prim = c->slot_prim.as<int32_t>(); v.prim_bid = c->prim_bid.as<int32_t>();
     v.prims = c->prims.as<int32_t>(); v.verts = c->verts.as<double>();
     v.nf2 = c->nf2.as<NodeF2>(); v.nf3 = c->nf3.as<NodeF3>();
     v.segf = c->segf.as<SegF>(); v.seg_inv_vv = c->seg_inv.as<float>(); v.trif = c->trif.as<TriF>();
-    v.root_ref = L.root_ref;
     // conservative FP32 radius margin: 32 ulp of the largest coordinate
     v.shrink_abs = (float)(32.0 * std::ldexp(1.0, -24) * L.coord_scale);
     c->view = v;
-    c->mesh = std::move(m);
-    c->bvh = std::move(b);
+    c->mesh = std::move(all);
     c->L = std::move(L);
     c->has_mesh = true;
     return WOS_OK;
+}
+
+int wos_ctx_set_mesh(wos_ctx *c, int dim, const double *verts, int64_t n_verts,
+                     const int32_t *prims, int64_t n_prims, const int32_t *boundary_ids) {
+    if (!c) return fail(WOS_ERR_INPUT, "context is NULL");
+    if (!verts || !prims || n_verts <= 0 || n_prims <= 0)
+        return fail(WOS_ERR_MESH, "mesh has no vertices or primitives");
+    return wos_ctx_set_domains(c, dim, 1, &verts, &n_verts, &prims, &n_prims, &boundary_ids);
 }
 
 int64_t wos_bvh_build_host(int dim, const double *verts, int64_t n_verts, const int32_t *prims,
@@ -1203,14 +1609,48 @@ int wos_solve_grid(wos_ctx *c, int nx, int ny, int nz, const wos_eq_desc *eq,
     if (c->mesh.dim == 2) nz = 1;  // runner.hpp:83
     GridSpec g{};
     g.nx = nx; g.ny = ny; g.nz = nz;
-    return solve(c, &g, nullptr, (int64_t)nx * ny * nz, eq, params, out, summary);
+    return solve(c, {Part{0, &g, nullptr, (int64_t)nx * ny * nz}}, eq, params, out, summary);
+}
+
+int wos_solve_grid_batch(wos_ctx *c, int nx, int ny, int nz, const wos_eq_desc *eq,
+                         const wos_run_params *params, wos_point_stats *out,
+                         wos_run_summary *domain_summaries, wos_run_summary *summary) {
+    if (!c || !c->has_mesh) return fail(c ? WOS_ERR_STATE : WOS_ERR_INPUT, "no mesh set on this context");
+    if (nx < 1 || ny < 1 || nz < 1) return fail(WOS_ERR_INPUT, "grid sizes must be >= 1");
+    if (c->mesh.dim == 2) nz = 1;
+    GridSpec g{};
+    g.nx = nx; g.ny = ny; g.nz = nz;
+    std::vector<Part> parts;
+    for (int d = 0; d < (int)c->L.domains.size(); ++d)
+        parts.push_back(Part{d, &g, nullptr, (int64_t)nx * ny * nz});
+    return solve(c, parts, eq, params, out, summary, nullptr, domain_summaries);
 }
 
 int wos_solve_points(wos_ctx *c, const double *points, int64_t n, const wos_eq_desc *eq,
                      const wos_run_params *params, wos_point_stats *out,
                      wos_run_summary *summary) {
     if (!points) return fail(WOS_ERR_INPUT, "points is NULL");
-    return solve(c, nullptr, points, n, eq, params, out, summary);
+    return solve(c, {Part{0, nullptr, points, n}}, eq, params, out, summary);
+}
+
+int wos_adjoint_points(wos_ctx *c, const double *points, int64_t n, const wos_eq_desc *eq,
+                       const wos_run_params *params, const double *dl_du, double *grad_vertex,
+                       double *grad_params, wos_point_stats *out, wos_run_summary *summary) {
+    if (!points) return fail(WOS_ERR_INPUT, "points is NULL");
+    AdjRequest r{dl_du, grad_vertex, grad_params};
+    return solve(c, {Part{0, nullptr, points, n}}, eq, params, out, summary, &r);
+}
+
+int wos_adjoint_grid(wos_ctx *c, int nx, int ny, int nz, const wos_eq_desc *eq,
+                     const wos_run_params *params, const double *dl_du, double *grad_vertex,
+                     double *grad_params, wos_point_stats *out, wos_run_summary *summary) {
+    if (!c || !c->has_mesh) return fail(c ? WOS_ERR_STATE : WOS_ERR_INPUT, "no mesh set on this context");
+    if (nx < 1 || ny < 1 || nz < 1) return fail(WOS_ERR_INPUT, "grid sizes must be >= 1");
+    if (c->mesh.dim == 2) nz = 1;
+    GridSpec g{};
+    g.nx = nx; g.ny = ny; g.nz = nz;
+    AdjRequest r{dl_du, grad_vertex, grad_params};
+    return solve(c, {Part{0, &g, nullptr, (int64_t)nx * ny * nz}}, eq, params, out, summary, &r);
 }
 
 }  // extern "C"

@@ -31,6 +31,10 @@ WOS_FP64_PLAIN, WOS_FP32_PERSISTENT = range(2)
 WOS_MAX_BND_VALUES = 16
 
 LOC_OUTSIDE, LOC_INSIDE, LOC_BOUNDARY, LOC_INDETERMINATE = range(4)
+# enum wos_grad_index
+WOS_GRAD_ALPHA, WOS_GRAD_SRC_VALUE, WOS_GRAD_SRC_RADIUS, WOS_GRAD_SRC_CENTER = 0, 1, 2, 3
+WOS_GRAD_BND_VALUES = 6
+WOS_N_GRAD = 6 + WOS_MAX_BND_VALUES
 
 
 class EqDesc(C.Structure):
@@ -169,6 +173,24 @@ def load(path: str | None = None):
     lib.wos_solve_points.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.c_int64,
                                      C.POINTER(EqDesc), C.POINTER(RunParams),
                                      C.POINTER(PointStats), C.POINTER(RunSummary)]
+    for name in ("wos_adjoint_points",):
+        getattr(lib, name).argtypes = [C.c_void_p, C.POINTER(C.c_double), C.c_int64,
+                                       C.POINTER(EqDesc), C.POINTER(RunParams),
+                                       C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                       C.POINTER(C.c_double), C.POINTER(PointStats),
+                                       C.POINTER(RunSummary)]
+    lib.wos_adjoint_grid.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.POINTER(EqDesc),
+                                     C.POINTER(RunParams), C.POINTER(C.c_double),
+                                     C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                     C.POINTER(PointStats), C.POINTER(RunSummary)]
+    lib.wos_ctx_set_domains.argtypes = [C.c_void_p, C.c_int, C.c_int,
+                                        C.POINTER(C.POINTER(C.c_double)), C.POINTER(C.c_int64),
+                                        C.POINTER(C.POINTER(C.c_int32)), C.POINTER(C.c_int64),
+                                        C.POINTER(C.POINTER(C.c_int32))]
+    lib.wos_solve_grid_batch.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int,
+                                         C.POINTER(EqDesc), C.POINTER(RunParams),
+                                         C.POINTER(PointStats), C.POINTER(RunSummary),
+                                         C.POINTER(RunSummary)]
     lib.wos_bvh_build_host.restype = C.c_int64
     lib.wos_bvh_build_host.argtypes = [C.c_int, C.POINTER(C.c_double), C.c_int64,
                                        C.POINTER(C.c_int32), C.c_int64, C.POINTER(C.c_int32),

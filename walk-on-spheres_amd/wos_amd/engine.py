@@ -58,6 +58,45 @@ class Context:
         capi.check(self.lib, rc)
         self.dim = dim
         self.n_verts = len(v)
+        self.n_domains = 1
+
+    def set_domains(self, meshes):
+        """Several domains [(verts, prims, boundary_ids or None), ...] of one
+        dimension, solved together by solve_grid_batch."""
+        vs = [np.ascontiguousarray(m[0], np.float64) for m in meshes]
+        ps = [np.ascontiguousarray(m[1], np.int32) for m in meshes]
+        dim = ps[0].shape[1]
+        vs = [np.ascontiguousarray(v[:, :dim]) for v in vs]
+        bs = [None if len(m) < 3 or m[2] is None else np.ascontiguousarray(m[2], np.int32)
+              for m in meshes]
+        n = len(meshes)
+        vp = (C.POINTER(C.c_double) * n)(*[ptr(v, C.c_double) for v in vs])
+        pp = (C.POINTER(C.c_int32) * n)(*[ptr(p, C.c_int32) for p in ps])
+        bp = (C.POINTER(C.c_int32) * n)(*[ptr(b, C.c_int32) if b is not None else None for b in bs])
+        nv = (C.c_int64 * n)(*[len(v) for v in vs])
+        npr = (C.c_int64 * n)(*[len(p) for p in ps])
+        rc = self.lib.wos_ctx_set_domains(self.h, dim, n, vp, nv, pp, npr, bp)
+        capi.check(self.lib, rc)
+        self.dim = dim
+        self.n_verts = int(sum(len(v) for v in vs))
+        self.n_domains = n
+
+    def solve_grid_batch(self, nx, ny, nz, eq: EqDesc, prm: RunParams):
+        """Returns (list of per-domain result dicts, total summary)."""
+        if self.dim == 2:
+            nz = 1
+        m = nx * ny * nz
+        out, st, summ = self._outputs(m * self.n_domains)
+        per = (RunSummary * self.n_domains)()
+        rc = self.lib.wos_solve_grid_batch(self.h, nx, ny, nz, C.byref(eq), C.byref(prm),
+                                           C.byref(st), per, C.byref(summ))
+        capi.check(self.lib, rc)
+        res = []
+        for d in range(self.n_domains):
+            r = {k: v[d * m:(d + 1) * m] for k, v in out.items()}
+            r["summary"] = self._summary(per[d])
+            res.append(r)
+        return res, self._summary(summ)
 
     def set_tuning(self, block_threads=256, lds_nodes=-1, chunk=256, blocks_per_cu=0):
         t = capi.Tuning(block_threads, lds_nodes, chunk, blocks_per_cu)
@@ -82,6 +121,54 @@ class Context:
         p.max_walks = max_walks
         p.tune = tune
         return p
+
+    def adjoint_points(self, points, eq: EqDesc, prm: RunParams, dl_du):
+        """Replay the walks of solve_points and return (forward stats,
+        grad_vertex [n_verts], grad_params [WOS_N_GRAD])."""
+        pts = np.ascontiguousarray(points, np.float64)
+        return self._adjoint(self.lib.wos_adjoint_points, len(pts), eq, prm, dl_du,
+                             ptr(pts, C.c_double), len(pts))
+
+    def adjoint_grid(self, nx, ny, nz, eq: EqDesc, prm: RunParams, dl_du):
+        if self.dim == 2:
+            nz = 1
+        return self._adjoint(self.lib.wos_adjoint_grid, nx * ny * nz, eq, prm, dl_du, nx, ny, nz)
+
+    def _adjoint(self, fn, n, eq, prm, dl_du, *args):
+        dl = np.ascontiguousarray(dl_du, np.float64)
+        if dl.shape != (n,):
+            raise ValueError(f"dl_du must have shape ({n},)")
+        gv = np.zeros(self.n_verts)
+        gp = np.zeros(capi.WOS_N_GRAD)
+        out, st, summ = self._outputs(n)
+        rc = fn(self.h, *args, C.byref(eq), C.byref(prm), ptr(dl, C.c_double),
+                ptr(gv, C.c_double), ptr(gp, C.c_double), C.byref(st), C.byref(summ))
+        capi.check(self.lib, rc)
+        out["summary"] = self._summary(summ)
+        return out, gv, gp
+
+    @staticmethod
+    def _summary(summ):
+        s = {k: getattr(summ, k) for k, _ in RunSummary._fields_}
+        s["grid_bounds"] = list(summ.grid_bounds)
+        return s
+
+    @staticmethod
+    def _outputs(n):
+        out = dict(mean=np.zeros(n), variance=np.zeros(n), standard_error=np.zeros(n),
+                   mean_steps=np.zeros(n), location=np.zeros(n, np.uint8),
+                   max_steps_hits=np.zeros(n, np.int32), walks=np.zeros(n, np.int32),
+                   ambiguous_ray_retries=np.zeros(n, np.int32))
+        st = PointStats()
+        st.mean = ptr(out["mean"], C.c_double)
+        st.variance = ptr(out["variance"], C.c_double)
+        st.standard_error = ptr(out["standard_error"], C.c_double)
+        st.mean_steps = ptr(out["mean_steps"], C.c_double)
+        st.location = ptr(out["location"], C.c_uint8)
+        st.max_steps_hits = ptr(out["max_steps_hits"], C.c_int32)
+        st.walks = ptr(out["walks"], C.c_int32)
+        st.ambiguous_ray_retries = ptr(out["ambiguous_ray_retries"], C.c_int32)
+        return out, st, RunSummary()
 
     def _solve(self, fn, n, eq: EqDesc, prm: RunParams, *args):
         out = dict(mean=np.zeros(n), variance=np.zeros(n), standard_error=np.zeros(n),
