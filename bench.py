@@ -12,6 +12,11 @@ n_p = ceil(var_p / 1e-6) from the FP64 run (shared by every form) and
 t_walk,p = (T / total steps) * mean_steps_p of the form being measured.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+       python bench.py --fp32-only --steps 1 --warmup 0   (walker launches only,
+                                                          for counter runs)
+       python bench.py --copy-calibration 1024            (one 1024 MiB fp32
+                                                          read+write kernel, the
+                                                          scale for HBM counters)
 """
 import argparse
 import json
@@ -84,6 +89,20 @@ def cpu_baseline(V, F, eq, grid, inside_idx, var_ref, eps, seed, budget_s=12.0):
                        f"{walks} walks each, {dt:.1f} s on {cores} threads")
 
 
+def copy_calibration(mib):
+    """One elementwise fp32 kernel that reads and writes `mib` MiB each: the
+    known-size transfer the FETCH_SIZE / WRITE_SIZE counters are scaled by."""
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    torch.mul(src, 2.0, out=dst)
+    torch.cuda.synchronize()
+    assert float(dst[-1]) == 2.0
+    print(json.dumps({"copy_calibration_bytes_read": n * 4, "copy_calibration_bytes_written": n * 4}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -91,11 +110,17 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--walks", type=int, default=4096)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--fp32-only", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(the single-GPU kernel at full shape is the measured unit).", file=sys.stderr)
         sys.exit(2)
+
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
 
     from wos_amd import Context, meshgen, problems
     from wos_amd.capi import WOS_FP32_PERSISTENT, WOS_FP64_PLAIN
@@ -119,6 +144,12 @@ def main():
         walk_k.append(r32["summary"]["walk_kernel_seconds"])
     t32 = (time.perf_counter() - t0) / args.steps
     s32 = r32["summary"]
+    if args.fp32_only:
+        print(json.dumps({"fp32_ms_per_solve": t32 * 1e3, "walk_kernel_ms": np.mean(walk_k) * 1e3,
+                          "total_steps": s32["total_steps"], "total_walks": s32["total_walks"],
+                          "launch_blocks": s32["launch_blocks"],
+                          "launch_threads": s32["launch_threads"]}))
+        return
 
     t0 = time.perf_counter()
     r64 = ctx.solve_grid(*grid, eq, p64)

@@ -146,6 +146,15 @@ typedef struct wos_tuning {
                                     fit without lowering occupancy                  */
     int32_t chunk;               /* walks a wavefront claims per refill            */
     int32_t blocks_per_cu;       /* persistent blocks per CU; 0 = occupancy limit  */
+    int32_t min_ready;           /* lanes of a wavefront whose closest-point queries
+                                    have finished before they take their walk step
+                                    together (0..64; 64 = whole-wave lockstep)     */
+    float relax;                 /* relative slack r of step queries, 0 <= r < 0.5:
+                                    boxes farther than (1 - r) x the best distance
+                                    are skipped and the sphere radius is the
+                                    resulting lower bound (>= (1 - r) x distance);
+                                    0 = exact search.  Walks are ended and projected
+                                    with an exact search in every case.            */
 } wos_tuning;
 
 typedef struct wos_ctx wos_ctx;
@@ -197,6 +206,20 @@ int wos_solve_grid_batch(wos_ctx *ctx, int nx, int ny, int nz, const wos_eq_desc
 int64_t wos_bvh_build_host(int dim, const double *verts, int64_t n_verts,
                            const int32_t *prims, int64_t n_prims, int32_t *count,
                            int32_t *idx, double *lo, double *hi, int32_t *slots);
+
+/* Closest boundary point of n points (n x dim doubles) of domain `domain`
+ * (0 after wos_ctx_set_mesh): dist[i] = distance, nearest[i] (n x dim, may be
+ * NULL) = the nearest point, prim[i] (may be NULL) = its primitive.
+ * WOS_FP64_PLAIN runs the reference traversal and returns what bvh_npq
+ * returns (include/wos/bvh.hpp:38-39, src/bvh.cpp).  WOS_FP32_PERSISTENT runs
+ * the walker's FP32 query and projects onto the primitive it found in FP64;
+ * with relax > 0 (FP32 only) boxes farther than (1 - relax) x the best
+ * distance are skipped and lower[i] (may be NULL) receives the lower bound
+ * of the distance that the walker would use as its sphere radius
+ * (lower[i] = dist[i] for an exact search). */
+int wos_closest_points(wos_ctx *ctx, int domain, const double *points, int64_t n,
+                       int precision, double relax, double *dist, double *lower,
+                       double *nearest, int32_t *prim);
 
 /* Launch shape of the FP32 walker.  No reference counterpart (the reference
  * has one CPU code path); used by the autotuner. */

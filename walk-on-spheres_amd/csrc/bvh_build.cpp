@@ -147,7 +147,8 @@ DeviceLayouts make_layouts(const std::vector<const HostMesh *> &meshes,
                 else if (v > hi[k]) hi[k] = v;
             }
         for (int k = 0; k < 3; ++k) { D.bounds[2 * k] = lo[k]; D.bounds[2 * k + 1] = hi[k]; }
-        for (double v : V) L.coord_scale = std::max(L.coord_scale, std::fabs(v));
+        for (double v : V) D.coord_scale = std::max(D.coord_scale, std::fabs(v));
+        L.coord_scale = std::max(L.coord_scale, D.coord_scale);
     }
     const size_t ns = (size_t)po;
     L.slot_prim.resize(ns);
@@ -198,8 +199,10 @@ DeviceLayouts make_layouts(const std::vector<const HostMesh *> &meshes,
                 f.e1x = (float)t.e1x; f.e1y = (float)t.e1y; f.e1z = (float)t.e1z;
                 f.e2x = (float)t.e2x; f.e2y = (float)t.e2y; f.e2z = (float)t.e2z;
                 f.a = (float)t.a; f.b = (float)t.b; f.c = (float)t.c;
-                f.det = (float)t.det; f.denom = (float)t.denom;
+                f.inv_a = t.a != 0.0 ? (float)(1.0 / t.a) : 0.0f;
+                f.inv_c = t.c != 0.0 ? (float)(1.0 / t.c) : 0.0f;
                 f.inv_det = t.det != 0.0 ? (float)(1.0 / t.det) : 0.0f;
+                f.inv_denom = t.denom != 0.0 ? (float)(1.0 / t.denom) : 0.0f;
                 L.trif[gs] = f;
             }
         }

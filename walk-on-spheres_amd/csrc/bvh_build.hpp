@@ -67,12 +67,13 @@ struct alignas(16) NodeF2 {
     float r_lox, r_loy, r_hix, r_hiy;
     int32_t left, right, pad0, pad1;
 };
-// FP32 leaf slots
+// FP32 leaf slots.  a = |e1|^2, b = e1.e2, c = |e2|^2, det = ac - b^2,
+// denom = a - 2b + c; reciprocals are 0 where the quantity is 0.
 struct alignas(16) TriF {
-    float v0x, v0y, v0z, a;
+    float v0x, v0y, v0z, inv_a;
     float e1x, e1y, e1z, b;
     float e2x, e2y, e2z, c;
-    float det, denom, inv_det, pad;
+    float a, inv_c, inv_det, inv_denom;
 };
 struct alignas(16) SegF { float s0x, s0y, vx, vy; };
 
@@ -84,6 +85,7 @@ struct DomainInfo {
     int32_t root_ref = 0;                // FP32 root reference
     int depth = 0;
     double bounds[6] = {0, 0, 0, 0, 0, 0};  // bbox of the domain's vertices (mesh.cpp:296-322)
+    double coord_scale = 1.0;               // max |coordinate| of this domain, >= 1
 };
 
 struct DeviceLayouts {

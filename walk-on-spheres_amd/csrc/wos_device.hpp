@@ -41,7 +41,6 @@ struct MeshView {
     const SegF *segf;
     const float *seg_inv_vv;
     const TriF *trif;
-    float shrink_abs;          // absolute conservative margin of FP32 radii
 };
 
 struct EqView {
@@ -63,7 +62,7 @@ struct StartInfo {
     int32_t slot;         // leaf slot of the nearest primitive
     int32_t root64;       // the point's domain: FP64 root node
     int32_t root32;       // the point's domain: FP32 root reference
-    int32_t pad;
+    float shrink_abs;     // the point's domain: absolute margin of FP32 radii
     uint32_t key0, key1;  // walk-stream key
 };
 
@@ -616,6 +615,7 @@ __device__ __forceinline__ double eq_boundary(const EqView &e, const MeshView &m
 // FP32 child-box traversal
 // --------------------------------------------------------------------------
 __device__ __forceinline__ float fmaxf3(float a, float b, float c) { return fmaxf(fmaxf(a, b), c); }
+__device__ __forceinline__ float clamp01(float x) { return fminf(fmaxf(x, 0.0f), 1.0f); }
 
 // squared distance point -> box, FP32
 __device__ __forceinline__ float box_d2f(float lox, float loy, float loz, float hix, float hiy,
@@ -632,149 +632,167 @@ __device__ __forceinline__ float box_d2f2(float lox, float loy, float hix, float
     return __builtin_fmaf(dx, dx, dy * dy);
 }
 
-__device__ __forceinline__ float rcpf(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float sq3(float x, float y, float z) {
+    return __builtin_fmaf(x, x, __builtin_fmaf(y, y, z * z));
+}
 
-// FP32 Eberly (same regions as npq.cpp:52-189), squared distance only
-__device__ __forceinline__ float eberly_f(const TriF &T, float px, float py, float pz) {
-    const float a = T.a, b = T.b, c = T.c, det = T.det, denom = T.denom;
-    const float bpx = T.v0x - px, bpy = T.v0y - py, bpz = T.v0z - pz;
-    const float d = __builtin_fmaf(T.e1x, bpx, __builtin_fmaf(T.e1y, bpy, T.e1z * bpz));
-    const float e = __builtin_fmaf(T.e2x, bpx, __builtin_fmaf(T.e2y, bpy, T.e2z * bpz));
-    float s = __builtin_fmaf(b, e, -c * d);
-    float t = __builtin_fmaf(b, d, -a * e);
-    if (s + t <= det) {
-        if (s < 0.0f) {
-            if (t < 0.0f) {
-                if (d < 0.0f) {
-                    t = 0.0f;
-                    s = (-d >= a) ? 1.0f : -d * rcpf(a);
-                } else {
-                    s = 0.0f;
-                    t = (e >= 0.0f) ? 0.0f : ((-e >= c) ? 1.0f : -e * rcpf(c));
-                }
-            } else {
-                s = 0.0f;
-                t = (e >= 0.0f) ? 0.0f : ((-e >= c) ? 1.0f : -e * rcpf(c));
-            }
-        } else if (t < 0.0f) {
-            t = 0.0f;
-            s = (d >= 0.0f) ? 0.0f : ((-d >= a) ? 1.0f : -d * rcpf(a));
-        } else {
-            s *= T.inv_det;
-            t *= T.inv_det;
-        }
-    } else {
-        if (s < 0.0f) {
-            const float bd = b + d, ce = c + e;
-            if (ce > bd) {
-                const float numer = ce - bd;
-                s = (numer >= denom) ? 1.0f : numer * rcpf(denom);
-                t = 1.0f - s;
-            } else {
-                s = 0.0f;
-                t = (ce <= 0.0f) ? 1.0f : ((e >= 0.0f) ? 0.0f : -e * rcpf(c));
-            }
-        } else if (t < 0.0f) {
-            const float be = b + e, ad = a + d;
-            if (ad > be) {
-                const float numer = ad - be;
-                t = (numer >= denom) ? 1.0f : numer * rcpf(denom);
-                s = 1.0f - t;
-            } else {
-                t = 0.0f;
-                s = (ad <= 0.0f) ? 1.0f : ((d >= 0.0f) ? 0.0f : -d * rcpf(a));
-            }
-        } else {
-            const float numer = (c + e) - (b + d);
-            s = (numer <= 0.0f) ? 0.0f : ((numer >= denom) ? 1.0f : numer * rcpf(denom));
-            t = 1.0f - s;
-        }
-    }
-    // nearest - p = (v0 - p) + s e1 + t e2
-    const float dx = __builtin_fmaf(t, T.e2x, __builtin_fmaf(s, T.e1x, bpx));
-    const float dy = __builtin_fmaf(t, T.e2y, __builtin_fmaf(s, T.e1y, bpy));
-    const float dz = __builtin_fmaf(t, T.e2z, __builtin_fmaf(s, T.e1z, bpz));
-    return __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+// Squared distance point -> triangle in FP32 without branches.  The nearest
+// point is the plane projection when its barycentrics lie in the triangle,
+// otherwise the nearest of the three clamped edge projections; the minimum of
+// the in-triangle projection (when valid) and the three edge candidates is
+// that distance in every case, degenerate triangles included (their
+// reciprocals are 0, which collapses a candidate onto a vertex).  The same
+// minimum as the region walk of src/npq.cpp:52-189.
+__device__ __forceinline__ float tri_d2f(const TriF &T, float px, float py, float pz) {
+    const float bx = T.v0x - px, by = T.v0y - py, bz = T.v0z - pz;
+    const float d = __builtin_fmaf(T.e1x, bx, __builtin_fmaf(T.e1y, by, T.e1z * bz));
+    const float e = __builtin_fmaf(T.e2x, bx, __builtin_fmaf(T.e2y, by, T.e2z * bz));
+    // plane projection v0 + s e1 + t e2
+    const float s = __builtin_fmaf(T.b, e, -T.c * d) * T.inv_det;
+    const float t = __builtin_fmaf(T.b, d, -T.a * e) * T.inv_det;
+    const bool inside = s >= 0.0f && t >= 0.0f && s + t <= 1.0f;
+    const float di = sq3(__builtin_fmaf(t, T.e2x, __builtin_fmaf(s, T.e1x, bx)),
+                         __builtin_fmaf(t, T.e2y, __builtin_fmaf(s, T.e1y, by)),
+                         __builtin_fmaf(t, T.e2z, __builtin_fmaf(s, T.e1z, bz)));
+    // edge v0-v1
+    const float u0 = clamp01(-d * T.inv_a);
+    const float d0 = sq3(__builtin_fmaf(u0, T.e1x, bx), __builtin_fmaf(u0, T.e1y, by),
+                         __builtin_fmaf(u0, T.e1z, bz));
+    // edge v0-v2
+    const float u1 = clamp01(-e * T.inv_c);
+    const float d1 = sq3(__builtin_fmaf(u1, T.e2x, bx), __builtin_fmaf(u1, T.e2y, by),
+                         __builtin_fmaf(u1, T.e2z, bz));
+    // edge v1-v2: v0 + w e1 + (1 - w) e2
+    const float w = clamp01(((T.c + e) - (T.b + d)) * T.inv_denom);
+    const float w1 = 1.0f - w;
+    const float d2 = sq3(__builtin_fmaf(w1, T.e2x, __builtin_fmaf(w, T.e1x, bx)),
+                         __builtin_fmaf(w1, T.e2y, __builtin_fmaf(w, T.e1y, by)),
+                         __builtin_fmaf(w1, T.e2z, __builtin_fmaf(w, T.e1z, bz)));
+    return fminf(inside ? di : INFINITY, fminf(d0, fminf(d1, d2)));
 }
 
 __device__ __forceinline__ float seg_d2f(const SegF &g, float inv_vv, float px, float py) {
     const float wx = px - g.s0x, wy = py - g.s0y;
-    float par = __builtin_fmaf(wx, g.vx, wy * g.vy) * inv_vv;
-    par = fminf(fmaxf(par, 0.0f), 1.0f);
+    const float par = clamp01(__builtin_fmaf(wx, g.vx, wy * g.vy) * inv_vv);
     const float dx = __builtin_fmaf(-par, g.vx, wx), dy = __builtin_fmaf(-par, g.vy, wy);
     return __builtin_fmaf(dx, dx, dy * dy);
 }
 
-struct StackEntry { int32_t ref; float d2; };
-
-// Closest point FP32: near child first, far child with its box distance on
-// the LDS stack (popped entries whose distance is no longer below the best
-// are dropped without a node load).  Internal nodes are in breadth-first
-// order; the first n_cached of them are read from the block's LDS copy.
-// Returns the squared distance and the best leaf slot.
 template <int DIM>
 __device__ __forceinline__ float leaf_d2f(const MeshView &m, int slot, float px, float py, float pz) {
-    if constexpr (DIM == 3) return eberly_f(m.trif[slot], px, py, pz);
+    if constexpr (DIM == 3) return tri_d2f(m.trif[slot], px, py, pz);
     else return seg_d2f(m.segf[slot], m.seg_inv_vv[slot], px, py);
 }
 
-// `best_slot` on entry: a primitive to test first (the previous step's
-// nearest, -1 for none).  Its distance bounds the search from the start, so
-// the traversal only opens boxes that can still hold a closer primitive; the
-// result is the same minimum as a search from infinity.
-template <int DIM>
-__device__ __forceinline__ float npq_f(const MeshView &m, int32_t root, float px, float py,
-                                       float pz, int &best_slot, StackEntry *stk, int stride,
-                                       const void *cache, int n_cached) {
-    float best = best_slot >= 0 ? leaf_d2f<DIM>(m, best_slot, px, py, pz) : INFINITY;
-    int sp = 0;
-    int32_t ref = root;
-    for (;;) {
-        if (ref >= 0) {
-            float dl, dr;
-            int32_t cl, cr;
-            if constexpr (DIM == 3) {
-                const NodeF3 n = ref < n_cached ? static_cast<const NodeF3 *>(cache)[ref] : m.nf3[ref];
-                dl = box_d2f(n.l_lox, n.l_loy, n.l_loz, n.l_hix, n.l_hiy, n.l_hiz, px, py, pz);
-                dr = box_d2f(n.r_lox, n.r_loy, n.r_loz, n.r_hix, n.r_hiy, n.r_hiz, px, py, pz);
-                cl = n.left; cr = n.right;
-            } else {
-                const NodeF2 n = ref < n_cached ? static_cast<const NodeF2 *>(cache)[ref] : m.nf2[ref];
-                dl = box_d2f2(n.l_lox, n.l_loy, n.l_hix, n.l_hiy, px, py);
-                dr = box_d2f2(n.r_lox, n.r_loy, n.r_hix, n.r_hiy, px, py);
-                cl = n.left; cr = n.right;
-            }
-            const bool hl = dl < best, hr = dr < best;
-            if (hl && hr) {
-                const bool left_first = dl <= dr;
-                const int32_t near = left_first ? cl : cr;
-                StackEntry far;
-                far.ref = left_first ? cr : cl;
-                far.d2 = left_first ? dr : dl;
-                stk[(sp++) * stride] = far;
-                ref = near;
-                continue;
-            }
-            if (hl) { ref = cl; continue; }
-            if (hr) { ref = cr; continue; }
-        } else {
-            const int32_t code = -ref - 1;
-            const int first = code >> 3, count = code & 7;
-            for (int i = 0; i < count; ++i) {
-                const int slot = first + i;
-                const float d = leaf_d2f<DIM>(m, slot, px, py, pz);
-                if (d < best) { best = d; best_slot = slot; }
-            }
-        }
-        // pop
-        bool found = false;
-        while (sp > 0) {
-            const StackEntry e = stk[(--sp) * stride];
-            if (e.d2 < best) { ref = e.ref; found = true; break; }
-        }
-        if (!found) break;
+struct StackEntry { int32_t ref; float d2; };
+
+// Resumable closest-point query of one lane.  A query advances by one unit
+// per call of step(): one internal node (both child boxes) or one leaf
+// primitive, so lanes of a wave whose queries differ in length by orders of
+// magnitude stay busy together; the walker interleaves its walk steps with
+// these units.  Order: near child first, far child with its box distance on
+// the lane's LDS stack.
+//
+// Pruning: a box is opened while its squared distance is below cf x best.
+// cf = 1 is the exact search.  cf < 1 skips boxes between cf x best and best;
+// the smallest such box distance is kept in lb, so min(best, lb) is a lower
+// bound of the true squared distance and best (reached by best_slot) an upper
+// bound; both are equal whenever lb >= best.
+struct QueryF {
+    int32_t cur;      // internal node to open (when lcount == 0)
+    int32_t lslot;    // next leaf primitive
+    int32_t lcount;   // leaf primitives left
+    int32_t sp;       // stack depth
+    float best;       // smallest squared primitive distance so far
+    float lb;         // smallest skipped box distance
+    float cf;         // pruning factor
+    int32_t best_slot;
+};
+
+// Set the next unit from a child reference.
+__device__ __forceinline__ void query_enter(QueryF &q, int32_t ref) {
+    if (ref >= 0) {
+        q.cur = ref;
+        q.lcount = 0;
+    } else {
+        const int32_t code = -ref - 1;
+        q.lslot = code >> 3;
+        q.lcount = code & 7;
     }
-    return best;
+}
+
+// Start at `root`; best starts from the primitive in q.best_slot (the
+// previous step's nearest, -1 for none), which bounds the search from the
+// first node on.
+template <int DIM>
+__device__ __forceinline__ void query_begin(QueryF &q, const MeshView &m, int32_t root, float cf,
+                                            float px, float py, float pz) {
+    q.best = q.best_slot >= 0 ? leaf_d2f<DIM>(m, q.best_slot, px, py, pz) : INFINITY;
+    q.lb = INFINITY;
+    q.cf = cf;
+    q.sp = 0;
+    query_enter(q, root);
+}
+
+// One unit.  Returns false when the query is complete.
+template <int DIM>
+__device__ __forceinline__ bool query_step(QueryF &q, const MeshView &m, float px, float py,
+                                           float pz, StackEntry *stk, int stride,
+                                           const void *cache, int n_cached) {
+    bool pop;
+    if (q.lcount > 0) {
+        const float d = leaf_d2f<DIM>(m, q.lslot, px, py, pz);
+        if (d < q.best) { q.best = d; q.best_slot = q.lslot; }
+        ++q.lslot;
+        --q.lcount;
+        pop = q.lcount == 0;
+    } else {
+        float dl, dr;
+        int32_t cl, cr;
+        const int32_t ref = q.cur;
+        if constexpr (DIM == 3) {
+            const NodeF3 n = ref < n_cached ? static_cast<const NodeF3 *>(cache)[ref] : m.nf3[ref];
+            dl = box_d2f(n.l_lox, n.l_loy, n.l_loz, n.l_hix, n.l_hiy, n.l_hiz, px, py, pz);
+            dr = box_d2f(n.r_lox, n.r_loy, n.r_loz, n.r_hix, n.r_hiy, n.r_hiz, px, py, pz);
+            cl = n.left; cr = n.right;
+        } else {
+            const NodeF2 n = ref < n_cached ? static_cast<const NodeF2 *>(cache)[ref] : m.nf2[ref];
+            dl = box_d2f2(n.l_lox, n.l_loy, n.l_hix, n.l_hiy, px, py);
+            dr = box_d2f2(n.r_lox, n.r_loy, n.r_hix, n.r_hiy, px, py);
+            cl = n.left; cr = n.right;
+        }
+        const float thr = q.best * q.cf;
+        const bool hl = dl < thr, hr = dr < thr;
+        if (!hl && dl < q.best) q.lb = fminf(q.lb, dl);
+        if (!hr && dr < q.best) q.lb = fminf(q.lb, dr);
+        pop = false;
+        if (hl && hr) {
+            const bool left_first = dl <= dr;
+            StackEntry far;
+            far.ref = left_first ? cr : cl;
+            far.d2 = left_first ? dr : dl;
+            stk[(q.sp++) * stride] = far;
+            query_enter(q, left_first ? cl : cr);
+        } else if (hl) {
+            query_enter(q, cl);
+        } else if (hr) {
+            query_enter(q, cr);
+        } else {
+            pop = true;
+        }
+    }
+    if (pop) {
+        while (q.sp > 0) {
+            const StackEntry e = stk[(--q.sp) * stride];
+            if (e.d2 < q.best * q.cf) {
+                query_enter(q, e.ref);
+                return true;
+            }
+            if (e.d2 < q.best) q.lb = fminf(q.lb, e.d2);
+        }
+        return false;
+    }
+    return true;
 }
 
 }  // namespace wosx

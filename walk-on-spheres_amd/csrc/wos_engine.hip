@@ -205,6 +205,7 @@ struct DomainView {
     int s0, s1;        // leaf-slot range (winding number)
     double tol;        // boundary tolerance 1e-10 max(1, extent) (runner.hpp:86-88)
     int64_t out_off;   // first output index of this domain's points
+    float shrink_abs;  // absolute margin of FP32 radii in this domain
 };
 
 __device__ __forceinline__ double grid_coordinate(double lo, double hi, int index, int count) {
@@ -275,7 +276,7 @@ __global__ __launch_bounds__(kBlock) void k_classify(MeshView m, EqView e, GridS
     S.slot = q.slot;
     S.root64 = dom.root64;
     S.root32 = dom.root32;
-    S.pad = 0;
+    S.shrink_abs = dom.shrink_abs;
     S.bid = (DIM == 2 && q.prim >= 0) ? m.prim_bid[q.prim] : -1;
     S.key0 = (uint32_t)key;
     S.key1 = (uint32_t)(key >> 32);
@@ -284,6 +285,50 @@ __global__ __launch_bounds__(kBlock) void k_classify(MeshView m, EqView e, GridS
     loc[o] = (uint8_t)code;
     amb[o] = retries;
     bvalue[o] = gval;
+}
+
+// --------------------------------------------------------------------------
+// closest boundary point of a point list: bvh_npq (include/wos/bvh.hpp:38-39)
+// per lane.  FP64 runs the reference traversal; FP32 runs the walker's query
+// (pruning factor cf) and projects onto the primitive it found in FP64.
+// --------------------------------------------------------------------------
+template <int DIM>
+__global__ __launch_bounds__(kBlock) void k_closest(MeshView m, int32_t root64, int32_t root32,
+                                                    int fp32, float cf, const double *pts,
+                                                    int64_t n, double *dist, double *lower,
+                                                    double *nearest, int32_t *prim) {
+    extern __shared__ __align__(16) unsigned char lds_c[];
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double x = pts[i * DIM], y = pts[i * DIM + 1], z = DIM == 3 ? pts[i * DIM + 2] : 0.0;
+    double nx = 0.0, ny = 0.0, nz = 0.0;
+    if (!fp32) {
+        int *stk = reinterpret_cast<int *>(lds_c) + threadIdx.x;
+        const NpqD q = DIM == 2 ? npq_ref_2d(m, root64, x, y, stk, blockDim.x)
+                                : npq_ref_3d(m, root64, x, y, z, stk, blockDim.x);
+        dist[i] = sqrt(q.d2);
+        lower[i] = dist[i];
+        prim[i] = q.prim;
+        nx = q.px; ny = q.py; nz = q.pz;
+    } else {
+        StackEntry *stk = reinterpret_cast<StackEntry *>(lds_c) + threadIdx.x;
+        const float fx = (float)x, fy = (float)y, fz = (float)z;
+        QueryF q{};
+        q.best_slot = -1;
+        query_begin<DIM>(q, m, root32, cf, fx, fy, fz);
+        while (query_step<DIM>(q, m, fx, fy, fz, stk, blockDim.x, nullptr, 0)) {
+        }
+        dist[i] = sqrt((double)q.best);
+        lower[i] = sqrt((double)fminf(q.best, q.lb));
+        const int slot = q.best_slot;
+        prim[i] = m.slot_prim[slot];
+        double s, t;
+        if (DIM == 2) npq_seg_d(m.segd[slot], x, y, nx, ny, s);
+        else eberly_d(m.trid[slot], x, y, z, nx, ny, nz, s, t);
+    }
+    nearest[i * DIM] = nx;
+    nearest[i * DIM + 1] = ny;
+    if (DIM == 3) nearest[i * DIM + 2] = nz;
 }
 
 // --------------------------------------------------------------------------
@@ -363,10 +408,13 @@ __global__ __launch_bounds__(kBlock) void k_walk_fp64(MeshView m, EqView e, Roun
     const int64_t item = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (item >= rv.total) return;
     uint32_t w;
-    const StartInfo S = round_item(rv, item, w);
-    double px = S.x, py = S.y, pz = S.z, R = S.r;
-    NpqD q;
-    q.px = S.nx; q.py = S.ny; q.pz = S.nz; q.s = S.s; q.t = S.t; q.prim = S.prim;
+    // the start record is read field by field where needed (a register copy
+    // of all of it would stay live across the walk)
+    const StartInfo *S = &round_item(rv, item, w);
+    double px = S->x, py = S->y, pz = S->z, R = S->r;
+    const uint32_t k0 = S->key0, k1 = S->key1;
+    const int root = S->root64;
+    NpqD q{};
     double sample = 0.0, weight = 1.0;
     int steps = 0;
     const double alpha = e.alpha;
@@ -375,9 +423,9 @@ __global__ __launch_bounds__(kBlock) void k_walk_fp64(MeshView m, EqView e, Roun
         if (SRC && e.source_kind != WOS_SRC_ZERO) {
             if constexpr (ADJ) {
                 sample += source_term_adj<DIM, double, 1>(e, px, py, pz, R, weight, w,
-                                                          (uint32_t)steps, S.key0, S.key1, D);
+                                                          (uint32_t)steps, k0, k1, D);
             } else {
-                CursorD src(w, (uint32_t)steps, 1, S.key0, S.key1);
+                CursorD src(w, (uint32_t)steps, 1, k0, k1);
                 sample += source_term<DIM, double>(e, px, py, pz, R, weight, src);
             }
         }
@@ -386,7 +434,7 @@ __global__ __launch_bounds__(kBlock) void k_walk_fp64(MeshView m, EqView e, Roun
             weight *= DIM == 2 ? screened_weight_2d<double>(alpha, R)
                                : screened_weight_3d<double>(alpha, R);
         }
-        CursorD wk(w, (uint32_t)steps, 0, S.key0, S.key1);
+        CursorD wk(w, (uint32_t)steps, 0, k0, k1);
         if (DIM == 2) {
             const double theta = wk.unit() * 2 * kPi;
             px = px + R * cos(theta);
@@ -400,17 +448,17 @@ __global__ __launch_bounds__(kBlock) void k_walk_fp64(MeshView m, EqView e, Roun
             pz = pz + R * z;
         }
         ++steps;
-        q = DIM == 2 ? npq_ref_2d(m, S.root64, px, py, stk, stride)
-                     : npq_ref_3d(m, S.root64, px, py, pz, stk, stride);
+        q = DIM == 2 ? npq_ref_2d(m, root, px, py, stk, stride)
+                     : npq_ref_3d(m, root, px, py, pz, stk, stride);
         R = sqrt(q.d2);
     }
     const bool hit = R > eps;
-    const double g = steps == 0 ? S.g : eq_boundary(e, m, q.px, q.py, q.pz, q.prim, q.s, q.t);
+    const double g = steps == 0 ? S->g : eq_boundary(e, m, q.px, q.py, q.pz, q.prim, q.s, q.t);
     vals[item] = sample + weight * g;
     steps_out[item] = steps | (hit ? (int32_t)0x80000000 : 0);
     if constexpr (ADJ) {
-        const int prim = steps == 0 ? S.prim : q.prim;
-        const double s = steps == 0 ? S.s : q.s, t = steps == 0 ? S.t : q.t;
+        const int prim = steps == 0 ? S->prim : q.prim;
+        const double s = steps == 0 ? S->s : q.s, t = steps == 0 ? S->t : q.t;
         store_adjoint(adj, item, D, weight, g, prim, s, DIM == 3 ? t : 0.0);
     }
 }
@@ -597,16 +645,35 @@ __device__ double source_term_adj(const EqView &e, T px, T py, T pz, T R, T weig
     return S;
 }
 
+// zero n 64-bit words (a kernel rather than a memset node, so that the
+// launch sequence can be captured into a graph and replayed)
+__global__ void k_zero_u64(unsigned long long *p, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = 0ull;
+}
+
 // --------------------------------------------------------------------------
-// kernel 2b: FP32 persistent walker with ballot-driven lane refill
+// kernel 2b: FP32 persistent walker
 // --------------------------------------------------------------------------
+// Each lane holds one walk at a time and advances its closest-point query by
+// one unit (node or primitive) per trip of the inner loop.  When min_ready
+// lanes of the wave have finished their queries, or no lane is searching any
+// more, the finished lanes take their walk step together: end the walk or go
+// on, refill from the wave's pool of work items (ballot + mbcnt rank, one
+// atomicAdd per chunk), move to the next sphere point and begin the next
+// query.  Step queries prune with the factor cf; a walk that the relaxed
+// bound would end, or that reaches max_steps, searches again exactly first,
+// so the stopping rule and the terminal projection see the exact nearest
+// primitive.
 struct PersistArgs {
     RoundView rv;
     int max_steps;
     double eps;
     float eps_f;
     float shrink_rel;
-    uint64_t total;
+    float cf;            // pruning factor of step queries, (1 - relax)^2
+    int min_ready;       // finished lanes that start a walk-step phase
     unsigned long long *counter;
     int chunk;
     int n_cached;        // internal BVH nodes staged in LDS (breadth-first prefix)
@@ -634,13 +701,16 @@ __global__ __launch_bounds__(kBlock) void k_walk_fp32(MeshView m, EqView e, Pers
     const int lane = __lane_id();
 
     // lane state
-    bool active = false;
-    float px = 0, py = 0, pz = 0, R = 0, weight = 1.0f;
+    bool active = false;     // the lane holds a walk
+    bool searching = false;  // its query is in progress
+    float px = 0, py = 0, pz = 0, R = 0, weight = 1.0f, margin = 0.0f;
     double sample = 0.0;
-    int steps = 0, slot = -1;
+    int steps = 0;
     int32_t root = 0;
     uint32_t w = 0, k0 = 0, k1 = 0;
     size_t out_index = 0;
+    QueryF q{};
+    q.best_slot = -1;
     const float alpha = (float)e.alpha;
     WalkDer D{};
 
@@ -649,6 +719,34 @@ __global__ __launch_bounds__(kBlock) void k_walk_fp32(MeshView m, EqView e, Pers
     bool exhausted = false;
 
     for (;;) {
+        // ---- finished queries: radius, then end of walk or exact re-query
+        if (active && !searching) {
+            const bool exact = !(q.lb < q.best);
+            R = sqrtf(exact ? q.best : q.lb) * a.shrink_rel - margin;
+            const bool stop = !(R > a.eps_f) || steps >= a.max_steps;
+            if (stop && !exact) {
+                query_begin<DIM>(q, m, root, 1.0f, px, py, pz);
+                searching = true;
+            } else if (stop) {
+                // terminal value in FP64: nearest point on the nearest primitive
+                const bool hit = R > a.eps_f;
+                double g = 0.0, s = 0.0, t = 0.0;
+                int prim = -1;
+                const int slot = q.best_slot;
+                if (slot >= 0) {
+                    prim = m.slot_prim[slot];
+                    double nx, ny, nz = 0.0;
+                    if (DIM == 2) npq_seg_d(m.segd[slot], (double)px, (double)py, nx, ny, s);
+                    else eberly_d(m.trid[slot], (double)px, (double)py, (double)pz, nx, ny, nz, s, t);
+                    g = eq_boundary(e, m, nx, ny, nz, prim, s, t);
+                }
+                a.vals[out_index] = sample + (double)weight * g;
+                a.steps_out[out_index] = steps | (hit ? (int32_t)0x80000000 : 0);
+                if constexpr (ADJ) store_adjoint(adj, (int64_t)out_index, D, (double)weight, g, prim, s, t);
+                active = false;
+            }
+        }
+
         // ---- refill free lanes from the wave pool
         const uint64_t free_mask = __ballot(!active);
         if (free_mask != 0 && !exhausted) {
@@ -671,11 +769,12 @@ __global__ __launch_bounds__(kBlock) void k_walk_fp32(MeshView m, EqView e, Pers
                 if (!active && item < pool_end) {
                     const StartInfo &S = round_item(a.rv, (int64_t)item, w);
                     px = (float)S.x; py = (float)S.y; pz = (float)S.z;
-                    R = (float)S.r * a.shrink_rel - m.shrink_abs;
+                    margin = S.shrink_abs;
+                    R = (float)S.r * a.shrink_rel - margin;
                     weight = 1.0f;
                     sample = 0.0;
                     steps = 0;
-                    slot = S.slot;
+                    q.best_slot = S.slot;
                     root = S.root32;
                     k0 = S.key0;
                     k1 = S.key1;
@@ -696,62 +795,56 @@ __global__ __launch_bounds__(kBlock) void k_walk_fp32(MeshView m, EqView e, Pers
                 pool_next = pool_next + nfree < pool_end ? pool_next + nfree : pool_end;
             }
         }
+
+        // ---- one walk step (include/wos/wos.hpp:68-127), then the next query
+        if (active && !searching) {
+            if (SMODE != 0 && e.source_kind != WOS_SRC_ZERO) {
+                if constexpr (ADJ)
+                    sample += source_term_adj<DIM, float, SMODE>(e, px, py, pz, R, weight, w,
+                                                                 (uint32_t)steps, k0, k1, D);
+                else
+                    sample += source_term_f<DIM, SMODE>(e, px, py, pz, R, weight, w,
+                                                        (uint32_t)steps, k0, k1);
+            }
+            if (SCR) {
+                if constexpr (ADJ && DIM == 3) D.L += dlog_w3_dalpha(e.alpha, (double)R);
+                weight *= DIM == 2 ? screened_weight_2d<float>(alpha, R)
+                                   : screened_weight_3d<float>(alpha, R);
+            }
+            Cursor wk(w, (uint32_t)steps, 0, k0, k1);
+            if (DIM == 2) {
+                const float u = wk.unitf();
+                float sn, cs;
+                __sincosf(u * (float)(2.0 * kPi), &sn, &cs);
+                px = __builtin_fmaf(R, cs, px);
+                py = __builtin_fmaf(R, sn, py);
+            } else {
+                const float u = wk.unitf(), v = wk.unitf();
+                const float z = 2.0f * u - 1.0f;
+                const float rr = sqrtf(fmaxf(0.0f, 1.0f - z * z));
+                float sn, cs;
+                __sincosf(v * (float)(2.0 * kPi), &sn, &cs);
+                px = __builtin_fmaf(R * rr, cs, px);
+                py = __builtin_fmaf(R * rr, sn, py);
+                pz = __builtin_fmaf(R, z, pz);
+            }
+            ++steps;
+            query_begin<DIM>(q, m, root, a.cf, px, py, pz);
+            searching = true;
+        }
         if (__ballot(active) == 0) {
             if (exhausted) break;
             continue;
         }
-        if (active) {
-        // ---- one walk step (include/wos/wos.hpp:68-127)
-        if (SMODE != 0 && e.source_kind != WOS_SRC_ZERO) {
-            if constexpr (ADJ)
-                sample += source_term_adj<DIM, float, SMODE>(e, px, py, pz, R, weight, w,
-                                                             (uint32_t)steps, k0, k1, D);
-            else
-                sample += source_term_f<DIM, SMODE>(e, px, py, pz, R, weight, w, (uint32_t)steps, k0, k1);
+
+        // ---- advance the queries until enough lanes are ready to step
+        for (;;) {
+            if (searching)
+                searching = query_step<DIM>(q, m, px, py, pz, stk, stride, cache, a.n_cached);
+            if (__ballot(searching) == 0) break;
+            const uint64_t ready = __ballot(!searching && (active || !exhausted));
+            if (__popcll(ready) >= a.min_ready) break;
         }
-        if (SCR) {
-            if constexpr (ADJ && DIM == 3) D.L += dlog_w3_dalpha(e.alpha, (double)R);
-            weight *= DIM == 2 ? screened_weight_2d<float>(alpha, R)
-                               : screened_weight_3d<float>(alpha, R);
-        }
-        Cursor wk(w, (uint32_t)steps, 0, k0, k1);
-        if (DIM == 2) {
-            const float u = wk.unitf();
-            float sn, cs;
-            __sincosf(u * (float)(2.0 * kPi), &sn, &cs);
-            px = __builtin_fmaf(R, cs, px);
-            py = __builtin_fmaf(R, sn, py);
-        } else {
-            const float u = wk.unitf(), v = wk.unitf();
-            const float z = 2.0f * u - 1.0f;
-            const float rr = sqrtf(fmaxf(0.0f, 1.0f - z * z));
-            float sn, cs;
-            __sincosf(v * (float)(2.0 * kPi), &sn, &cs);
-            px = __builtin_fmaf(R * rr, cs, px);
-            py = __builtin_fmaf(R * rr, sn, py);
-            pz = __builtin_fmaf(R, z, pz);
-        }
-        ++steps;
-        const float d2 = npq_f<DIM>(m, root, px, py, pz, slot, stk, stride, cache, a.n_cached);
-        R = sqrtf(d2) * a.shrink_rel - m.shrink_abs;
-        if (!(R > a.eps_f) || steps >= a.max_steps) {
-            // terminal value in FP64: nearest point on the nearest primitive
-            const bool hit = R > a.eps_f;
-            double g = 0.0, s = 0.0, t = 0.0;
-            int prim = -1;
-            if (slot >= 0) {
-                prim = m.slot_prim[slot];
-                double nx, ny, nz = 0.0;
-                if (DIM == 2) npq_seg_d(m.segd[slot], (double)px, (double)py, nx, ny, s);
-                else eberly_d(m.trid[slot], (double)px, (double)py, (double)pz, nx, ny, nz, s, t);
-                g = eq_boundary(e, m, nx, ny, nz, prim, s, t);
-            }
-            a.vals[out_index] = sample + (double)weight * g;
-            a.steps_out[out_index] = steps | (hit ? (int32_t)0x80000000 : 0);
-            if constexpr (ADJ) store_adjoint(adj, (int64_t)out_index, D, (double)weight, g, prim, s, t);
-            active = false;
-        }
-        }  // active
     }
 }
 
@@ -889,13 +982,14 @@ struct wos_ctx {
     DeviceLayouts L;
     bool has_mesh = false;
     DevBuf nd2, nd3, segd, trid, slot_prim, prim_bid, prims, verts, nf2, nf3, segf, seg_inv, trif;
+    DevBuf cq_out, cq_prim;
     DevBuf plist_in, loc, starts, amb, bvalue, inside, vals, steps, mean, m2, count, ssum, hits,
         counter, vv, pid_all, r_pid, r_wbeg, r_off, coef, scal_sum, grad_v, grad_b, dl_du,
         adj_scal, adj_vw, adj_prim;
     int last_rounds = 0;
     MeshView view{};
     hipEvent_t ev[6]{};
-    wos_tuning tune{256, -1, 256, 0};
+    wos_tuning tune{256, -1, 256, 0, 16, 0.0f};
     size_t lds_per_cu = 65536;
     size_t lds_per_block = 65536;
     int last_threads = 0;
@@ -965,11 +1059,14 @@ hipError_t launch_walk(wos_ctx *c, const EqView &ev, int precision, const RoundV
     a.eps = eps;
     a.eps_f = (float)eps;
     a.shrink_rel = 1.0f - 1.0f / 131072.0f;
+    a.cf = (1.0f - c->tune.relax) * (1.0f - c->tune.relax);
+    a.min_ready = c->tune.min_ready;
     a.counter = c->counter.as<unsigned long long>();
     a.chunk = c->tune.chunk;
     a.vals = c->vals.as<double>();
     a.steps_out = c->steps.as<int32_t>();
-    e = hipMemsetAsync(c->counter.p, 0, sizeof(unsigned long long), c->stream);
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, c->stream, a.counter, 1);
+    e = hipGetLastError();
     if (e != hipSuccess) return e;
     if (green)
         hipLaunchKernelGGL((k_walk_fp32<DIM, 2, SCR, ADJ>), dim3(blocks), dim3(threads), lds,
@@ -1116,8 +1213,10 @@ int solve(wos_ctx *c, const std::vector<Part> &parts, const wos_eq_desc *eq,
                 g = *p.grid;
                 for (int k = 0; k < 6; ++k) g.b[k] = gb[k];
             }
+            // conservative FP32 radius margin: 32 ulp of the domain's largest coordinate
             DomainView dv{D.node_off, D.root_ref, (int)D.prim_off, (int)(D.prim_off + D.n_prims),
-                          part_tol[pi], off};
+                          part_tol[pi], off,
+                          (float)(32.0 * std::ldexp(1.0, -24) * D.coord_scale)};
             const double *dp = p.points ? c->plist_in.as<double>() + poff * dim : nullptr;
             hipError_t le = dim == 2 ? launch_classify(c, k_classify<2>, ev, g, dv, dp, p.count,
                                                        prm->seed, prm->max_ray_attempts)
@@ -1534,8 +1633,6 @@ int wos_ctx_set_domains(wos_ctx *c, int dim, int n_domains, const double *const 
     v.prims = c->prims.as<int32_t>(); v.verts = c->verts.as<double>();
     v.nf2 = c->nf2.as<NodeF2>(); v.nf3 = c->nf3.as<NodeF3>();
     v.segf = c->segf.as<SegF>(); v.seg_inv_vv = c->seg_inv.as<float>(); v.trif = c->trif.as<TriF>();
-    // conservative FP32 radius margin: 32 ulp of the largest coordinate
-    v.shrink_abs = (float)(32.0 * std::ldexp(1.0, -24) * L.coord_scale);
     c->view = v;
     c->mesh = std::move(all);
     c->L = std::move(L);
@@ -1592,6 +1689,8 @@ int wos_ctx_set_tuning(wos_ctx *c, const wos_tuning *t) {
     if (t->chunk < 1 || t->chunk > (1 << 20)) return fail(WOS_ERR_INPUT, "chunk out of range");
     if (t->lds_nodes < -1 || t->blocks_per_cu < 0)
         return fail(WOS_ERR_INPUT, "lds_nodes must be >= -1 and blocks_per_cu >= 0");
+    if (t->min_ready < 0 || t->min_ready > 64) return fail(WOS_ERR_INPUT, "min_ready must be in 0..64");
+    if (!(t->relax >= 0.0f && t->relax < 0.5f)) return fail(WOS_ERR_INPUT, "relax must be in [0, 0.5)");
     c->tune = *t;
     return WOS_OK;
 }
@@ -1631,6 +1730,56 @@ int wos_solve_points(wos_ctx *c, const double *points, int64_t n, const wos_eq_d
                      wos_run_summary *summary) {
     if (!points) return fail(WOS_ERR_INPUT, "points is NULL");
     return solve(c, {Part{0, nullptr, points, n}}, eq, params, out, summary);
+}
+
+int wos_closest_points(wos_ctx *c, int domain, const double *points, int64_t n, int precision,
+                       double relax, double *dist, double *lower, double *nearest,
+                       int32_t *prim) {
+    if (!c || !c->has_mesh) return fail(c ? WOS_ERR_STATE : WOS_ERR_INPUT, "no mesh set on this context");
+    if (domain < 0 || domain >= (int)c->L.domains.size()) return fail(WOS_ERR_INPUT, "domain out of range");
+    if (n < 0) return fail(WOS_ERR_INPUT, "n must be >= 0");
+    if (n > 0 && (!points || !dist)) return fail(WOS_ERR_INPUT, "points and dist must not be NULL");
+    if (precision != WOS_FP64_PLAIN && precision != WOS_FP32_PERSISTENT)
+        return fail(WOS_ERR_INPUT, "unknown precision");
+    if (!(relax >= 0.0 && relax < 0.5)) return fail(WOS_ERR_INPUT, "relax must be in [0, 0.5)");
+    if (precision == WOS_FP64_PLAIN && relax != 0.0)
+        return fail(WOS_ERR_INPUT, "the FP64 search is always exact (relax must be 0)");
+    if (n == 0) return WOS_OK;
+    const int dim = c->mesh.dim;
+    for (int64_t i = 0; i < n * dim; ++i)
+        if (!std::isfinite(points[i])) return fail(WOS_ERR_INPUT, "points must be finite");
+    HIPX(hipSetDevice(c->device));
+    std::vector<double> pts(points, points + n * dim);
+    HIPX(upload(c->plist_in, pts, c->stream));
+    HIPX(c->cq_out.reserve((size_t)n * (2 + dim) * sizeof(double)));
+    HIPX(c->cq_prim.reserve((size_t)n * sizeof(int32_t)));
+    double *d_dist = c->cq_out.as<double>(), *d_low = d_dist + n, *d_near = d_low + n;
+    const DomainInfo &D = c->L.domains[domain];
+    const float cf = (float)((1.0 - relax) * (1.0 - relax));
+    const int blocks = (int)((n + kBlock - 1) / kBlock);
+    const size_t lds = (size_t)kBlock * std::max((c->L.depth + 2) * sizeof(int),
+                                                 (c->L.depth + 1) * sizeof(StackEntry));
+    const int fp32 = precision == WOS_FP32_PERSISTENT;
+    if (dim == 2)
+        hipLaunchKernelGGL(k_closest<2>, dim3(blocks), dim3(kBlock), lds, c->stream, c->view,
+                           D.node_off, D.root_ref, fp32, cf, c->plist_in.as<double>(), n, d_dist,
+                           d_low, d_near, c->cq_prim.as<int32_t>());
+    else
+        hipLaunchKernelGGL(k_closest<3>, dim3(blocks), dim3(kBlock), lds, c->stream, c->view,
+                           D.node_off, D.root_ref, fp32, cf, c->plist_in.as<double>(), n, d_dist,
+                           d_low, d_near, c->cq_prim.as<int32_t>());
+    HIPX(hipGetLastError());
+    HIPX(hipMemcpyAsync(dist, d_dist, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    if (lower)
+        HIPX(hipMemcpyAsync(lower, d_low, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    if (nearest)
+        HIPX(hipMemcpyAsync(nearest, d_near, n * dim * sizeof(double), hipMemcpyDeviceToHost,
+                            c->stream));
+    if (prim)
+        HIPX(hipMemcpyAsync(prim, c->cq_prim.p, n * sizeof(int32_t), hipMemcpyDeviceToHost,
+                            c->stream));
+    HIPX(hipStreamSynchronize(c->stream));
+    return WOS_OK;
 }
 
 int wos_adjoint_points(wos_ctx *c, const double *points, int64_t n, const wos_eq_desc *eq,

@@ -110,6 +110,8 @@ class Tuning(C.Structure):
         ("lds_nodes", C.c_int32),
         ("chunk", C.c_int32),
         ("blocks_per_cu", C.c_int32),
+        ("min_ready", C.c_int32),
+        ("relax", C.c_float),
     ]
 
 
@@ -196,6 +198,10 @@ def load(path: str | None = None):
                                        C.POINTER(C.c_int32), C.c_int64, C.POINTER(C.c_int32),
                                        C.POINTER(C.c_int32), C.POINTER(C.c_double),
                                        C.POINTER(C.c_double), C.POINTER(C.c_int32)]
+    lib.wos_closest_points.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.c_int64,
+                                       C.c_int, C.c_double, C.POINTER(C.c_double),
+                                       C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                       C.POINTER(C.c_int32)]
     lib.wos_ctx_set_tuning.argtypes = [C.c_void_p, C.POINTER(Tuning)]
     lib.wos_ctx_get_tuning.argtypes = [C.c_void_p, C.POINTER(Tuning)]
     if path is None:

@@ -98,8 +98,9 @@ class Context:
             res.append(r)
         return res, self._summary(summ)
 
-    def set_tuning(self, block_threads=256, lds_nodes=-1, chunk=256, blocks_per_cu=0):
-        t = capi.Tuning(block_threads, lds_nodes, chunk, blocks_per_cu)
+    def set_tuning(self, block_threads=256, lds_nodes=-1, chunk=256, blocks_per_cu=0,
+                   min_ready=16, relax=0.0):
+        t = capi.Tuning(block_threads, lds_nodes, chunk, blocks_per_cu, min_ready, relax)
         capi.check(self.lib, self.lib.wos_ctx_set_tuning(self.h, C.byref(t)))
 
     def get_tuning(self):
@@ -121,6 +122,22 @@ class Context:
         p.max_walks = max_walks
         p.tune = tune
         return p
+
+    def closest_points(self, points, precision=WOS_FP64_PLAIN, relax=0.0, domain=0):
+        """Closest boundary point per point: dict(dist, lower, nearest, prim)
+        (bvh_npq of the reference, include/wos/bvh.hpp:38-39)."""
+        dim = self.dim
+        pts = np.ascontiguousarray(points, np.float64)
+        n = len(pts)
+        pts = np.ascontiguousarray(pts.reshape(n, -1)[:, :dim]) if n else np.zeros((0, dim))
+        dist, lower = np.zeros(n), np.zeros(n)
+        nearest = np.zeros((n, dim))
+        prim = np.zeros(n, np.int32)
+        rc = self.lib.wos_closest_points(self.h, domain, ptr(pts, C.c_double), n, precision, relax,
+                                         ptr(dist, C.c_double), ptr(lower, C.c_double),
+                                         ptr(nearest, C.c_double), ptr(prim, C.c_int32))
+        capi.check(self.lib, rc)
+        return dict(dist=dist, lower=lower, nearest=nearest, prim=prim)
 
     def adjoint_points(self, points, eq: EqDesc, prm: RunParams, dl_du):
         """Replay the walks of solve_points and return (forward stats,
