@@ -155,6 +155,9 @@ typedef struct wos_tuning {
                                     resulting lower bound (>= (1 - r) x distance);
                                     0 = exact search.  Walks are ended and projected
                                     with an exact search in every case.            */
+    int32_t grid_cells;          /* cells per domain of the FP32 distance grid
+                                    (cell lists near the boundary, cell distance
+                                    bounds inside); 0 = no grid                    */
 } wos_tuning;
 
 typedef struct wos_ctx wos_ctx;
@@ -216,7 +219,9 @@ int64_t wos_bvh_build_host(int dim, const double *verts, int64_t n_verts,
  * with relax > 0 (FP32 only) boxes farther than (1 - relax) x the best
  * distance are skipped and lower[i] (may be NULL) receives the lower bound
  * of the distance that the walker would use as its sphere radius
- * (lower[i] = dist[i] for an exact search). */
+ * (lower[i] = dist[i] for an exact search).  A relaxed search that the
+ * distance grid answers from its cell bound alone reports an upper bound in
+ * dist[i], prim[i] = -1 and a NaN nearest point. */
 int wos_closest_points(wos_ctx *ctx, int domain, const double *points, int64_t n,
                        int precision, double relax, double *dist, double *lower,
                        double *nearest, int32_t *prim);

@@ -55,38 +55,50 @@ def test_closest_points(gpu_ctx, oracle, name, mesh_fn):
     opt = np.array([r["point"] for r in o])
     assert np.array_equal(g64["dist"], od), name
     assert np.array_equal(g64["lower"], g64["dist"])
-    # points on vertices are equally near several primitives; the oracle
-    # orders primitives inside a leaf differently from the reference's
-    # nth_element, so it may name another of them
-    unique = od > 0
-    assert np.array_equal(g64["prim"][unique], op[unique]), name
-    assert np.array_equal(g64["nearest"][unique], opt[unique]), name
+    # Points on or next to a vertex are (up to rounding) equally near several
+    # primitives.  The oracle orders the primitives inside a leaf differently
+    # from the reference's nth_element and may name another of them, so
+    # against the oracle the reported point only has to be a nearest point;
+    # against the compiled reference everything is bit-exact.
+    dn = np.linalg.norm(g64["nearest"] - pts, axis=1)
+    assert np.allclose(dn, od, rtol=1e-12, atol=1e-15), name
+    same = g64["prim"] == op
+    print(f"{name}: fp64 prim == oracle prim for {same.mean():.4f} of the points")
+    assert same.mean() > 0.8, name
+    assert np.array_equal(g64["nearest"][same], opt[same]), name
     if pyoracle_reference_built():
         import pyoracle
         rm = pyoracle.Reference().mesh(V, F, B)
         rr = [rm.npq(p) for p in pts]
         assert np.array_equal(g64["dist"], np.array([r["distance"] for r in rr])), name
-        assert np.array_equal(g64["prim"], np.array([r["prim"] for r in rr])), name
         assert np.array_equal(g64["nearest"], np.array([r["point"][:dim] for r in rr])), name
+        if dim == 2:  # the reference's 3D bvh_npq reports no primitive
+            assert np.array_equal(g64["prim"], np.array([r["prim"] for r in rr])), name
 
     margin = 32 * 2.0 ** -24 * max(1.0, np.abs(V).max())
-    g32 = gpu_ctx.closest_points(pts, WOS_FP32_PERSISTENT)
-    err = np.abs(g32["dist"] - od)
-    print(f"{name}: fp32 max|d - d64| = {err.max():.3e} (margin {margin:.3e})")
-    assert np.all(err <= margin), name
-    assert np.array_equal(g32["lower"], g32["dist"])
-    # the FP64 projection on the FP32 choice is a nearest point up to the margin
-    dproj = np.linalg.norm(g32["nearest"] - pts, axis=1)
-    assert np.all(np.abs(dproj - od) <= margin), name
+    for grid_cells in (0, 1 << 10, 1 << 16, 1 << 21):
+        gpu_ctx.set_tuning(grid_cells=grid_cells)
+        g32 = gpu_ctx.closest_points(pts, WOS_FP32_PERSISTENT)
+        err = np.abs(g32["dist"] - od)
+        print(f"{name} grid {grid_cells}: fp32 max|d - d64| = {err.max():.3e} (margin {margin:.3e})")
+        assert np.all(err <= margin), (name, grid_cells)
+        assert np.array_equal(g32["lower"], g32["dist"])
+        # the FP64 projection on the FP32 choice is a nearest point up to the margin
+        dproj = np.linalg.norm(g32["nearest"] - pts, axis=1)
+        assert np.all(np.abs(dproj - od) <= margin), (name, grid_cells)
 
-    for relax in (0.05, 0.2, 0.45):
-        r = gpu_ctx.closest_points(pts, WOS_FP32_PERSISTENT, relax=relax)
-        assert np.all(r["lower"] <= od + margin), (name, relax)
-        assert np.all(r["lower"] >= (1 - relax) * r["dist"] * (1 - 1e-6)), (name, relax)
-        assert np.all(r["dist"] >= od - margin), (name, relax)
-        frac = np.mean(r["lower"] < r["dist"])
-        print(f"{name} relax={relax}: inexact fraction {frac:.3f}, "
-              f"mean lower/d64 {np.mean(r['lower'][od > 0] / od[od > 0]):.4f}")
+        for relax in (0.05, 0.2, 0.45):
+            r = gpu_ctx.closest_points(pts, WOS_FP32_PERSISTENT, relax=relax)
+            assert np.all(r["lower"] <= od + margin), (name, grid_cells, relax)
+            assert np.all(r["lower"] >= (1 - relax) * r["dist"] * (1 - 1e-6)), (name, grid_cells, relax)
+            assert np.all(r["dist"] >= od - margin), (name, grid_cells, relax)
+            found = r["prim"] >= 0
+            assert np.all(np.isnan(r["nearest"][~found])), (name, grid_cells, relax)
+            frac = np.mean(r["lower"] < r["dist"])
+            print(f"{name} grid {grid_cells} relax={relax}: inexact {frac:.3f}, "
+                  f"grid-bound answers {np.mean(~found):.3f}, "
+                  f"mean lower/d64 {np.mean(r['lower'][od > 0] / od[od > 0]):.4f}")
+    gpu_ctx.set_tuning()
 
 
 def test_closest_points_validation(gpu_ctx):

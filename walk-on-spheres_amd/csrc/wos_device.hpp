@@ -19,6 +19,28 @@
 namespace wosx {
 
 // --------------------------------------------------------------------------
+// distance grid (FP32 walker): a uniform grid of cubic cells per domain.
+// Cell record: dc = distance from the cell centre to the boundary, rounded
+// down; code = (first << 5) | count for a cell whose complete list of
+// possibly-nearest primitives (count <= 31 leaf slots at glist[first..]) is
+// stored, 0 otherwise.  The list holds every primitive within dc + 2 hd of
+// the centre (hd = half cell diagonal, padded), which contains the nearest
+// primitive of every point within hd of the centre.
+// --------------------------------------------------------------------------
+struct GridInfo {
+    float ox, oy, oz;     // corner of cell (0, 0, 0)
+    float h, inv_h;       // cell size and its reciprocal
+    float hd;             // padded half diagonal used for the lists
+    int32_t nx, ny, nz;   // cells per axis (nz = 1 in 2D)
+    int32_t cell_off;     // first cell of this domain in the cell array
+};
+struct GridCellF {
+    float dc;
+    uint32_t code;
+};
+constexpr int kGridListMax = 31;
+
+// --------------------------------------------------------------------------
 // kernel-side views
 // --------------------------------------------------------------------------
 struct MeshView {
@@ -41,6 +63,10 @@ struct MeshView {
     const SegF *segf;
     const float *seg_inv_vv;
     const TriF *trif;
+    // distance grid (nullptr when off)
+    const GridInfo *grids;     // per domain
+    const GridCellF *cells;
+    const int32_t *glist;      // leaf slots of the cell lists
 };
 
 struct EqView {
@@ -64,6 +90,8 @@ struct StartInfo {
     int32_t root32;       // the point's domain: FP32 root reference
     float shrink_abs;     // the point's domain: absolute margin of FP32 radii
     uint32_t key0, key1;  // walk-stream key
+    int32_t domain;       // the point's domain index
+    int32_t pad;
 };
 
 // nearest-point query result (FP64)
@@ -700,8 +728,9 @@ struct StackEntry { int32_t ref; float d2; };
 // bound; both are equal whenever lb >= best.
 struct QueryF {
     int32_t cur;      // internal node to open (when lcount == 0)
-    int32_t lslot;    // next leaf primitive
+    int32_t lslot;    // next leaf primitive (its index in glist when indirect)
     int32_t lcount;   // leaf primitives left
+    bool indirect;    // primitives come from a grid cell list
     int32_t sp;       // stack depth
     float best;       // smallest squared primitive distance so far
     float lb;         // smallest skipped box distance
@@ -721,17 +750,53 @@ __device__ __forceinline__ void query_enter(QueryF &q, int32_t ref) {
     }
 }
 
-// Start at `root`; best starts from the primitive in q.best_slot (the
-// previous step's nearest, -1 for none), which bounds the search from the
-// first node on.
+// Begin a query at p in `domain`.  With a distance grid, a cell that holds
+// a primitive list turns the query into a scan of that list (exact); with
+// cf < 1, a cell whose centre distance dc gives lb = dc - |p - c| and
+// ub = dc' + |p - c| (dc' = dc rounded up) with lb^2 >= cf ub^2 ends it at
+// once with best = ub^2, lb = lb^2.  Otherwise the search starts at `root`
+// with best from the primitive in q.best_slot (the previous step's nearest,
+// -1 for none), which bounds it from the first node on.  Returns false when
+// the query is already complete.
 template <int DIM>
-__device__ __forceinline__ void query_begin(QueryF &q, const MeshView &m, int32_t root, float cf,
-                                            float px, float py, float pz) {
-    q.best = q.best_slot >= 0 ? leaf_d2f<DIM>(m, q.best_slot, px, py, pz) : INFINITY;
+__device__ __forceinline__ bool query_begin(QueryF &q, const MeshView &m, int32_t root,
+                                            int domain, float cf, float px, float py, float pz) {
     q.lb = INFINITY;
     q.cf = cf;
     q.sp = 0;
+    q.indirect = false;
+    if (m.grids) {
+        const GridInfo G = m.grids[domain];
+        const float fx = (px - G.ox) * G.inv_h, fy = (py - G.oy) * G.inv_h;
+        const float fz = DIM == 3 ? (pz - G.oz) * G.inv_h : 0.5f;
+        const int ix = (int)floorf(fx), iy = (int)floorf(fy), iz = (int)floorf(fz);
+        if (ix >= 0 && iy >= 0 && iz >= 0 && ix < G.nx && iy < G.ny && iz < G.nz) {
+            const GridCellF C = m.cells[G.cell_off + (ix * G.ny + iy) * G.nz + iz];
+            if (C.code != 0u) {
+                q.indirect = true;
+                q.lslot = (int32_t)(C.code >> 5);
+                q.lcount = (int32_t)(C.code & 31u);
+                q.best = INFINITY;
+                return true;
+            }
+            if (cf < 1.0f) {
+                const float dx = px - (G.ox + ((float)ix + 0.5f) * G.h);
+                const float dy = py - (G.oy + ((float)iy + 0.5f) * G.h);
+                const float dz = DIM == 3 ? pz - (G.oz + ((float)iz + 0.5f) * G.h) : 0.0f;
+                const float t = sqrtf(__builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz)));
+                const float lb = C.dc - t;
+                const float ub = __builtin_fmaf(C.dc, 1.0f + 1.2e-7f, t);
+                if (lb > 0.0f && lb * lb >= cf * (ub * ub)) {
+                    q.best = ub * ub;
+                    q.lb = lb * lb;
+                    return false;
+                }
+            }
+        }
+    }
+    q.best = q.best_slot >= 0 ? leaf_d2f<DIM>(m, q.best_slot, px, py, pz) : INFINITY;
     query_enter(q, root);
+    return true;
 }
 
 // One unit.  Returns false when the query is complete.
@@ -741,8 +806,9 @@ __device__ __forceinline__ bool query_step(QueryF &q, const MeshView &m, float p
                                            const void *cache, int n_cached) {
     bool pop;
     if (q.lcount > 0) {
-        const float d = leaf_d2f<DIM>(m, q.lslot, px, py, pz);
-        if (d < q.best) { q.best = d; q.best_slot = q.lslot; }
+        const int32_t slot = q.indirect ? m.glist[q.lslot] : q.lslot;
+        const float d = leaf_d2f<DIM>(m, slot, px, py, pz);
+        if (d < q.best) { q.best = d; q.best_slot = slot; }
         ++q.lslot;
         --q.lcount;
         pop = q.lcount == 0;

@@ -206,6 +206,7 @@ struct DomainView {
     double tol;        // boundary tolerance 1e-10 max(1, extent) (runner.hpp:86-88)
     int64_t out_off;   // first output index of this domain's points
     float shrink_abs;  // absolute margin of FP32 radii in this domain
+    int index;         // domain index
 };
 
 __device__ __forceinline__ double grid_coordinate(double lo, double hi, int index, int count) {
@@ -277,6 +278,8 @@ __global__ __launch_bounds__(kBlock) void k_classify(MeshView m, EqView e, GridS
     S.root64 = dom.root64;
     S.root32 = dom.root32;
     S.shrink_abs = dom.shrink_abs;
+    S.domain = dom.index;
+    S.pad = 0;
     S.bid = (DIM == 2 && q.prim >= 0) ? m.prim_bid[q.prim] : -1;
     S.key0 = (uint32_t)key;
     S.key1 = (uint32_t)(key >> 32);
@@ -288,13 +291,92 @@ __global__ __launch_bounds__(kBlock) void k_classify(MeshView m, EqView e, GridS
 }
 
 // --------------------------------------------------------------------------
+// distance grid build: FP64 reference search from each cell centre (formed in
+// FP32 exactly as the walker forms it), then the primitives within
+// dc + 2 hd of the centre, counted (phase 0) and written (phase 1).
+// --------------------------------------------------------------------------
+template <int DIM>
+__device__ int range_collect(const MeshView &m, int root, double cx, double cy, double cz,
+                             double r2, int *stk, int stride, int32_t *out, int cap) {
+    int sp = 1;
+    stk[0] = root;
+    int count = 0;
+    while (sp > 0) {
+        const int curr = stk[(--sp) * stride];
+        int cnt, idx;
+        if constexpr (DIM == 3) {
+            const NodeD3 n = m.nd3[curr];
+            if (box_d2_3(n, cx, cy, cz) > r2) continue;
+            cnt = n.count; idx = n.idx;
+        } else {
+            const NodeD2 n = m.nd2[curr];
+            if (box_d2_2(n, cx, cy) > r2) continue;
+            cnt = n.count; idx = n.idx;
+        }
+        if (cnt > 0) {
+            for (int i = 0; i < cnt; ++i) {
+                const int slot = idx + i;
+                double nx, ny, nz, s, t, d;
+                if constexpr (DIM == 3) d = eberly_d(m.trid[slot], cx, cy, cz, nx, ny, nz, s, t);
+                else d = npq_seg_d(m.segd[slot], cx, cy, nx, ny, s);
+                if (d <= r2) {
+                    if (count >= cap) return cap + 1;
+                    if (out) out[count] = slot;
+                    ++count;
+                }
+            }
+        } else {
+            stk[(sp++) * stride] = idx;       // right
+            stk[(sp++) * stride] = curr + 1;  // left
+        }
+    }
+    return count;
+}
+
+template <int DIM>
+__global__ __launch_bounds__(kBlock) void k_grid_build(MeshView m, GridInfo G, int root64,
+                                                       double pad, int phase, GridCellF *cells,
+                                                       int32_t *counts, double *rl2,
+                                                       const int64_t *offsets, int32_t *glist) {
+    extern __shared__ int lds_g[];
+    int *stk = lds_g + threadIdx.x;
+    const int n = G.nx * G.ny * G.nz;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int ix = i / (G.ny * G.nz), iy = (i / G.nz) % G.ny, iz = i % G.nz;
+    const double cx = (double)(G.ox + ((float)ix + 0.5f) * G.h);
+    const double cy = (double)(G.oy + ((float)iy + 0.5f) * G.h);
+    const double cz = DIM == 3 ? (double)(G.oz + ((float)iz + 0.5f) * G.h) : 0.0;
+    const int c = G.cell_off + i;
+    if (phase == 0) {
+        const NpqD q = DIM == 2 ? npq_ref_2d(m, root64, cx, cy, stk, blockDim.x)
+                                : npq_ref_3d(m, root64, cx, cy, cz, stk, blockDim.x);
+        const double dc = sqrt(q.d2);
+        float f = (float)dc;
+        if ((double)f > dc) f = nextafterf(f, 0.0f);
+        const double rl = (dc + pad) * (1.0 + 1e-6);
+        const int cnt = range_collect<DIM>(m, root64, cx, cy, cz, rl * rl, stk, blockDim.x,
+                                           nullptr, kGridListMax);
+        cells[c].dc = f;
+        cells[c].code = 0u;
+        counts[c] = cnt <= kGridListMax ? cnt : 0;
+        rl2[c] = rl * rl;
+    } else if (counts[c] > 0) {
+        const int64_t off = offsets[c];
+        const int cnt = range_collect<DIM>(m, root64, cx, cy, cz, rl2[c], stk, blockDim.x,
+                                           glist + off, kGridListMax);
+        cells[c].code = ((uint32_t)off << 5) | (uint32_t)cnt;
+    }
+}
+
+// --------------------------------------------------------------------------
 // closest boundary point of a point list: bvh_npq (include/wos/bvh.hpp:38-39)
 // per lane.  FP64 runs the reference traversal; FP32 runs the walker's query
 // (pruning factor cf) and projects onto the primitive it found in FP64.
 // --------------------------------------------------------------------------
 template <int DIM>
 __global__ __launch_bounds__(kBlock) void k_closest(MeshView m, int32_t root64, int32_t root32,
-                                                    int fp32, float cf, const double *pts,
+                                                    int domain, int fp32, float cf, const double *pts,
                                                     int64_t n, double *dist, double *lower,
                                                     double *nearest, int32_t *prim) {
     extern __shared__ __align__(16) unsigned char lds_c[];
@@ -315,16 +397,21 @@ __global__ __launch_bounds__(kBlock) void k_closest(MeshView m, int32_t root64, 
         const float fx = (float)x, fy = (float)y, fz = (float)z;
         QueryF q{};
         q.best_slot = -1;
-        query_begin<DIM>(q, m, root32, cf, fx, fy, fz);
-        while (query_step<DIM>(q, m, fx, fy, fz, stk, blockDim.x, nullptr, 0)) {
-        }
+        bool more = query_begin<DIM>(q, m, root32, domain, cf, fx, fy, fz);
+        while (more) more = query_step<DIM>(q, m, fx, fy, fz, stk, blockDim.x, nullptr, 0);
         dist[i] = sqrt((double)q.best);
         lower[i] = sqrt((double)fminf(q.best, q.lb));
         const int slot = q.best_slot;
-        prim[i] = m.slot_prim[slot];
-        double s, t;
-        if (DIM == 2) npq_seg_d(m.segd[slot], x, y, nx, ny, s);
-        else eberly_d(m.trid[slot], x, y, z, nx, ny, nz, s, t);
+        if (slot >= 0) {
+            prim[i] = m.slot_prim[slot];
+            double s, t;
+            if (DIM == 2) npq_seg_d(m.segd[slot], x, y, nx, ny, s);
+            else eberly_d(m.trid[slot], x, y, z, nx, ny, nz, s, t);
+        } else {
+            // the distance grid's bound ended the search: no primitive
+            prim[i] = -1;
+            nx = ny = nz = NAN;
+        }
     }
     nearest[i * DIM] = nx;
     nearest[i * DIM + 1] = ny;
@@ -682,9 +769,13 @@ struct PersistArgs {
     int32_t *steps_out;
 };
 
+// Forward walks: at least 4 wavefronts per SIMD (at most 128 VGPRs), which
+// every forward variant fits without spilling (more wavefronts measured no
+// faster).  Adjoint replays carry the FP64 parameter derivatives and may use
+// up to 256 VGPRs rather than spill them.
 template <int DIM, int SMODE, bool SCR, bool ADJ>
-__global__ __launch_bounds__(kBlock) void k_walk_fp32(MeshView m, EqView e, PersistArgs a,
-                                                      AdjOut adj) {
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(ADJ ? 2 : 4, 8)))
+void k_walk_fp32(MeshView m, EqView e, PersistArgs a, AdjOut adj) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     // stage the top of the tree once per block
     {
@@ -706,7 +797,7 @@ __global__ __launch_bounds__(kBlock) void k_walk_fp32(MeshView m, EqView e, Pers
     float px = 0, py = 0, pz = 0, R = 0, weight = 1.0f, margin = 0.0f;
     double sample = 0.0;
     int steps = 0;
-    int32_t root = 0;
+    int32_t root = 0, dom = 0;
     uint32_t w = 0, k0 = 0, k1 = 0;
     size_t out_index = 0;
     QueryF q{};
@@ -725,8 +816,7 @@ __global__ __launch_bounds__(kBlock) void k_walk_fp32(MeshView m, EqView e, Pers
             R = sqrtf(exact ? q.best : q.lb) * a.shrink_rel - margin;
             const bool stop = !(R > a.eps_f) || steps >= a.max_steps;
             if (stop && !exact) {
-                query_begin<DIM>(q, m, root, 1.0f, px, py, pz);
-                searching = true;
+                searching = query_begin<DIM>(q, m, root, dom, 1.0f, px, py, pz);
             } else if (stop) {
                 // terminal value in FP64: nearest point on the nearest primitive
                 const bool hit = R > a.eps_f;
@@ -776,6 +866,7 @@ __global__ __launch_bounds__(kBlock) void k_walk_fp32(MeshView m, EqView e, Pers
                     steps = 0;
                     q.best_slot = S.slot;
                     root = S.root32;
+                    dom = S.domain;
                     k0 = S.key0;
                     k1 = S.key1;
                     out_index = (size_t)item;
@@ -829,8 +920,7 @@ __global__ __launch_bounds__(kBlock) void k_walk_fp32(MeshView m, EqView e, Pers
                 pz = __builtin_fmaf(R, z, pz);
             }
             ++steps;
-            query_begin<DIM>(q, m, root, a.cf, px, py, pz);
-            searching = true;
+            searching = query_begin<DIM>(q, m, root, dom, a.cf, px, py, pz);
         }
         if (__ballot(active) == 0) {
             if (exhausted) break;
@@ -983,13 +1073,16 @@ struct wos_ctx {
     bool has_mesh = false;
     DevBuf nd2, nd3, segd, trid, slot_prim, prim_bid, prims, verts, nf2, nf3, segf, seg_inv, trif;
     DevBuf cq_out, cq_prim;
+    DevBuf g_info, g_cells, g_list;   // distance grid
+    int grid_built_cells = 0;         // target cells per domain of the built grid (0: none)
+    int64_t grid_list_len = 0;
     DevBuf plist_in, loc, starts, amb, bvalue, inside, vals, steps, mean, m2, count, ssum, hits,
         counter, vv, pid_all, r_pid, r_wbeg, r_off, coef, scal_sum, grad_v, grad_b, dl_du,
         adj_scal, adj_vw, adj_prim;
     int last_rounds = 0;
     MeshView view{};
     hipEvent_t ev[6]{};
-    wos_tuning tune{256, -1, 256, 0, 16, 0.0f};
+    wos_tuning tune{256, -1, 64, 0, 32, 0.1f, 1 << 22};
     size_t lds_per_cu = 65536;
     size_t lds_per_block = 65536;
     int last_threads = 0;
@@ -997,6 +1090,98 @@ struct wos_ctx {
 };
 
 namespace {
+
+// Build the distance grid of every domain for the FP32 walker (once per mesh
+// and grid size) and point the kernel view at it; grid_cells == 0 turns it
+// off.  Cubic cells of size h = (bbox volume / grid_cells)^(1/dim) over the
+// domain's bounding box padded by half a cell.
+hipError_t ensure_grid(wos_ctx *c) {
+    const int target = c->tune.grid_cells;
+    if (target <= 0) {
+        c->view.grids = nullptr; c->view.cells = nullptr; c->view.glist = nullptr;
+        return hipSuccess;
+    }
+    if (c->grid_built_cells != target) {
+        const int dim = c->mesh.dim;
+        const int nd = (int)c->L.domains.size();
+        std::vector<GridInfo> info(nd);
+        int64_t total = 0;
+        for (int d = 0; d < nd; ++d) {
+            const DomainInfo &D = c->L.domains[d];
+            double ext[3], emax = 0.0;
+            for (int k = 0; k < dim; ++k) {
+                ext[k] = D.bounds[2 * k + 1] - D.bounds[2 * k];
+                emax = std::max(emax, ext[k]);
+            }
+            if (!(emax > 0.0)) emax = 1.0;
+            double vol = 1.0;
+            for (int k = 0; k < dim; ++k) vol *= std::max(ext[k], 1e-3 * emax);
+            const double h = std::pow(vol / (double)target, 1.0 / dim);
+            GridInfo &G = info[d];
+            int32_t nk[3] = {1, 1, 1};
+            for (int k = 0; k < dim; ++k)
+                nk[k] = (int32_t)std::min<double>(std::floor(ext[k] / h) + 2.0, 1 << 15);
+            G.nx = nk[0]; G.ny = nk[1]; G.nz = nk[2];
+            G.h = (float)h;
+            G.inv_h = 1.0f / G.h;
+            G.ox = (float)(D.bounds[0] - 0.5 * h);
+            G.oy = (float)(D.bounds[2] - 0.5 * h);
+            G.oz = dim == 3 ? (float)(D.bounds[4] - 0.5 * h) : 0.0f;
+            G.hd = (float)(0.5 * h * std::sqrt((double)dim));
+            G.cell_off = (int32_t)total;
+            total += (int64_t)G.nx * G.ny * G.nz;
+            if (total > (int64_t)1 << 30) return hipErrorOutOfMemory;
+        }
+        DevBuf counts, rl2, offs;
+        hipError_t e;
+        if ((e = c->g_cells.reserve((size_t)total * sizeof(GridCellF))) != hipSuccess) return e;
+        if ((e = counts.reserve((size_t)total * sizeof(int32_t))) != hipSuccess) return e;
+        if ((e = rl2.reserve((size_t)total * sizeof(double))) != hipSuccess) return e;
+        const size_t lds = (size_t)kBlock * (c->L.depth + 2) * sizeof(int);
+        MeshView v = c->view;
+        for (int phase = 0; phase < 2; ++phase) {
+            if (phase == 1) {
+                std::vector<int32_t> cnt((size_t)total);
+                if ((e = hipMemcpyAsync(cnt.data(), counts.p, cnt.size() * sizeof(int32_t),
+                                        hipMemcpyDeviceToHost, c->stream)) != hipSuccess) return e;
+                if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) return e;
+                std::vector<int64_t> off((size_t)total);
+                int64_t run = 0;
+                for (size_t i = 0; i < cnt.size(); ++i) { off[i] = run; run += cnt[i]; }
+                if (run >= ((int64_t)1 << 27)) return hipErrorOutOfMemory;
+                c->grid_list_len = run;
+                if ((e = upload(offs, off, c->stream)) != hipSuccess) return e;
+                if ((e = c->g_list.reserve((size_t)std::max<int64_t>(run, 1) * sizeof(int32_t))) != hipSuccess)
+                    return e;
+            }
+            for (int d = 0; d < nd; ++d) {
+                const GridInfo &G = info[d];
+                const DomainInfo &D = c->L.domains[d];
+                const int n = G.nx * G.ny * G.nz;
+                const double pad = 2.0 * (0.5 * (double)G.h * std::sqrt((double)dim) * 1.0001 +
+                                          64.0 * std::ldexp(1.0, -24) * D.coord_scale);
+                const int blocks = (n + kBlock - 1) / kBlock;
+                if (dim == 2)
+                    hipLaunchKernelGGL(k_grid_build<2>, dim3(blocks), dim3(kBlock), lds, c->stream, v,
+                                       G, D.node_off, pad, phase, c->g_cells.as<GridCellF>(),
+                                       counts.as<int32_t>(), rl2.as<double>(), offs.as<int64_t>(),
+                                       c->g_list.as<int32_t>());
+                else
+                    hipLaunchKernelGGL(k_grid_build<3>, dim3(blocks), dim3(kBlock), lds, c->stream, v,
+                                       G, D.node_off, pad, phase, c->g_cells.as<GridCellF>(),
+                                       counts.as<int32_t>(), rl2.as<double>(), offs.as<int64_t>(),
+                                       c->g_list.as<int32_t>());
+                if ((e = hipGetLastError()) != hipSuccess) return e;
+            }
+        }
+        if ((e = upload(c->g_info, info, c->stream)) != hipSuccess) return e;
+        c->grid_built_cells = target;
+    }
+    c->view.grids = c->g_info.as<GridInfo>();
+    c->view.cells = c->g_cells.as<GridCellF>();
+    c->view.glist = c->g_list.as<int32_t>();
+    return hipSuccess;
+}
 
 template <typename K>
 hipError_t launch_classify(wos_ctx *c, K kernel, const EqView &ev, const GridSpec &g,
@@ -1022,6 +1207,8 @@ hipError_t launch_walk(wos_ctx *c, const EqView &ev, int precision, const RoundV
         *blocks_out = blocks;
         return hipGetLastError();
     }
+    hipError_t ge = ensure_grid(c);
+    if (ge != hipSuccess) return ge;
     constexpr int SMODE = SRC ? 1 : 0;
     const bool green = ev.source_mode == WOS_SOURCE_GREEN && SRC;
     const void *kfn = green ? (const void *)k_walk_fp32<DIM, 2, SCR, ADJ>
@@ -1216,7 +1403,7 @@ int solve(wos_ctx *c, const std::vector<Part> &parts, const wos_eq_desc *eq,
             // conservative FP32 radius margin: 32 ulp of the domain's largest coordinate
             DomainView dv{D.node_off, D.root_ref, (int)D.prim_off, (int)(D.prim_off + D.n_prims),
                           part_tol[pi], off,
-                          (float)(32.0 * std::ldexp(1.0, -24) * D.coord_scale)};
+                          (float)(32.0 * std::ldexp(1.0, -24) * D.coord_scale), p.domain};
             const double *dp = p.points ? c->plist_in.as<double>() + poff * dim : nullptr;
             hipError_t le = dim == 2 ? launch_classify(c, k_classify<2>, ev, g, dv, dp, p.count,
                                                        prm->seed, prm->max_ray_attempts)
@@ -1637,6 +1824,7 @@ int wos_ctx_set_domains(wos_ctx *c, int dim, int n_domains, const double *const 
     c->mesh = std::move(all);
     c->L = std::move(L);
     c->has_mesh = true;
+    c->grid_built_cells = 0;
     return WOS_OK;
 }
 
@@ -1691,6 +1879,8 @@ int wos_ctx_set_tuning(wos_ctx *c, const wos_tuning *t) {
         return fail(WOS_ERR_INPUT, "lds_nodes must be >= -1 and blocks_per_cu >= 0");
     if (t->min_ready < 0 || t->min_ready > 64) return fail(WOS_ERR_INPUT, "min_ready must be in 0..64");
     if (!(t->relax >= 0.0f && t->relax < 0.5f)) return fail(WOS_ERR_INPUT, "relax must be in [0, 0.5)");
+    if (t->grid_cells < 0 || t->grid_cells > (1 << 26))
+        return fail(WOS_ERR_INPUT, "grid_cells must be in 0..2^26");
     c->tune = *t;
     return WOS_OK;
 }
@@ -1760,13 +1950,14 @@ int wos_closest_points(wos_ctx *c, int domain, const double *points, int64_t n, 
     const size_t lds = (size_t)kBlock * std::max((c->L.depth + 2) * sizeof(int),
                                                  (c->L.depth + 1) * sizeof(StackEntry));
     const int fp32 = precision == WOS_FP32_PERSISTENT;
+    if (fp32) HIPX(ensure_grid(c));
     if (dim == 2)
         hipLaunchKernelGGL(k_closest<2>, dim3(blocks), dim3(kBlock), lds, c->stream, c->view,
-                           D.node_off, D.root_ref, fp32, cf, c->plist_in.as<double>(), n, d_dist,
+                           D.node_off, D.root_ref, domain, fp32, cf, c->plist_in.as<double>(), n, d_dist,
                            d_low, d_near, c->cq_prim.as<int32_t>());
     else
         hipLaunchKernelGGL(k_closest<3>, dim3(blocks), dim3(kBlock), lds, c->stream, c->view,
-                           D.node_off, D.root_ref, fp32, cf, c->plist_in.as<double>(), n, d_dist,
+                           D.node_off, D.root_ref, domain, fp32, cf, c->plist_in.as<double>(), n, d_dist,
                            d_low, d_near, c->cq_prim.as<int32_t>());
     HIPX(hipGetLastError());
     HIPX(hipMemcpyAsync(dist, d_dist, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));

@@ -112,6 +112,7 @@ class Tuning(C.Structure):
         ("blocks_per_cu", C.c_int32),
         ("min_ready", C.c_int32),
         ("relax", C.c_float),
+        ("grid_cells", C.c_int32),
     ]
 
 

@@ -98,9 +98,10 @@ class Context:
             res.append(r)
         return res, self._summary(summ)
 
-    def set_tuning(self, block_threads=256, lds_nodes=-1, chunk=256, blocks_per_cu=0,
-                   min_ready=16, relax=0.0):
-        t = capi.Tuning(block_threads, lds_nodes, chunk, blocks_per_cu, min_ready, relax)
+    def set_tuning(self, block_threads=256, lds_nodes=-1, chunk=64, blocks_per_cu=0,
+                   min_ready=32, relax=0.1, grid_cells=1 << 22):
+        t = capi.Tuning(block_threads, lds_nodes, chunk, blocks_per_cu, min_ready, relax,
+                        grid_cells)
         capi.check(self.lib, self.lib.wos_ctx_set_tuning(self.h, C.byref(t)))
 
     def get_tuning(self):
