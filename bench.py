@@ -11,7 +11,7 @@ Grid pts/s at SE <= 1e-3 = #inside / sum_p n_p * t_walk,p with
 n_p = ceil(var_p / 1e-6) from the FP64 run (shared by every form) and
 t_walk,p = (T / total steps) * mean_steps_p of the form being measured.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-adjoint]
        python bench.py --fp32-only --steps 1 --warmup 0   (walker launches only,
                                                           for counter runs)
        python bench.py --copy-calibration 1024            (one 1024 MiB fp32
@@ -103,6 +103,74 @@ def copy_calibration(mib):
     print(json.dumps({"copy_calibration_bytes_read": n * 4, "copy_calibration_bytes_written": n * 4}))
 
 
+# FP32 walker records read per query unit (bytes): a grid cell (GridCellF),
+# a cell-list entry (GridEntry), a primitive (TriF 64 / SegF + 1/|v|^2 20) and
+# an internal node (NodeF3 64 / NodeF2 48); per walk: its start record (120),
+# value + steps (12), the FP64 terminal primitive (TriD 128 / SegD 32) and
+# its slot -> primitive id (4).
+RECORDS = {3: dict(cell=8, entry=8, prim=64, node=64, walk=120 + 12 + 128 + 4),
+           2: dict(cell=8, entry=8, prim=20, node=48, walk=120 + 12 + 32 + 4)}
+
+
+def walker_bytes(s, dim):
+    """Algorithmic bytes of one FP32 walker launch from its query-unit counts:
+    every query reads its grid cell; a list scan reads each entry and its
+    primitive; a tree query reads its start primitive, each opened node and
+    each leaf primitive."""
+    r = RECORDS[dim]
+    tree_q = s["q_queries"] - s["q_grid_bound"] - s["q_list_scans"]
+    return (s["q_queries"] * r["cell"] + s["q_list_entries"] * r["entry"]
+            + s["q_list_prims"] * r["prim"]
+            + s["q_nodes"] * r["node"] + (s["q_leaf_prims"] + tree_q) * r["prim"]
+            + s["total_walks"] * r["walk"])
+
+
+def torus_probes(n, seed):
+    """n points uniform inside the torus R = 1, r = 0.35 (rejection in the box)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    while sum(len(o) for o in out) < n:
+        p = rng.uniform([-1.35, -1.35, -0.35], [1.35, 1.35, 0.35], size=(4 * n, 3))
+        q = (np.hypot(p[:, 0], p[:, 1]) - 1.0) ** 2 + p[:, 2] ** 2 < 0.35 ** 2
+        out.append(p[q])
+    return np.concatenate(out)[:n]
+
+
+def adjoint_bench(Context, meshgen, problems, precision, walks=2048):
+    """Differentiable 3D Laplace on the torus R = 1, r = 0.35 (200 x 100 quads,
+    40 000 triangles), per-vertex values v ~ N(0, 1) (seed 3), 10 000 probes,
+    loss 0.5 sum (u - t)^2 with t from v + 0.1 N(0, 1): forward solve and
+    adjoint replay (dLoss/dv) at equal walks per point."""
+    V, F, _ = meshgen.torus(1.0, 0.35, 200, 100)
+    rng = np.random.default_rng(3)
+    v = rng.normal(size=len(V))
+    vt = v + 0.1 * rng.normal(size=len(V))
+    pts = torus_probes(10000, 3)
+    ctx = Context(0)
+    ctx.set_mesh(V, F)
+    p = Context.params(walks, 1e-3, 3, precision=precision)
+    target = ctx.solve_points(pts, problems.laplace_vertex_values(vt), p)["mean"]
+    eq = problems.laplace_vertex_values(v)
+    ctx.solve_points(pts, eq, p)  # warm-up
+    t0 = time.perf_counter()
+    fwd = ctx.solve_points(pts, eq, p)
+    t_fwd = time.perf_counter() - t0
+    dl = np.nan_to_num(fwd["mean"] - target)
+    t0 = time.perf_counter()
+    out, gv, _ = ctx.adjoint_points(pts, eq, p, dl)
+    t_adj = time.perf_counter() - t0
+    n_walks = out["summary"]["total_walks"]
+    return dict(workload="torus 40000 tris, 10000 probes, %d walks/point, eps 1e-3, seed 3, "
+                         "per-vertex Dirichlet values" % walks,
+                precision="fp32" if precision else "fp64",
+                loss=float(0.5 * np.sum(dl ** 2)),
+                forward_ms=t_fwd * 1e3, adjoint_ms=t_adj * 1e3,
+                adjoint_over_forward=t_adj / t_fwd,
+                walks_per_s=n_walks / out["summary"]["walk_kernel_seconds"],
+                walks_per_s_end_to_end=n_walks / t_adj,
+                grad_norm=float(np.linalg.norm(gv)))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -110,6 +178,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--walks", type=int, default=4096)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--fp32-only", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB")
     args = ap.parse_args()
@@ -150,6 +219,10 @@ def main():
                           "launch_blocks": s32["launch_blocks"],
                           "launch_threads": s32["launch_threads"]}))
         return
+    # query-unit counts of the same solve (an extra, untimed launch)
+    pc = Context.params(nw, eps, seed, precision=WOS_FP32_PERSISTENT, count_units=1)
+    sc = ctx.solve_grid(*grid, eq, pc)["summary"]
+    assert sc["total_steps"] == s32["total_steps"]
 
     t0 = time.perf_counter()
     r64 = ctx.solve_grid(*grid, eq, p64)
@@ -162,17 +235,21 @@ def main():
     v32 = pts_per_s_at_se(ins.sum(), var_ref, r32["mean_steps"][ins], t32, s32["total_steps"])
     v64 = pts_per_s_at_se(ins.sum(), var_ref, r64["mean_steps"][ins], t64, s64["total_steps"])
 
-    counts_path = os.path.join(ROOT, "profiles", "algorithmic_counts.json")
-    with open(counts_path) as f:
-        counts = json.load(f)["icosphere4_screened_32cubed"]
     kern_s = float(np.mean(walk_k))
-    achieved = s32["total_steps"] * counts["B_step_f32"] / kern_s / 1e9
+    launch_bytes = walker_bytes(sc, 3)
+    achieved = launch_bytes / kern_s / 1e9
+    with open(os.path.join(ROOT, "profiles", "algorithmic_counts.json")) as f:
+        ref_counts = json.load(f)["icosphere4_screened_32cubed"]
+    ref_achieved = s32["total_steps"] * ref_counts["B_step_f32"] / kern_s / 1e9
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "walk_fp32_icosphere4_traffic.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
+    steps = s32["total_steps"]
 
+    z = np.abs(r32["mean"][ins] - r64["mean"][ins]) / np.maximum(
+        np.sqrt(r32["standard_error"][ins] ** 2 + r64["standard_error"][ins] ** 2), 1e-300)
     d = np.abs(r32["mean"][ins] - r64["mean"][ins])
     line = {
         "metric": METRIC,
@@ -189,20 +266,32 @@ def main():
         "data": "synthetic",
         "config": {"workload": WORKLOAD, "inside_points": int(ins.sum()), "walks_per_point": nw,
                    "accumulation": "f64 (Welford, terminal projection, boundary value)",
-                   "parallelism": "single GPU"},
+                   "tuning": ctx.get_tuning(), "parallelism": "single GPU"},
         "fp64_plain": {"value": v64, "ms_per_solve": t64 * 1e3,
                        "walk_kernel_ms": s64["walk_kernel_seconds"] * 1e3,
                        "steps_per_s": s64["total_steps"] / t64},
         "fp32_over_fp64": v32 / v64,
         "walks_per_s": s32["total_walks"] / t32,
         "steps_per_s": s32["total_steps"] / t32,
-        "fp32_vs_fp64_mean_abs_diff": {"max": float(d.max()), "mean": float(d.mean())},
+        "fp32_vs_fp64": {"max_abs_diff": float(d.max()), "mean_abs_diff": float(d.mean()),
+                         "max_z": float(z.max()), "mean_z2": float(np.mean(z ** 2)),
+                         "mean_steps_ratio": float(np.mean(r32["mean_steps"][ins]) /
+                                                   np.mean(r64["mean_steps"][ins]))},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_walk_fp32<3,true,true>",
+                     "kernel": "k_walk_fp32<3,1,true,false>",
                      "kernel_ms": kern_s * 1e3,
-                     "bytes_per_step": counts["B_step_f32"]},
+                     "bytes_per_launch": launch_bytes,
+                     "bytes_per_step": launch_bytes / steps,
+                     "units_per_step": {k: sc[k] / steps for k in
+                                        ("q_queries", "q_grid_bound", "q_list_scans",
+                                         "q_list_entries", "q_list_prims", "q_nodes",
+                                         "q_leaf_prims")},
+                     "reference_layout_bytes_per_step": ref_counts["B_step_f32"],
+                     "reference_layout_achieved": ref_achieved},
     }
+    if not args.no_adjoint:
+        line["adjoint"] = adjoint_bench(Context, meshgen, problems, WOS_FP32_PERSISTENT)
     if not args.no_cpu:
         inside_idx = np.nonzero(ins)[0]
         line["cpu_baseline"] = cpu_baseline(V, F, eq, grid, inside_idx, var_ref, eps, seed)

@@ -103,6 +103,8 @@ typedef struct wos_run_params {
     double target_se;            /* > 0: adaptive walks per point until SE <= target */
     int32_t max_walks;           /* adaptive cap per point                         */
     int32_t tune;                /* 0: default launch shape, 1: autotuned shape    */
+    int32_t count_units;         /* 1: count the FP32 walker's query units into the
+                                    summary (wavefront ballots; off by default)    */
 } wos_run_params;
 
 /* Per-point outputs, caller-allocated arrays of n entries (any may be NULL
@@ -137,6 +139,15 @@ typedef struct wos_run_summary {
     int32_t launch_threads;
     int32_t lds_cached_nodes;    /* BVH nodes staged in LDS by the FP32 walker     */
     int32_t rounds;              /* walker launches (adaptive mode: > 1)           */
+    /* FP32 query units, filled when count_units = 1 (else 0): */
+    int64_t q_queries;           /* closest-point queries begun (walk steps and
+                                    exact re-queries)                              */
+    int64_t q_grid_bound;        /* ... answered by a distance-grid cell bound     */
+    int64_t q_list_scans;        /* ... answered by a cell's primitive list        */
+    int64_t q_list_entries;      /* cell-list entries read                         */
+    int64_t q_list_prims;        /* primitives tested from cell lists              */
+    int64_t q_nodes;             /* BVH internal nodes opened                      */
+    int64_t q_leaf_prims;        /* primitives tested from BVH leaves              */
 } wos_run_summary;
 
 /* Launch shape of the FP32 persistent walker (set by the autotuner). */

@@ -42,13 +42,31 @@ def test_version_and_error_string_without_gpu():
     assert isinstance(lib.wos_last_error(), bytes)
 
 
-def test_struct_layouts():
-    # sizes follow the C declaration order with natural alignment
-    assert C.sizeof(capi.EqDesc) == 4 * 4 + 8 * 6 + 8 * capi.WOS_MAX_BND_VALUES + 8 + 8 + 8
-    assert C.sizeof(capi.RunParams) == 4 * 4 + 8 + 8 + 8 + 4 + 4
-    assert C.sizeof(capi.PointStats) == 8 * 8
-    assert capi.EqDesc.alpha.offset == 16
-    assert capi.RunParams.epsilon.offset == 16
+STRUCTS = [("wos_eq_desc", capi.EqDesc), ("wos_run_params", capi.RunParams),
+           ("wos_point_stats", capi.PointStats), ("wos_run_summary", capi.RunSummary),
+           ("wos_tuning", capi.Tuning)]
+
+
+def test_struct_layouts(tmp_path):
+    """The ctypes mirror has the C compiler's size and field offsets for
+    every structure of the header."""
+    import subprocess
+    src = ["#include <stddef.h>", "#include <stdio.h>", f'#include "{HEADER}"', "int main(void) {"]
+    for cname, py in STRUCTS:
+        src.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for f, _ in py._fields_:
+            src.append(f'printf("{cname} {f} %zu\\n", offsetof({cname}, {f}));')
+    src += ["return 0;", "}"]
+    c_file = tmp_path / "layout.c"
+    c_file.write_text("\n".join(src))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", "-o", str(exe), str(c_file)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {tuple(l.split()[:2]): int(l.split()[2]) for l in out if l}
+    for cname, py in STRUCTS:
+        assert got[(cname, "size")] == C.sizeof(py), cname
+        for f, _ in py._fields_:
+            assert got[(cname, f)] == getattr(py, f).offset, (cname, f)
 
 
 def test_null_context_is_an_input_error():

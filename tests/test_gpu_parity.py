@@ -73,8 +73,10 @@ def test_annulus_fp32_tracks_fp64_and_analytic(gpu_ctx):
     gpu_ctx.set_tuning(relax=0.0)
     c = gpu_ctx.solve_grid(64, 64, 1, eq, Context.params(n, 1e-3, 12345,
                                                          precision=WOS_FP32_PERSISTENT))
-    # default FP32 (relaxed queries, distance grid): smaller spheres give
-    # other walks, i.e. an independent estimate of the same expectation
+    # default FP32 (relaxed queries, distance grid): where a query is relaxed
+    # the walk takes a smaller sphere and becomes an independent estimate of
+    # the same expectation (here nearly every grid cell holds a primitive
+    # list, which answers exactly, so most walks stay the FP64 ones)
     gpu_ctx.set_tuning()
     b = gpu_ctx.solve_grid(64, 64, 1, eq, Context.params(n, 1e-3, 12345,
                                                          precision=WOS_FP32_PERSISTENT))
@@ -86,11 +88,13 @@ def test_annulus_fp32_tracks_fp64_and_analytic(gpu_ctx):
     print(f"fp32 exact vs fp64: max|d|={d.max():.3e} mean|d|={d.mean():.3e} "
           f"max d/se={np.max(d / comb):.3f}")
     assert np.all(d <= 3.0 * comb)
-    z = (a["mean"][ins] - b["mean"][ins]) / np.sqrt(a["standard_error"][ins] ** 2 +
-                                                    b["standard_error"][ins] ** 2)
+    comb = np.sqrt(a["standard_error"][ins] ** 2 + b["standard_error"][ins] ** 2)
+    d = a["mean"][ins] - b["mean"][ins]
+    assert np.all(np.abs(d[comb == 0]) <= 1e-12)  # every walk ends at once: both exact
+    z = d[comb > 0] / comb[comb > 0]
     print(f"fp32 default vs fp64: max|z|={np.abs(z).max():.3f} mean z^2={np.mean(z ** 2):.3f}")
     assert np.all(np.abs(z) <= 5.0)
-    assert 0.6 < np.mean(z ** 2) < 1.4
+    assert np.mean(z ** 2) < 1.4
     # exact solution u = ln r / ln 0.25; bias budget: epsilon shell and the 128-gon
     xs = np.linspace(-1, 1, 64)
     X, Y = np.meshgrid(xs, xs, indexing="ij")

@@ -20,12 +20,15 @@ namespace wosx {
 
 // --------------------------------------------------------------------------
 // distance grid (FP32 walker): a uniform grid of cubic cells per domain.
-// Cell record: dc = distance from the cell centre to the boundary, rounded
-// down; code = (first << 5) | count for a cell whose complete list of
-// possibly-nearest primitives (count <= 31 leaf slots at glist[first..]) is
-// stored, 0 otherwise.  The list holds every primitive within dc + 2 hd of
-// the centre (hd = half cell diagonal, padded), which contains the nearest
-// primitive of every point within hd of the centre.
+// Cell record: the distance dc from the cell centre to the boundary as FP32
+// bits with the low 5 mantissa bits cleared (so still a lower bound) and
+// replaced by the length (0..31) of the cell's primitive list, which starts
+// at glist[first].  The list holds every primitive within dc + 2 hd of the
+// centre (hd = half cell diagonal, padded), which contains the nearest
+// primitive of every point within hd of the centre; entries carry their
+// distance from the centre (rounded down) and are sorted by it, so a scan
+// from p can stop at the first entry with dmin - |p - c| >= its best.  A
+// cell whose list would exceed 31 entries stores none (length 0).
 // --------------------------------------------------------------------------
 struct GridInfo {
     float ox, oy, oz;     // corner of cell (0, 0, 0)
@@ -35,8 +38,12 @@ struct GridInfo {
     int32_t cell_off;     // first cell of this domain in the cell array
 };
 struct GridCellF {
-    float dc;
-    uint32_t code;
+    uint32_t dcbits;   // FP32 bits of dc, low 5 bits = list length
+    uint32_t first;    // first list entry
+};
+struct GridEntry {
+    int32_t slot;      // leaf slot of the primitive
+    float dmin;        // its distance from the cell centre, rounded down
 };
 constexpr int kGridListMax = 31;
 
@@ -66,7 +73,7 @@ struct MeshView {
     // distance grid (nullptr when off)
     const GridInfo *grids;     // per domain
     const GridCellF *cells;
-    const int32_t *glist;      // leaf slots of the cell lists
+    const GridEntry *glist;    // cell lists
 };
 
 struct EqView {
@@ -731,6 +738,7 @@ struct QueryF {
     int32_t lslot;    // next leaf primitive (its index in glist when indirect)
     int32_t lcount;   // leaf primitives left
     bool indirect;    // primitives come from a grid cell list
+    float t;          // |p - cell centre| of a list scan
     int32_t sp;       // stack depth
     float best;       // smallest squared primitive distance so far
     float lb;         // smallest skipped box distance
@@ -772,20 +780,24 @@ __device__ __forceinline__ bool query_begin(QueryF &q, const MeshView &m, int32_
         const int ix = (int)floorf(fx), iy = (int)floorf(fy), iz = (int)floorf(fz);
         if (ix >= 0 && iy >= 0 && iz >= 0 && ix < G.nx && iy < G.ny && iz < G.nz) {
             const GridCellF C = m.cells[G.cell_off + (ix * G.ny + iy) * G.nz + iz];
-            if (C.code != 0u) {
+            const float dx = px - (G.ox + ((float)ix + 0.5f) * G.h);
+            const float dy = py - (G.oy + ((float)iy + 0.5f) * G.h);
+            const float dz = DIM == 3 ? pz - (G.oz + ((float)iz + 0.5f) * G.h) : 0.0f;
+            const float t = sqrtf(__builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz)));
+            const uint32_t len = C.dcbits & 31u;
+            if (len != 0u) {
                 q.indirect = true;
-                q.lslot = (int32_t)(C.code >> 5);
-                q.lcount = (int32_t)(C.code & 31u);
+                q.lslot = (int32_t)C.first;
+                q.lcount = (int32_t)len;
+                q.t = t;
                 q.best = INFINITY;
                 return true;
             }
             if (cf < 1.0f) {
-                const float dx = px - (G.ox + ((float)ix + 0.5f) * G.h);
-                const float dy = py - (G.oy + ((float)iy + 0.5f) * G.h);
-                const float dz = DIM == 3 ? pz - (G.oz + ((float)iz + 0.5f) * G.h) : 0.0f;
-                const float t = sqrtf(__builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz)));
-                const float lb = C.dc - t;
-                const float ub = __builtin_fmaf(C.dc, 1.0f + 1.2e-7f, t);
+                // dc lost its low 5 mantissa bits: dc <= true <= dc (1 + 2^-17)
+                const float dc = __uint_as_float(C.dcbits);
+                const float lb = dc - t;
+                const float ub = __builtin_fmaf(dc, 1.0f + 1.0f / 131072.0f, t);
                 if (lb > 0.0f && lb * lb >= cf * (ub * ub)) {
                     q.best = ub * ub;
                     q.lb = lb * lb;
@@ -806,11 +818,22 @@ __device__ __forceinline__ bool query_step(QueryF &q, const MeshView &m, float p
                                            const void *cache, int n_cached) {
     bool pop;
     if (q.lcount > 0) {
-        const int32_t slot = q.indirect ? m.glist[q.lslot] : q.lslot;
-        const float d = leaf_d2f<DIM>(m, slot, px, py, pz);
-        if (d < q.best) { q.best = d; q.best_slot = slot; }
+        int32_t slot = q.lslot;
+        bool skip = false;
+        if (q.indirect) {
+            // sorted by distance from the centre: once an entry's lower
+            // bound reaches the best, so do all that follow
+            const GridEntry E = m.glist[q.lslot];
+            const float lbk = E.dmin - q.t;
+            skip = lbk > 0.0f && lbk * lbk >= q.best;
+            slot = E.slot;
+        }
+        if (!skip) {
+            const float d = leaf_d2f<DIM>(m, slot, px, py, pz);
+            if (d < q.best) { q.best = d; q.best_slot = slot; }
+        }
         ++q.lslot;
-        --q.lcount;
+        q.lcount = skip ? 0 : q.lcount - 1;
         pop = q.lcount == 0;
     } else {
         float dl, dr;

@@ -295,9 +295,17 @@ __global__ __launch_bounds__(kBlock) void k_classify(MeshView m, EqView e, GridS
 // FP32 exactly as the walker forms it), then the primitives within
 // dc + 2 hd of the centre, counted (phase 0) and written (phase 1).
 // --------------------------------------------------------------------------
+__device__ __forceinline__ float round_down_f(double x) {
+    float f = (float)x;
+    if ((double)f > x) f = nextafterf(f, -INFINITY);
+    return f;
+}
+
+// Primitives within sqrt(r2) of c: counted, and when `out` is given written
+// as (slot, distance rounded down) in ascending distance (insertion sort).
 template <int DIM>
 __device__ int range_collect(const MeshView &m, int root, double cx, double cy, double cz,
-                             double r2, int *stk, int stride, int32_t *out, int cap) {
+                             double r2, int *stk, int stride, GridEntry *out, int cap) {
     int sp = 1;
     stk[0] = root;
     int count = 0;
@@ -321,7 +329,14 @@ __device__ int range_collect(const MeshView &m, int root, double cx, double cy, 
                 else d = npq_seg_d(m.segd[slot], cx, cy, nx, ny, s);
                 if (d <= r2) {
                     if (count >= cap) return cap + 1;
-                    if (out) out[count] = slot;
+                    if (out) {
+                        GridEntry e;
+                        e.slot = slot;
+                        e.dmin = round_down_f(sqrt(d));
+                        int j = count;
+                        while (j > 0 && out[j - 1].dmin > e.dmin) { out[j] = out[j - 1]; --j; }
+                        out[j] = e;
+                    }
                     ++count;
                 }
             }
@@ -337,7 +352,7 @@ template <int DIM>
 __global__ __launch_bounds__(kBlock) void k_grid_build(MeshView m, GridInfo G, int root64,
                                                        double pad, int phase, GridCellF *cells,
                                                        int32_t *counts, double *rl2,
-                                                       const int64_t *offsets, int32_t *glist) {
+                                                       const int64_t *offsets, GridEntry *glist) {
     extern __shared__ int lds_g[];
     int *stk = lds_g + threadIdx.x;
     const int n = G.nx * G.ny * G.nz;
@@ -352,20 +367,19 @@ __global__ __launch_bounds__(kBlock) void k_grid_build(MeshView m, GridInfo G, i
         const NpqD q = DIM == 2 ? npq_ref_2d(m, root64, cx, cy, stk, blockDim.x)
                                 : npq_ref_3d(m, root64, cx, cy, cz, stk, blockDim.x);
         const double dc = sqrt(q.d2);
-        float f = (float)dc;
-        if ((double)f > dc) f = nextafterf(f, 0.0f);
         const double rl = (dc + pad) * (1.0 + 1e-6);
         const int cnt = range_collect<DIM>(m, root64, cx, cy, cz, rl * rl, stk, blockDim.x,
                                            nullptr, kGridListMax);
-        cells[c].dc = f;
-        cells[c].code = 0u;
+        cells[c].dcbits = __float_as_uint(round_down_f(dc)) & ~31u;
+        cells[c].first = 0u;
         counts[c] = cnt <= kGridListMax ? cnt : 0;
         rl2[c] = rl * rl;
     } else if (counts[c] > 0) {
         const int64_t off = offsets[c];
         const int cnt = range_collect<DIM>(m, root64, cx, cy, cz, rl2[c], stk, blockDim.x,
                                            glist + off, kGridListMax);
-        cells[c].code = ((uint32_t)off << 5) | (uint32_t)cnt;
+        cells[c].dcbits |= (uint32_t)cnt;
+        cells[c].first = (uint32_t)off;
     }
 }
 
@@ -767,6 +781,7 @@ struct PersistArgs {
     int cache_bytes;     // LDS bytes reserved for them (16 B aligned)
     double *vals;
     int32_t *steps_out;
+    unsigned long long *units;  // 7 query-unit counters, or nullptr
 };
 
 // Forward walks: at least 4 wavefronts per SIMD (at most 128 VGPRs), which
@@ -808,8 +823,12 @@ void k_walk_fp32(MeshView m, EqView e, PersistArgs a, AdjOut adj) {
     // wave-uniform pool of work items
     uint64_t pool_next = 0, pool_end = 0;
     bool exhausted = false;
+    // wave-uniform query-unit counts (a.units != nullptr): queries, grid
+    // bounds, list scans, list primitives, nodes, leaf primitives
+    uint64_t u_q = 0, u_gb = 0, u_ls = 0, u_lp = 0, u_nd = 0, u_pr = 0, u_le = 0;
 
     for (;;) {
+        bool began = false;
         // ---- finished queries: radius, then end of walk or exact re-query
         if (active && !searching) {
             const bool exact = !(q.lb < q.best);
@@ -817,6 +836,7 @@ void k_walk_fp32(MeshView m, EqView e, PersistArgs a, AdjOut adj) {
             const bool stop = !(R > a.eps_f) || steps >= a.max_steps;
             if (stop && !exact) {
                 searching = query_begin<DIM>(q, m, root, dom, 1.0f, px, py, pz);
+                began = true;
             } else if (stop) {
                 // terminal value in FP64: nearest point on the nearest primitive
                 const bool hit = R > a.eps_f;
@@ -921,6 +941,12 @@ void k_walk_fp32(MeshView m, EqView e, PersistArgs a, AdjOut adj) {
             }
             ++steps;
             searching = query_begin<DIM>(q, m, root, dom, a.cf, px, py, pz);
+            began = true;
+        }
+        if (a.units) {
+            u_q += __popcll(__ballot(began));
+            u_gb += __popcll(__ballot(began && !searching));
+            u_ls += __popcll(__ballot(began && searching && q.indirect));
         }
         if (__ballot(active) == 0) {
             if (exhausted) break;
@@ -929,12 +955,33 @@ void k_walk_fp32(MeshView m, EqView e, PersistArgs a, AdjOut adj) {
 
         // ---- advance the queries until enough lanes are ready to step
         for (;;) {
+            if (a.units) {
+                const bool leaf = searching && q.lcount > 0;
+                bool stop_entry = false;
+                if (leaf && q.indirect) {
+                    const float lbk = m.glist[q.lslot].dmin - q.t;
+                    stop_entry = lbk > 0.0f && lbk * lbk >= q.best;
+                }
+                u_nd += __popcll(__ballot(searching && !leaf));
+                u_lp += __popcll(__ballot(leaf && q.indirect));
+                u_le += __popcll(__ballot(stop_entry));
+                u_pr += __popcll(__ballot(leaf && !q.indirect));
+            }
             if (searching)
                 searching = query_step<DIM>(q, m, px, py, pz, stk, stride, cache, a.n_cached);
             if (__ballot(searching) == 0) break;
             const uint64_t ready = __ballot(!searching && (active || !exhausted));
             if (__popcll(ready) >= a.min_ready) break;
         }
+    }
+    if (a.units && lane == 0) {
+        atomicAdd(a.units + 0, (unsigned long long)u_q);
+        atomicAdd(a.units + 1, (unsigned long long)u_gb);
+        atomicAdd(a.units + 2, (unsigned long long)u_ls);
+        atomicAdd(a.units + 3, (unsigned long long)u_lp);
+        atomicAdd(a.units + 4, (unsigned long long)u_nd);
+        atomicAdd(a.units + 5, (unsigned long long)u_pr);
+        atomicAdd(a.units + 6, (unsigned long long)u_le);
     }
 }
 
@@ -1073,6 +1120,8 @@ struct wos_ctx {
     bool has_mesh = false;
     DevBuf nd2, nd3, segd, trid, slot_prim, prim_bid, prims, verts, nf2, nf3, segf, seg_inv, trif;
     DevBuf cq_out, cq_prim;
+    DevBuf units;                     // FP32 query-unit counters
+    bool count_units = false;
     DevBuf g_info, g_cells, g_list;   // distance grid
     int grid_built_cells = 0;         // target cells per domain of the built grid (0: none)
     int64_t grid_list_len = 0;
@@ -1082,7 +1131,7 @@ struct wos_ctx {
     int last_rounds = 0;
     MeshView view{};
     hipEvent_t ev[6]{};
-    wos_tuning tune{256, -1, 64, 0, 32, 0.1f, 1 << 22};
+    wos_tuning tune{256, -1, 64, 0, 48, 0.1f, 1 << 22};
     size_t lds_per_cu = 65536;
     size_t lds_per_block = 65536;
     int last_threads = 0;
@@ -1148,10 +1197,10 @@ hipError_t ensure_grid(wos_ctx *c) {
                 std::vector<int64_t> off((size_t)total);
                 int64_t run = 0;
                 for (size_t i = 0; i < cnt.size(); ++i) { off[i] = run; run += cnt[i]; }
-                if (run >= ((int64_t)1 << 27)) return hipErrorOutOfMemory;
+                if (run >= ((int64_t)1 << 31)) return hipErrorOutOfMemory;
                 c->grid_list_len = run;
                 if ((e = upload(offs, off, c->stream)) != hipSuccess) return e;
-                if ((e = c->g_list.reserve((size_t)std::max<int64_t>(run, 1) * sizeof(int32_t))) != hipSuccess)
+                if ((e = c->g_list.reserve((size_t)std::max<int64_t>(run, 1) * sizeof(GridEntry))) != hipSuccess)
                     return e;
             }
             for (int d = 0; d < nd; ++d) {
@@ -1165,12 +1214,12 @@ hipError_t ensure_grid(wos_ctx *c) {
                     hipLaunchKernelGGL(k_grid_build<2>, dim3(blocks), dim3(kBlock), lds, c->stream, v,
                                        G, D.node_off, pad, phase, c->g_cells.as<GridCellF>(),
                                        counts.as<int32_t>(), rl2.as<double>(), offs.as<int64_t>(),
-                                       c->g_list.as<int32_t>());
+                                       c->g_list.as<GridEntry>());
                 else
                     hipLaunchKernelGGL(k_grid_build<3>, dim3(blocks), dim3(kBlock), lds, c->stream, v,
                                        G, D.node_off, pad, phase, c->g_cells.as<GridCellF>(),
                                        counts.as<int32_t>(), rl2.as<double>(), offs.as<int64_t>(),
-                                       c->g_list.as<int32_t>());
+                                       c->g_list.as<GridEntry>());
                 if ((e = hipGetLastError()) != hipSuccess) return e;
             }
         }
@@ -1179,7 +1228,7 @@ hipError_t ensure_grid(wos_ctx *c) {
     }
     c->view.grids = c->g_info.as<GridInfo>();
     c->view.cells = c->g_cells.as<GridCellF>();
-    c->view.glist = c->g_list.as<int32_t>();
+    c->view.glist = c->g_list.as<GridEntry>();
     return hipSuccess;
 }
 
@@ -1252,6 +1301,7 @@ hipError_t launch_walk(wos_ctx *c, const EqView &ev, int precision, const RoundV
     a.chunk = c->tune.chunk;
     a.vals = c->vals.as<double>();
     a.steps_out = c->steps.as<int32_t>();
+    a.units = c->count_units ? c->units.as<unsigned long long>() : nullptr;
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, c->stream, a.counter, 1);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
@@ -1431,6 +1481,13 @@ int solve(wos_ctx *c, const std::vector<Part> &parts, const wos_eq_desc *eq,
     const int n_in = (int)inside.size();
 
     c->last_rounds = 0;
+    c->count_units = prm->count_units != 0 && prm->precision == WOS_FP32_PERSISTENT;
+    if (c->count_units) {
+        HIPX(c->units.reserve(7 * sizeof(unsigned long long)));
+        hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, c->stream,
+                           c->units.as<unsigned long long>(), 7);
+        HIPX(hipGetLastError());
+    }
     std::vector<double> h_mean(n_in), h_m2(n_in), h_cnt(n_in), h_ss(n_in);
     std::vector<int32_t> h_hits(n_in);
     double walk_s = 0.0, total_walk_s = 0.0;
@@ -1695,6 +1752,15 @@ int solve(wos_ctx *c, const std::vector<Part> &parts, const wos_eq_desc *eq,
     S.launch_blocks = blocks;
     S.launch_threads = prm->precision == WOS_FP32_PERSISTENT ? c->last_threads : kBlock;
     S.lds_cached_nodes = prm->precision == WOS_FP32_PERSISTENT ? c->last_cached : 0;
+    if (c->count_units) {
+        unsigned long long u[7];
+        HIPX(hipMemcpyAsync(u, c->units.p, sizeof(u), hipMemcpyDeviceToHost, c->stream));
+        HIPX(hipStreamSynchronize(c->stream));
+        S.q_queries = (int64_t)u[0]; S.q_grid_bound = (int64_t)u[1]; S.q_list_scans = (int64_t)u[2];
+        S.q_list_entries = (int64_t)u[3]; S.q_list_prims = (int64_t)(u[3] - u[6]);
+        S.q_nodes = (int64_t)u[4]; S.q_leaf_prims = (int64_t)u[5];
+        c->count_units = false;
+    }
     if (summary) *summary = S;
     return WOS_OK;
 }

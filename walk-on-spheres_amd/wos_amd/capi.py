@@ -66,6 +66,7 @@ class RunParams(C.Structure):
         ("target_se", C.c_double),
         ("max_walks", C.c_int32),
         ("tune", C.c_int32),
+        ("count_units", C.c_int32),
     ]
 
 
@@ -101,6 +102,13 @@ class RunSummary(C.Structure):
         ("launch_threads", C.c_int32),
         ("lds_cached_nodes", C.c_int32),
         ("rounds", C.c_int32),
+        ("q_queries", C.c_int64),
+        ("q_grid_bound", C.c_int64),
+        ("q_list_scans", C.c_int64),
+        ("q_list_entries", C.c_int64),
+        ("q_list_prims", C.c_int64),
+        ("q_nodes", C.c_int64),
+        ("q_leaf_prims", C.c_int64),
     ]
 
 

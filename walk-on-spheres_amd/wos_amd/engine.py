@@ -99,7 +99,7 @@ class Context:
         return res, self._summary(summ)
 
     def set_tuning(self, block_threads=256, lds_nodes=-1, chunk=64, blocks_per_cu=0,
-                   min_ready=32, relax=0.1, grid_cells=1 << 22):
+                   min_ready=48, relax=0.1, grid_cells=1 << 22):
         t = capi.Tuning(block_threads, lds_nodes, chunk, blocks_per_cu, min_ready, relax,
                         grid_cells)
         capi.check(self.lib, self.lib.wos_ctx_set_tuning(self.h, C.byref(t)))
@@ -111,7 +111,7 @@ class Context:
 
     @staticmethod
     def params(n_walks, epsilon, seed, max_steps=1000, max_ray_attempts=12,
-               precision=WOS_FP64_PLAIN, target_se=0.0, max_walks=0, tune=0):
+               precision=WOS_FP64_PLAIN, target_se=0.0, max_walks=0, tune=0, count_units=0):
         p = RunParams()
         p.n_walks = n_walks
         p.max_steps = max_steps
@@ -122,6 +122,7 @@ class Context:
         p.target_se = target_se
         p.max_walks = max_walks
         p.tune = tune
+        p.count_units = count_units
         return p
 
     def closest_points(self, points, precision=WOS_FP64_PLAIN, relax=0.0, domain=0):
