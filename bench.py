@@ -14,6 +14,7 @@ t_walk,p = (T / total steps) * mean_steps_p of the form being measured.
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-adjoint]
        python bench.py --fp32-only --steps 1 --warmup 0   (walker launches only,
                                                           for counter runs)
+       python bench.py --config 3 | --config 5             (batched stars, plate)
        python bench.py --copy-calibration 1024            (one 1024 MiB fp32
                                                           read+write kernel, the
                                                           scale for HBM counters)
@@ -171,6 +172,87 @@ def adjoint_bench(Context, meshgen, problems, precision, walks=2048):
                 grad_norm=float(np.linalg.norm(gv)))
 
 
+def config3(Context, meshgen, problems, caps):
+    """Batched uneven 2D Poisson, Green-function source sampling: 16 star
+    domains r = 1 + a cos(k theta) (a ~ U(0.1, 0.4), k ~ U{3..9}, 512..16384
+    segments, meshgen seed 2024), f = exp(-|x|^2/0.1), g = sin 3x cos 2y,
+    256^2 grid each, walks per point adaptive from 64 to 16384 until
+    SE <= 1e-3, FP32, eps 1e-3, seed 11: every round is one walker launch over
+    all domains.  Points reaching SE <= 1e-3 per second, capped points apart."""
+    fam = meshgen.star_family(16, seed=2024, n_min=512, n_max=16384)
+    ctx = Context(0)
+    ctx.set_domains([f["mesh"] for f in fam])
+    eq = problems.star_poisson_gaussian(1)
+    p = Context.params(64, 1e-3, 11, precision=caps.WOS_FP32_PERSISTENT, target_se=SE_TARGET,
+                       max_walks=16384)
+    ctx.solve_grid_batch(16, 16, 1, eq, p)  # builds the distance grids
+    t0 = time.perf_counter()
+    res, tot = ctx.solve_grid_batch(256, 256, 1, eq, p)
+    dt = time.perf_counter() - t0
+    inside = reached = capped = 0
+    per = []
+    for f, r in zip(fam, res):
+        ins = r["location"] == 1
+        ok = ins & (r["standard_error"] <= SE_TARGET)
+        cap = ins & ~ok
+        inside += int(ins.sum()); reached += int(ok.sum()); capped += int(cap.sum())
+        per.append(dict(segments=f["n"], a=round(f["a"], 4), k=f["k"], inside=int(ins.sum()),
+                        reached=int(ok.sum()), capped=int(cap.sum()),
+                        mean_walks=float(r["walks"][ins].mean())))
+    return dict(config=3, workload=config3.__doc__.split(":")[0].strip(),
+                metric="points reaching SE<=1e-3 per s (FP32, batched, adaptive)",
+                value=reached / dt, unit="pts/s", seconds=dt, inside=inside, reached=reached,
+                capped=capped, rounds=tot["rounds"], total_walks=tot["total_walks"],
+                total_steps=tot["total_steps"], steps_per_s=tot["total_steps"] / dt,
+                walk_kernel_s=tot["walk_kernel_seconds"], domains=per)
+
+
+def config5(Context, meshgen, problems, caps):
+    """Welding-style stationary screened Poisson: box plate 120 x 60 x 10 mm,
+    200 316 triangles, alpha = 0.05, f = Q exp(-|x - x0|^2 / sigma^2) with
+    Q = 1, sigma = 3 mm, x0 = (40, 30, 10), uniform ball sampling, g = 0,
+    64 x 32 x 16 grid, 8192 walks per point, eps 0.01, seed 5: FP32 and FP64
+    grid pts/s at SE <= 1e-3 and the FP32 adjoint of L = sum u (gradients
+    with respect to Q, sigma, x0, alpha)."""
+    V, F, _ = meshgen.plate_200k()
+    ctx = Context(0)
+    ctx.set_mesh(V, F)
+    eq = problems.plate_gaussian(alpha=0.05, q=1.0, sigma=3.0, x0=(40.0, 30.0, 10.0))
+    grid, nw, eps, seed = (64, 32, 16), 8192, 0.01, 5
+    p32 = Context.params(nw, eps, seed, precision=caps.WOS_FP32_PERSISTENT)
+    p64 = Context.params(nw, eps, seed, precision=caps.WOS_FP64_PLAIN)
+    ctx.solve_grid(*grid, eq, p32)
+    t0 = time.perf_counter()
+    r32 = ctx.solve_grid(*grid, eq, p32)
+    t32 = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    r64 = ctx.solve_grid(*grid, eq, p64)
+    t64 = time.perf_counter() - t0
+    ins = r64["location"] == 1
+    var = r64["variance"][ins]
+    v32 = pts_per_s_at_se(ins.sum(), var, r32["mean_steps"][ins], t32, r32["summary"]["total_steps"])
+    v64 = pts_per_s_at_se(ins.sum(), var, r64["mean_steps"][ins], t64, r64["summary"]["total_steps"])
+    dl = np.where(ins, 1.0, 0.0)
+    t0 = time.perf_counter()
+    _, _, gp = ctx.adjoint_grid(*grid, eq, p32, dl)
+    tadj = time.perf_counter() - t0
+    # a floor for points where the source underflows FP32 (f ~ 1e-170 far
+    # from x0: both estimates are 0 to 1e-160 and their SEs vanish)
+    floor = 1e-9 * np.abs(r64["mean"][ins]).max()
+    z = np.abs(r32["mean"][ins] - r64["mean"][ins]) / np.maximum(
+        np.sqrt(r32["standard_error"][ins] ** 2 + r64["standard_error"][ins] ** 2), floor)
+    return dict(config=5, workload=config5.__doc__.split(":")[0].strip(),
+                metric="grid pts/s at SE<=1e-3 (FP32 vs FP64), adjoint time",
+                value=v32, unit="grid pts/s", fp64_value=v64, fp32_over_fp64=v32 / v64,
+                inside=int(ins.sum()), fp32_ms=t32 * 1e3, fp64_ms=t64 * 1e3,
+                fp32_steps_per_s=r32["summary"]["total_steps"] / t32,
+                fp32_vs_fp64_max_z=float(z.max()), fp32_vs_fp64_mean_z2=float(np.mean(z ** 2)),
+                adjoint_ms=tadj * 1e3, adjoint_over_forward=tadj / t32,
+                grad={"Q": gp[caps.WOS_GRAD_SRC_VALUE], "sigma": gp[caps.WOS_GRAD_SRC_RADIUS],
+                      "x0": list(gp[caps.WOS_GRAD_SRC_CENTER:caps.WOS_GRAD_SRC_CENTER + 3]),
+                      "alpha": gp[caps.WOS_GRAD_ALPHA]})
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -181,6 +263,8 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--fp32-only", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB")
+    ap.add_argument("--config", type=int, default=2, choices=(2, 3, 5),
+                    help="2: the headline line (default); 3, 5: the batched and plate workloads")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -192,7 +276,14 @@ def main():
         return
 
     from wos_amd import Context, meshgen, problems
+    from wos_amd import capi
     from wos_amd.capi import WOS_FP32_PERSISTENT, WOS_FP64_PLAIN
+    if args.config == 3:
+        print(json.dumps(config3(Context, meshgen, problems, capi)))
+        return
+    if args.config == 5:
+        print(json.dumps(config5(Context, meshgen, problems, capi)))
+        return
 
     V, F, _ = meshgen.icosphere(4)
     eq = problems.sphere_screened_constant(alpha=10.0, f=1.0)

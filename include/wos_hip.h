@@ -102,7 +102,9 @@ typedef struct wos_run_params {
     uint64_t seed;               /* global seed (runner.hpp:163-168)               */
     double target_se;            /* > 0: adaptive walks per point until SE <= target */
     int32_t max_walks;           /* adaptive cap per point                         */
-    int32_t tune;                /* 0: default launch shape, 1: autotuned shape    */
+    int32_t tune;                /* 0: the context's launch shape; 1 (FP32): the
+                                    autotuned one (wos_ctx_autotune), found on
+                                    first use of a problem; results are the same  */
     int32_t count_units;         /* 1: count the FP32 walker's query units into the
                                     summary (wavefront ballots; off by default)    */
 } wos_run_params;
@@ -169,6 +171,8 @@ typedef struct wos_tuning {
     int32_t grid_cells;          /* cells per domain of the FP32 distance grid
                                     (cell lists near the boundary, cell distance
                                     bounds inside); 0 = no grid                    */
+    int32_t grid_list_max;       /* longest cell list kept (1..127); a cell with
+                                    more candidate primitives keeps none           */
 } wos_tuning;
 
 typedef struct wos_ctx wos_ctx;
@@ -241,6 +245,18 @@ int wos_closest_points(wos_ctx *ctx, int domain, const double *points, int64_t n
  * has one CPU code path); used by the autotuner. */
 int wos_ctx_set_tuning(wos_ctx *ctx, const wos_tuning *tuning);
 int wos_ctx_get_tuning(const wos_ctx *ctx, wos_tuning *tuning);
+
+/* Autotune the FP32 walker's launch shape on an Nx x Ny x Nz grid problem
+ * over every domain of the context (the problem wos_solve_grid_batch would
+ * solve): pilot solves of about 16 M walks, coordinate descent over walkers
+ * per block, lanes per walk-step phase, refill chunk and LDS node cache.
+ * These fields never change a result, so a tuned solve is bit-identical to
+ * an untuned one; grid_cells, grid_list_max and relax are kept as set.  The
+ * chosen shape becomes the context's tuning, is written to `chosen` (may be
+ * NULL) and is cached for solves with params->tune = 1, which run this
+ * search themselves on first use of a problem.  No reference counterpart. */
+int wos_ctx_autotune(wos_ctx *ctx, int nx, int ny, int nz, const wos_eq_desc *eq,
+                     const wos_run_params *params, wos_tuning *chosen);
 
 /* Sweep an Nx x Ny x Nz grid over the mesh bounding box (Nz forced to 1 in
  * 2D), classify every point, walk every inside point and write row-major

@@ -123,3 +123,26 @@ def test_tuning_validation(ctx):
     ctx.set_tuning(min_ready=7, relax=0.25, grid_cells=4096)
     t = ctx.get_tuning()
     assert t["min_ready"] == 7 and abs(t["relax"] - 0.25) < 1e-7 and t["grid_cells"] == 4096
+
+
+@pytest.mark.parametrize("name,mesh_fn,eq_fn,grid,eps", CASES[:2], ids=[c[0] for c in CASES[:2]])
+def test_autotune_is_result_neutral(ctx, name, mesh_fn, eq_fn, grid, eps):
+    V, F, B = mesh_fn()
+    ctx.set_mesh(V, F, B)
+    eq = eq_fn()
+    ctx.set_tuning()
+    p = Context.params(64, eps, 5, precision=WOS_FP32_PERSISTENT)
+    base = ctx.solve_grid(*grid, eq, p)
+    chosen = ctx.autotune(*grid, eq, p)
+    print(name, "autotuned", chosen)
+    assert chosen["block_threads"] in (128, 192, 256)
+    assert chosen["grid_cells"] == 1 << 22 and abs(chosen["relax"] - 0.1) < 1e-7
+    assert ctx.get_tuning() == chosen
+    after = ctx.solve_grid(*grid, eq, p)
+    ctx.set_tuning()
+    tuned = ctx.solve_grid(*grid, eq, Context.params(64, eps, 5, precision=WOS_FP32_PERSISTENT,
+                                                     tune=1))
+    for r in (after, tuned):
+        for k in ("mean", "variance", "mean_steps", "location"):
+            assert np.array_equal(base[k], r[k], equal_nan=True), (name, k)
+    assert ctx.get_tuning()["min_ready"] == 48  # a tune=1 solve leaves the context's shape

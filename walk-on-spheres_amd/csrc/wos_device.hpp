@@ -21,14 +21,15 @@ namespace wosx {
 // --------------------------------------------------------------------------
 // distance grid (FP32 walker): a uniform grid of cubic cells per domain.
 // Cell record: the distance dc from the cell centre to the boundary as FP32
-// bits with the low 5 mantissa bits cleared (so still a lower bound) and
-// replaced by the length (0..31) of the cell's primitive list, which starts
+// bits with the low 7 mantissa bits cleared (so still a lower bound) and
+// replaced by the length (0..127) of the cell's primitive list, which starts
 // at glist[first].  The list holds every primitive within dc + 2 hd of the
 // centre (hd = half cell diagonal, padded), which contains the nearest
 // primitive of every point within hd of the centre; entries carry their
 // distance from the centre (rounded down) and are sorted by it, so a scan
 // from p can stop at the first entry with dmin - |p - c| >= its best.  A
-// cell whose list would exceed 31 entries stores none (length 0).
+// cell whose list would exceed the build's cap stores none (length 0), and
+// so does a cell whose bound dc already answers every relaxed query.
 // --------------------------------------------------------------------------
 struct GridInfo {
     float ox, oy, oz;     // corner of cell (0, 0, 0)
@@ -38,14 +39,15 @@ struct GridInfo {
     int32_t cell_off;     // first cell of this domain in the cell array
 };
 struct GridCellF {
-    uint32_t dcbits;   // FP32 bits of dc, low 5 bits = list length
+    uint32_t dcbits;   // FP32 bits of dc, low 7 bits = list length
     uint32_t first;    // first list entry
 };
 struct GridEntry {
     int32_t slot;      // leaf slot of the primitive
     float dmin;        // its distance from the cell centre, rounded down
 };
-constexpr int kGridListMax = 31;
+constexpr int kGridListMax = 127;
+constexpr uint32_t kGridLenMask = 127u;
 
 // --------------------------------------------------------------------------
 // kernel-side views
@@ -784,7 +786,18 @@ __device__ __forceinline__ bool query_begin(QueryF &q, const MeshView &m, int32_
             const float dy = py - (G.oy + ((float)iy + 0.5f) * G.h);
             const float dz = DIM == 3 ? pz - (G.oz + ((float)iz + 0.5f) * G.h) : 0.0f;
             const float t = sqrtf(__builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz)));
-            const uint32_t len = C.dcbits & 31u;
+            if (cf < 1.0f) {
+                // dc lost its low 7 mantissa bits: dc <= true <= dc (1 + 2^-15)
+                const float dc = __uint_as_float(C.dcbits & ~kGridLenMask);
+                const float lb = dc - t;
+                const float ub = __builtin_fmaf(dc, 1.0f + 1.0f / 32768.0f, t);
+                if (lb > 0.0f && lb * lb >= cf * (ub * ub)) {
+                    q.best = ub * ub;
+                    q.lb = lb * lb;
+                    return false;
+                }
+            }
+            const uint32_t len = C.dcbits & kGridLenMask;
             if (len != 0u) {
                 q.indirect = true;
                 q.lslot = (int32_t)C.first;
@@ -792,17 +805,6 @@ __device__ __forceinline__ bool query_begin(QueryF &q, const MeshView &m, int32_
                 q.t = t;
                 q.best = INFINITY;
                 return true;
-            }
-            if (cf < 1.0f) {
-                // dc lost its low 5 mantissa bits: dc <= true <= dc (1 + 2^-17)
-                const float dc = __uint_as_float(C.dcbits);
-                const float lb = dc - t;
-                const float ub = __builtin_fmaf(dc, 1.0f + 1.0f / 131072.0f, t);
-                if (lb > 0.0f && lb * lb >= cf * (ub * ub)) {
-                    q.best = ub * ub;
-                    q.lb = lb * lb;
-                    return false;
-                }
             }
         }
     }
@@ -822,10 +824,13 @@ __device__ __forceinline__ bool query_step(QueryF &q, const MeshView &m, float p
         bool skip = false;
         if (q.indirect) {
             // sorted by distance from the centre: once an entry's lower
-            // bound reaches the best, so do all that follow
+            // bound reaches cf x best, so do all that follow; below best it
+            // is kept in lb (a relaxed stop)
             const GridEntry E = m.glist[q.lslot];
-            const float lbk = E.dmin - q.t;
-            skip = lbk > 0.0f && lbk * lbk >= q.best;
+            const float lbk = fmaxf(E.dmin - q.t, 0.0f);
+            const float lbk2 = lbk * lbk;
+            skip = lbk2 >= q.best * q.cf;
+            if (skip && lbk2 < q.best) q.lb = fminf(q.lb, lbk2);
             slot = E.slot;
         }
         if (!skip) {

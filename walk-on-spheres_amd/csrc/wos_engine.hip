@@ -350,7 +350,8 @@ __device__ int range_collect(const MeshView &m, int root, double cx, double cy, 
 
 template <int DIM>
 __global__ __launch_bounds__(kBlock) void k_grid_build(MeshView m, GridInfo G, int root64,
-                                                       double pad, int phase, GridCellF *cells,
+                                                       double pad, int cap, double band,
+                                                       int phase, GridCellF *cells,
                                                        int32_t *counts, double *rl2,
                                                        const int64_t *offsets, GridEntry *glist) {
     extern __shared__ int lds_g[];
@@ -368,16 +369,18 @@ __global__ __launch_bounds__(kBlock) void k_grid_build(MeshView m, GridInfo G, i
                                 : npq_ref_3d(m, root64, cx, cy, cz, stk, blockDim.x);
         const double dc = sqrt(q.d2);
         const double rl = (dc + pad) * (1.0 + 1e-6);
-        const int cnt = range_collect<DIM>(m, root64, cx, cy, cz, rl * rl, stk, blockDim.x,
-                                           nullptr, kGridListMax);
-        cells[c].dcbits = __float_as_uint(round_down_f(dc)) & ~31u;
+        // a cell beyond `band` is answered by its bound alone: no list
+        const int cnt = dc < band ? range_collect<DIM>(m, root64, cx, cy, cz, rl * rl, stk,
+                                                       blockDim.x, nullptr, cap)
+                                  : cap + 1;
+        cells[c].dcbits = __float_as_uint(round_down_f(dc)) & ~kGridLenMask;
         cells[c].first = 0u;
-        counts[c] = cnt <= kGridListMax ? cnt : 0;
+        counts[c] = cnt <= cap ? cnt : 0;
         rl2[c] = rl * rl;
     } else if (counts[c] > 0) {
         const int64_t off = offsets[c];
         const int cnt = range_collect<DIM>(m, root64, cx, cy, cz, rl2[c], stk, blockDim.x,
-                                           glist + off, kGridListMax);
+                                           glist + off, cap);
         cells[c].dcbits |= (uint32_t)cnt;
         cells[c].first = (uint32_t)off;
     }
@@ -959,8 +962,8 @@ void k_walk_fp32(MeshView m, EqView e, PersistArgs a, AdjOut adj) {
                 const bool leaf = searching && q.lcount > 0;
                 bool stop_entry = false;
                 if (leaf && q.indirect) {
-                    const float lbk = m.glist[q.lslot].dmin - q.t;
-                    stop_entry = lbk > 0.0f && lbk * lbk >= q.best;
+                    const float lbk = fmaxf(m.glist[q.lslot].dmin - q.t, 0.0f);
+                    stop_entry = lbk * lbk >= q.best * q.cf;
                 }
                 u_nd += __popcll(__ballot(searching && !leaf));
                 u_lp += __popcll(__ballot(leaf && q.indirect));
@@ -1111,6 +1114,18 @@ hipError_t upload(DevBuf &b, const std::vector<T> &v, hipStream_t s) {
 
 }  // namespace
 
+// what the autotuner's cached shapes are keyed by
+struct TuneKey {
+    int dim = 0, n_domains = 0;
+    int64_t n_prims = 0, n_points = 0;
+    int kind = 0, source_mode = 0, source_kind = 0, grid_cells = 0;
+    bool operator==(const TuneKey &o) const {
+        return dim == o.dim && n_domains == o.n_domains && n_prims == o.n_prims &&
+               n_points == o.n_points && kind == o.kind && source_mode == o.source_mode &&
+               source_kind == o.source_kind && grid_cells == o.grid_cells;
+    }
+};
+
 struct wos_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -1121,9 +1136,12 @@ struct wos_ctx {
     DevBuf nd2, nd3, segd, trid, slot_prim, prim_bid, prims, verts, nf2, nf3, segf, seg_inv, trif;
     DevBuf cq_out, cq_prim;
     DevBuf units;                     // FP32 query-unit counters
+    std::vector<std::pair<TuneKey, wos_tuning>> tuned;  // autotuned shapes
     bool count_units = false;
     DevBuf g_info, g_cells, g_list;   // distance grid
     int grid_built_cells = 0;         // target cells per domain of the built grid (0: none)
+    int grid_built_cap = 0;           // its list cap
+    float grid_built_relax = -1.0f;   // and the relax it was built for
     int64_t grid_list_len = 0;
     DevBuf plist_in, loc, starts, amb, bvalue, inside, vals, steps, mean, m2, count, ssum, hits,
         counter, vv, pid_all, r_pid, r_wbeg, r_off, coef, scal_sum, grad_v, grad_b, dl_du,
@@ -1131,7 +1149,7 @@ struct wos_ctx {
     int last_rounds = 0;
     MeshView view{};
     hipEvent_t ev[6]{};
-    wos_tuning tune{256, -1, 64, 0, 48, 0.1f, 1 << 22};
+    wos_tuning tune{256, -1, 64, 0, 48, 0.1f, 1 << 22, 31};
     size_t lds_per_cu = 65536;
     size_t lds_per_block = 65536;
     int last_threads = 0;
@@ -1150,7 +1168,8 @@ hipError_t ensure_grid(wos_ctx *c) {
         c->view.grids = nullptr; c->view.cells = nullptr; c->view.glist = nullptr;
         return hipSuccess;
     }
-    if (c->grid_built_cells != target) {
+    if (c->grid_built_cells != target || c->grid_built_cap != c->tune.grid_list_max ||
+        c->grid_built_relax != c->tune.relax) {
         const int dim = c->mesh.dim;
         const int nd = (int)c->L.domains.size();
         std::vector<GridInfo> info(nd);
@@ -1207,17 +1226,23 @@ hipError_t ensure_grid(wos_ctx *c) {
                 const GridInfo &G = info[d];
                 const DomainInfo &D = c->L.domains[d];
                 const int n = G.nx * G.ny * G.nz;
-                const double pad = 2.0 * (0.5 * (double)G.h * std::sqrt((double)dim) * 1.0001 +
-                                          64.0 * std::ldexp(1.0, -24) * D.coord_scale);
+                const double hd = 0.5 * (double)G.h * std::sqrt((double)dim) * 1.0001 +
+                                  64.0 * std::ldexp(1.0, -24) * D.coord_scale;
+                const double pad = 2.0 * hd;
+                // with relax r, a cell with dc >= hd (2 - r) / r answers every
+                // query from its bound; lists are kept below that (padded)
+                const double r = (double)c->tune.relax;
+                const double band = r > 0.0 ? 1.05 * hd * (2.0 - r) / r + pad : INFINITY;
+                const int cap = c->tune.grid_list_max;
                 const int blocks = (n + kBlock - 1) / kBlock;
                 if (dim == 2)
                     hipLaunchKernelGGL(k_grid_build<2>, dim3(blocks), dim3(kBlock), lds, c->stream, v,
-                                       G, D.node_off, pad, phase, c->g_cells.as<GridCellF>(),
+                                       G, D.node_off, pad, cap, band, phase, c->g_cells.as<GridCellF>(),
                                        counts.as<int32_t>(), rl2.as<double>(), offs.as<int64_t>(),
                                        c->g_list.as<GridEntry>());
                 else
                     hipLaunchKernelGGL(k_grid_build<3>, dim3(blocks), dim3(kBlock), lds, c->stream, v,
-                                       G, D.node_off, pad, phase, c->g_cells.as<GridCellF>(),
+                                       G, D.node_off, pad, cap, band, phase, c->g_cells.as<GridCellF>(),
                                        counts.as<int32_t>(), rl2.as<double>(), offs.as<int64_t>(),
                                        c->g_list.as<GridEntry>());
                 if ((e = hipGetLastError()) != hipSuccess) return e;
@@ -1225,6 +1250,8 @@ hipError_t ensure_grid(wos_ctx *c) {
         }
         if ((e = upload(c->g_info, info, c->stream)) != hipSuccess) return e;
         c->grid_built_cells = target;
+        c->grid_built_cap = c->tune.grid_list_max;
+        c->grid_built_relax = c->tune.relax;
     }
     c->view.grids = c->g_info.as<GridInfo>();
     c->view.cells = c->g_cells.as<GridCellF>();
@@ -1379,9 +1406,9 @@ struct Part {
     int64_t count;
 };
 
-int solve(wos_ctx *c, const std::vector<Part> &parts, const wos_eq_desc *eq,
-          const wos_run_params *prm, wos_point_stats *out, wos_run_summary *summary,
-          const AdjRequest *adjr = nullptr, wos_run_summary *part_summaries = nullptr) {
+int solve_once(wos_ctx *c, const std::vector<Part> &parts, const wos_eq_desc *eq,
+               const wos_run_params *prm, wos_point_stats *out, wos_run_summary *summary,
+               const AdjRequest *adjr = nullptr, wos_run_summary *part_summaries = nullptr) {
     int64_t n = 0;
     for (const Part &p : parts) n += p.count;
     if (!c) return fail(WOS_ERR_INPUT, "context is NULL");
@@ -1765,6 +1792,117 @@ int solve(wos_ctx *c, const std::vector<Part> &parts, const wos_eq_desc *eq,
     return WOS_OK;
 }
 
+// ---------------------------------------------------------------------------
+// autotuner.  It searches only launch-shape fields that never change a
+// result (walkers per block, lanes per walk-step phase, refill chunk, LDS
+// node cache): every walk's draws and queries are independent of them, so a
+// tuned solve is bit-identical to an untuned one.  The distance grid, its
+// list cap and relax stay as set.  Coordinate descent from the current
+// shape over a fixed candidate list, each candidate timed twice (walker
+// kernel time, the smaller counts) on a pilot solve of the same problem with
+// about 16 M walks.
+// ---------------------------------------------------------------------------
+TuneKey tune_key(const wos_ctx *c, const std::vector<Part> &parts, const wos_eq_desc *eq) {
+    TuneKey k{};
+    k.dim = c->mesh.dim;
+    k.n_domains = (int)c->L.domains.size();
+    k.n_prims = c->view.n_prims;
+    for (const Part &p : parts) k.n_points += p.count;
+    k.kind = eq ? eq->kind : -1;
+    k.source_mode = eq ? eq->source_mode : -1;
+    k.source_kind = eq ? eq->source_kind : -1;
+    k.grid_cells = c->tune.grid_cells;
+    return k;
+}
+
+int autotune(wos_ctx *c, const std::vector<Part> &parts, const wos_eq_desc *eq,
+             const wos_run_params *prm, wos_tuning *chosen) {
+    int64_t n = 0;
+    for (const Part &p : parts) n += p.count;
+    if (n <= 0) return fail(WOS_ERR_INPUT, "point count out of range");
+    wos_run_params p = *prm;
+    p.tune = 0;
+    p.target_se = 0.0;
+    p.count_units = 0;
+    p.precision = WOS_FP32_PERSISTENT;
+    const int64_t want = ((int64_t)1 << 24) / n + 1;
+    p.n_walks = (int)std::max<int64_t>(16, std::min<int64_t>(want, std::max(16, prm->n_walks)));
+    std::vector<double> mean((size_t)n);
+    wos_point_stats st{};
+    st.mean = mean.data();
+    const wos_tuning saved = c->tune;
+    auto timed = [&](const wos_tuning &t, double *sec) -> int {
+        c->tune = t;
+        double best = INFINITY;
+        for (int r = 0; r < 2; ++r) {
+            wos_run_summary S{};
+            const int rc = solve_once(c, parts, eq, &p, &st, &S);
+            if (rc != WOS_OK) return rc;
+            best = std::min(best, S.walk_kernel_seconds);
+        }
+        *sec = best;
+        return WOS_OK;
+    };
+    wos_tuning cur = saved;
+    double cur_s = 0.0;
+    int rc = timed(cur, &cur_s);
+    if (rc != WOS_OK) { c->tune = saved; return rc; }
+    static const int kReady[] = {16, 24, 32, 40, 48, 56};
+    static const int kThreads[] = {128, 192, 256};
+    static const int kChunk[] = {32, 64, 256};
+    static const int kLds[] = {0, -1};
+    auto descend = [&](int wos_tuning::*field, const int *vals, int nv) -> int {
+        for (int i = 0; i < nv; ++i) {
+            if (cur.*field == vals[i]) continue;
+            wos_tuning t = cur;
+            t.*field = vals[i];
+            double sec = 0.0;
+            const int r = timed(t, &sec);
+            if (r != WOS_OK) return r;
+            if (sec < cur_s) { cur = t; cur_s = sec; }
+        }
+        return WOS_OK;
+    };
+    if ((rc = descend(&wos_tuning::min_ready, kReady, 6)) == WOS_OK &&
+        (rc = descend(&wos_tuning::block_threads, kThreads, 3)) == WOS_OK &&
+        (rc = descend(&wos_tuning::chunk, kChunk, 3)) == WOS_OK)
+        rc = descend(&wos_tuning::lds_nodes, kLds, 2);
+    c->tune = saved;
+    if (rc != WOS_OK) return rc;
+    *chosen = cur;
+    c->tuned.emplace_back(tune_key(c, parts, eq), cur);
+    return WOS_OK;
+}
+
+// solve_once with the autotuned launch shape when params->tune = 1 (FP32):
+// the shape cached for this problem, found by autotune() on first use.
+int solve(wos_ctx *c, const std::vector<Part> &parts, const wos_eq_desc *eq,
+          const wos_run_params *prm, wos_point_stats *out, wos_run_summary *summary,
+          const AdjRequest *adjr = nullptr, wos_run_summary *part_summaries = nullptr) {
+    if (!c || !prm || prm->tune == 0 || prm->precision != WOS_FP32_PERSISTENT || !c->has_mesh)
+        return solve_once(c, parts, eq, prm, out, summary, adjr, part_summaries);
+    if (prm->tune != 1) return fail(WOS_ERR_INPUT, "tune must be 0 or 1");
+    const TuneKey key = tune_key(c, parts, eq);
+    const wos_tuning *found = nullptr;
+    for (const auto &e : c->tuned)
+        if (e.first == key) found = &e.second;
+    wos_tuning shape;
+    if (found) {
+        shape = *found;
+    } else {
+        const int rc = autotune(c, parts, eq, prm, &shape);
+        if (rc != WOS_OK) return rc;
+    }
+    const wos_tuning saved = c->tune;
+    c->tune.block_threads = shape.block_threads;
+    c->tune.min_ready = shape.min_ready;
+    c->tune.chunk = shape.chunk;
+    c->tune.lds_nodes = shape.lds_nodes;
+    const int rc = solve_once(c, parts, eq, prm, out, summary, adjr, part_summaries);
+    c->tune = saved;
+    return rc;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1891,6 +2029,7 @@ int wos_ctx_set_domains(wos_ctx *c, int dim, int n_domains, const double *const 
     c->L = std::move(L);
     c->has_mesh = true;
     c->grid_built_cells = 0;
+    c->tuned.clear();
     return WOS_OK;
 }
 
@@ -1947,6 +2086,8 @@ int wos_ctx_set_tuning(wos_ctx *c, const wos_tuning *t) {
     if (!(t->relax >= 0.0f && t->relax < 0.5f)) return fail(WOS_ERR_INPUT, "relax must be in [0, 0.5)");
     if (t->grid_cells < 0 || t->grid_cells > (1 << 26))
         return fail(WOS_ERR_INPUT, "grid_cells must be in 0..2^26");
+    if (t->grid_list_max < 1 || t->grid_list_max > kGridListMax)
+        return fail(WOS_ERR_INPUT, "grid_list_max must be in 1..127");
     c->tune = *t;
     return WOS_OK;
 }
@@ -1954,6 +2095,25 @@ int wos_ctx_set_tuning(wos_ctx *c, const wos_tuning *t) {
 int wos_ctx_get_tuning(const wos_ctx *c, wos_tuning *t) {
     if (!c || !t) return fail(WOS_ERR_INPUT, "context or tuning is NULL");
     *t = c->tune;
+    return WOS_OK;
+}
+
+int wos_ctx_autotune(wos_ctx *c, int nx, int ny, int nz, const wos_eq_desc *eq,
+                     const wos_run_params *params, wos_tuning *chosen) {
+    if (!c || !c->has_mesh) return fail(c ? WOS_ERR_STATE : WOS_ERR_INPUT, "no mesh set on this context");
+    if (!params) return fail(WOS_ERR_INPUT, "params is NULL");
+    if (nx < 1 || ny < 1 || nz < 1) return fail(WOS_ERR_INPUT, "grid sizes must be >= 1");
+    if (c->mesh.dim == 2) nz = 1;
+    GridSpec g{};
+    g.nx = nx; g.ny = ny; g.nz = nz;
+    std::vector<Part> parts;
+    for (int d = 0; d < (int)c->L.domains.size(); ++d)
+        parts.push_back(Part{d, &g, nullptr, (int64_t)nx * ny * nz});
+    wos_tuning best{};
+    const int rc = autotune(c, parts, eq, params, &best);
+    if (rc != WOS_OK) return rc;
+    c->tune = best;
+    if (chosen) *chosen = best;
     return WOS_OK;
 }
 

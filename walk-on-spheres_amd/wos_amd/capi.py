@@ -121,6 +121,7 @@ class Tuning(C.Structure):
         ("min_ready", C.c_int32),
         ("relax", C.c_float),
         ("grid_cells", C.c_int32),
+        ("grid_list_max", C.c_int32),
     ]
 
 
@@ -211,6 +212,8 @@ def load(path: str | None = None):
                                        C.c_int, C.c_double, C.POINTER(C.c_double),
                                        C.POINTER(C.c_double), C.POINTER(C.c_double),
                                        C.POINTER(C.c_int32)]
+    lib.wos_ctx_autotune.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.POINTER(EqDesc),
+                                     C.POINTER(RunParams), C.POINTER(Tuning)]
     lib.wos_ctx_set_tuning.argtypes = [C.c_void_p, C.POINTER(Tuning)]
     lib.wos_ctx_get_tuning.argtypes = [C.c_void_p, C.POINTER(Tuning)]
     if path is None:

@@ -99,10 +99,20 @@ class Context:
         return res, self._summary(summ)
 
     def set_tuning(self, block_threads=256, lds_nodes=-1, chunk=64, blocks_per_cu=0,
-                   min_ready=48, relax=0.1, grid_cells=1 << 22):
+                   min_ready=48, relax=0.1, grid_cells=1 << 22, grid_list_max=31):
         t = capi.Tuning(block_threads, lds_nodes, chunk, blocks_per_cu, min_ready, relax,
-                        grid_cells)
+                        grid_cells, grid_list_max)
         capi.check(self.lib, self.lib.wos_ctx_set_tuning(self.h, C.byref(t)))
+
+    def autotune(self, nx, ny, nz, eq: EqDesc, prm: RunParams):
+        """Search the FP32 walker's launch shape on this grid problem (results
+        are unchanged by it); returns and installs the chosen tuning."""
+        if self.dim == 2:
+            nz = 1
+        t = capi.Tuning()
+        capi.check(self.lib, self.lib.wos_ctx_autotune(self.h, nx, ny, nz, C.byref(eq),
+                                                       C.byref(prm), C.byref(t)))
+        return {k: getattr(t, k) for k, _ in capi.Tuning._fields_}
 
     def get_tuning(self):
         t = capi.Tuning()
