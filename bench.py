@@ -211,12 +211,14 @@ def config5(Context, meshgen, problems, caps):
     """Welding-style stationary screened Poisson: box plate 120 x 60 x 10 mm,
     200 316 triangles, alpha = 0.05, f = Q exp(-|x - x0|^2 / sigma^2) with
     Q = 1, sigma = 3 mm, x0 = (40, 30, 10), uniform ball sampling, g = 0,
-    64 x 32 x 16 grid, 8192 walks per point, eps 0.01, seed 5: FP32 and FP64
+    64 x 32 x 16 grid, 8192 walks per point, eps 0.01, seed 5, distance grid
+    of 2^23 cells: FP32 and FP64
     grid pts/s at SE <= 1e-3 and the FP32 adjoint of L = sum u (gradients
     with respect to Q, sigma, x0, alpha)."""
     V, F, _ = meshgen.plate_200k()
     ctx = Context(0)
     ctx.set_mesh(V, F)
+    ctx.set_tuning(grid_cells=1 << 23)  # 200k triangles: a finer distance grid
     eq = problems.plate_gaussian(alpha=0.05, q=1.0, sigma=3.0, x0=(40.0, 30.0, 10.0))
     grid, nw, eps, seed = (64, 32, 16), 8192, 0.01, 5
     p32 = Context.params(nw, eps, seed, precision=caps.WOS_FP32_PERSISTENT)
@@ -242,6 +244,7 @@ def config5(Context, meshgen, problems, caps):
     z = np.abs(r32["mean"][ins] - r64["mean"][ins]) / np.maximum(
         np.sqrt(r32["standard_error"][ins] ** 2 + r64["standard_error"][ins] ** 2), floor)
     return dict(config=5, workload=config5.__doc__.split(":")[0].strip(),
+                tuning=ctx.get_tuning(),
                 metric="grid pts/s at SE<=1e-3 (FP32 vs FP64), adjoint time",
                 value=v32, unit="grid pts/s", fp64_value=v64, fp32_over_fp64=v32 / v64,
                 inside=int(ins.sum()), fp32_ms=t32 * 1e3, fp64_ms=t64 * 1e3,
@@ -310,6 +313,14 @@ def main():
                           "launch_blocks": s32["launch_blocks"],
                           "launch_threads": s32["launch_threads"]}))
         return
+    # the same solve as a captured graph (captured by the first call)
+    pg = Context.params(nw, eps, seed, precision=WOS_FP32_PERSISTENT, use_graph=1)
+    rg = ctx.solve_grid(*grid, eq, pg)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        rg = ctx.solve_grid(*grid, eq, pg)
+    tg = (time.perf_counter() - t0) / args.steps
+    graph_same = bool(np.array_equal(rg["mean"], r32["mean"], equal_nan=True))
     # query-unit counts of the same solve (an extra, untimed launch)
     pc = Context.params(nw, eps, seed, precision=WOS_FP32_PERSISTENT, count_units=1)
     sc = ctx.solve_grid(*grid, eq, pc)["summary"]
@@ -362,6 +373,7 @@ def main():
                        "walk_kernel_ms": s64["walk_kernel_seconds"] * 1e3,
                        "steps_per_s": s64["total_steps"] / t64},
         "fp32_over_fp64": v32 / v64,
+        "fp32_graph": {"ms_per_solve": tg * 1e3, "results_equal_eager": graph_same},
         "walks_per_s": s32["total_walks"] / t32,
         "steps_per_s": s32["total_steps"] / t32,
         "fp32_vs_fp64": {"max_abs_diff": float(d.max()), "mean_abs_diff": float(d.mean()),

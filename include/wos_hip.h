@@ -107,6 +107,11 @@ typedef struct wos_run_params {
                                     first use of a problem; results are the same  */
     int32_t count_units;         /* 1: count the FP32 walker's query units into the
                                     summary (wavefront ballots; off by default)    */
+    int32_t use_graph;           /* 1: run a fixed-walk FP32 grid solve as one
+                                    captured hipGraph (classification, compaction,
+                                    walker, Welford), captured on first use and
+                                    replayed while problem, tuning and buffers stay
+                                    the same; results equal use_graph = 0          */
 } wos_run_params;
 
 /* Per-point outputs, caller-allocated arrays of n entries (any may be NULL

@@ -480,6 +480,10 @@ struct RoundView {
     int cw;
     int w0;
     int64_t total;
+    // when set (graph replay), n and total are read from device memory: the
+    // inside count is known only on the device
+    const int32_t *n_dev;
+    const int64_t *total_dev;
 };
 
 __device__ __forceinline__ const StartInfo &round_item(const RoundView &r, int64_t i, uint32_t &walk) {
@@ -757,6 +761,64 @@ __global__ void k_zero_u64(unsigned long long *p, int64_t n) {
         p[i] = 0ull;
 }
 
+__global__ void k_zero_u32(uint32_t *p, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = 0u;
+}
+
+// Inside-point compaction on the device (graph replay, where the host does
+// not see the classification): per block of kBlock points its number of
+// inside points, one scan over the blocks, then every block writes its
+// inside points in ascending order -- the same list the host builds.
+__global__ __launch_bounds__(kBlock) void k_inside_count(const uint8_t *loc, int64_t n,
+                                                         int32_t *bcount) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = __syncthreads_count(i < n && loc[i] == 1);
+    if (threadIdx.x == 0) bcount[blockIdx.x] = c;
+}
+
+__global__ __launch_bounds__(1024) void k_inside_scan(const int32_t *bcount, int64_t nb,
+                                                      int64_t *boff, int32_t *n_in,
+                                                      int64_t *total, int64_t walks_per_point) {
+    __shared__ int64_t part[1024];
+    const int t = threadIdx.x;
+    const int64_t per = (nb + 1023) / 1024;
+    const int64_t b0 = min(nb, (int64_t)t * per), b1 = min(nb, b0 + per);
+    int64_t sum = 0;
+    for (int64_t b = b0; b < b1; ++b) sum += bcount[b];
+    part[t] = sum;
+    __syncthreads();
+    for (int d = 1; d < 1024; d <<= 1) {
+        const int64_t v = t >= d ? part[t - d] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    int64_t run = t > 0 ? part[t - 1] : 0;
+    for (int64_t b = b0; b < b1; ++b) { boff[b] = run; run += bcount[b]; }
+    if (t == 1023) {
+        *n_in = (int32_t)part[1023];
+        *total = part[1023] * walks_per_point;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_inside_scatter(const uint8_t *loc, int64_t n,
+                                                           const int64_t *boff, int32_t *inside) {
+    __shared__ int scan[kBlock];
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int f = (i < n && loc[i] == 1) ? 1 : 0;
+    scan[threadIdx.x] = f;
+    __syncthreads();
+    for (int d = 1; d < kBlock; d <<= 1) {
+        const int v = (int)threadIdx.x >= d ? scan[threadIdx.x - d] : 0;
+        __syncthreads();
+        scan[threadIdx.x] += v;
+        __syncthreads();
+    }
+    if (f) inside[boff[blockIdx.x] + scan[threadIdx.x] - 1] = (int32_t)i;
+}
+
 // --------------------------------------------------------------------------
 // kernel 2b: FP32 persistent walker
 // --------------------------------------------------------------------------
@@ -826,6 +888,7 @@ void k_walk_fp32(MeshView m, EqView e, PersistArgs a, AdjOut adj) {
     // wave-uniform pool of work items
     uint64_t pool_next = 0, pool_end = 0;
     bool exhausted = false;
+    const uint64_t total = a.rv.total_dev ? (uint64_t)*a.rv.total_dev : (uint64_t)a.rv.total;
     // wave-uniform query-unit counts (a.units != nullptr): queries, grid
     // bounds, list scans, list primitives, nodes, leaf primitives
     uint64_t u_q = 0, u_gb = 0, u_ls = 0, u_lp = 0, u_nd = 0, u_pr = 0, u_le = 0;
@@ -867,12 +930,12 @@ void k_walk_fp32(MeshView m, EqView e, PersistArgs a, AdjOut adj) {
                 unsigned long long base = 0;
                 if (lane == 0) base = atomicAdd(a.counter, (unsigned long long)a.chunk);
                 base = __shfl(base, 0);
-                if (base >= (unsigned long long)a.rv.total) {
+                if (base >= (unsigned long long)total) {
                     exhausted = true;
                 } else {
                     pool_next = base;
                     pool_end = base + (uint64_t)a.chunk;
-                    if (pool_end > (uint64_t)a.rv.total) pool_end = (uint64_t)a.rv.total;
+                    if (pool_end > total) pool_end = total;
                 }
             }
             if (!exhausted) {
@@ -996,7 +1059,7 @@ __global__ __launch_bounds__(kBlock) void k_welford(RoundView rv, const double *
                                                     double *count, double *steps_sum,
                                                     int32_t *hits) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= rv.n) return;
+    if (k >= (rv.n_dev ? *rv.n_dev : rv.n)) return;
     const int p = rv.pid[k];
     const int64_t base = rv.off ? rv.off[k] : (int64_t)k * rv.cw;
     const int cnt = rv.off ? (int)(rv.off[k + 1] - rv.off[k]) : rv.cw;
@@ -1137,6 +1200,13 @@ struct wos_ctx {
     DevBuf cq_out, cq_prim;
     DevBuf units;                     // FP32 query-unit counters
     std::vector<std::pair<TuneKey, wos_tuning>> tuned;  // autotuned shapes
+    // graph replay: the captured solve, what it was captured for, its tolerances
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    std::vector<uint8_t> graph_key;
+    std::vector<double> graph_tol;
+    int graph_blocks = 0;
+    DevBuf g_bcount, g_boff, g_misc;
     bool count_units = false;
     DevBuf g_info, g_cells, g_list;   // distance grid
     int grid_built_cells = 0;         // target cells per domain of the built grid (0: none)
@@ -1149,7 +1219,7 @@ struct wos_ctx {
     int last_rounds = 0;
     MeshView view{};
     hipEvent_t ev[6]{};
-    wos_tuning tune{256, -1, 64, 0, 48, 0.1f, 1 << 22, 31};
+    wos_tuning tune{256, -1, 64, 0, 48, 0.2f, 1 << 22, 127};
     size_t lds_per_cu = 65536;
     size_t lds_per_block = 65536;
     int last_threads = 0;
@@ -1406,14 +1476,15 @@ struct Part {
     int64_t count;
 };
 
-int solve_once(wos_ctx *c, const std::vector<Part> &parts, const wos_eq_desc *eq,
-               const wos_run_params *prm, wos_point_stats *out, wos_run_summary *summary,
-               const AdjRequest *adjr = nullptr, wos_run_summary *part_summaries = nullptr) {
+// Validate a solve, build its equation view and stage its inputs.
+int prepare_solve(wos_ctx *c, const std::vector<Part> &parts, const wos_eq_desc *eq,
+                  const wos_run_params *prm, const AdjRequest *adjr, EqView &ev,
+                  int64_t &n_out) {
     int64_t n = 0;
     for (const Part &p : parts) n += p.count;
     if (!c) return fail(WOS_ERR_INPUT, "context is NULL");
     if (!c->has_mesh) return fail(WOS_ERR_STATE, "no mesh set on this context");
-    if (!prm || !out || !out->mean) return fail(WOS_ERR_INPUT, "params and out->mean are required");
+    if (!prm) return fail(WOS_ERR_INPUT, "params are required");
     if (adjr && !adjr->dl_du) return fail(WOS_ERR_INPUT, "dl_du is required");
     if (adjr && prm->target_se > 0.0)
         return fail(WOS_ERR_INPUT, "the adjoint replays a fixed number of walks (target_se = 0)");
@@ -1435,7 +1506,7 @@ int solve_once(wos_ctx *c, const std::vector<Part> &parts, const wos_eq_desc *eq
     const int dim = c->mesh.dim;
     HIPX(hipSetDevice(c->device));
 
-    EqView ev{};
+    ev = EqView{};
     ev.kind = eq->kind; ev.source_mode = eq->source_mode; ev.source_kind = eq->source_kind;
     ev.boundary_kind = eq->boundary_kind; ev.alpha = eq->alpha; ev.src_value = eq->src_value;
     ev.cx = eq->src_center[0]; ev.cy = eq->src_center[1]; ev.cz = eq->src_center[2];
@@ -1460,10 +1531,16 @@ int solve_once(wos_ctx *c, const std::vector<Part> &parts, const wos_eq_desc *eq
         if (!pts.empty()) HIPX(upload(c->plist_in, pts, c->stream));
     }
 
-    // per part: the domain's bbox grid and tolerance (runner.hpp:82-88), then
-    // classification of its points into the part's slice of the outputs
-    HIPX(hipEventRecord(c->ev[0], c->stream));
-    std::vector<double> part_tol(parts.size());
+    n_out = n;
+    return WOS_OK;
+}
+
+// Classify every part's points into its slice of loc/starts/amb/bvalue
+// (stream-ordered, no synchronisation); part_tol receives the tolerances.
+int classify_parts(wos_ctx *c, const std::vector<Part> &parts, const EqView &ev,
+                   const wos_run_params *prm, std::vector<double> &part_tol) {
+    const int dim = c->mesh.dim;
+    part_tol.assign(parts.size(), 0.0);
     {
         int64_t off = 0, poff = 0;
         for (size_t pi = 0; pi < parts.size(); ++pi) {
@@ -1491,6 +1568,110 @@ int solve_once(wos_ctx *c, const std::vector<Part> &parts, const wos_eq_desc *eq
             if (p.points) poff += p.count;
         }
     }
+    return WOS_OK;
+}
+
+// Per-point result policy (runner.hpp:195-235) and the summary sums in
+// sweep order (runner.hpp:224-228), for the whole solve and per part, from
+// the host copies of the classification and the Welford state.
+int finish_solve(wos_ctx *c, const std::vector<Part> &parts,
+                 const std::vector<double> &part_tol, int64_t n,
+                 const std::vector<uint8_t> &loc, const std::vector<int32_t> &amb,
+                 const std::vector<double> &bval, const std::vector<double> &h_mean,
+                 const std::vector<double> &h_m2, const std::vector<double> &h_cnt,
+                 const std::vector<double> &h_ss, const std::vector<int32_t> &h_hits,
+                 double classify_s, double walk_s, double total_walk_s, int blocks,
+                 const wos_run_params *prm, wos_point_stats *out,
+                 wos_run_summary *summary, wos_run_summary *part_summaries) {
+    wos_run_summary S{};
+    int ii = 0;
+    size_t pi = 0;
+    int64_t part_end = parts.empty() ? 0 : parts[0].count;
+    wos_run_summary P{};
+    auto close_part = [&](size_t k) {
+        const DomainInfo &D = c->L.domains[parts[k].domain];
+        for (int j = 0; j < 6; ++j) P.grid_bounds[j] = D.bounds[j];
+        P.boundary_tolerance = part_tol[k];
+        if (part_summaries) part_summaries[k] = P;
+        P = wos_run_summary{};
+    };
+    for (int64_t i = 0; i < n; ++i) {
+        while (i >= part_end && pi + 1 < parts.size()) {
+            close_part(pi);
+            ++pi;
+            part_end += parts[pi].count;
+        }
+        double mu = NAN, var = NAN, se = NAN, ms = NAN;
+        int32_t hits = 0, walks = 0;
+        if (loc[i] == 2) {
+            mu = bval[i]; var = se = ms = 0.0;
+        } else if (loc[i] == 1) {
+            const double N = h_cnt[ii];
+            mu = h_mean[ii];
+            var = N > 1 ? h_m2[ii] / (N - 1.0) : 0.0;
+            se = N > 0 ? std::sqrt(var / N) : 0.0;
+            ms = N > 0 ? h_ss[ii] / N : 0.0;
+            hits = h_hits[ii];
+            walks = (int32_t)N;
+            ++ii;
+            for (wos_run_summary *T : {&S, &P}) {
+                T->inside_count++;
+                T->max_steps_hits += hits;
+                T->sum_variance += var;
+                T->sum_standard_error += se;
+                T->sum_mean_steps += ms;
+                T->total_walks += walks;
+                T->total_steps += (int64_t)h_ss[ii - 1];
+            }
+        }
+        if (loc[i] == 3) { S.indeterminate_points++; P.indeterminate_points++; }
+        S.ambiguous_ray_retries += amb[i];
+        P.ambiguous_ray_retries += amb[i];
+        out->mean[i] = mu;
+        if (out->variance) out->variance[i] = var;
+        if (out->standard_error) out->standard_error[i] = se;
+        if (out->mean_steps) out->mean_steps[i] = ms;
+        if (out->location) out->location[i] = loc[i];
+        if (out->max_steps_hits) out->max_steps_hits[i] = hits;
+        if (out->walks) out->walks[i] = walks;
+        if (out->ambiguous_ray_retries) out->ambiguous_ray_retries[i] = amb[i];
+    }
+    if (!parts.empty()) close_part(pi);
+    S.wos_seconds = classify_s + total_walk_s;
+    S.walk_kernel_seconds = walk_s;
+    S.rounds = c->last_rounds;
+    const DomainInfo &D0 = c->L.domains[parts[0].domain];
+    for (int k = 0; k < 6; ++k) S.grid_bounds[k] = D0.bounds[k];
+    S.boundary_tolerance = part_tol[0];
+    S.launch_blocks = blocks;
+    S.launch_threads = prm->precision == WOS_FP32_PERSISTENT ? c->last_threads : kBlock;
+    S.lds_cached_nodes = prm->precision == WOS_FP32_PERSISTENT ? c->last_cached : 0;
+    if (c->count_units) {
+        unsigned long long u[7];
+        HIPX(hipMemcpyAsync(u, c->units.p, sizeof(u), hipMemcpyDeviceToHost, c->stream));
+        HIPX(hipStreamSynchronize(c->stream));
+        S.q_queries = (int64_t)u[0]; S.q_grid_bound = (int64_t)u[1]; S.q_list_scans = (int64_t)u[2];
+        S.q_list_entries = (int64_t)u[3]; S.q_list_prims = (int64_t)(u[3] - u[6]);
+        S.q_nodes = (int64_t)u[4]; S.q_leaf_prims = (int64_t)u[5];
+        c->count_units = false;
+    }
+    if (summary) *summary = S;
+    return WOS_OK;
+}
+
+int solve_once(wos_ctx *c, const std::vector<Part> &parts, const wos_eq_desc *eq,
+               const wos_run_params *prm, wos_point_stats *out, wos_run_summary *summary,
+               const AdjRequest *adjr = nullptr, wos_run_summary *part_summaries = nullptr) {
+    if (!out || !out->mean) return fail(WOS_ERR_INPUT, "out->mean is required");
+    EqView ev{};
+    int64_t n = 0;
+    int rc = prepare_solve(c, parts, eq, prm, adjr, ev, n);
+    if (rc != WOS_OK) return rc;
+    const int dim = c->mesh.dim;
+    HIPX(hipEventRecord(c->ev[0], c->stream));
+    std::vector<double> part_tol;
+    rc = classify_parts(c, parts, ev, prm, part_tol);
+    if (rc != WOS_OK) return rc;
     HIPX(hipEventRecord(c->ev[1], c->stream));
 
     std::vector<uint8_t> loc(n);
@@ -1716,80 +1897,164 @@ int solve_once(wos_ctx *c, const std::vector<Part> &parts, const wos_eq_desc *eq
 
     // per-point result policy (runner.hpp:195-235) and the summary sums in
     // sweep order (runner.hpp:224-228), for the whole solve and per part
-    wos_run_summary S{};
-    int ii = 0;
-    size_t pi = 0;
-    int64_t part_end = parts.empty() ? 0 : parts[0].count;
-    wos_run_summary P{};
-    auto close_part = [&](size_t k) {
-        const DomainInfo &D = c->L.domains[parts[k].domain];
-        for (int j = 0; j < 6; ++j) P.grid_bounds[j] = D.bounds[j];
-        P.boundary_tolerance = part_tol[k];
-        if (part_summaries) part_summaries[k] = P;
-        P = wos_run_summary{};
-    };
-    for (int64_t i = 0; i < n; ++i) {
-        while (i >= part_end && pi + 1 < parts.size()) {
-            close_part(pi);
-            ++pi;
-            part_end += parts[pi].count;
-        }
-        double mu = NAN, var = NAN, se = NAN, ms = NAN;
-        int32_t hits = 0, walks = 0;
-        if (loc[i] == 2) {
-            mu = bval[i]; var = se = ms = 0.0;
-        } else if (loc[i] == 1) {
-            const double N = h_cnt[ii];
-            mu = h_mean[ii];
-            var = N > 1 ? h_m2[ii] / (N - 1.0) : 0.0;
-            se = N > 0 ? std::sqrt(var / N) : 0.0;
-            ms = N > 0 ? h_ss[ii] / N : 0.0;
-            hits = h_hits[ii];
-            walks = (int32_t)N;
-            ++ii;
-            for (wos_run_summary *T : {&S, &P}) {
-                T->inside_count++;
-                T->max_steps_hits += hits;
-                T->sum_variance += var;
-                T->sum_standard_error += se;
-                T->sum_mean_steps += ms;
-                T->total_walks += walks;
-                T->total_steps += (int64_t)h_ss[ii - 1];
-            }
-        }
-        if (loc[i] == 3) { S.indeterminate_points++; P.indeterminate_points++; }
-        S.ambiguous_ray_retries += amb[i];
-        P.ambiguous_ray_retries += amb[i];
-        out->mean[i] = mu;
-        if (out->variance) out->variance[i] = var;
-        if (out->standard_error) out->standard_error[i] = se;
-        if (out->mean_steps) out->mean_steps[i] = ms;
-        if (out->location) out->location[i] = loc[i];
-        if (out->max_steps_hits) out->max_steps_hits[i] = hits;
-        if (out->walks) out->walks[i] = walks;
-        if (out->ambiguous_ray_retries) out->ambiguous_ray_retries[i] = amb[i];
+    return finish_solve(c, parts, part_tol, n, loc, amb, bval, h_mean, h_m2, h_cnt, h_ss, h_hits,
+                        classify_s, walk_s, total_walk_s, blocks, prm, out, summary,
+                        part_summaries);
+}
+
+// ---------------------------------------------------------------------------
+// graph replay (params->use_graph = 1): a fixed-walk FP32 grid solve whose
+// whole device sequence -- classification, inside-point compaction, Welford
+// reset, persistent walker, Welford merge -- is captured once into a
+// hipGraph and replayed while the problem, the tuning and every buffer it
+// touches stay the same (any change is detected and captured again).  No
+// memset nodes: buffers are cleared by kernels.  Results equal solve_once's
+// bit for bit.
+// ---------------------------------------------------------------------------
+template <typename T>
+void key_put(std::vector<uint8_t> &k, const T &v) {
+    const uint8_t *b = reinterpret_cast<const uint8_t *>(&v);
+    k.insert(k.end(), b, b + sizeof(T));
+}
+
+void drop_graph(wos_ctx *c) {
+    if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
+    c->graph_exec = nullptr;
+    c->graph = nullptr;
+    c->graph_key.clear();
+}
+
+int solve_graph(wos_ctx *c, const std::vector<Part> &parts, const wos_eq_desc *eq,
+                const wos_run_params *prm, wos_point_stats *out, wos_run_summary *summary,
+                wos_run_summary *part_summaries) {
+    if (!out || !out->mean) return fail(WOS_ERR_INPUT, "out->mean is required");
+    if (!prm) return fail(WOS_ERR_INPUT, "params are required");
+    if (prm->precision != WOS_FP32_PERSISTENT || prm->target_se > 0.0)
+        return fail(WOS_ERR_INPUT, "graph replay runs fixed-walk FP32 persistent solves "
+                                   "(precision FP32, target_se = 0)");
+    for (const Part &p : parts)
+        if (!p.grid) return fail(WOS_ERR_INPUT, "graph replay runs grid solves");
+    EqView ev{};
+    int64_t n = 0;
+    int rc = prepare_solve(c, parts, eq, prm, nullptr, ev, n);
+    if (rc != WOS_OK) return rc;
+    const int64_t nw = prm->n_walks;
+    if (n * nw > ((int64_t)4 << 30) / 12)
+        return fail(WOS_ERR_INPUT, "graph replay needs points x n_walks <= 357913941 (one round)");
+    const int dim = c->mesh.dim;
+    const int64_t nb = (n + kBlock - 1) / kBlock;
+    HIPX(c->inside.reserve((size_t)n * sizeof(int32_t)));
+    HIPX(c->g_bcount.reserve((size_t)nb * sizeof(int32_t)));
+    HIPX(c->g_boff.reserve((size_t)nb * sizeof(int64_t)));
+    HIPX(c->g_misc.reserve(64));
+    HIPX(c->mean.reserve((size_t)n * sizeof(double)));
+    HIPX(c->m2.reserve((size_t)n * sizeof(double)));
+    HIPX(c->count.reserve((size_t)n * sizeof(double)));
+    HIPX(c->ssum.reserve((size_t)n * sizeof(double)));
+    HIPX(c->hits.reserve((size_t)n * sizeof(int32_t)));
+    HIPX(c->vals.reserve((size_t)(n * nw) * sizeof(double)));
+    HIPX(c->steps.reserve((size_t)(n * nw) * sizeof(int32_t)));
+    HIPX(c->counter.reserve(sizeof(unsigned long long)));
+    if (c->pid_all.bytes < (size_t)n * sizeof(int32_t)) {
+        std::vector<int32_t> ids((size_t)n);
+        for (int64_t k = 0; k < n; ++k) ids[(size_t)k] = (int32_t)k;
+        HIPX(upload(c->pid_all, ids, c->stream));
     }
-    if (!parts.empty()) close_part(pi);
-    S.wos_seconds = classify_s + total_walk_s;
-    S.walk_kernel_seconds = walk_s;
-    S.rounds = c->last_rounds;
-    const DomainInfo &D0 = c->L.domains[parts[0].domain];
-    for (int k = 0; k < 6; ++k) S.grid_bounds[k] = D0.bounds[k];
-    S.boundary_tolerance = part_tol[0];
-    S.launch_blocks = blocks;
-    S.launch_threads = prm->precision == WOS_FP32_PERSISTENT ? c->last_threads : kBlock;
-    S.lds_cached_nodes = prm->precision == WOS_FP32_PERSISTENT ? c->last_cached : 0;
-    if (c->count_units) {
-        unsigned long long u[7];
-        HIPX(hipMemcpyAsync(u, c->units.p, sizeof(u), hipMemcpyDeviceToHost, c->stream));
-        HIPX(hipStreamSynchronize(c->stream));
-        S.q_queries = (int64_t)u[0]; S.q_grid_bound = (int64_t)u[1]; S.q_list_scans = (int64_t)u[2];
-        S.q_list_entries = (int64_t)u[3]; S.q_list_prims = (int64_t)(u[3] - u[6]);
-        S.q_nodes = (int64_t)u[4]; S.q_leaf_prims = (int64_t)u[5];
-        c->count_units = false;
+    HIPX(ensure_grid(c));
+    c->count_units = false;
+    int32_t *n_in_dev = c->g_misc.as<int32_t>();
+    int64_t *total_dev = reinterpret_cast<int64_t *>(c->g_misc.as<uint8_t>() + 8);
+
+    std::vector<uint8_t> key;
+    key_put(key, n);
+    for (const Part &p : parts) { key_put(key, p.domain); key_put(key, *p.grid); }
+    key_put(key, ev);
+    key_put(key, *prm);
+    key_put(key, c->tune);
+    key_put(key, c->view);
+    for (const DevBuf *b : {&c->loc, &c->starts, &c->amb, &c->bvalue, &c->inside, &c->g_bcount,
+                            &c->g_boff, &c->g_misc, &c->mean, &c->m2, &c->count, &c->ssum,
+                            &c->hits, &c->vals, &c->steps, &c->counter, &c->pid_all, &c->vv})
+        key_put(key, b->p);
+    if (key != c->graph_key) {
+        drop_graph(c);
+        HIPX(hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
+        auto body = [&]() -> int {
+            int r = classify_parts(c, parts, ev, prm, c->graph_tol);
+            if (r != WOS_OK) return r;
+            hipLaunchKernelGGL(k_inside_count, dim3((unsigned)nb), dim3(kBlock), 0, c->stream,
+                               c->loc.as<uint8_t>(), n, c->g_bcount.as<int32_t>());
+            hipLaunchKernelGGL(k_inside_scan, dim3(1), dim3(1024), 0, c->stream,
+                               c->g_bcount.as<int32_t>(), nb, c->g_boff.as<int64_t>(), n_in_dev,
+                               total_dev, nw);
+            hipLaunchKernelGGL(k_inside_scatter, dim3((unsigned)nb), dim3(kBlock), 0, c->stream,
+                               c->loc.as<uint8_t>(), n, c->g_boff.as<int64_t>(),
+                               c->inside.as<int32_t>());
+            for (DevBuf *b : {&c->mean, &c->m2, &c->count, &c->ssum})
+                hipLaunchKernelGGL(k_zero_u64, dim3(256), dim3(256), 0, c->stream,
+                                   b->as<unsigned long long>(), n);
+            hipLaunchKernelGGL(k_zero_u32, dim3(256), dim3(256), 0, c->stream,
+                               c->hits.as<uint32_t>(), n);
+            HIPX(hipGetLastError());
+            RoundView rv{};
+            rv.starts = c->starts.as<StartInfo>();
+            rv.inside = c->inside.as<int32_t>();
+            rv.pid = c->pid_all.as<int32_t>();
+            rv.n = (int)n;
+            rv.cw = (int)nw;
+            rv.w0 = 0;
+            rv.total = n * nw;
+            rv.n_dev = n_in_dev;
+            rv.total_dev = total_dev;
+            HIPX(dispatch_walk(c, dim, false, ev, eq->source_kind != WOS_SRC_NONE,
+                               eq->kind == WOS_EQ_SCREENED, WOS_FP32_PERSISTENT, rv, prm->epsilon,
+                               prm->max_steps, AdjOut{}, &c->graph_blocks));
+            hipLaunchKernelGGL(k_welford, dim3((unsigned)nb), dim3(kBlock), 0, c->stream, rv,
+                               c->vals.as<double>(), c->steps.as<int32_t>(), c->mean.as<double>(),
+                               c->m2.as<double>(), c->count.as<double>(), c->ssum.as<double>(),
+                               c->hits.as<int32_t>());
+            HIPX(hipGetLastError());
+            return WOS_OK;
+        };
+        const int brc = body();
+        hipGraph_t g = nullptr;
+        const hipError_t ec = hipStreamEndCapture(c->stream, &g);
+        if (brc != WOS_OK) { if (g) (void)hipGraphDestroy(g); return brc; }
+        HIPX(ec);
+        c->graph = g;
+        HIPX(hipGraphInstantiate(&c->graph_exec, c->graph, nullptr, nullptr, 0));
+        c->graph_key = key;
     }
-    if (summary) *summary = S;
-    return WOS_OK;
+    HIPX(hipEventRecord(c->ev[0], c->stream));
+    HIPX(hipGraphLaunch(c->graph_exec, c->stream));
+    HIPX(hipEventRecord(c->ev[1], c->stream));
+    HIPX(hipStreamSynchronize(c->stream));
+    const double graph_s = elapsed(c->ev[0], c->ev[1]);
+
+    int32_t n_in = 0;
+    HIPX(hipMemcpy(&n_in, n_in_dev, sizeof(int32_t), hipMemcpyDeviceToHost));
+    std::vector<uint8_t> loc((size_t)n);
+    std::vector<int32_t> amb((size_t)n), inside((size_t)n_in), h_hits((size_t)n_in);
+    std::vector<double> bval((size_t)n), h_mean((size_t)n_in), h_m2((size_t)n_in),
+        h_cnt((size_t)n_in), h_ss((size_t)n_in);
+    HIPX(hipMemcpyAsync(loc.data(), c->loc.p, (size_t)n, hipMemcpyDeviceToHost, c->stream));
+    HIPX(hipMemcpyAsync(amb.data(), c->amb.p, (size_t)n * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    HIPX(hipMemcpyAsync(bval.data(), c->bvalue.p, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    if (n_in > 0) {
+        HIPX(hipMemcpyAsync(h_mean.data(), c->mean.p, (size_t)n_in * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPX(hipMemcpyAsync(h_m2.data(), c->m2.p, (size_t)n_in * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPX(hipMemcpyAsync(h_cnt.data(), c->count.p, (size_t)n_in * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPX(hipMemcpyAsync(h_ss.data(), c->ssum.p, (size_t)n_in * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPX(hipMemcpyAsync(h_hits.data(), c->hits.p, (size_t)n_in * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    }
+    HIPX(hipStreamSynchronize(c->stream));
+    c->last_rounds = 1;
+    // the graph runs classification and walks back to back: its time is
+    // reported as both the solve and the walker time
+    return finish_solve(c, parts, c->graph_tol, n, loc, amb, bval, h_mean, h_m2, h_cnt, h_ss,
+                        h_hits, 0.0, graph_s, graph_s, c->graph_blocks, prm, out, summary,
+                        part_summaries);
 }
 
 // ---------------------------------------------------------------------------
@@ -1824,6 +2089,7 @@ int autotune(wos_ctx *c, const std::vector<Part> &parts, const wos_eq_desc *eq,
     p.tune = 0;
     p.target_se = 0.0;
     p.count_units = 0;
+    p.use_graph = 0;
     p.precision = WOS_FP32_PERSISTENT;
     const int64_t want = ((int64_t)1 << 24) / n + 1;
     p.n_walks = (int)std::max<int64_t>(16, std::min<int64_t>(want, std::max(16, prm->n_walks)));
@@ -1879,8 +2145,17 @@ int autotune(wos_ctx *c, const std::vector<Part> &parts, const wos_eq_desc *eq,
 int solve(wos_ctx *c, const std::vector<Part> &parts, const wos_eq_desc *eq,
           const wos_run_params *prm, wos_point_stats *out, wos_run_summary *summary,
           const AdjRequest *adjr = nullptr, wos_run_summary *part_summaries = nullptr) {
+    if (c && prm && prm->use_graph) {
+        if (adjr) return fail(WOS_ERR_INPUT, "graph replay runs forward solves only");
+        if (prm->use_graph != 1) return fail(WOS_ERR_INPUT, "use_graph must be 0 or 1");
+    }
+    auto run = [&]() {
+        return prm && prm->use_graph ? solve_graph(c, parts, eq, prm, out, summary, part_summaries)
+                                     : solve_once(c, parts, eq, prm, out, summary, adjr,
+                                                  part_summaries);
+    };
     if (!c || !prm || prm->tune == 0 || prm->precision != WOS_FP32_PERSISTENT || !c->has_mesh)
-        return solve_once(c, parts, eq, prm, out, summary, adjr, part_summaries);
+        return run();
     if (prm->tune != 1) return fail(WOS_ERR_INPUT, "tune must be 0 or 1");
     const TuneKey key = tune_key(c, parts, eq);
     const wos_tuning *found = nullptr;
@@ -1898,7 +2173,7 @@ int solve(wos_ctx *c, const std::vector<Part> &parts, const wos_eq_desc *eq,
     c->tune.min_ready = shape.min_ready;
     c->tune.chunk = shape.chunk;
     c->tune.lds_nodes = shape.lds_nodes;
-    const int rc = solve_once(c, parts, eq, prm, out, summary, adjr, part_summaries);
+    const int rc = run();
     c->tune = saved;
     return rc;
 }
@@ -1948,6 +2223,8 @@ void wos_ctx_destroy(wos_ctx *c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
+    if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
     for (auto &e : c->ev) if (e) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(c->stream);
     delete c;
@@ -2030,6 +2307,7 @@ int wos_ctx_set_domains(wos_ctx *c, int dim, int n_domains, const double *const 
     c->has_mesh = true;
     c->grid_built_cells = 0;
     c->tuned.clear();
+    drop_graph(c);
     return WOS_OK;
 }
 

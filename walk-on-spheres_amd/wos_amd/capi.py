@@ -67,6 +67,7 @@ class RunParams(C.Structure):
         ("max_walks", C.c_int32),
         ("tune", C.c_int32),
         ("count_units", C.c_int32),
+        ("use_graph", C.c_int32),
     ]
 
 

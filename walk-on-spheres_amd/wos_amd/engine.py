@@ -99,7 +99,7 @@ class Context:
         return res, self._summary(summ)
 
     def set_tuning(self, block_threads=256, lds_nodes=-1, chunk=64, blocks_per_cu=0,
-                   min_ready=48, relax=0.1, grid_cells=1 << 22, grid_list_max=31):
+                   min_ready=48, relax=0.2, grid_cells=1 << 22, grid_list_max=127):
         t = capi.Tuning(block_threads, lds_nodes, chunk, blocks_per_cu, min_ready, relax,
                         grid_cells, grid_list_max)
         capi.check(self.lib, self.lib.wos_ctx_set_tuning(self.h, C.byref(t)))
@@ -121,7 +121,8 @@ class Context:
 
     @staticmethod
     def params(n_walks, epsilon, seed, max_steps=1000, max_ray_attempts=12,
-               precision=WOS_FP64_PLAIN, target_se=0.0, max_walks=0, tune=0, count_units=0):
+               precision=WOS_FP64_PLAIN, target_se=0.0, max_walks=0, tune=0, count_units=0,
+               use_graph=0):
         p = RunParams()
         p.n_walks = n_walks
         p.max_steps = max_steps
@@ -133,6 +134,7 @@ class Context:
         p.max_walks = max_walks
         p.tune = tune
         p.count_units = count_units
+        p.use_graph = use_graph
         return p
 
     def closest_points(self, points, precision=WOS_FP64_PLAIN, relax=0.0, domain=0):
