@@ -14,8 +14,12 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HEADER = os.path.join(ROOT, "include", "wos_hip.h")
 
 
-def declared_functions():
-    text = open(HEADER).read()
+RUN_HEADER = os.path.join(ROOT, "include", "wos_run.h")
+RUN_LIB = os.path.join(ROOT, "walk-on-spheres_amd", "csrc", "libwos_run.so")
+
+
+def declared_functions(header=HEADER):
+    text = open(header).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     names = re.findall(r"^\s*(?:const\s+)?[\w\s\*]+?\b(wos_\w+)\s*\(", text, flags=re.M)
     return sorted(set(names))
@@ -34,6 +38,29 @@ def test_library_exports_every_declared_symbol():
     lib = C.CDLL(capi.LIB_PATH)
     missing = [n for n in declared_functions() if not hasattr(lib, n)]
     assert not missing, missing
+
+
+def test_run_library_exports_the_registry_entry_point():
+    """libwos_run.so exports what include/wos_run.h declares, and its
+    registry lists the reference's equations in the reference's order."""
+    if not os.path.exists(RUN_LIB):
+        pytest.fail("libwos_run.so is not built; run __graft_entry__.build()")
+    lib = C.CDLL(RUN_LIB)
+    names = declared_functions(RUN_HEADER)
+    assert "wos_hip_run" in names
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    lib.wos_hip_equation_name.restype = C.c_char_p
+    n = lib.wos_hip_equation_count()
+    assert [lib.wos_hip_equation_name(i) for i in range(n)] == [b"laplace", b"poisson",
+                                                                  b"screened_poisson"]
+    assert lib.wos_hip_equation_name(n) is None
+    # argument errors return 1 before any device use
+    lib.wos_hip_run.restype = C.c_int
+    assert lib.wos_hip_run(4, b"m.obj", b"o.h5", 2, 2, 1, 1, C.c_double(1e-3), 10, 12,
+                           C.c_uint64(1), C.c_double(5.0), 0, 0, None, None) == 1
+    assert lib.wos_hip_run(2, b"m.obj", b"o.h5", 2, 2, 1, 1, C.c_double(1e-3), 10, 12,
+                           C.c_uint64(1), C.c_double(5.0), 0, 7, None, None) == 1
 
 
 def test_version_and_error_string_without_gpu():

@@ -1,6 +1,7 @@
 """The `wos` tool end to end on the GPU: progress/summary lines, the HDF5
 layout of include/wos/hdf5_io.hpp:147-322 (read back with h5dump), and the
 stored fields equal to the library's own solve with the same seed."""
+import ctypes
 import os
 import subprocess
 
@@ -77,3 +78,41 @@ def test_cli_writes_reference_layout(tmp_path, dim):
     assert x[0] == r["summary"]["grid_bounds"][0] and x[-1] == r["summary"]["grid_bounds"][1]
     assert h5_array(str(out), "/metadata/N_walks", "<i4", str(tmp_path))[0] == 32
     assert h5_array(str(out), "/metadata/seed", "<u8", str(tmp_path))[0] == 4242
+
+
+class ExecOpts(ctypes.Structure):
+    """wos_exec_opts of include/wos_run.h."""
+    _fields_ = [("precision", ctypes.c_int32), ("device", ctypes.c_int32),
+                ("target_se", ctypes.c_double), ("max_walks", ctypes.c_int32),
+                ("reserved", ctypes.c_int32)]
+
+
+def test_registry_entry_point_matches_library(tmp_path):
+    """wos_hip_run (libwos_run.so) with the reference's run arguments writes the
+    same fields as a direct solve; a problem descriptor and FP32 options pass
+    through."""
+    C = ctypes
+    from wos_amd.capi import WOS_FP32_PERSISTENT
+    if not os.path.exists(H5DUMP):
+        pytest.skip("h5dump not available")
+    lib = C.CDLL(os.path.join(ROOT, "walk-on-spheres_amd", "csrc", "libwos_run.so"))
+    lib.wos_hip_run.restype = C.c_int
+    V, F, B = M.annulus(64)
+    mesh = tmp_path / "a.obj"
+    M.write_obj(str(mesh), V, F, B)
+    ctx = Context(0)
+    ctx.set_mesh(V, F, B)
+    for label, eq, opts, prec in (
+            ("builtin fp64", None, None, 0),
+            ("annulus data fp32", Pr.annulus_laplace(), ExecOpts(WOS_FP32_PERSISTENT, 0, 0.0, 0, 0),
+             WOS_FP32_PERSISTENT)):
+        out = tmp_path / f"{label.replace(' ', '_')}.h5"
+        rc = lib.wos_hip_run(2, str(mesh).encode(), str(out).encode(), 10, 9, 1, 24,
+                             C.c_double(1e-3), 1000, 12, C.c_uint64(77), C.c_double(5.0), 0, 0,
+                             C.byref(eq) if eq is not None else None,
+                             C.byref(opts) if opts is not None else None)
+        assert rc == 0, label
+        mean = h5_array(str(out), "/mean", "<f8", str(tmp_path))
+        want = ctx.solve_grid(10, 9, 1, eq if eq is not None else Pr.laplace_builtin(2),
+                              Context.params(24, 1e-3, 77, precision=prec))
+        assert np.array_equal(mean, want["mean"], equal_nan=True), label
