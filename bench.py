@@ -14,7 +14,7 @@ t_walk,p = (T / total steps) * mean_steps_p of the form being measured.
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-adjoint]
        python bench.py --fp32-only --steps 1 --warmup 0   (walker launches only,
                                                           for counter runs)
-       python bench.py --config 3 | --config 5             (batched stars, plate)
+       python bench.py --workload stars | --workload plate  (batched stars, plate)
        python bench.py --copy-calibration 1024            (one 1024 MiB fp32
                                                           read+write kernel, the
                                                           scale for HBM counters)
@@ -172,7 +172,7 @@ def adjoint_bench(Context, meshgen, problems, precision, walks=2048):
                 grad_norm=float(np.linalg.norm(gv)))
 
 
-def config3(Context, meshgen, problems, caps):
+def workload_stars(Context, meshgen, problems, caps):
     """Batched uneven 2D Poisson, Green-function source sampling: 16 star
     domains r = 1 + a cos(k theta) (a ~ U(0.1, 0.4), k ~ U{3..9}, 512..16384
     segments, meshgen seed 2024), f = exp(-|x|^2/0.1), g = sin 3x cos 2y,
@@ -199,7 +199,7 @@ def config3(Context, meshgen, problems, caps):
         per.append(dict(segments=f["n"], a=round(f["a"], 4), k=f["k"], inside=int(ins.sum()),
                         reached=int(ok.sum()), capped=int(cap.sum()),
                         mean_walks=float(r["walks"][ins].mean())))
-    return dict(config=3, workload=config3.__doc__.split(":")[0].strip(),
+    return dict(workload="stars", description=workload_stars.__doc__.split(":")[0].strip(),
                 metric="points reaching SE<=1e-3 per s (FP32, batched, adaptive)",
                 value=reached / dt, unit="pts/s", seconds=dt, inside=inside, reached=reached,
                 capped=capped, rounds=tot["rounds"], total_walks=tot["total_walks"],
@@ -207,7 +207,7 @@ def config3(Context, meshgen, problems, caps):
                 walk_kernel_s=tot["walk_kernel_seconds"], domains=per)
 
 
-def config5(Context, meshgen, problems, caps):
+def workload_plate(Context, meshgen, problems, caps):
     """Welding-style stationary screened Poisson: box plate 120 x 60 x 10 mm,
     200 316 triangles, alpha = 0.05, f = Q exp(-|x - x0|^2 / sigma^2) with
     Q = 1, sigma = 3 mm, x0 = (40, 30, 10), uniform ball sampling, g = 0,
@@ -243,7 +243,7 @@ def config5(Context, meshgen, problems, caps):
     floor = 1e-9 * np.abs(r64["mean"][ins]).max()
     z = np.abs(r32["mean"][ins] - r64["mean"][ins]) / np.maximum(
         np.sqrt(r32["standard_error"][ins] ** 2 + r64["standard_error"][ins] ** 2), floor)
-    return dict(config=5, workload=config5.__doc__.split(":")[0].strip(),
+    return dict(workload="plate", description=workload_plate.__doc__.split(":")[0].strip(),
                 tuning=ctx.get_tuning(),
                 metric="grid pts/s at SE<=1e-3 (FP32 vs FP64), adjoint time",
                 value=v32, unit="grid pts/s", fp64_value=v64, fp32_over_fp64=v32 / v64,
@@ -266,8 +266,9 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--fp32-only", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB")
-    ap.add_argument("--config", type=int, default=2, choices=(2, 3, 5),
-                    help="2: the headline line (default); 3, 5: the batched and plate workloads")
+    ap.add_argument("--workload", default="icosphere", choices=("icosphere", "stars", "plate"),
+                    help="icosphere: the headline line (default); stars: 16 batched star "
+                         "domains, adaptive; plate: the 200k-triangle plate with gradients")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -281,11 +282,11 @@ def main():
     from wos_amd import Context, meshgen, problems
     from wos_amd import capi
     from wos_amd.capi import WOS_FP32_PERSISTENT, WOS_FP64_PLAIN
-    if args.config == 3:
-        print(json.dumps(config3(Context, meshgen, problems, capi)))
+    if args.workload == "stars":
+        print(json.dumps(workload_stars(Context, meshgen, problems, capi)))
         return
-    if args.config == 5:
-        print(json.dumps(config5(Context, meshgen, problems, capi)))
+    if args.workload == "plate":
+        print(json.dumps(workload_plate(Context, meshgen, problems, capi)))
         return
 
     V, F, _ = meshgen.icosphere(4)
@@ -343,11 +344,14 @@ def main():
     with open(os.path.join(ROOT, "profiles", "algorithmic_counts.json")) as f:
         ref_counts = json.load(f)["icosphere4_screened_32cubed"]
     ref_achieved = s32["total_steps"] * ref_counts["B_step_f32"] / kern_s / 1e9
-    traffic = None
+    traffic, counters = None, {}
     tpath = os.path.join(ROOT, "profiles", "walk_fp32_icosphere4_traffic.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+            prof = json.load(f)
+        traffic = prof.get("hbm_bytes_per_launch")
+        counters = {k: prof[k] for k in ("valu_busy", "valu_lane_utilisation", "l2_hit_rate")
+                    if k in prof}
     steps = s32["total_steps"]
 
     z = np.abs(r32["mean"][ins] - r64["mean"][ins]) / np.maximum(
@@ -391,7 +395,8 @@ def main():
                                          "q_list_entries", "q_list_prims", "q_nodes",
                                          "q_leaf_prims")},
                      "reference_layout_bytes_per_step": ref_counts["B_step_f32"],
-                     "reference_layout_achieved": ref_achieved},
+                     "reference_layout_achieved": ref_achieved,
+                     "profiled_counters": counters},
     }
     if not args.no_adjoint:
         line["adjoint"] = adjoint_bench(Context, meshgen, problems, WOS_FP32_PERSISTENT)
