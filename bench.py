@@ -211,14 +211,12 @@ def workload_plate(Context, meshgen, problems, caps):
     """Welding-style stationary screened Poisson: box plate 120 x 60 x 10 mm,
     200 316 triangles, alpha = 0.05, f = Q exp(-|x - x0|^2 / sigma^2) with
     Q = 1, sigma = 3 mm, x0 = (40, 30, 10), uniform ball sampling, g = 0,
-    64 x 32 x 16 grid, 8192 walks per point, eps 0.01, seed 5, distance grid
-    of 2^23 cells: FP32 and FP64
+    64 x 32 x 16 grid, 8192 walks per point, eps 0.01, seed 5: FP32 and FP64
     grid pts/s at SE <= 1e-3 and the FP32 adjoint of L = sum u (gradients
     with respect to Q, sigma, x0, alpha)."""
     V, F, _ = meshgen.plate_200k()
     ctx = Context(0)
     ctx.set_mesh(V, F)
-    ctx.set_tuning(grid_cells=1 << 23)  # 200k triangles: a finer distance grid
     eq = problems.plate_gaussian(alpha=0.05, q=1.0, sigma=3.0, x0=(40.0, 30.0, 10.0))
     grid, nw, eps, seed = (64, 32, 16), 8192, 0.01, 5
     p32 = Context.params(nw, eps, seed, precision=caps.WOS_FP32_PERSISTENT)

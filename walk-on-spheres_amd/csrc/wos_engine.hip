@@ -1219,7 +1219,7 @@ struct wos_ctx {
     int last_rounds = 0;
     MeshView view{};
     hipEvent_t ev[6]{};
-    wos_tuning tune{256, -1, 64, 0, 48, 0.2f, 1 << 22, 127};
+    wos_tuning tune{256, -1, 64, 0, 48, 0.2f, 1 << 23, 127};
     size_t lds_per_cu = 65536;
     size_t lds_per_block = 65536;
     int last_threads = 0;
