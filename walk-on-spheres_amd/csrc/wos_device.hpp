@@ -813,32 +813,84 @@ __device__ __forceinline__ bool query_begin(QueryF &q, const MeshView &m, int32_
     return true;
 }
 
-// One unit.  Returns false when the query is complete.
+// What one query_step did (bits of its return value).
+constexpr uint32_t kQsMore = 1u;         // the query continues
+constexpr uint32_t kQsNode = 2u;         // opened an internal node
+constexpr uint32_t kQsLeaf = 4u;         // tested a BVH leaf primitive
+constexpr uint32_t kQsEntryShift = 3u;   // cell-list entries read, 0..2 (bits 3-4)
+constexpr uint32_t kQsPrimShift = 5u;    // cell-list primitives tested, 0..2 (bits 5-6)
+
+// a leaf primitive's FP32 record, loaded apart from its distance test so
+// that two loads can be in flight together
+template <int DIM> struct LeafRec;
+template <> struct LeafRec<3> { TriF t; };
+template <> struct LeafRec<2> { SegF s; float inv; };
 template <int DIM>
-__device__ __forceinline__ bool query_step(QueryF &q, const MeshView &m, float px, float py,
-                                           float pz, StackEntry *stk, int stride,
-                                           const void *cache, int n_cached) {
+__device__ __forceinline__ LeafRec<DIM> leaf_load(const MeshView &m, int slot) {
+    LeafRec<DIM> r;
+    if constexpr (DIM == 3) r.t = m.trif[slot];
+    else { r.s = m.segf[slot]; r.inv = m.seg_inv_vv[slot]; }
+    return r;
+}
+template <int DIM>
+__device__ __forceinline__ float leaf_dist(const LeafRec<DIM> &r, float px, float py, float pz) {
+    if constexpr (DIM == 3) return tri_d2f(r.t, px, py, pz);
+    else return seg_d2f(r.s, r.inv, px, py);
+}
+
+// One unit: an internal node, a BVH leaf primitive, or the next two entries
+// of a cell list.  Returns the kQs* bits of what it did; kQsMore is clear
+// when the query is complete.
+template <int DIM>
+__device__ __forceinline__ uint32_t query_step(QueryF &q, const MeshView &m, float px, float py,
+                                               float pz, StackEntry *stk, int stride,
+                                               const void *cache, int n_cached) {
     bool pop;
-    if (q.lcount > 0) {
-        int32_t slot = q.lslot;
-        bool skip = false;
-        if (q.indirect) {
-            // sorted by distance from the centre: once an entry's lower
-            // bound reaches cf x best, so do all that follow; below best it
-            // is kept in lb (a relaxed stop)
-            const GridEntry E = m.glist[q.lslot];
-            const float lbk = fmaxf(E.dmin - q.t, 0.0f);
-            const float lbk2 = lbk * lbk;
-            skip = lbk2 >= q.best * q.cf;
-            if (skip && lbk2 < q.best) q.lb = fminf(q.lb, lbk2);
-            slot = E.slot;
+    uint32_t did = 0u;
+    if (q.lcount > 0 && q.indirect) {
+        // The next two sorted entries at once, both primitive records in
+        // flight together.  An entry whose lower bound dmin - |p - c| reaches
+        // cf x best ends the scan (all later ones are farther), its bound
+        // kept in lb when below best (a relaxed stop).  The second entry is
+        // tested when its bound is below the threshold the first was checked
+        // against: a primitive the first one's result would have let the scan
+        // skip cannot lower an exact minimum.
+        const bool two = q.lcount > 1;
+        const GridEntry E0 = m.glist[q.lslot];
+        const GridEntry E1 = two ? m.glist[q.lslot + 1] : E0;
+        const LeafRec<DIM> R0 = leaf_load<DIM>(m, E0.slot);
+        const LeafRec<DIM> R1 = leaf_load<DIM>(m, E1.slot);
+        did |= (two ? 2u : 1u) << kQsEntryShift;
+        const float thr = q.best * q.cf;
+        const float l0 = fmaxf(E0.dmin - q.t, 0.0f), l1 = fmaxf(E1.dmin - q.t, 0.0f);
+        const float l02 = l0 * l0, l12 = l1 * l1;
+        if (l02 >= thr) {
+            if (l02 < q.best) q.lb = fminf(q.lb, l02);
+            q.lcount = 0;
+        } else {
+            const bool t1 = two && l12 < thr;
+            const float d0 = leaf_dist<DIM>(R0, px, py, pz);
+            const float d1 = t1 ? leaf_dist<DIM>(R1, px, py, pz) : INFINITY;
+            if (d0 < q.best) { q.best = d0; q.best_slot = E0.slot; }
+            if (d1 < q.best) { q.best = d1; q.best_slot = E1.slot; }
+            did |= (t1 ? 2u : 1u) << kQsPrimShift;
+            if (two && !t1) {
+                if (l12 < q.best) q.lb = fminf(q.lb, l12);
+                q.lcount = 0;
+            } else {
+                const int used = t1 ? 2 : 1;
+                q.lslot += used;
+                q.lcount -= used;
+            }
         }
-        if (!skip) {
-            const float d = leaf_d2f<DIM>(m, slot, px, py, pz);
-            if (d < q.best) { q.best = d; q.best_slot = slot; }
-        }
+        pop = q.lcount == 0;
+    } else if (q.lcount > 0) {
+        const int32_t slot = q.lslot;
+        const float d = leaf_d2f<DIM>(m, slot, px, py, pz);
+        if (d < q.best) { q.best = d; q.best_slot = slot; }
         ++q.lslot;
-        q.lcount = skip ? 0 : q.lcount - 1;
+        --q.lcount;
+        did |= kQsLeaf;
         pop = q.lcount == 0;
     } else {
         float dl, dr;
@@ -855,6 +907,7 @@ __device__ __forceinline__ bool query_step(QueryF &q, const MeshView &m, float p
             dr = box_d2f2(n.r_lox, n.r_loy, n.r_hix, n.r_hiy, px, py);
             cl = n.left; cr = n.right;
         }
+        did |= kQsNode;
         const float thr = q.best * q.cf;
         const bool hl = dl < thr, hr = dr < thr;
         if (!hl && dl < q.best) q.lb = fminf(q.lb, dl);
@@ -880,13 +933,13 @@ __device__ __forceinline__ bool query_step(QueryF &q, const MeshView &m, float p
             const StackEntry e = stk[(--q.sp) * stride];
             if (e.d2 < q.best * q.cf) {
                 query_enter(q, e.ref);
-                return true;
+                return did | kQsMore;
             }
             if (e.d2 < q.best) q.lb = fminf(q.lb, e.d2);
         }
-        return false;
+        return did;
     }
-    return true;
+    return did | kQsMore;
 }
 
 }  // namespace wosx

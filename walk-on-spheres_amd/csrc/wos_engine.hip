@@ -415,7 +415,8 @@ __global__ __launch_bounds__(kBlock) void k_closest(MeshView m, int32_t root64, 
         QueryF q{};
         q.best_slot = -1;
         bool more = query_begin<DIM>(q, m, root32, domain, cf, fx, fy, fz);
-        while (more) more = query_step<DIM>(q, m, fx, fy, fz, stk, blockDim.x, nullptr, 0);
+        while (more)
+            more = (query_step<DIM>(q, m, fx, fy, fz, stk, blockDim.x, nullptr, 0) & kQsMore) != 0u;
         dist[i] = sqrt((double)q.best);
         lower[i] = sqrt((double)fminf(q.best, q.lb));
         const int slot = q.best_slot;
@@ -890,7 +891,8 @@ void k_walk_fp32(MeshView m, EqView e, PersistArgs a, AdjOut adj) {
     bool exhausted = false;
     const uint64_t total = a.rv.total_dev ? (uint64_t)*a.rv.total_dev : (uint64_t)a.rv.total;
     // wave-uniform query-unit counts (a.units != nullptr): queries, grid
-    // bounds, list scans, list primitives, nodes, leaf primitives
+    // bounds, list scans, list entries read, nodes, leaf primitives, list
+    // primitives tested
     uint64_t u_q = 0, u_gb = 0, u_ls = 0, u_lp = 0, u_nd = 0, u_pr = 0, u_le = 0;
 
     for (;;) {
@@ -1021,20 +1023,18 @@ void k_walk_fp32(MeshView m, EqView e, PersistArgs a, AdjOut adj) {
 
         // ---- advance the queries until enough lanes are ready to step
         for (;;) {
-            if (a.units) {
-                const bool leaf = searching && q.lcount > 0;
-                bool stop_entry = false;
-                if (leaf && q.indirect) {
-                    const float lbk = fmaxf(m.glist[q.lslot].dmin - q.t, 0.0f);
-                    stop_entry = lbk * lbk >= q.best * q.cf;
-                }
-                u_nd += __popcll(__ballot(searching && !leaf));
-                u_lp += __popcll(__ballot(leaf && q.indirect));
-                u_le += __popcll(__ballot(stop_entry));
-                u_pr += __popcll(__ballot(leaf && !q.indirect));
+            uint32_t did = 0u;
+            if (searching) {
+                did = query_step<DIM>(q, m, px, py, pz, stk, stride, cache, a.n_cached);
+                searching = (did & kQsMore) != 0u;
             }
-            if (searching)
-                searching = query_step<DIM>(q, m, px, py, pz, stk, stride, cache, a.n_cached);
+            if (a.units) {
+                const uint32_t ne = (did >> kQsEntryShift) & 3u, np = (did >> kQsPrimShift) & 3u;
+                u_nd += __popcll(__ballot((did & kQsNode) != 0u));
+                u_pr += __popcll(__ballot((did & kQsLeaf) != 0u));
+                u_lp += __popcll(__ballot(ne & 1u)) + 2 * __popcll(__ballot(ne & 2u));
+                u_le += __popcll(__ballot(np & 1u)) + 2 * __popcll(__ballot(np & 2u));
+            }
             if (__ballot(searching) == 0) break;
             const uint64_t ready = __ballot(!searching && (active || !exhausted));
             if (__popcll(ready) >= a.min_ready) break;
@@ -1651,7 +1651,7 @@ int finish_solve(wos_ctx *c, const std::vector<Part> &parts,
         HIPX(hipMemcpyAsync(u, c->units.p, sizeof(u), hipMemcpyDeviceToHost, c->stream));
         HIPX(hipStreamSynchronize(c->stream));
         S.q_queries = (int64_t)u[0]; S.q_grid_bound = (int64_t)u[1]; S.q_list_scans = (int64_t)u[2];
-        S.q_list_entries = (int64_t)u[3]; S.q_list_prims = (int64_t)(u[3] - u[6]);
+        S.q_list_entries = (int64_t)u[3]; S.q_list_prims = (int64_t)u[6];
         S.q_nodes = (int64_t)u[4]; S.q_leaf_prims = (int64_t)u[5];
         c->count_units = false;
     }
