@@ -178,6 +178,11 @@ typedef struct wos_tuning {
                                     bounds inside); 0 = no grid                    */
     int32_t grid_list_max;       /* longest cell list kept (1..127); a cell with
                                     more candidate primitives keeps none           */
+    int32_t xcd_parts;           /* the walk items are split into this many
+                                    contiguous parts (regions of the domain) and
+                                    workgroup b starts on part b % xcd_parts, which
+                                    keeps each XCD's L2 on one region; 8 = the
+                                    XCDs of an MI355X, 1 = one shared range       */
 } wos_tuning;
 
 typedef struct wos_ctx wos_ctx;
@@ -254,7 +259,8 @@ int wos_ctx_get_tuning(const wos_ctx *ctx, wos_tuning *tuning);
 /* Autotune the FP32 walker's launch shape on an Nx x Ny x Nz grid problem
  * over every domain of the context (the problem wos_solve_grid_batch would
  * solve): pilot solves of about 16 M walks, coordinate descent over walkers
- * per block, lanes per walk-step phase, refill chunk and LDS node cache.
+ * per block, lanes per walk-step phase, refill chunk, LDS node cache and
+ * xcd_parts.
  * These fields never change a result, so a tuned solve is bit-identical to
  * an untuned one; grid_cells, grid_list_max and relax are kept as set.  The
  * chosen shape becomes the context's tuning, is written to `chosen` (may be

@@ -21,11 +21,13 @@ pytestmark = pytest.mark.gpu
 # nearest primitive of every point of its cell, so the grid changes how the
 # nearest primitive is found but not which distance comes out
 SHAPES = [
-    dict(block_threads=64, chunk=1, min_ready=0, lds_nodes=0, grid_cells=0),
-    dict(block_threads=128, chunk=4096, min_ready=64, lds_nodes=7, grid_cells=1 << 12),
-    dict(block_threads=192, chunk=33, min_ready=1, lds_nodes=-1, grid_cells=1 << 18),
+    dict(block_threads=64, chunk=1, min_ready=0, lds_nodes=0, grid_cells=0, xcd_parts=1),
+    dict(block_threads=128, chunk=4096, min_ready=64, lds_nodes=7, grid_cells=1 << 12,
+         xcd_parts=3),
+    dict(block_threads=192, chunk=33, min_ready=1, lds_nodes=-1, grid_cells=1 << 18,
+         xcd_parts=64),
     dict(block_threads=256, chunk=256, min_ready=32, lds_nodes=0, blocks_per_cu=1,
-         grid_cells=1 << 21),
+         grid_cells=1 << 21, xcd_parts=8),
 ]
 
 # relaxed queries: the grid (which decides the radii) stays, the rest varies
@@ -54,7 +56,7 @@ def test_fp32_independent_of_launch_shape(ctx, name, mesh_fn, eq_fn, grid, eps, 
     eq = eq_fn()
     p = Context.params(64, eps, 5, precision=WOS_FP32_PERSISTENT)
     ctx.set_tuning(**dict(shapes[0], relax=relax, block_threads=256, chunk=64, min_ready=16,
-                          lds_nodes=-1))
+                          lds_nodes=-1, xcd_parts=8))
     ref = ctx.solve_grid(*grid, eq, p)
     for shape in shapes:
         ctx.set_tuning(**dict(shape, relax=relax))
@@ -120,6 +122,10 @@ def test_tuning_validation(ctx):
         ctx.set_tuning(block_threads=96)
     with pytest.raises(RuntimeError):
         ctx.set_tuning(grid_cells=-1)
+    with pytest.raises(RuntimeError):
+        ctx.set_tuning(xcd_parts=0)
+    with pytest.raises(RuntimeError):
+        ctx.set_tuning(xcd_parts=65)
     ctx.set_tuning(min_ready=7, relax=0.25, grid_cells=4096)
     t = ctx.get_tuning()
     assert t["min_ready"] == 7 and abs(t["relax"] - 0.25) < 1e-7 and t["grid_cells"] == 4096

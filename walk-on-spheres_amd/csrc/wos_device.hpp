@@ -858,8 +858,6 @@ __device__ __forceinline__ uint32_t query_step(QueryF &q, const MeshView &m, flo
         const bool two = q.lcount > 1;
         const GridEntry E0 = m.glist[q.lslot];
         const GridEntry E1 = two ? m.glist[q.lslot + 1] : E0;
-        const LeafRec<DIM> R0 = leaf_load<DIM>(m, E0.slot);
-        const LeafRec<DIM> R1 = leaf_load<DIM>(m, E1.slot);
         did |= (two ? 2u : 1u) << kQsEntryShift;
         const float thr = q.best * q.cf;
         const float l0 = fmaxf(E0.dmin - q.t, 0.0f), l1 = fmaxf(E1.dmin - q.t, 0.0f);
@@ -868,7 +866,11 @@ __device__ __forceinline__ uint32_t query_step(QueryF &q, const MeshView &m, flo
             if (l02 < q.best) q.lb = fminf(q.lb, l02);
             q.lcount = 0;
         } else {
+            // only the records that are tested are loaded, both in flight
             const bool t1 = two && l12 < thr;
+            const LeafRec<DIM> R0 = leaf_load<DIM>(m, E0.slot);
+            LeafRec<DIM> R1 = R0;
+            if (t1) R1 = leaf_load<DIM>(m, E1.slot);
             const float d0 = leaf_dist<DIM>(R0, px, py, pz);
             const float d1 = t1 ? leaf_dist<DIM>(R1, px, py, pz) : INFINITY;
             if (d0 < q.best) { q.best = d0; q.best_slot = E0.slot; }

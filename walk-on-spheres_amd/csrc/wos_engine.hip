@@ -34,6 +34,7 @@ using namespace wosx;
 namespace {
 
 constexpr int kBlock = 256;
+constexpr int kMaxParts = 64;  // largest xcd_parts
 
 // --------------------------------------------------------------------------
 // walk-step helpers (FP64): src/wos.cpp:9-56, include/wos/wos.hpp:54-138
@@ -762,6 +763,15 @@ __global__ void k_zero_u64(unsigned long long *p, int64_t n) {
         p[i] = 0ull;
 }
 
+// first item of every part of the walker's item range
+__global__ void k_init_parts(unsigned long long *counter, int parts, int64_t total,
+                             const int64_t *total_dev) {
+    const int p = threadIdx.x;
+    if (p >= parts) return;
+    const uint64_t t = total_dev ? (uint64_t)*total_dev : (uint64_t)total;
+    counter[16 * p] = t * (uint64_t)p / (uint64_t)parts;
+}
+
 __global__ void k_zero_u32(uint32_t *p, int64_t n) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x)
@@ -841,7 +851,8 @@ struct PersistArgs {
     float shrink_rel;
     float cf;            // pruning factor of step queries, (1 - relax)^2
     int min_ready;       // finished lanes that start a walk-step phase
-    unsigned long long *counter;
+    unsigned long long *counter;  // one per part, 16 words apart (own cache line)
+    int parts;           // item range split into this many contiguous parts
     int chunk;
     int n_cached;        // internal BVH nodes staged in LDS (breadth-first prefix)
     int cache_bytes;     // LDS bytes reserved for them (16 B aligned)
@@ -886,10 +897,15 @@ void k_walk_fp32(MeshView m, EqView e, PersistArgs a, AdjOut adj) {
     const float alpha = (float)e.alpha;
     WalkDer D{};
 
-    // wave-uniform pool of work items
+    // wave-uniform pool of work items.  The item range is split into
+    // a.parts contiguous parts (points are in sweep order, so a part is a
+    // region of the domain); the hardware places workgroups on the XCDs
+    // round-robin, so taking part blockIdx.x % parts first keeps each XCD's
+    // L2 on one region.  An exhausted part sends the wave to the next one.
     uint64_t pool_next = 0, pool_end = 0;
     bool exhausted = false;
     const uint64_t total = a.rv.total_dev ? (uint64_t)*a.rv.total_dev : (uint64_t)a.rv.total;
+    int part = (int)(blockIdx.x % (unsigned)a.parts), parts_left = a.parts;
     // wave-uniform query-unit counts (a.units != nullptr): queries, grid
     // bounds, list scans, list entries read, nodes, leaf primitives, list
     // primitives tested
@@ -928,16 +944,21 @@ void k_walk_fp32(MeshView m, EqView e, PersistArgs a, AdjOut adj) {
         // ---- refill free lanes from the wave pool
         const uint64_t free_mask = __ballot(!active);
         if (free_mask != 0 && !exhausted) {
-            if (pool_next >= pool_end) {
+            while (pool_next >= pool_end) {
+                const uint64_t part_end = total * (uint64_t)(part + 1) / (uint64_t)a.parts;
                 unsigned long long base = 0;
-                if (lane == 0) base = atomicAdd(a.counter, (unsigned long long)a.chunk);
+                if (lane == 0)
+                    base = atomicAdd(a.counter + 16 * part, (unsigned long long)a.chunk);
                 base = __shfl(base, 0);
-                if (base >= (unsigned long long)total) {
-                    exhausted = true;
-                } else {
+                if (base < part_end) {
                     pool_next = base;
                     pool_end = base + (uint64_t)a.chunk;
-                    if (pool_end > total) pool_end = total;
+                    if (pool_end > part_end) pool_end = part_end;
+                } else if (--parts_left == 0) {
+                    exhausted = true;
+                    break;
+                } else {
+                    part = part + 1 == a.parts ? 0 : part + 1;
                 }
             }
             if (!exhausted) {
@@ -1219,7 +1240,7 @@ struct wos_ctx {
     int last_rounds = 0;
     MeshView view{};
     hipEvent_t ev[6]{};
-    wos_tuning tune{256, -1, 64, 0, 48, 0.2f, 1 << 23, 127};
+    wos_tuning tune{256, -1, 64, 0, 48, 0.2f, 1 << 23, 127, 8};
     size_t lds_per_cu = 65536;
     size_t lds_per_block = 65536;
     int last_threads = 0;
@@ -1399,7 +1420,9 @@ hipError_t launch_walk(wos_ctx *c, const EqView &ev, int precision, const RoundV
     a.vals = c->vals.as<double>();
     a.steps_out = c->steps.as<int32_t>();
     a.units = c->count_units ? c->units.as<unsigned long long>() : nullptr;
-    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, c->stream, a.counter, 1);
+    a.parts = c->tune.xcd_parts;
+    hipLaunchKernelGGL(k_init_parts, dim3(1), dim3(64), 0, c->stream, a.counter, a.parts,
+                       (int64_t)rv.total, rv.total_dev);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
     if (green)
@@ -1712,7 +1735,7 @@ int solve_once(wos_ctx *c, const std::vector<Part> &parts, const wos_eq_desc *eq
         HIPX(hipMemsetAsync(c->count.p, 0, n_in * sizeof(double), c->stream));
         HIPX(hipMemsetAsync(c->ssum.p, 0, n_in * sizeof(double), c->stream));
         HIPX(hipMemsetAsync(c->hits.p, 0, n_in * sizeof(int32_t), c->stream));
-        HIPX(c->counter.reserve(sizeof(unsigned long long)));
+        HIPX(c->counter.reserve(16 * kMaxParts * sizeof(unsigned long long)));
         const bool src = eq->source_kind != WOS_SRC_NONE;
         const bool scr = eq->kind == WOS_EQ_SCREENED;
         const bool adaptive = prm->target_se > 0.0;
@@ -1955,7 +1978,7 @@ int solve_graph(wos_ctx *c, const std::vector<Part> &parts, const wos_eq_desc *e
     HIPX(c->hits.reserve((size_t)n * sizeof(int32_t)));
     HIPX(c->vals.reserve((size_t)(n * nw) * sizeof(double)));
     HIPX(c->steps.reserve((size_t)(n * nw) * sizeof(int32_t)));
-    HIPX(c->counter.reserve(sizeof(unsigned long long)));
+    HIPX(c->counter.reserve(16 * kMaxParts * sizeof(unsigned long long)));
     if (c->pid_all.bytes < (size_t)n * sizeof(int32_t)) {
         std::vector<int32_t> ids((size_t)n);
         for (int64_t k = 0; k < n; ++k) ids[(size_t)k] = (int32_t)k;
@@ -2117,6 +2140,7 @@ int autotune(wos_ctx *c, const std::vector<Part> &parts, const wos_eq_desc *eq,
     static const int kThreads[] = {128, 192, 256};
     static const int kChunk[] = {32, 64, 256};
     static const int kLds[] = {0, -1};
+    static const int kParts[] = {1, 8};
     auto descend = [&](int wos_tuning::*field, const int *vals, int nv) -> int {
         for (int i = 0; i < nv; ++i) {
             if (cur.*field == vals[i]) continue;
@@ -2131,8 +2155,9 @@ int autotune(wos_ctx *c, const std::vector<Part> &parts, const wos_eq_desc *eq,
     };
     if ((rc = descend(&wos_tuning::min_ready, kReady, 6)) == WOS_OK &&
         (rc = descend(&wos_tuning::block_threads, kThreads, 3)) == WOS_OK &&
-        (rc = descend(&wos_tuning::chunk, kChunk, 3)) == WOS_OK)
-        rc = descend(&wos_tuning::lds_nodes, kLds, 2);
+        (rc = descend(&wos_tuning::chunk, kChunk, 3)) == WOS_OK &&
+        (rc = descend(&wos_tuning::lds_nodes, kLds, 2)) == WOS_OK)
+        rc = descend(&wos_tuning::xcd_parts, kParts, 2);
     c->tune = saved;
     if (rc != WOS_OK) return rc;
     *chosen = cur;
@@ -2173,6 +2198,7 @@ int solve(wos_ctx *c, const std::vector<Part> &parts, const wos_eq_desc *eq,
     c->tune.min_ready = shape.min_ready;
     c->tune.chunk = shape.chunk;
     c->tune.lds_nodes = shape.lds_nodes;
+    c->tune.xcd_parts = shape.xcd_parts;
     const int rc = run();
     c->tune = saved;
     return rc;
@@ -2366,6 +2392,8 @@ int wos_ctx_set_tuning(wos_ctx *c, const wos_tuning *t) {
         return fail(WOS_ERR_INPUT, "grid_cells must be in 0..2^26");
     if (t->grid_list_max < 1 || t->grid_list_max > kGridListMax)
         return fail(WOS_ERR_INPUT, "grid_list_max must be in 1..127");
+    if (t->xcd_parts < 1 || t->xcd_parts > kMaxParts)
+        return fail(WOS_ERR_INPUT, "xcd_parts must be in 1..64");
     c->tune = *t;
     return WOS_OK;
 }

@@ -123,6 +123,7 @@ class Tuning(C.Structure):
         ("relax", C.c_float),
         ("grid_cells", C.c_int32),
         ("grid_list_max", C.c_int32),
+        ("xcd_parts", C.c_int32),
     ]
 
 
