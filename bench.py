@@ -109,8 +109,8 @@ def copy_calibration(mib):
 # an internal node (NodeF3 64 / NodeF2 48); per walk: its start record (120),
 # value + steps (12), the FP64 terminal primitive (TriD 128 / SegD 32) and
 # its slot -> primitive id (4).
-RECORDS = {3: dict(cell=8, entry=8, prim=64, node=64, walk=120 + 12 + 128 + 4),
-           2: dict(cell=8, entry=8, prim=20, node=48, walk=120 + 12 + 32 + 4)}
+RECORDS = {3: dict(cell=16, entry=8, prim=64, node=64, walk=120 + 12 + 128 + 4),
+           2: dict(cell=16, entry=8, prim=20, node=48, walk=120 + 12 + 32 + 4)}
 
 
 def walker_bytes(s, dim):

@@ -29,7 +29,10 @@ namespace wosx {
 // distance from the centre (rounded down) and are sorted by it, so a scan
 // from p can stop at the first entry with dmin - |p - c| >= its best.  A
 // cell whose list would exceed the build's cap stores none (length 0), and
-// so does a cell whose bound dc already answers every relaxed query.
+// so does a cell whose bound dc already answers every relaxed query.  Entry
+// 0 (always tested: the scan starts from an infinite best) and the bound of
+// entry 1 are repeated in the cell record, so a scan that entry 1 cannot
+// improve never reads the list.
 // --------------------------------------------------------------------------
 struct GridInfo {
     float ox, oy, oz;     // corner of cell (0, 0, 0)
@@ -38,9 +41,11 @@ struct GridInfo {
     int32_t nx, ny, nz;   // cells per axis (nz = 1 in 2D)
     int32_t cell_off;     // first cell of this domain in the cell array
 };
-struct GridCellF {
+struct alignas(16) GridCellF {
     uint32_t dcbits;   // FP32 bits of dc, low 7 bits = list length
     uint32_t first;    // first list entry
+    int32_t slot0;     // leaf slot of list entry 0 (nearest to the centre), -1 if none
+    float dmin1;       // list entry 1's distance from the centre, +inf if none
 };
 struct GridEntry {
     int32_t slot;      // leaf slot of the primitive
@@ -760,17 +765,26 @@ __device__ __forceinline__ void query_enter(QueryF &q, int32_t ref) {
     }
 }
 
-// Begin a query at p in `domain`.  With a distance grid, a cell that holds
-// a primitive list turns the query into a scan of that list (exact); with
-// cf < 1, a cell whose centre distance dc gives lb = dc - |p - c| and
-// ub = dc' + |p - c| (dc' = dc rounded up) with lb^2 >= cf ub^2 ends it at
-// once with best = ub^2, lb = lb^2.  Otherwise the search starts at `root`
-// with best from the primitive in q.best_slot (the previous step's nearest,
-// -1 for none), which bounds it from the first node on.  Returns false when
-// the query is already complete.
+// What one query_step or query_begin did (bits of its return value).
+constexpr uint32_t kQsMore = 1u;         // the query continues
+constexpr uint32_t kQsNode = 2u;         // opened an internal node
+constexpr uint32_t kQsLeaf = 4u;         // tested a BVH leaf primitive
+constexpr uint32_t kQsEntryShift = 3u;   // cell-list entries read, 0..2 (bits 3-4)
+constexpr uint32_t kQsPrimShift = 5u;    // cell-list primitives tested, 0..2 (bits 5-6)
+
+// Begin a query at p in `domain`.  With a distance grid: with cf < 1, a cell
+// whose centre distance dc gives lb = dc - |p - c| and ub = dc' + |p - c|
+// (dc' = dc rounded up) with lb^2 >= cf ub^2 ends it at once with
+// best = ub^2, lb = lb^2; otherwise a cell that holds a primitive list turns
+// it into a scan of that list (exact for cf = 1), whose entry 0 is tested
+// here.  Otherwise the search starts at `root` with best from the primitive
+// in q.best_slot (the previous step's nearest, -1 for none), which bounds it
+// from the first node on.  Returns the kQs* bits; kQsMore is clear when the
+// query is already complete.
 template <int DIM>
-__device__ __forceinline__ bool query_begin(QueryF &q, const MeshView &m, int32_t root,
-                                            int domain, float cf, float px, float py, float pz) {
+__device__ __forceinline__ uint32_t query_begin(QueryF &q, const MeshView &m, int32_t root,
+                                                int domain, float cf, float px, float py,
+                                                float pz) {
     q.lb = INFINITY;
     q.cf = cf;
     q.sp = 0;
@@ -794,31 +808,35 @@ __device__ __forceinline__ bool query_begin(QueryF &q, const MeshView &m, int32_
                 if (lb > 0.0f && lb * lb >= cf * (ub * ub)) {
                     q.best = ub * ub;
                     q.lb = lb * lb;
-                    return false;
+                    return 0u;
                 }
             }
             const uint32_t len = C.dcbits & kGridLenMask;
             if (len != 0u) {
+                // entry 0 now; the list only if entry 1's bound is below cf x best
                 q.indirect = true;
-                q.lslot = (int32_t)C.first;
-                q.lcount = (int32_t)len;
                 q.t = t;
-                q.best = INFINITY;
-                return true;
+                q.best = leaf_d2f<DIM>(m, C.slot0, px, py, pz);
+                q.best_slot = C.slot0;
+                q.lslot = (int32_t)C.first + 1;
+                q.lcount = (int32_t)len - 1;
+                if (q.lcount > 0) {
+                    const float l1 = fmaxf(C.dmin1 - t, 0.0f);
+                    const float l12 = l1 * l1;
+                    if (l12 >= q.best * cf) {
+                        if (l12 < q.best) q.lb = fminf(q.lb, l12);
+                        q.lcount = 0;
+                    }
+                }
+                const uint32_t did = 1u << kQsPrimShift;
+                return q.lcount > 0 ? did | kQsMore : did;
             }
         }
     }
     q.best = q.best_slot >= 0 ? leaf_d2f<DIM>(m, q.best_slot, px, py, pz) : INFINITY;
     query_enter(q, root);
-    return true;
+    return kQsMore;
 }
-
-// What one query_step did (bits of its return value).
-constexpr uint32_t kQsMore = 1u;         // the query continues
-constexpr uint32_t kQsNode = 2u;         // opened an internal node
-constexpr uint32_t kQsLeaf = 4u;         // tested a BVH leaf primitive
-constexpr uint32_t kQsEntryShift = 3u;   // cell-list entries read, 0..2 (bits 3-4)
-constexpr uint32_t kQsPrimShift = 5u;    // cell-list primitives tested, 0..2 (bits 5-6)
 
 // a leaf primitive's FP32 record, loaded apart from its distance test so
 // that two loads can be in flight together

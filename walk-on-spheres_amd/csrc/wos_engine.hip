@@ -376,6 +376,8 @@ __global__ __launch_bounds__(kBlock) void k_grid_build(MeshView m, GridInfo G, i
                                   : cap + 1;
         cells[c].dcbits = __float_as_uint(round_down_f(dc)) & ~kGridLenMask;
         cells[c].first = 0u;
+        cells[c].slot0 = -1;
+        cells[c].dmin1 = INFINITY;
         counts[c] = cnt <= cap ? cnt : 0;
         rl2[c] = rl * rl;
     } else if (counts[c] > 0) {
@@ -384,6 +386,8 @@ __global__ __launch_bounds__(kBlock) void k_grid_build(MeshView m, GridInfo G, i
                                            glist + off, cap);
         cells[c].dcbits |= (uint32_t)cnt;
         cells[c].first = (uint32_t)off;
+        cells[c].slot0 = glist[off].slot;
+        cells[c].dmin1 = cnt > 1 ? glist[off + 1].dmin : INFINITY;
     }
 }
 
@@ -415,7 +419,7 @@ __global__ __launch_bounds__(kBlock) void k_closest(MeshView m, int32_t root64, 
         const float fx = (float)x, fy = (float)y, fz = (float)z;
         QueryF q{};
         q.best_slot = -1;
-        bool more = query_begin<DIM>(q, m, root32, domain, cf, fx, fy, fz);
+        bool more = (query_begin<DIM>(q, m, root32, domain, cf, fx, fy, fz) & kQsMore) != 0u;
         while (more)
             more = (query_step<DIM>(q, m, fx, fy, fz, stk, blockDim.x, nullptr, 0) & kQsMore) != 0u;
         dist[i] = sqrt((double)q.best);
@@ -909,17 +913,19 @@ void k_walk_fp32(MeshView m, EqView e, PersistArgs a, AdjOut adj) {
     // wave-uniform query-unit counts (a.units != nullptr): queries, grid
     // bounds, list scans, list entries read, nodes, leaf primitives, list
     // primitives tested
-    uint64_t u_q = 0, u_gb = 0, u_ls = 0, u_lp = 0, u_nd = 0, u_pr = 0, u_le = 0;
+    uint64_t u_q = 0, u_gb = 0, u_ls = 0, u_lent = 0, u_nd = 0, u_pr = 0, u_lprim = 0;
 
     for (;;) {
         bool began = false;
+        uint32_t qb = 0u;
         // ---- finished queries: radius, then end of walk or exact re-query
         if (active && !searching) {
             const bool exact = !(q.lb < q.best);
             R = sqrtf(exact ? q.best : q.lb) * a.shrink_rel - margin;
             const bool stop = !(R > a.eps_f) || steps >= a.max_steps;
             if (stop && !exact) {
-                searching = query_begin<DIM>(q, m, root, dom, 1.0f, px, py, pz);
+                qb = query_begin<DIM>(q, m, root, dom, 1.0f, px, py, pz);
+                searching = (qb & kQsMore) != 0u;
                 began = true;
             } else if (stop) {
                 // terminal value in FP64: nearest point on the nearest primitive
@@ -1029,13 +1035,15 @@ void k_walk_fp32(MeshView m, EqView e, PersistArgs a, AdjOut adj) {
                 pz = __builtin_fmaf(R, z, pz);
             }
             ++steps;
-            searching = query_begin<DIM>(q, m, root, dom, a.cf, px, py, pz);
+            qb = query_begin<DIM>(q, m, root, dom, a.cf, px, py, pz);
+            searching = (qb & kQsMore) != 0u;
             began = true;
         }
         if (a.units) {
             u_q += __popcll(__ballot(began));
-            u_gb += __popcll(__ballot(began && !searching));
-            u_ls += __popcll(__ballot(began && searching && q.indirect));
+            u_gb += __popcll(__ballot(began && !q.indirect && !searching));
+            u_ls += __popcll(__ballot(began && q.indirect));
+            u_lprim += __popcll(__ballot(((qb >> kQsPrimShift) & 3u) != 0u));
         }
         if (__ballot(active) == 0) {
             if (exhausted) break;
@@ -1053,8 +1061,8 @@ void k_walk_fp32(MeshView m, EqView e, PersistArgs a, AdjOut adj) {
                 const uint32_t ne = (did >> kQsEntryShift) & 3u, np = (did >> kQsPrimShift) & 3u;
                 u_nd += __popcll(__ballot((did & kQsNode) != 0u));
                 u_pr += __popcll(__ballot((did & kQsLeaf) != 0u));
-                u_lp += __popcll(__ballot(ne & 1u)) + 2 * __popcll(__ballot(ne & 2u));
-                u_le += __popcll(__ballot(np & 1u)) + 2 * __popcll(__ballot(np & 2u));
+                u_lent += __popcll(__ballot(ne & 1u)) + 2 * __popcll(__ballot(ne & 2u));
+                u_lprim += __popcll(__ballot(np & 1u)) + 2 * __popcll(__ballot(np & 2u));
             }
             if (__ballot(searching) == 0) break;
             const uint64_t ready = __ballot(!searching && (active || !exhausted));
@@ -1065,10 +1073,10 @@ void k_walk_fp32(MeshView m, EqView e, PersistArgs a, AdjOut adj) {
         atomicAdd(a.units + 0, (unsigned long long)u_q);
         atomicAdd(a.units + 1, (unsigned long long)u_gb);
         atomicAdd(a.units + 2, (unsigned long long)u_ls);
-        atomicAdd(a.units + 3, (unsigned long long)u_lp);
+        atomicAdd(a.units + 3, (unsigned long long)u_lent);
         atomicAdd(a.units + 4, (unsigned long long)u_nd);
         atomicAdd(a.units + 5, (unsigned long long)u_pr);
-        atomicAdd(a.units + 6, (unsigned long long)u_le);
+        atomicAdd(a.units + 6, (unsigned long long)u_lprim);
     }
 }
 
