@@ -751,6 +751,8 @@ struct QueryF {
     float lb;         // smallest skipped box distance
     float cf;         // pruning factor
     int32_t best_slot;
+    uint32_t ckey;    // 1 + index of the grid cell held in `cell`, 0 for none
+    GridCellF cell;   // the last grid cell read (cells do not change during a launch)
 };
 
 // Set the next unit from a child reference.
@@ -772,6 +774,27 @@ constexpr uint32_t kQsLeaf = 4u;         // tested a BVH leaf primitive
 constexpr uint32_t kQsEntryShift = 3u;   // cell-list entries read, 0..2 (bits 3-4)
 constexpr uint32_t kQsPrimShift = 5u;    // cell-list primitives tested, 0..2 (bits 5-6)
 
+// A domain's grid parameters.  When every calling lane is in one domain
+// (always for a single mesh) the record is read through the constant address
+// space with a wave-uniform index, i.e. by scalar loads, which keeps 48 B per
+// lane off the vector memory path of every query.
+__device__ __forceinline__ GridInfo grid_info(const GridInfo *grids, int domain) {
+    static_assert(sizeof(GridInfo) == 40, "GridInfo is ten 32-bit words");
+    const int d0 = __builtin_amdgcn_readfirstlane(domain);
+    GridInfo G;
+    if (__ballot(domain != d0) == 0) {
+        typedef __attribute__((address_space(4))) const uint32_t ConstWord;
+        const ConstWord *src = (const ConstWord *)(grids + d0);
+        uint32_t w[10];
+#pragma unroll
+        for (int k = 0; k < 10; ++k) w[k] = (uint32_t)__builtin_amdgcn_readfirstlane((int)src[k]);
+        __builtin_memcpy(&G, w, sizeof(G));
+    } else {
+        G = grids[domain];
+    }
+    return G;
+}
+
 // Begin a query at p in `domain`.  With a distance grid: with cf < 1, a cell
 // whose centre distance dc gives lb = dc - |p - c| and ub = dc' + |p - c|
 // (dc' = dc rounded up) with lb^2 >= cf ub^2 ends it at once with
@@ -790,12 +813,18 @@ __device__ __forceinline__ uint32_t query_begin(QueryF &q, const MeshView &m, in
     q.sp = 0;
     q.indirect = false;
     if (m.grids) {
-        const GridInfo G = m.grids[domain];
+        const GridInfo G = grid_info(m.grids, domain);
         const float fx = (px - G.ox) * G.inv_h, fy = (py - G.oy) * G.inv_h;
         const float fz = DIM == 3 ? (pz - G.oz) * G.inv_h : 0.5f;
         const int ix = (int)floorf(fx), iy = (int)floorf(fy), iz = (int)floorf(fz);
         if (ix >= 0 && iy >= 0 && iz >= 0 && ix < G.nx && iy < G.ny && iz < G.nz) {
-            const GridCellF C = m.cells[G.cell_off + (ix * G.ny + iy) * G.nz + iz];
+            // steps near the boundary are short and often stay in one cell
+            const uint32_t key = (uint32_t)(G.cell_off + (ix * G.ny + iy) * G.nz + iz) + 1u;
+            if (key != q.ckey) {
+                q.cell = m.cells[key - 1u];
+                q.ckey = key;
+            }
+            const GridCellF C = q.cell;
             const float dx = px - (G.ox + ((float)ix + 0.5f) * G.h);
             const float dy = py - (G.oy + ((float)iy + 0.5f) * G.h);
             const float dz = DIM == 3 ? pz - (G.oz + ((float)iz + 0.5f) * G.h) : 0.0f;
