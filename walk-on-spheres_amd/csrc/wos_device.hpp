@@ -57,6 +57,12 @@ constexpr uint32_t kGridLenMask = 127u;
 // --------------------------------------------------------------------------
 // kernel-side views
 // --------------------------------------------------------------------------
+// per-domain constants of the FP32 walker
+struct alignas(8) DomainF {
+    int32_t root32;       // FP32 root reference
+    float shrink_abs;     // absolute margin of FP32 radii
+};
+
 struct MeshView {
     int dim;
     int n_prims;
@@ -81,6 +87,7 @@ struct MeshView {
     const GridInfo *grids;     // per domain
     const GridCellF *cells;
     const GridEntry *glist;    // cell lists
+    const DomainF *doms;       // per domain
 };
 
 struct EqView {
@@ -106,6 +113,15 @@ struct StartInfo {
     uint32_t key0, key1;  // walk-stream key
     int32_t domain;       // the point's domain index
     int32_t pad;
+};
+
+// the FP32 walker's copy of a start (two 16 B loads), one per inside point
+struct alignas(16) StartF {
+    float x, y, z;        // the point, rounded to FP32
+    float r;              // distance to the boundary, rounded to FP32
+    int32_t slot;         // leaf slot of the nearest primitive
+    int32_t domain;
+    uint32_t key0, key1;  // walk-stream key
 };
 
 // nearest-point query result (FP64)
@@ -774,25 +790,27 @@ constexpr uint32_t kQsLeaf = 4u;         // tested a BVH leaf primitive
 constexpr uint32_t kQsEntryShift = 3u;   // cell-list entries read, 0..2 (bits 3-4)
 constexpr uint32_t kQsPrimShift = 5u;    // cell-list primitives tested, 0..2 (bits 5-6)
 
-// A domain's grid parameters.  When every calling lane is in one domain
-// (always for a single mesh) the record is read through the constant address
-// space with a wave-uniform index, i.e. by scalar loads, which keeps 48 B per
-// lane off the vector memory path of every query.
-__device__ __forceinline__ GridInfo grid_info(const GridInfo *grids, int domain) {
-    static_assert(sizeof(GridInfo) == 40, "GridInfo is ten 32-bit words");
-    const int d0 = __builtin_amdgcn_readfirstlane(domain);
-    GridInfo G;
-    if (__ballot(domain != d0) == 0) {
+// arr[index] of a small read-only per-domain table.  When every calling lane
+// has the same index (always for a single mesh) the record is read through
+// the constant address space with a wave-uniform index, i.e. by scalar
+// loads, which keeps it off the vector memory path.
+template <typename T>
+__device__ __forceinline__ T domain_record(const T *arr, int index) {
+    static_assert(sizeof(T) % 4 == 0, "whole 32-bit words");
+    constexpr int kWords = (int)(sizeof(T) / 4);
+    const int i0 = __builtin_amdgcn_readfirstlane(index);
+    T rec;
+    if (__ballot(index != i0) == 0) {
         typedef __attribute__((address_space(4))) const uint32_t ConstWord;
-        const ConstWord *src = (const ConstWord *)(grids + d0);
-        uint32_t w[10];
+        const ConstWord *src = (const ConstWord *)(arr + i0);
+        uint32_t w[kWords];
 #pragma unroll
-        for (int k = 0; k < 10; ++k) w[k] = (uint32_t)__builtin_amdgcn_readfirstlane((int)src[k]);
-        __builtin_memcpy(&G, w, sizeof(G));
+        for (int k = 0; k < kWords; ++k) w[k] = (uint32_t)__builtin_amdgcn_readfirstlane((int)src[k]);
+        __builtin_memcpy(&rec, w, sizeof(rec));
     } else {
-        G = grids[domain];
+        rec = arr[index];
     }
-    return G;
+    return rec;
 }
 
 // Begin a query at p in `domain`.  With a distance grid: with cf < 1, a cell
@@ -813,7 +831,7 @@ __device__ __forceinline__ uint32_t query_begin(QueryF &q, const MeshView &m, in
     q.sp = 0;
     q.indirect = false;
     if (m.grids) {
-        const GridInfo G = grid_info(m.grids, domain);
+        const GridInfo G = domain_record(m.grids, domain);
         const float fx = (px - G.ox) * G.inv_h, fy = (py - G.oy) * G.inv_h;
         const float fz = DIM == 3 ? (pz - G.oz) * G.inv_h : 0.5f;
         const int ix = (int)floorf(fx), iy = (int)floorf(fy), iz = (int)floorf(fz);

@@ -476,12 +476,17 @@ __device__ double source_term_adj(const EqView &e, T px, T py, T pz, T R, T weig
 // off[k] <= i < off[k+1], and its result goes to staging slot i (point major).
 // off == nullptr: every point runs cw walks starting at w0.
 // --------------------------------------------------------------------------
+constexpr int kItemBlockShift = 6;   // items per entry of RoundView::kblk: 64
+
 struct RoundView {
     const StartInfo *starts;
+    const StartF *sf;        // inside id -> FP32 start copy
     const int32_t *inside;   // inside id -> point index
     const int32_t *pid;      // active k -> inside id
+    int pid_identity;        // pid[k] == k for every k
     const int32_t *wbeg;     // active k -> first walk index (off != nullptr)
     const int64_t *off;      // exclusive prefix of counts, n + 1 entries, or nullptr
+    const int32_t *kblk;     // (off != nullptr) item block b -> the k holding item b * 64
     int n;
     int cw;
     int w0;
@@ -492,21 +497,26 @@ struct RoundView {
     const int64_t *total_dev;
 };
 
-__device__ __forceinline__ const StartInfo &round_item(const RoundView &r, int64_t i, uint32_t &walk) {
+// The active point k of item i and the walk index it runs.
+__device__ __forceinline__ int round_point(const RoundView &r, int64_t i, uint32_t &walk) {
     int k;
     if (!r.off) {
         k = (int)(i / r.cw);
         walk = (uint32_t)(r.w0 + (int)(i - (int64_t)k * r.cw));
     } else {
-        int lo = 0, hi = r.n - 1;
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (r.off[mid] <= i) lo = mid; else hi = mid - 1;
-        }
-        k = lo;
+        k = r.kblk[i >> kItemBlockShift];
+        while (r.off[k + 1] <= i) ++k;
         walk = (uint32_t)(r.wbeg[k] + (int)(i - r.off[k]));
     }
-    return r.starts[r.inside[r.pid[k]]];
+    return k;
+}
+
+__device__ __forceinline__ int round_inside_id(const RoundView &r, int k) {
+    return r.pid_identity ? k : r.pid[k];
+}
+
+__device__ __forceinline__ const StartInfo &round_item(const RoundView &r, int64_t i, uint32_t &walk) {
+    return r.starts[r.inside[round_inside_id(r, round_point(r, i, walk))]];
 }
 
 // --------------------------------------------------------------------------
@@ -818,6 +828,24 @@ __global__ __launch_bounds__(1024) void k_inside_scan(const int32_t *bcount, int
     }
 }
 
+// FP32 start copies of the inside points (n_dev: inside count on the device)
+__global__ __launch_bounds__(kBlock) void k_start_f(const StartInfo *starts, const int32_t *inside,
+                                                    int64_t n, const int32_t *n_dev, StartF *sf) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= (n_dev ? (int64_t)*n_dev : n)) return;
+    const StartInfo &S = starts[inside[j]];
+    StartF f;
+    f.x = (float)S.x;
+    f.y = (float)S.y;
+    f.z = (float)S.z;
+    f.r = (float)S.r;
+    f.slot = S.slot;
+    f.domain = S.domain;
+    f.key0 = S.key0;
+    f.key1 = S.key1;
+    sf[j] = f;
+}
+
 __global__ __launch_bounds__(kBlock) void k_inside_scatter(const uint8_t *loc, int64_t n,
                                                            const int64_t *boff, int32_t *inside) {
     __shared__ int scan[kBlock];
@@ -933,7 +961,7 @@ void k_walk_fp32(MeshView m, EqView e, PersistArgs a, AdjOut adj) {
                 double g = 0.0, s = 0.0, t = 0.0;
                 int prim = -1;
                 const int slot = q.best_slot;
-                if (slot >= 0) {
+                if (slot >= 0 && (ADJ || e.boundary_kind != WOS_BND_ZERO)) {
                     prim = m.slot_prim[slot];
                     double nx, ny, nz = 0.0;
                     if (DIM == 2) npq_seg_d(m.segd[slot], (double)px, (double)py, nx, ny, s);
@@ -972,16 +1000,18 @@ void k_walk_fp32(MeshView m, EqView e, PersistArgs a, AdjOut adj) {
                 const int rank = __builtin_amdgcn_mbcnt_hi(hi, __builtin_amdgcn_mbcnt_lo(lo, 0));
                 const uint64_t item = pool_next + (uint64_t)rank;
                 if (!active && item < pool_end) {
-                    const StartInfo &S = round_item(a.rv, (int64_t)item, w);
-                    px = (float)S.x; py = (float)S.y; pz = (float)S.z;
-                    margin = S.shrink_abs;
-                    R = (float)S.r * a.shrink_rel - margin;
+                    const int j = round_inside_id(a.rv, round_point(a.rv, (int64_t)item, w));
+                    const StartF S = a.rv.sf[j];
+                    dom = S.domain;
+                    const DomainF DF = domain_record(m.doms, dom);
+                    px = S.x; py = S.y; pz = S.z;
+                    margin = DF.shrink_abs;
+                    R = S.r * a.shrink_rel - margin;
                     weight = 1.0f;
                     sample = 0.0;
                     steps = 0;
                     q.best_slot = S.slot;
-                    root = S.root32;
-                    dom = S.domain;
+                    root = DF.root32;
                     k0 = S.key0;
                     k1 = S.key1;
                     out_index = (size_t)item;
@@ -989,11 +1019,12 @@ void k_walk_fp32(MeshView m, EqView e, PersistArgs a, AdjOut adj) {
                     if constexpr (ADJ) D = WalkDer{};
                     if (!(R > a.eps_f)) {
                         // zero-step walk: g at the start's nearest point
-                        a.vals[out_index] = S.g;
+                        const StartInfo &F = a.rv.starts[a.rv.inside[j]];
+                        a.vals[out_index] = F.g;
                         a.steps_out[out_index] = 0;
                         if constexpr (ADJ)
-                            store_adjoint(adj, (int64_t)item, D, 1.0, S.g, S.prim, S.s,
-                                          DIM == 3 ? S.t : 0.0);
+                            store_adjoint(adj, (int64_t)item, D, 1.0, F.g, F.prim, F.s,
+                                          DIM == 3 ? F.t : 0.0);
                         active = false;
                     }
                 }
@@ -1242,6 +1273,8 @@ struct wos_ctx {
     int grid_built_cap = 0;           // its list cap
     float grid_built_relax = -1.0f;   // and the relax it was built for
     int64_t grid_list_len = 0;
+    DevBuf doms;                      // DomainF per domain
+    DevBuf starts_f, r_kblk;          // FP32 start copies; item block -> point (uneven rounds)
     DevBuf plist_in, loc, starts, amb, bvalue, inside, vals, steps, mean, m2, count, ssum, hits,
         counter, vv, pid_all, r_pid, r_wbeg, r_off, coef, scal_sum, grad_v, grad_b, dl_du,
         adj_scal, adj_vw, adj_prim;
@@ -1568,6 +1601,11 @@ int prepare_solve(wos_ctx *c, const std::vector<Part> &parts, const wos_eq_desc 
 
 // Classify every part's points into its slice of loc/starts/amb/bvalue
 // (stream-ordered, no synchronisation); part_tol receives the tolerances.
+// conservative FP32 radius margin of a domain: 32 ulp of its largest coordinate
+float fp32_margin(const DomainInfo &D) {
+    return (float)(32.0 * std::ldexp(1.0, -24) * D.coord_scale);
+}
+
 int classify_parts(wos_ctx *c, const std::vector<Part> &parts, const EqView &ev,
                    const wos_run_params *prm, std::vector<double> &part_tol) {
     const int dim = c->mesh.dim;
@@ -1585,10 +1623,8 @@ int classify_parts(wos_ctx *c, const std::vector<Part> &parts, const EqView &ev,
                 g = *p.grid;
                 for (int k = 0; k < 6; ++k) g.b[k] = gb[k];
             }
-            // conservative FP32 radius margin: 32 ulp of the domain's largest coordinate
             DomainView dv{D.node_off, D.root_ref, (int)D.prim_off, (int)(D.prim_off + D.n_prims),
-                          part_tol[pi], off,
-                          (float)(32.0 * std::ldexp(1.0, -24) * D.coord_scale), p.domain};
+                          part_tol[pi], off, fp32_margin(D), p.domain};
             const double *dp = p.points ? c->plist_in.as<double>() + poff * dim : nullptr;
             hipError_t le = dim == 2 ? launch_classify(c, k_classify<2>, ev, g, dv, dp, p.count,
                                                        prm->seed, prm->max_ray_attempts)
@@ -1733,6 +1769,11 @@ int solve_once(wos_ctx *c, const std::vector<Part> &parts, const wos_eq_desc *eq
     int blocks = 0;
     if (n_in > 0) {
         HIPX(upload(c->inside, inside, c->stream));
+        HIPX(c->starts_f.reserve((size_t)n_in * sizeof(StartF)));
+        hipLaunchKernelGGL(k_start_f, dim3((unsigned)((n_in + kBlock - 1) / kBlock)), dim3(kBlock),
+                           0, c->stream, c->starts.as<StartInfo>(), c->inside.as<int32_t>(),
+                           (int64_t)n_in, nullptr, c->starts_f.as<StartF>());
+        HIPX(hipGetLastError());
         HIPX(c->mean.reserve(n_in * sizeof(double)));
         HIPX(c->m2.reserve(n_in * sizeof(double)));
         HIPX(c->count.reserve(n_in * sizeof(double)));
@@ -1767,7 +1808,7 @@ int solve_once(wos_ctx *c, const std::vector<Part> &parts, const wos_eq_desc *eq
             HIPX(hipMemsetAsync(c->grad_b.p, 0, WOS_MAX_BND_VALUES * sizeof(double), c->stream));
         }
         std::vector<int64_t> done(n_in, 0), want(n_in, std::min<int64_t>(prm->n_walks, cap));
-        std::vector<int32_t> pid, wbeg;
+        std::vector<int32_t> pid, wbeg, kblk;
         std::vector<int64_t> off;
         std::vector<double> a_cnt(n_in), a_m2(n_in);
         std::vector<int32_t> all_ids(n_in);
@@ -1803,10 +1844,12 @@ int solve_once(wos_ctx *c, const std::vector<Part> &parts, const wos_eq_desc *eq
             }
             RoundView rv{};
             rv.starts = c->starts.as<StartInfo>();
+            rv.sf = c->starts_f.as<StartF>();
             rv.inside = c->inside.as<int32_t>();
             if (uniform) {
                 const int64_t cw = std::min<int64_t>(want[0], std::max<int64_t>(1, budget / n_in));
                 rv.pid = c->pid_all.as<int32_t>();
+                rv.pid_identity = 1;
                 rv.n = n_in;
                 rv.cw = (int)cw;
                 rv.w0 = (int)done[0];
@@ -1825,12 +1868,21 @@ int solve_once(wos_ctx *c, const std::vector<Part> &parts, const wos_eq_desc *eq
                     done[k] += cnt;
                     want[k] -= cnt;
                 }
+                // the point holding the first item of every block of 64 items
+                kblk.assign((size_t)((tot + (1 << kItemBlockShift) - 1) >> kItemBlockShift), 0);
+                for (size_t b = 0, k = 0; b < kblk.size(); ++b) {
+                    const int64_t item = (int64_t)b << kItemBlockShift;
+                    while (off[k + 1] <= item) ++k;
+                    kblk[b] = (int32_t)k;
+                }
                 HIPX(upload(c->r_pid, pid, c->stream));
                 HIPX(upload(c->r_wbeg, wbeg, c->stream));
                 HIPX(upload(c->r_off, off, c->stream));
+                HIPX(upload(c->r_kblk, kblk, c->stream));
                 rv.pid = c->r_pid.as<int32_t>();
                 rv.wbeg = c->r_wbeg.as<int32_t>();
                 rv.off = c->r_off.as<int64_t>();
+                rv.kblk = c->r_kblk.as<int32_t>();
                 rv.n = (int)pid.size();
                 rv.total = tot;
             }
@@ -1976,6 +2028,7 @@ int solve_graph(wos_ctx *c, const std::vector<Part> &parts, const wos_eq_desc *e
     const int dim = c->mesh.dim;
     const int64_t nb = (n + kBlock - 1) / kBlock;
     HIPX(c->inside.reserve((size_t)n * sizeof(int32_t)));
+    HIPX(c->starts_f.reserve((size_t)n * sizeof(StartF)));
     HIPX(c->g_bcount.reserve((size_t)nb * sizeof(int32_t)));
     HIPX(c->g_boff.reserve((size_t)nb * sizeof(int64_t)));
     HIPX(c->g_misc.reserve(64));
@@ -2004,7 +2057,7 @@ int solve_graph(wos_ctx *c, const std::vector<Part> &parts, const wos_eq_desc *e
     key_put(key, *prm);
     key_put(key, c->tune);
     key_put(key, c->view);
-    for (const DevBuf *b : {&c->loc, &c->starts, &c->amb, &c->bvalue, &c->inside, &c->g_bcount,
+    for (const DevBuf *b : {&c->loc, &c->starts, &c->starts_f, &c->amb, &c->bvalue, &c->inside, &c->g_bcount,
                             &c->g_boff, &c->g_misc, &c->mean, &c->m2, &c->count, &c->ssum,
                             &c->hits, &c->vals, &c->steps, &c->counter, &c->pid_all, &c->vv})
         key_put(key, b->p);
@@ -2022,6 +2075,9 @@ int solve_graph(wos_ctx *c, const std::vector<Part> &parts, const wos_eq_desc *e
             hipLaunchKernelGGL(k_inside_scatter, dim3((unsigned)nb), dim3(kBlock), 0, c->stream,
                                c->loc.as<uint8_t>(), n, c->g_boff.as<int64_t>(),
                                c->inside.as<int32_t>());
+            hipLaunchKernelGGL(k_start_f, dim3((unsigned)nb), dim3(kBlock), 0, c->stream,
+                               c->starts.as<StartInfo>(), c->inside.as<int32_t>(), n, n_in_dev,
+                               c->starts_f.as<StartF>());
             for (DevBuf *b : {&c->mean, &c->m2, &c->count, &c->ssum})
                 hipLaunchKernelGGL(k_zero_u64, dim3(256), dim3(256), 0, c->stream,
                                    b->as<unsigned long long>(), n);
@@ -2030,8 +2086,10 @@ int solve_graph(wos_ctx *c, const std::vector<Part> &parts, const wos_eq_desc *e
             HIPX(hipGetLastError());
             RoundView rv{};
             rv.starts = c->starts.as<StartInfo>();
+            rv.sf = c->starts_f.as<StartF>();
             rv.inside = c->inside.as<int32_t>();
             rv.pid = c->pid_all.as<int32_t>();
+            rv.pid_identity = 1;
             rv.n = (int)n;
             rv.cw = (int)nw;
             rv.w0 = 0;
@@ -2320,6 +2378,9 @@ int wos_ctx_set_domains(wos_ctx *c, int dim, int n_domains, const double *const 
         HIPX(upload(c->nf3, L.nf3, s));
         HIPX(upload(c->trif, L.trif, s));
     }
+    std::vector<DomainF> doms;
+    for (const DomainInfo &D : L.domains) doms.push_back(DomainF{D.root_ref, fp32_margin(D)});
+    HIPX(upload(c->doms, doms, s));
     HIPX(upload(c->slot_prim, L.slot_prim, s));
     HIPX(upload(c->prim_bid, all.bids, s));
     HIPX(upload(c->prims, all.prims, s));
@@ -2335,6 +2396,7 @@ int wos_ctx_set_domains(wos_ctx *c, int dim, int n_domains, const double *const 
     v.prims = c->prims.as<int32_t>(); v.verts = c->verts.as<double>();
     v.nf2 = c->nf2.as<NodeF2>(); v.nf3 = c->nf3.as<NodeF3>();
     v.segf = c->segf.as<SegF>(); v.seg_inv_vv = c->seg_inv.as<float>(); v.trif = c->trif.as<TriF>();
+    v.doms = c->doms.as<DomainF>();
     c->view = v;
     c->mesh = std::move(all);
     c->L = std::move(L);
