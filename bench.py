@@ -109,21 +109,24 @@ def copy_calibration(mib):
 # an internal node (NodeF3 64 / NodeF2 48); per walk: its start record (120),
 # value + steps (12), the FP64 terminal primitive (TriD 128 / SegD 32) and
 # its slot -> primitive id (4).
-RECORDS = {3: dict(cell=16, entry=8, prim=64, node=64, walk=120 + 12 + 128 + 4),
-           2: dict(cell=16, entry=8, prim=20, node=48, walk=120 + 12 + 32 + 4)}
+# walk: FP32 start copy + the value and step count written;
+# terminal: FP64 primitive record + slot -> primitive id (read unless g = 0)
+RECORDS = {3: dict(cell=16, entry=8, prim=48, node=64, walk=32 + 12, terminal=128 + 4),
+           2: dict(cell=16, entry=8, prim=16, node=48, walk=32 + 12, terminal=32 + 4)}
 
 
-def walker_bytes(s, dim):
+def walker_bytes(s, dim, terminal=True):
     """Algorithmic bytes of one FP32 walker launch from its query-unit counts:
     every query reads its grid cell; a list scan reads each entry and its
     primitive; a tree query reads its start primitive, each opened node and
-    each leaf primitive."""
+    each leaf primitive; every walk reads its start and writes its result, and
+    reads its terminal primitive when the boundary data is not zero."""
     r = RECORDS[dim]
     tree_q = s["q_queries"] - s["q_grid_bound"] - s["q_list_scans"]
     return (s["q_queries"] * r["cell"] + s["q_list_entries"] * r["entry"]
             + s["q_list_prims"] * r["prim"]
             + s["q_nodes"] * r["node"] + (s["q_leaf_prims"] + tree_q) * r["prim"]
-            + s["total_walks"] * r["walk"])
+            + s["total_walks"] * (r["walk"] + (r["terminal"] if terminal else 0)))
 
 
 def torus_probes(n, seed):
@@ -337,7 +340,7 @@ def main():
     v64 = pts_per_s_at_se(ins.sum(), var_ref, r64["mean_steps"][ins], t64, s64["total_steps"])
 
     kern_s = float(np.mean(walk_k))
-    launch_bytes = walker_bytes(sc, 3)
+    launch_bytes = walker_bytes(sc, 3, terminal=False)  # g = 0 on the sphere
     achieved = launch_bytes / kern_s / 1e9
     with open(os.path.join(ROOT, "profiles", "algorithmic_counts.json")) as f:
         ref_counts = json.load(f)["icosphere4_screened_32cubed"]

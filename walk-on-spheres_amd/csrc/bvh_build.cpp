@@ -153,7 +153,7 @@ DeviceLayouts make_layouts(const std::vector<const HostMesh *> &meshes,
     const size_t ns = (size_t)po;
     L.slot_prim.resize(ns);
     if (dim == 2) {
-        L.nd2.resize(n_nodes); L.segd.resize(ns); L.segf.resize(ns); L.seg_inv_vv.resize(ns);
+        L.nd2.resize(n_nodes); L.segd.resize(ns); L.segf.resize(ns);
     } else {
         L.nd3.resize(n_nodes); L.trid.resize(ns); L.trif.resize(ns);
     }
@@ -179,8 +179,6 @@ DeviceLayouts make_layouts(const std::vector<const HostMesh *> &meshes,
                 const float ax = (float)V(a, 0), ay = (float)V(a, 1);
                 const float vx = (float)V(c, 0) - ax, vy = (float)V(c, 1) - ay;
                 L.segf[gs] = SegF{ax, ay, vx, vy};
-                const float vv = vx * vx + vy * vy;
-                L.seg_inv_vv[gs] = vv > 0.0f ? 1.0f / vv : 0.0f;
             } else {
                 const int32_t i0 = m.prims[3 * p], i1 = m.prims[3 * p + 1], i2 = m.prims[3 * p + 2];
                 TriD t{};
@@ -198,11 +196,9 @@ DeviceLayouts make_layouts(const std::vector<const HostMesh *> &meshes,
                 f.v0x = (float)t.v0x; f.v0y = (float)t.v0y; f.v0z = (float)t.v0z;
                 f.e1x = (float)t.e1x; f.e1y = (float)t.e1y; f.e1z = (float)t.e1z;
                 f.e2x = (float)t.e2x; f.e2y = (float)t.e2y; f.e2z = (float)t.e2z;
-                f.a = (float)t.a; f.b = (float)t.b; f.c = (float)t.c;
                 f.inv_a = t.a != 0.0 ? (float)(1.0 / t.a) : 0.0f;
                 f.inv_c = t.c != 0.0 ? (float)(1.0 / t.c) : 0.0f;
                 f.inv_det = t.det != 0.0 ? (float)(1.0 / t.det) : 0.0f;
-                f.inv_denom = t.denom != 0.0 ? (float)(1.0 / t.denom) : 0.0f;
                 L.trif[gs] = f;
             }
         }

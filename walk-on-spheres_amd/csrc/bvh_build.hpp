@@ -67,13 +67,14 @@ struct alignas(16) NodeF2 {
     float r_lox, r_loy, r_hix, r_hiy;
     int32_t left, right, pad0, pad1;
 };
-// FP32 leaf slots.  a = |e1|^2, b = e1.e2, c = |e2|^2, det = ac - b^2,
-// denom = a - 2b + c; reciprocals are 0 where the quantity is 0.
+// FP32 leaf slots (48 B: three 16 B loads).  With a = |e1|^2, c = |e2|^2,
+// b = e1.e2 and det = ac - b^2 formed in FP64, the record keeps 1/det (the
+// one quantity that cancels), 1/a and 1/c; reciprocals are 0 where the
+// quantity is 0.  The distance test forms a, b, c and 1/|e1 - e2|^2 itself.
 struct alignas(16) TriF {
-    float v0x, v0y, v0z, inv_a;
-    float e1x, e1y, e1z, b;
-    float e2x, e2y, e2z, c;
-    float a, inv_c, inv_det, inv_denom;
+    float v0x, v0y, v0z, inv_det;
+    float e1x, e1y, e1z, inv_a;
+    float e2x, e2y, e2z, inv_c;
 };
 struct alignas(16) SegF { float s0x, s0y, vx, vy; };
 
@@ -97,7 +98,6 @@ struct DeviceLayouts {
     std::vector<NodeF3> nf3;
     std::vector<SegF> segf;
     std::vector<TriF> trif;
-    std::vector<float> seg_inv_vv;  // 2D: 1 / |v|^2 per slot
     std::vector<int32_t> slot_prim; // global slot -> global primitive
     std::vector<DomainInfo> domains;
     int depth = 0;             // deepest domain

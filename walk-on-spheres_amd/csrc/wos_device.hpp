@@ -81,7 +81,6 @@ struct MeshView {
     const NodeF2 *nf2;
     const NodeF3 *nf3;
     const SegF *segf;
-    const float *seg_inv_vv;
     const TriF *trif;
     // distance grid (nullptr when off)
     const GridInfo *grids;     // per domain
@@ -702,12 +701,17 @@ __device__ __forceinline__ float sq3(float x, float y, float z) {
 // reciprocals are 0, which collapses a candidate onto a vertex).  The same
 // minimum as the region walk of src/npq.cpp:52-189.
 __device__ __forceinline__ float tri_d2f(const TriF &T, float px, float py, float pz) {
+    const float a = sq3(T.e1x, T.e1y, T.e1z), c = sq3(T.e2x, T.e2y, T.e2z);
+    const float b = __builtin_fmaf(T.e1x, T.e2x, __builtin_fmaf(T.e1y, T.e2y, T.e1z * T.e2z));
+    // 1/|e1 - e2|^2 (the edge v1-v2); +inf for a degenerate edge, which the
+    // clamp below turns into an end point
+    const float inv_denom = __builtin_amdgcn_rcpf(sq3(T.e2x - T.e1x, T.e2y - T.e1y, T.e2z - T.e1z));
     const float bx = T.v0x - px, by = T.v0y - py, bz = T.v0z - pz;
     const float d = __builtin_fmaf(T.e1x, bx, __builtin_fmaf(T.e1y, by, T.e1z * bz));
     const float e = __builtin_fmaf(T.e2x, bx, __builtin_fmaf(T.e2y, by, T.e2z * bz));
     // plane projection v0 + s e1 + t e2
-    const float s = __builtin_fmaf(T.b, e, -T.c * d) * T.inv_det;
-    const float t = __builtin_fmaf(T.b, d, -T.a * e) * T.inv_det;
+    const float s = __builtin_fmaf(b, e, -c * d) * T.inv_det;
+    const float t = __builtin_fmaf(b, d, -a * e) * T.inv_det;
     const bool inside = s >= 0.0f && t >= 0.0f && s + t <= 1.0f;
     const float di = sq3(__builtin_fmaf(t, T.e2x, __builtin_fmaf(s, T.e1x, bx)),
                          __builtin_fmaf(t, T.e2y, __builtin_fmaf(s, T.e1y, by)),
@@ -721,7 +725,7 @@ __device__ __forceinline__ float tri_d2f(const TriF &T, float px, float py, floa
     const float d1 = sq3(__builtin_fmaf(u1, T.e2x, bx), __builtin_fmaf(u1, T.e2y, by),
                          __builtin_fmaf(u1, T.e2z, bz));
     // edge v1-v2: v0 + w e1 + (1 - w) e2
-    const float w = clamp01(((T.c + e) - (T.b + d)) * T.inv_denom);
+    const float w = clamp01(((c + e) - (b + d)) * inv_denom);
     const float w1 = 1.0f - w;
     const float d2 = sq3(__builtin_fmaf(w1, T.e2x, __builtin_fmaf(w, T.e1x, bx)),
                          __builtin_fmaf(w1, T.e2y, __builtin_fmaf(w, T.e1y, by)),
@@ -729,8 +733,10 @@ __device__ __forceinline__ float tri_d2f(const TriF &T, float px, float py, floa
     return fminf(inside ? di : INFINITY, fminf(d0, fminf(d1, d2)));
 }
 
-__device__ __forceinline__ float seg_d2f(const SegF &g, float inv_vv, float px, float py) {
+// (a zero-length segment gives 0 x inf = NaN, which the clamp turns into 0)
+__device__ __forceinline__ float seg_d2f(const SegF &g, float px, float py) {
     const float wx = px - g.s0x, wy = py - g.s0y;
+    const float inv_vv = __builtin_amdgcn_rcpf(__builtin_fmaf(g.vx, g.vx, g.vy * g.vy));
     const float par = clamp01(__builtin_fmaf(wx, g.vx, wy * g.vy) * inv_vv);
     const float dx = __builtin_fmaf(-par, g.vx, wx), dy = __builtin_fmaf(-par, g.vy, wy);
     return __builtin_fmaf(dx, dx, dy * dy);
@@ -739,7 +745,7 @@ __device__ __forceinline__ float seg_d2f(const SegF &g, float inv_vv, float px, 
 template <int DIM>
 __device__ __forceinline__ float leaf_d2f(const MeshView &m, int slot, float px, float py, float pz) {
     if constexpr (DIM == 3) return tri_d2f(m.trif[slot], px, py, pz);
-    else return seg_d2f(m.segf[slot], m.seg_inv_vv[slot], px, py);
+    else return seg_d2f(m.segf[slot], px, py);
 }
 
 struct StackEntry { int32_t ref; float d2; };
@@ -889,18 +895,18 @@ __device__ __forceinline__ uint32_t query_begin(QueryF &q, const MeshView &m, in
 // that two loads can be in flight together
 template <int DIM> struct LeafRec;
 template <> struct LeafRec<3> { TriF t; };
-template <> struct LeafRec<2> { SegF s; float inv; };
+template <> struct LeafRec<2> { SegF s; };
 template <int DIM>
 __device__ __forceinline__ LeafRec<DIM> leaf_load(const MeshView &m, int slot) {
     LeafRec<DIM> r;
     if constexpr (DIM == 3) r.t = m.trif[slot];
-    else { r.s = m.segf[slot]; r.inv = m.seg_inv_vv[slot]; }
+    else r.s = m.segf[slot];
     return r;
 }
 template <int DIM>
 __device__ __forceinline__ float leaf_dist(const LeafRec<DIM> &r, float px, float py, float pz) {
     if constexpr (DIM == 3) return tri_d2f(r.t, px, py, pz);
-    else return seg_d2f(r.s, r.inv, px, py);
+    else return seg_d2f(r.s, px, py);
 }
 
 // One unit: an internal node, a BVH leaf primitive, or the next two entries
@@ -920,9 +926,13 @@ __device__ __forceinline__ uint32_t query_step(QueryF &q, const MeshView &m, flo
         // tested when its bound is below the threshold the first was checked
         // against: a primitive the first one's result would have let the scan
         // skip cannot lower an exact minimum.
+        // A scan starts at entry 1 of a list that begins at an odd index and
+        // continues only by whole pairs, so lslot is even: one 16 B load
+        // (past a list's end it reads padding or the next list, unused).
         const bool two = q.lcount > 1;
-        const GridEntry E0 = m.glist[q.lslot];
-        const GridEntry E1 = two ? m.glist[q.lslot + 1] : E0;
+        const int4 pair = *reinterpret_cast<const int4 *>(m.glist + q.lslot);
+        const GridEntry E0{pair.x, __int_as_float(pair.y)};
+        const GridEntry E1{pair.z, __int_as_float(pair.w)};
         did |= (two ? 2u : 1u) << kQsEntryShift;
         const float thr = q.best * q.cf;
         const float l0 = fmaxf(E0.dmin - q.t, 0.0f), l1 = fmaxf(E1.dmin - q.t, 0.0f);

@@ -1256,7 +1256,7 @@ struct wos_ctx {
     HostMesh mesh;
     DeviceLayouts L;
     bool has_mesh = false;
-    DevBuf nd2, nd3, segd, trid, slot_prim, prim_bid, prims, verts, nf2, nf3, segf, seg_inv, trif;
+    DevBuf nd2, nd3, segd, trid, slot_prim, prim_bid, prims, verts, nf2, nf3, segf, trif;
     DevBuf cq_out, cq_prim;
     DevBuf units;                     // FP32 query-unit counters
     std::vector<std::pair<TuneKey, wos_tuning>> tuned;  // autotuned shapes
@@ -1345,13 +1345,22 @@ hipError_t ensure_grid(wos_ctx *c) {
                 if ((e = hipMemcpyAsync(cnt.data(), counts.p, cnt.size() * sizeof(int32_t),
                                         hipMemcpyDeviceToHost, c->stream)) != hipSuccess) return e;
                 if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) return e;
+                // every list starts at an odd entry index (lengths padded to
+                // even), so that entry 1 and each later pair is one 16 B
+                // load; one spare entry past the end for the last pair
                 std::vector<int64_t> off((size_t)total);
-                int64_t run = 0;
-                for (size_t i = 0; i < cnt.size(); ++i) { off[i] = run; run += cnt[i]; }
+                int64_t run = 1;
+                for (size_t i = 0; i < cnt.size(); ++i) {
+                    off[i] = run;
+                    run += cnt[i] + (cnt[i] & 1);
+                }
                 if (run >= ((int64_t)1 << 31)) return hipErrorOutOfMemory;
-                c->grid_list_len = run;
+                c->grid_list_len = run + 1;
                 if ((e = upload(offs, off, c->stream)) != hipSuccess) return e;
-                if ((e = c->g_list.reserve((size_t)std::max<int64_t>(run, 1) * sizeof(GridEntry))) != hipSuccess)
+                if ((e = c->g_list.reserve((size_t)(run + 1) * sizeof(GridEntry))) != hipSuccess)
+                    return e;
+                if ((e = hipMemsetAsync(c->g_list.p, 0, (size_t)(run + 1) * sizeof(GridEntry),
+                                        c->stream)) != hipSuccess)
                     return e;
             }
             for (int d = 0; d < nd; ++d) {
@@ -2371,7 +2380,6 @@ int wos_ctx_set_domains(wos_ctx *c, int dim, int n_domains, const double *const 
         HIPX(upload(c->segd, L.segd, s));
         HIPX(upload(c->nf2, L.nf2, s));
         HIPX(upload(c->segf, L.segf, s));
-        HIPX(upload(c->seg_inv, L.seg_inv_vv, s));
     } else {
         HIPX(upload(c->nd3, L.nd3, s));
         HIPX(upload(c->trid, L.trid, s));
@@ -2395,7 +2403,7 @@ int wos_ctx_set_domains(wos_ctx *c, int dim, int n_domains, const double *const 
     v.slot_prim = c->slot_prim.as<int32_t>(); v.prim_bid = c->prim_bid.as<int32_t>();
     v.prims = c->prims.as<int32_t>(); v.verts = c->verts.as<double>();
     v.nf2 = c->nf2.as<NodeF2>(); v.nf3 = c->nf3.as<NodeF3>();
-    v.segf = c->segf.as<SegF>(); v.seg_inv_vv = c->seg_inv.as<float>(); v.trif = c->trif.as<TriF>();
+    v.segf = c->segf.as<SegF>(); v.trif = c->trif.as<TriF>();
     v.doms = c->doms.as<DomainF>();
     c->view = v;
     c->mesh = std::move(all);
