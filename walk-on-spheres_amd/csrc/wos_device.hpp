@@ -410,6 +410,56 @@ __device__ __forceinline__ int ray_cast_d(const MeshView &m, int root, double px
 // --------------------------------------------------------------------------
 // special functions (FP64: the oracle's methods; FP32: same, fewer terms)
 // --------------------------------------------------------------------------
+// Elementary functions of the walk-step arithmetic (weights, Green
+// functions, source draws) by precision.  FP64, the reference arithmetic,
+// calls the C library and divides.  FP32 uses the hardware exp2, log2,
+// reciprocal, sin and cos instructions with a few FMAs; relative errors stay
+// near 1e-7 (exp of an argument x: about |x| 6e-8 more), far below the
+// statistical tolerance of the FP32 walker.
+template <typename T> struct Fx;
+template <> struct Fx<double> {
+    static __device__ __forceinline__ double exp(double x) { return ::exp(x); }
+    static __device__ __forceinline__ double log(double x) { return ::log(x); }
+    static __device__ __forceinline__ double expm1(double x) { return ::expm1(x); }
+    static __device__ __forceinline__ double cbrt(double x) { return ::cbrt(x); }
+    static __device__ __forceinline__ double cos(double x) { return ::cos(x); }
+    static __device__ __forceinline__ double sin(double x) { return ::sin(x); }
+    static __device__ __forceinline__ double div(double a, double b) { return a / b; }
+};
+template <> struct Fx<float> {
+    static __device__ __forceinline__ float exp(float x) {
+        return __builtin_amdgcn_exp2f(x * 1.44269504088896341f);
+    }
+    static __device__ __forceinline__ float log(float x) {
+        return __builtin_amdgcn_logf(x) * 0.693147180559945309f;
+    }
+    // e^x - 1: for |x| < 1/4 the Taylor polynomial of degree 7 (remainder
+    // below x^8/8! < 2e-9 |x|), else e^x - 1 (cancellation at most 1/(1 - e^-1/4))
+    static __device__ __forceinline__ float expm1(float x) {
+        float p = __builtin_fmaf(x, 1.0f / 5040.0f, 1.0f / 720.0f);
+        p = __builtin_fmaf(x, p, 1.0f / 120.0f);
+        p = __builtin_fmaf(x, p, 1.0f / 24.0f);
+        p = __builtin_fmaf(x, p, 1.0f / 6.0f);
+        p = __builtin_fmaf(x, p, 0.5f);
+        p = __builtin_fmaf(x, p, 1.0f);
+        p *= x;
+        return fabsf(x) < 0.25f ? p : exp(x) - 1.0f;
+    }
+    // cube root of x > 0: y = 2^(log2(x) / 3), then one Newton step
+    // y - (y^3 - x) / (3 y^2); 0 for x <= 0
+    static __device__ __forceinline__ float cbrt(float x) {
+        const float y = __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(x) * (1.0f / 3.0f));
+        const float y2 = y * y;
+        const float r = __builtin_fmaf(-(y2 * y - x), __builtin_amdgcn_rcpf(3.0f * y2), y);
+        return x > 0.0f ? r : 0.0f;
+    }
+    static __device__ __forceinline__ float cos(float x) { return __cosf(x); }
+    static __device__ __forceinline__ float sin(float x) { return __sinf(x); }
+    static __device__ __forceinline__ float div(float a, float b) {
+        return a * __builtin_amdgcn_rcpf(b);
+    }
+};
+
 template <typename T> struct Lim;
 template <> struct Lim<double> { static constexpr double eps = DBL_EPSILON; };
 template <> struct Lim<float> { static constexpr float eps = FLT_EPSILON; };
@@ -497,12 +547,12 @@ template <typename T> __device__ __forceinline__ T log_k0(T x) {
 template <typename T> __device__ __forceinline__ T poisson_green_2d(T R, T r) {
     if (!(r > (T)0) || r > R || R <= (T)0) return (T)NAN;
     if (r == R) return (T)0;
-    return log(R / r) / ((T)2 * (T)kPi);
+    return Fx<T>::div(Fx<T>::log(Fx<T>::div(R, r)), (T)2 * (T)kPi);
 }
 template <typename T> __device__ __forceinline__ T poisson_green_3d(T R, T r) {
     if (!(r > (T)0) || r > R || R <= (T)0) return (T)NAN;
     if (r == R) return (T)0;
-    return (R - r) / ((T)4 * (T)kPi * r * R);
+    return Fx<T>::div(R - r, (T)4 * (T)kPi * r * R);
 }
 // include/wos/screened.hpp:14-91
 template <typename T> __device__ __forceinline__ T screened_weight_2d(T alpha, T R) {
@@ -540,15 +590,16 @@ template <typename T> __device__ __forceinline__ T screened_weight_3d(T alpha, T
         const T x2 = x * x;
         return (T)1 - x2 / (T)6 + (T)7 * x2 * x2 / (T)360;
     }
-    return (T)2 * x * exp(-x) / (-expm1((T)-2 * x));
+    return Fx<T>::div((T)2 * x * Fx<T>::exp(-x), -Fx<T>::expm1((T)-2 * x));
 }
 template <typename T> __device__ __forceinline__ T screened_green_3d(T alpha, T R, T r) {
     if (!(r > (T)0) || r > R || R <= (T)0) return (T)NAN;
     if (r == R) return (T)0;
     const T x = alpha * r, X = alpha * R;
     if (X < (T)1e-5) return poisson_green_3d<T>(R, r);
-    const T ratio = exp(-x) * (-expm1((T)-2 * (X - x))) / (-expm1((T)-2 * X));
-    return ratio / ((T)4 * (T)kPi * r);
+    const T ratio = Fx<T>::div(Fx<T>::exp(-x) * (-Fx<T>::expm1((T)-2 * (X - x))),
+                               -Fx<T>::expm1((T)-2 * X));
+    return Fx<T>::div(ratio, (T)4 * (T)kPi * r);
 }
 template <typename T> __device__ __forceinline__ T screened_mass_3d(T alpha, T R) {
     const T x = alpha * R;
@@ -556,7 +607,7 @@ template <typename T> __device__ __forceinline__ T screened_mass_3d(T alpha, T R
         const T x2 = x * x;
         return R * R * ((T)1 / (T)6 - (T)7 * x2 / (T)360 + (T)31 * x2 * x2 / (T)15120);
     }
-    return ((T)1 - screened_weight_3d<T>(alpha, R)) / (alpha * alpha);
+    return Fx<T>::div((T)1 - screened_weight_3d<T>(alpha, R), alpha * alpha);
 }
 
 // --------------------------------------------------------------------------
@@ -625,7 +676,8 @@ __device__ __forceinline__ T eq_source(const EqView &e, T x, T y, T z, int dim) 
     case WOS_SRC_DISK_SQ: return acc <= (T)e.src_radius * (T)e.src_radius ? (T)e.src_value : (T)0;
     case WOS_SRC_DISK_SQRT: return sqrt(acc) <= (T)e.src_radius ? (T)e.src_value : (T)0;
     case WOS_SRC_CONSTANT: return (T)e.src_value;
-    case WOS_SRC_GAUSSIAN: return (T)e.src_value * exp(-acc / ((T)e.src_radius * (T)e.src_radius));
+    case WOS_SRC_GAUSSIAN:
+        return (T)e.src_value * Fx<T>::exp(-Fx<T>::div(acc, (T)e.src_radius * (T)e.src_radius));
     default: return (T)0;
     }
 }

@@ -67,10 +67,10 @@ __device__ __forceinline__ T poisson_radius_2d(T R, RNG &rng) {
 template <typename T, typename RNG>
 __device__ __forceinline__ T poisson_radius_3d(T R, RNG &rng) {
     const T u1 = rng.unit_open(), u2 = rng.unit_open();
-    const T a = -log(u1 * u2);
+    const T a = -Fx<T>::log(u1 * u2);
     const T v1 = rng.unit_open(), v2 = rng.unit_open();
-    const T b = -log(v1 * v2);
-    return R * a / (a + b);
+    const T b = -Fx<T>::log(v1 * v2);
+    return Fx<T>::div(R * a, a + b);
 }
 // include/wos/screened.hpp:97-101
 template <typename T, typename RNG>
@@ -143,15 +143,15 @@ __device__ __forceinline__ double source_term(const EqView &e, T px, T py, T pz,
         if (DIM == 2) {
             const T radius = screened ? screened_radius_2d<T>(alpha, R, rng) : poisson_radius_2d<T>(R, rng);
             const T angle = (T)2 * (T)kPi * rng.unit();
-            sx = px + radius * cos(angle);
-            sy = py + radius * sin(angle);
+            sx = px + radius * Fx<T>::cos(angle);
+            sy = py + radius * Fx<T>::sin(angle);
         } else {
             const T radius = screened ? screened_radius_3d<T>(alpha, R, rng) : poisson_radius_3d<T>(R, rng);
             const T z = (T)2 * rng.unit() - (T)1;
             const T angle = (T)2 * (T)kPi * rng.unit();
             const T xy = sqrt((T)1 - z * z);
-            sx = px + radius * xy * cos(angle);
-            sy = py + radius * xy * sin(angle);
+            sx = px + radius * xy * Fx<T>::cos(angle);
+            sy = py + radius * xy * Fx<T>::sin(angle);
             sz = pz + radius * z;
         }
         const T f = eq_source<T>(e, sx, sy, sz, DIM);
@@ -610,7 +610,7 @@ __device__ __forceinline__ double source_term_f(const EqView &e, float px, float
             dist = R * sqrtf(u01f_open(r.b));
         } else {
             const U64x2 r = philox(w, step, 1, 1, k0, k1);
-            dist = R * cbrtf(u01f_open(r.a));
+            dist = R * Fx<float>::cbrt(u01f_open(r.a));
         }
         f = (float)e.src_value;
     } else {
@@ -627,7 +627,7 @@ __device__ __forceinline__ double source_term_f(const EqView &e, float px, float
             const float u = rng.unit(), v = rng.unit(), ww = rng.unit_open();
             const float z = 2.0f * u - 1.0f;
             const float surf = sqrtf(fmaxf(0.0f, 1.0f - z * z));
-            dist = R * cbrtf(ww);
+            dist = R * Fx<float>::cbrt(ww);
             float sn, cs;
             __sincosf(v * (float)(2.0 * kPi), &sn, &cs);
             sx = __builtin_fmaf(dist * surf, cs, px);
@@ -667,16 +667,16 @@ __device__ double source_term_adj(const EqView &e, T px, T py, T pz, T R, T weig
                 radius = screened ? (T)screened_radius_2d<float>((float)alpha, (float)R, rng)
                                   : (T)poisson_radius_2d<float>((float)R, rng);
                 const T angle = (T)2 * (T)kPi * (T)rng.unit();
-                sx = px + radius * cos(angle);
-                sy = py + radius * sin(angle);
+                sx = px + radius * Fx<T>::cos(angle);
+                sy = py + radius * Fx<T>::sin(angle);
             } else {
                 radius = screened ? (T)screened_radius_3d<float>((float)alpha, (float)R, rng)
                                   : (T)poisson_radius_3d<float>((float)R, rng);
                 const T z = (T)2 * (T)rng.unit() - (T)1;
                 const T angle = (T)2 * (T)kPi * (T)rng.unit();
                 const T xy = sqrt((T)1 - z * z);
-                sx = px + radius * xy * cos(angle);
-                sy = py + radius * xy * sin(angle);
+                sx = px + radius * xy * Fx<T>::cos(angle);
+                sy = py + radius * xy * Fx<T>::sin(angle);
                 sz = pz + radius * z;
             }
         } else {
@@ -711,7 +711,7 @@ __device__ double source_term_adj(const EqView &e, T px, T py, T pz, T R, T weig
     if constexpr (F32) {
         if (e.source_kind == WOS_SRC_CONSTANT) {
             if (DIM == 2) dist = R * sqrtf(u01f_open(philox(w, step, 0, 1, k0, k1).b));
-            else dist = R * cbrtf(u01f_open(philox(w, step, 1, 1, k0, k1).a));
+            else dist = R * Fx<float>::cbrt(u01f_open(philox(w, step, 1, 1, k0, k1).a));
         } else {
             CursorF rng(w, step, 1, k0, k1);
             if (DIM == 2) {
@@ -725,7 +725,7 @@ __device__ double source_term_adj(const EqView &e, T px, T py, T pz, T R, T weig
                 const float u = rng.unit(), v = rng.unit(), ww = rng.unit_open();
                 const float z = 2.0f * u - 1.0f;
                 const float surf = sqrtf(fmaxf(0.0f, 1.0f - z * z));
-                dist = R * cbrtf(ww);
+                dist = R * Fx<float>::cbrt(ww);
                 float sn, cs;
                 __sincosf(v * (float)(2.0 * kPi), &sn, &cs);
                 sx = __builtin_fmaf(dist * surf, cs, px);
