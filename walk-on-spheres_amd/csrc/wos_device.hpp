@@ -425,6 +425,7 @@ template <> struct Fx<double> {
     static __device__ __forceinline__ double cos(double x) { return ::cos(x); }
     static __device__ __forceinline__ double sin(double x) { return ::sin(x); }
     static __device__ __forceinline__ double div(double a, double b) { return a / b; }
+    static __device__ __forceinline__ double sqrt(double x) { return ::sqrt(x); }
 };
 template <> struct Fx<float> {
     static __device__ __forceinline__ float exp(float x) {
@@ -458,6 +459,8 @@ template <> struct Fx<float> {
     static __device__ __forceinline__ float div(float a, float b) {
         return a * __builtin_amdgcn_rcpf(b);
     }
+    // the square root instruction (within 1 ulp, not correctly rounded)
+    static __device__ __forceinline__ float sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 };
 
 template <typename T> struct Lim;
@@ -588,7 +591,7 @@ template <typename T> __device__ __forceinline__ T screened_weight_3d(T alpha, T
     if (x == (T)0) return (T)1;
     if (x < (T)1e-3) {
         const T x2 = x * x;
-        return (T)1 - x2 / (T)6 + (T)7 * x2 * x2 / (T)360;
+        return (T)1 - Fx<T>::div(x2, (T)6) + Fx<T>::div((T)7 * x2 * x2, (T)360);
     }
     return Fx<T>::div((T)2 * x * Fx<T>::exp(-x), -Fx<T>::expm1((T)-2 * x));
 }
@@ -605,7 +608,8 @@ template <typename T> __device__ __forceinline__ T screened_mass_3d(T alpha, T R
     const T x = alpha * R;
     if (x < (T)1e-3) {
         const T x2 = x * x;
-        return R * R * ((T)1 / (T)6 - (T)7 * x2 / (T)360 + (T)31 * x2 * x2 / (T)15120);
+        return R * R * ((T)1 / (T)6 - Fx<T>::div((T)7 * x2, (T)360) +
+                        Fx<T>::div((T)31 * x2 * x2, (T)15120));
     }
     return Fx<T>::div((T)1 - screened_weight_3d<T>(alpha, R), alpha * alpha);
 }
@@ -904,7 +908,7 @@ __device__ __forceinline__ uint32_t query_begin(QueryF &q, const MeshView &m, in
             const float dx = px - (G.ox + ((float)ix + 0.5f) * G.h);
             const float dy = py - (G.oy + ((float)iy + 0.5f) * G.h);
             const float dz = DIM == 3 ? pz - (G.oz + ((float)iz + 0.5f) * G.h) : 0.0f;
-            const float t = sqrtf(__builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz)));
+            const float t = Fx<float>::sqrt(__builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz)));
             if (cf < 1.0f) {
                 // dc lost its low 7 mantissa bits: dc <= true <= dc (1 + 2^-15)
                 const float dc = __uint_as_float(C.dcbits & ~kGridLenMask);

@@ -62,7 +62,7 @@ template <typename T, typename RNG>
 __device__ __forceinline__ T poisson_radius_2d(T R, RNG &rng) {
     const T u = rng.unit_open();
     const T v = rng.unit_open();
-    return R * sqrt(u * v);
+    return R * Fx<T>::sqrt(u * v);
 }
 template <typename T, typename RNG>
 __device__ __forceinline__ T poisson_radius_3d(T R, RNG &rng) {
@@ -149,7 +149,7 @@ __device__ __forceinline__ double source_term(const EqView &e, T px, T py, T pz,
             const T radius = screened ? screened_radius_3d<T>(alpha, R, rng) : poisson_radius_3d<T>(R, rng);
             const T z = (T)2 * rng.unit() - (T)1;
             const T angle = (T)2 * (T)kPi * rng.unit();
-            const T xy = sqrt((T)1 - z * z);
+            const T xy = Fx<T>::sqrt((T)1 - z * z);
             sx = px + radius * xy * Fx<T>::cos(angle);
             sy = py + radius * xy * Fx<T>::sin(angle);
             sz = pz + radius * z;
@@ -490,6 +490,11 @@ struct RoundView {
     int n;
     int cw;
     int w0;
+    // i / cw for item indices i < 2^32 as (t + ((i - t) >> s1)) >> s2 with
+    // t = mulhi(cw_m, i) (division by an invariant integer, Granlund and
+    // Montgomery 1994; set by set_walks_per_point)
+    uint32_t cw_m;
+    int cw_s1, cw_s2;
     int64_t total;
     // when set (graph replay), n and total are read from device memory: the
     // inside count is known only on the device
@@ -501,8 +506,11 @@ struct RoundView {
 __device__ __forceinline__ int round_point(const RoundView &r, int64_t i, uint32_t &walk) {
     int k;
     if (!r.off) {
-        k = (int)(i / r.cw);
-        walk = (uint32_t)(r.w0 + (int)(i - (int64_t)k * r.cw));
+        const uint32_t n = (uint32_t)i;
+        const uint32_t t = __umulhi(r.cw_m, n);
+        const uint32_t q = (t + ((n - t) >> r.cw_s1)) >> r.cw_s2;
+        k = (int)q;
+        walk = (uint32_t)r.w0 + (n - q * (uint32_t)r.cw);
     } else {
         k = r.kblk[i >> kItemBlockShift];
         while (r.off[k + 1] <= i) ++k;
@@ -517,6 +525,17 @@ __device__ __forceinline__ int round_inside_id(const RoundView &r, int k) {
 
 __device__ __forceinline__ const StartInfo &round_item(const RoundView &r, int64_t i, uint32_t &walk) {
     return r.starts[r.inside[round_inside_id(r, round_point(r, i, walk))]];
+}
+
+// cw walks per point: the multiplier and shifts of round_point's division
+// (cw >= 1; item indices stay below 2^32, the staging limit of a round)
+void set_walks_per_point(RoundView &r, int cw) {
+    int l = 0;
+    while (((int64_t)1 << l) < (int64_t)cw) ++l;
+    r.cw = cw;
+    r.cw_m = (uint32_t)((((uint64_t)1 << 32) * (((uint64_t)1 << l) - (uint64_t)cw)) / (uint64_t)cw + 1);
+    r.cw_s1 = l < 1 ? l : 1;
+    r.cw_s2 = l > 1 ? l - 1 : 0;
 }
 
 // --------------------------------------------------------------------------
@@ -607,7 +626,7 @@ __device__ __forceinline__ double source_term_f(const EqView &e, float px, float
     if (e.source_kind == WOS_SRC_CONSTANT) {
         if (DIM == 2) {
             const U64x2 r = philox(w, step, 0, 1, k0, k1);
-            dist = R * sqrtf(u01f_open(r.b));
+            dist = R * Fx<float>::sqrt(u01f_open(r.b));
         } else {
             const U64x2 r = philox(w, step, 1, 1, k0, k1);
             dist = R * Fx<float>::cbrt(u01f_open(r.a));
@@ -618,7 +637,7 @@ __device__ __forceinline__ double source_term_f(const EqView &e, float px, float
         float sx, sy, sz = 0.0f;
         if (DIM == 2) {
             const float theta = rng.unit();
-            dist = R * sqrtf(rng.unit_open());
+            dist = R * Fx<float>::sqrt(rng.unit_open());
             float sn, cs;
             __sincosf(theta * (float)(2.0 * kPi), &sn, &cs);
             sx = __builtin_fmaf(dist, cs, px);
@@ -626,7 +645,7 @@ __device__ __forceinline__ double source_term_f(const EqView &e, float px, float
         } else {
             const float u = rng.unit(), v = rng.unit(), ww = rng.unit_open();
             const float z = 2.0f * u - 1.0f;
-            const float surf = sqrtf(fmaxf(0.0f, 1.0f - z * z));
+            const float surf = Fx<float>::sqrt(fmaxf(0.0f, 1.0f - z * z));
             dist = R * Fx<float>::cbrt(ww);
             float sn, cs;
             __sincosf(v * (float)(2.0 * kPi), &sn, &cs);
@@ -674,7 +693,7 @@ __device__ double source_term_adj(const EqView &e, T px, T py, T pz, T R, T weig
                                   : (T)poisson_radius_3d<float>((float)R, rng);
                 const T z = (T)2 * (T)rng.unit() - (T)1;
                 const T angle = (T)2 * (T)kPi * (T)rng.unit();
-                const T xy = sqrt((T)1 - z * z);
+                const T xy = Fx<T>::sqrt((T)1 - z * z);
                 sx = px + radius * xy * Fx<T>::cos(angle);
                 sy = py + radius * xy * Fx<T>::sin(angle);
                 sz = pz + radius * z;
@@ -710,13 +729,13 @@ __device__ double source_term_adj(const EqView &e, T px, T py, T pz, T R, T weig
     // uniform ball point
     if constexpr (F32) {
         if (e.source_kind == WOS_SRC_CONSTANT) {
-            if (DIM == 2) dist = R * sqrtf(u01f_open(philox(w, step, 0, 1, k0, k1).b));
+            if (DIM == 2) dist = R * Fx<float>::sqrt(u01f_open(philox(w, step, 0, 1, k0, k1).b));
             else dist = R * Fx<float>::cbrt(u01f_open(philox(w, step, 1, 1, k0, k1).a));
         } else {
             CursorF rng(w, step, 1, k0, k1);
             if (DIM == 2) {
                 const float theta = rng.unit();
-                dist = R * sqrtf(rng.unit_open());
+                dist = R * Fx<float>::sqrt(rng.unit_open());
                 float sn, cs;
                 __sincosf(theta * (float)(2.0 * kPi), &sn, &cs);
                 sx = __builtin_fmaf(dist, cs, px);
@@ -724,7 +743,7 @@ __device__ double source_term_adj(const EqView &e, T px, T py, T pz, T R, T weig
             } else {
                 const float u = rng.unit(), v = rng.unit(), ww = rng.unit_open();
                 const float z = 2.0f * u - 1.0f;
-                const float surf = sqrtf(fmaxf(0.0f, 1.0f - z * z));
+                const float surf = Fx<float>::sqrt(fmaxf(0.0f, 1.0f - z * z));
                 dist = R * Fx<float>::cbrt(ww);
                 float sn, cs;
                 __sincosf(v * (float)(2.0 * kPi), &sn, &cs);
@@ -949,7 +968,7 @@ void k_walk_fp32(MeshView m, EqView e, PersistArgs a, AdjOut adj) {
         // ---- finished queries: radius, then end of walk or exact re-query
         if (active && !searching) {
             const bool exact = !(q.lb < q.best);
-            R = sqrtf(exact ? q.best : q.lb) * a.shrink_rel - margin;
+            R = Fx<float>::sqrt(exact ? q.best : q.lb) * a.shrink_rel - margin;
             const bool stop = !(R > a.eps_f) || steps >= a.max_steps;
             if (stop && !exact) {
                 qb = query_begin<DIM>(q, m, root, dom, 1.0f, px, py, pz);
@@ -1058,7 +1077,7 @@ void k_walk_fp32(MeshView m, EqView e, PersistArgs a, AdjOut adj) {
             } else {
                 const float u = wk.unitf(), v = wk.unitf();
                 const float z = 2.0f * u - 1.0f;
-                const float rr = sqrtf(fmaxf(0.0f, 1.0f - z * z));
+                const float rr = Fx<float>::sqrt(fmaxf(0.0f, 1.0f - z * z));
                 float sn, cs;
                 __sincosf(v * (float)(2.0 * kPi), &sn, &cs);
                 px = __builtin_fmaf(R * rr, cs, px);
@@ -1860,7 +1879,7 @@ int solve_once(wos_ctx *c, const std::vector<Part> &parts, const wos_eq_desc *eq
                 rv.pid = c->pid_all.as<int32_t>();
                 rv.pid_identity = 1;
                 rv.n = n_in;
-                rv.cw = (int)cw;
+                set_walks_per_point(rv, (int)cw);
                 rv.w0 = (int)done[0];
                 rv.total = cw * n_in;
                 for (int k = 0; k < n_in; ++k) { done[k] += cw; want[k] -= cw; }
@@ -2100,7 +2119,7 @@ int solve_graph(wos_ctx *c, const std::vector<Part> &parts, const wos_eq_desc *e
             rv.pid = c->pid_all.as<int32_t>();
             rv.pid_identity = 1;
             rv.n = (int)n;
-            rv.cw = (int)nw;
+            set_walks_per_point(rv, (int)nw);
             rv.w0 = 0;
             rv.total = n * nw;
             rv.n_dev = n_in_dev;
