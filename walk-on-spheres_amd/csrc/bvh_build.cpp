@@ -116,6 +116,52 @@ HostBVH build_reference_bvh(const HostMesh &m, std::string *err) {
     return std::move(b.out);
 }
 
+// The FP32 frame record of triangle (i0, i1, i2) of m, in FP64.
+static TriF triangle_frame(const HostMesh &m, int32_t i0, int32_t i1, int32_t i2) {
+    const int32_t id[3] = {i0, i1, i2};
+    double P[3][3];
+    for (int k = 0; k < 3; ++k)
+        for (int j = 0; j < 3; ++j) P[k][j] = m.verts[(size_t)id[k] * 3 + j];
+    auto dot = [](const double *a, const double *b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; };
+    // the longest edge A -> B, C the third vertex
+    int ia = 0;
+    double best = -1.0;
+    for (int k = 0; k < 3; ++k) {
+        const double *a = P[k], *b = P[(k + 1) % 3];
+        const double d[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+        const double l = dot(d, d);
+        if (l > best) { best = l; ia = k; }
+    }
+    const double *A = P[ia], *B = P[(ia + 1) % 3], *C = P[(ia + 2) % 3];
+    double u[3] = {B[0] - A[0], B[1] - A[1], B[2] - A[2]};
+    const double l1 = std::sqrt(dot(u, u));
+    if (l1 > 0.0) for (double &x : u) x /= l1;
+    else { u[0] = 1.0; u[1] = 0.0; u[2] = 0.0; }
+    const double e[3] = {C[0] - A[0], C[1] - A[1], C[2] - A[2]};
+    const double x2 = l1 > 0.0 ? dot(e, u) : 0.0;
+    double w[3] = {e[0] - x2 * u[0], e[1] - x2 * u[1], e[2] - x2 * u[2]};
+    // once more against u: w stays orthogonal to u when it is much shorter than e
+    const double k2 = dot(w, u);
+    for (int j = 0; j < 3; ++j) w[j] -= k2 * u[j];
+    double y2 = std::sqrt(dot(w, w));
+    if (l1 > 0.0 && y2 > 1e-9 * l1) {
+        for (double &x : w) x /= y2;
+    } else {
+        // any unit vector orthogonal to u
+        y2 = 0.0;
+        const double ax[3] = {std::fabs(u[0]) < 0.9 ? 1.0 : 0.0, std::fabs(u[0]) < 0.9 ? 0.0 : 1.0, 0.0};
+        const double k = dot(ax, u);
+        double v[3] = {ax[0] - k * u[0], ax[1] - k * u[1], ax[2] - k * u[2]};
+        const double lv = std::sqrt(dot(v, v));
+        for (int j = 0; j < 3; ++j) w[j] = v[j] / lv;
+    }
+    TriF f;
+    f.ox = (float)A[0]; f.oy = (float)A[1]; f.oz = (float)A[2]; f.l1 = (float)l1;
+    f.ux = (float)u[0]; f.uy = (float)u[1]; f.uz = (float)u[2]; f.x2 = (float)x2;
+    f.wx = (float)w[0]; f.wy = (float)w[1]; f.wz = (float)w[2]; f.y2 = (float)y2;
+    return f;
+}
+
 DeviceLayouts make_layouts(const std::vector<const HostMesh *> &meshes,
                            const std::vector<const HostBVH *> &bvhs) {
     DeviceLayouts L;
@@ -192,14 +238,7 @@ DeviceLayouts make_layouts(const std::vector<const HostMesh *> &meshes,
                 t.det = t.a * t.c - t.b * t.b;
                 t.denom = t.a - 2 * t.b + t.c;
                 L.trid[gs] = t;
-                TriF f{};
-                f.v0x = (float)t.v0x; f.v0y = (float)t.v0y; f.v0z = (float)t.v0z;
-                f.e1x = (float)t.e1x; f.e1y = (float)t.e1y; f.e1z = (float)t.e1z;
-                f.e2x = (float)t.e2x; f.e2y = (float)t.e2y; f.e2z = (float)t.e2z;
-                f.inv_a = t.a != 0.0 ? (float)(1.0 / t.a) : 0.0f;
-                f.inv_c = t.c != 0.0 ? (float)(1.0 / t.c) : 0.0f;
-                f.inv_det = t.det != 0.0 ? (float)(1.0 / t.det) : 0.0f;
-                L.trif[gs] = f;
+                L.trif[gs] = triangle_frame(m, i0, i1, i2);
             }
         }
     }

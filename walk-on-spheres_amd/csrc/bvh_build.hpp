@@ -67,14 +67,16 @@ struct alignas(16) NodeF2 {
     float r_lox, r_loy, r_hix, r_hiy;
     int32_t left, right, pad0, pad1;
 };
-// FP32 leaf slots (48 B: three 16 B loads).  With a = |e1|^2, c = |e2|^2,
-// b = e1.e2 and det = ac - b^2 formed in FP64, the record keeps 1/det (the
-// one quantity that cancels), 1/a and 1/c; reciprocals are 0 where the
-// quantity is 0.  The distance test forms a, b, c and 1/|e1 - e2|^2 itself.
+// FP32 leaf slots.  A triangle (48 B: three 16 B loads) in its own frame,
+// formed in FP64: origin o at one end of its longest edge, unit vector u
+// along that edge, unit vector w in the plane towards the third vertex
+// (n = u x w is the normal); the triangle is (0, 0), (l1, 0), (x2, y2) in
+// (u, w) coordinates with y2 >= 0.  Degenerate triangles keep an orthonormal
+// frame (y2 = 0, or l1 = x2 = y2 = 0 for a point).
 struct alignas(16) TriF {
-    float v0x, v0y, v0z, inv_det;
-    float e1x, e1y, e1z, inv_a;
-    float e2x, e2y, e2z, inv_c;
+    float ox, oy, oz, l1;
+    float ux, uy, uz, x2;
+    float wx, wy, wz, y2;
 };
 struct alignas(16) SegF { float s0x, s0y, vx, vy; };
 

@@ -756,37 +756,39 @@ __device__ __forceinline__ float sq3(float x, float y, float z) {
 // that distance in every case, degenerate triangles included (their
 // reciprocals are 0, which collapses a candidate onto a vertex).  The same
 // minimum as the region walk of src/npq.cpp:52-189.
+// Squared distance from p to a triangle in its frame record: p - o in
+// frame coordinates (x, y, h), then h^2 plus the squared in-plane distance to
+// the 2D triangle (0, 0), (l1, 0), (x2, y2): zero strictly inside it, else
+// the nearest of its three edges (parameters clamped to [0, 1]).
 __device__ __forceinline__ float tri_d2f(const TriF &T, float px, float py, float pz) {
-    const float a = sq3(T.e1x, T.e1y, T.e1z), c = sq3(T.e2x, T.e2y, T.e2z);
-    const float b = __builtin_fmaf(T.e1x, T.e2x, __builtin_fmaf(T.e1y, T.e2y, T.e1z * T.e2z));
-    // 1/|e1 - e2|^2 (the edge v1-v2); +inf for a degenerate edge, which the
-    // clamp below turns into an end point
-    const float inv_denom = __builtin_amdgcn_rcpf(sq3(T.e2x - T.e1x, T.e2y - T.e1y, T.e2z - T.e1z));
-    const float bx = T.v0x - px, by = T.v0y - py, bz = T.v0z - pz;
-    const float d = __builtin_fmaf(T.e1x, bx, __builtin_fmaf(T.e1y, by, T.e1z * bz));
-    const float e = __builtin_fmaf(T.e2x, bx, __builtin_fmaf(T.e2y, by, T.e2z * bz));
-    // plane projection v0 + s e1 + t e2
-    const float s = __builtin_fmaf(b, e, -c * d) * T.inv_det;
-    const float t = __builtin_fmaf(b, d, -a * e) * T.inv_det;
-    const bool inside = s >= 0.0f && t >= 0.0f && s + t <= 1.0f;
-    const float di = sq3(__builtin_fmaf(t, T.e2x, __builtin_fmaf(s, T.e1x, bx)),
-                         __builtin_fmaf(t, T.e2y, __builtin_fmaf(s, T.e1y, by)),
-                         __builtin_fmaf(t, T.e2z, __builtin_fmaf(s, T.e1z, bz)));
-    // edge v0-v1
-    const float u0 = clamp01(-d * T.inv_a);
-    const float d0 = sq3(__builtin_fmaf(u0, T.e1x, bx), __builtin_fmaf(u0, T.e1y, by),
-                         __builtin_fmaf(u0, T.e1z, bz));
-    // edge v0-v2
-    const float u1 = clamp01(-e * T.inv_c);
-    const float d1 = sq3(__builtin_fmaf(u1, T.e2x, bx), __builtin_fmaf(u1, T.e2y, by),
-                         __builtin_fmaf(u1, T.e2z, bz));
-    // edge v1-v2: v0 + w e1 + (1 - w) e2
-    const float w = clamp01(((c + e) - (b + d)) * inv_denom);
-    const float w1 = 1.0f - w;
-    const float d2 = sq3(__builtin_fmaf(w1, T.e2x, __builtin_fmaf(w, T.e1x, bx)),
-                         __builtin_fmaf(w1, T.e2y, __builtin_fmaf(w, T.e1y, by)),
-                         __builtin_fmaf(w1, T.e2z, __builtin_fmaf(w, T.e1z, bz)));
-    return fminf(inside ? di : INFINITY, fminf(d0, fminf(d1, d2)));
+    const float rx = px - T.ox, ry = py - T.oy, rz = pz - T.oz;
+    const float x = __builtin_fmaf(rx, T.ux, __builtin_fmaf(ry, T.uy, rz * T.uz));
+    const float y = __builtin_fmaf(rx, T.wx, __builtin_fmaf(ry, T.wy, rz * T.wz));
+    const float nx = __builtin_fmaf(T.uy, T.wz, -T.uz * T.wy);
+    const float ny = __builtin_fmaf(T.uz, T.wx, -T.ux * T.wz);
+    const float nz = __builtin_fmaf(T.ux, T.wy, -T.uy * T.wx);
+    const float h = __builtin_fmaf(rx, nx, __builtin_fmaf(ry, ny, rz * nz));
+    const float yy = y * y;
+    // edge (0, 0)-(l1, 0)
+    const float xa = x - __builtin_amdgcn_fmed3f(x, 0.0f, T.l1);
+    const float dA = __builtin_fmaf(xa, xa, yy);
+    // edge (0, 0)-(x2, y2); the floor keeps a zero-length edge finite
+    const float invC = __builtin_amdgcn_rcpf(fmaxf(__builtin_fmaf(T.x2, T.x2, T.y2 * T.y2), 1e-30f));
+    const float tC = clamp01(__builtin_fmaf(x, T.x2, y * T.y2) * invC);
+    const float xc = __builtin_fmaf(-tC, T.x2, x), yc = __builtin_fmaf(-tC, T.y2, y);
+    const float dC = __builtin_fmaf(xc, xc, yc * yc);
+    // edge (l1, 0)-(x2, y2)
+    const float bx = T.x2 - T.l1, xl = x - T.l1;
+    const float invB = __builtin_amdgcn_rcpf(fmaxf(__builtin_fmaf(bx, bx, T.y2 * T.y2), 1e-30f));
+    const float tB = clamp01(__builtin_fmaf(xl, bx, y * T.y2) * invB);
+    const float xb = __builtin_fmaf(-tB, bx, xl), yb = __builtin_fmaf(-tB, T.y2, y);
+    const float dB = __builtin_fmaf(xb, xb, yb * yb);
+    // strictly left of the three edges of the counter-clockwise triangle
+    const float crossB = __builtin_fmaf(bx, y, -T.y2 * xl);
+    const float crossC = __builtin_fmaf(T.y2, x, -T.x2 * y);
+    const bool inside = fminf(y, fminf(crossB, crossC)) > 0.0f;
+    const float inplane = inside ? 0.0f : fminf(dA, fminf(dB, dC));
+    return __builtin_fmaf(h, h, inplane);
 }
 
 // (a zero-length segment gives 0 x inf = NaN, which the clamp turns into 0)
