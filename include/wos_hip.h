@@ -175,7 +175,8 @@ typedef struct wos_tuning {
                                     with an exact search in every case.            */
     int32_t grid_cells;          /* cells per domain of the FP32 distance grid
                                     (cell lists near the boundary, cell distance
-                                    bounds inside); 0 = no grid                    */
+                                    bounds inside); 0 = no grid, -1 = 2^24 for
+                                    triangle meshes and 2^22 for segment meshes    */
     int32_t grid_list_max;       /* longest cell list kept (1..127); a cell with
                                     more candidate primitives keeps none           */
     int32_t xcd_parts;           /* the walk items are split into this many

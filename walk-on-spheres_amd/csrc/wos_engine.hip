@@ -1300,7 +1300,7 @@ struct wos_ctx {
     int last_rounds = 0;
     MeshView view{};
     hipEvent_t ev[6]{};
-    wos_tuning tune{256, -1, 64, 0, 48, 0.2f, 1 << 23, 127, 8};
+    wos_tuning tune{256, -1, 64, 0, 48, 0.2f, -1, 127, 8};
     size_t lds_per_cu = 65536;
     size_t lds_per_block = 65536;
     int last_threads = 0;
@@ -1311,10 +1311,12 @@ namespace {
 
 // Build the distance grid of every domain for the FP32 walker (once per mesh
 // and grid size) and point the kernel view at it; grid_cells == 0 turns it
-// off.  Cubic cells of size h = (bbox volume / grid_cells)^(1/dim) over the
-// domain's bounding box padded by half a cell.
+// off, -1 takes 2^24 cells per domain in 3D and 2^22 in 2D.  Cubic cells of
+// size h = (bbox volume / cells)^(1/dim) over the domain's bounding box
+// padded by half a cell.
 hipError_t ensure_grid(wos_ctx *c) {
-    const int target = c->tune.grid_cells;
+    const int target = c->tune.grid_cells >= 0 ? c->tune.grid_cells
+                                               : (c->mesh.dim == 3 ? 1 << 24 : 1 << 22);
     if (target <= 0) {
         c->view.grids = nullptr; c->view.cells = nullptr; c->view.glist = nullptr;
         return hipSuccess;
@@ -2485,8 +2487,8 @@ int wos_ctx_set_tuning(wos_ctx *c, const wos_tuning *t) {
         return fail(WOS_ERR_INPUT, "lds_nodes must be >= -1 and blocks_per_cu >= 0");
     if (t->min_ready < 0 || t->min_ready > 64) return fail(WOS_ERR_INPUT, "min_ready must be in 0..64");
     if (!(t->relax >= 0.0f && t->relax < 0.5f)) return fail(WOS_ERR_INPUT, "relax must be in [0, 0.5)");
-    if (t->grid_cells < 0 || t->grid_cells > (1 << 26))
-        return fail(WOS_ERR_INPUT, "grid_cells must be in 0..2^26");
+    if (t->grid_cells < -1 || t->grid_cells > (1 << 26))
+        return fail(WOS_ERR_INPUT, "grid_cells must be -1 (default) or in 0..2^26");
     if (t->grid_list_max < 1 || t->grid_list_max > kGridListMax)
         return fail(WOS_ERR_INPUT, "grid_list_max must be in 1..127");
     if (t->xcd_parts < 1 || t->xcd_parts > kMaxParts)
