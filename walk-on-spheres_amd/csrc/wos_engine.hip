@@ -1052,8 +1052,11 @@ void k_walk_fp32(MeshView m, EqView e, PersistArgs a, AdjOut adj) {
             }
         }
 
-        // ---- one walk step (include/wos/wos.hpp:68-127), then the next query
-        if (active && !searching) {
+        // ---- one walk step (include/wos/wos.hpp:68-127), then the next query.
+        // A lane that began its exact re-query above does not step: its
+        // radius is the relaxed one, and the exact result (which query_begin
+        // may already have completed) decides the walk's end next iteration.
+        if (active && !searching && !began) {
             if (SMODE != 0 && e.source_kind != WOS_SRC_ZERO) {
                 if constexpr (ADJ)
                     sample += source_term_adj<DIM, float, SMODE>(e, px, py, pz, R, weight, w,
