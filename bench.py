@@ -351,7 +351,8 @@ def main():
         with open(tpath) as f:
             prof = json.load(f)
         traffic = prof.get("hbm_bytes_per_launch")
-        counters = {k: prof[k] for k in ("valu_busy", "valu_lane_utilisation", "l2_hit_rate")
+        counters = {k: prof[k] for k in ("valu_insts_per_cu_cycle", "valu_lane_utilisation",
+                                         "td_busy", "l2_hit_rate")
                     if k in prof}
     steps = s32["total_steps"]
 

@@ -639,6 +639,31 @@ __device__ __forceinline__ double dlog_g3_dalpha(double alpha, double R, double 
     return -r + h_screen(alpha, R - r) - h_screen(alpha, R);
 }
 
+// The FP32 replay forms the same two derivatives from
+// phi(y) = 2y / expm1(2 alpha y) - 1/alpha + y, the screened kernel's
+// deviation from its linear part: d/dalpha log w3 = -phi(R) and
+// d/dalpha log G3 = phi(R - r) - phi(R), which cancels no 1/alpha terms
+// (the forms above subtract quantities of size 1/alpha).  With
+// z = 2 alpha y, alpha phi = z/expm1(z) - 1 + z/2 = z^2/12 - z^4/720 +
+// z^6/30240 - z^8/1209600 + ... (Bernoulli series, used for |z| < 1).
+__device__ __forceinline__ float phi_screen_f(float alpha, float y) {
+    const float z = 2.0f * alpha * y;
+    const float z2 = z * z;
+    float s = __builtin_fmaf(z2, -1.0f / 1209600.0f, 1.0f / 30240.0f);
+    s = __builtin_fmaf(z2, s, -1.0f / 720.0f);
+    s = __builtin_fmaf(z2, s, 1.0f / 12.0f);
+    s *= z2;
+    const float d = Fx<float>::div(z, Fx<float>::expm1(z)) - 1.0f + 0.5f * z;
+    return Fx<float>::div(fabsf(z) < 1.0f ? s : d, alpha);
+}
+__device__ __forceinline__ float dlog_w3_dalpha_f(float alpha, float R) {
+    return alpha * R == 0.0f ? 0.0f : -phi_screen_f(alpha, R);
+}
+__device__ __forceinline__ float dlog_g3_dalpha_f(float alpha, float R, float r) {
+    if (alpha * R < 1e-5f) return 0.0f;
+    return phi_screen_f(alpha, R - r) - phi_screen_f(alpha, R);
+}
+
 // Per-walk derivatives accumulated by the adjoint replay.
 struct WalkDer {
     double d_alpha;           // d value / d alpha
@@ -649,7 +674,10 @@ struct WalkDer {
 };
 
 // source-term derivatives for a term S = base * f(y) (base = weight x
-// volume x G or weight x mass); dlogG = d log G / d alpha (0 if unused)
+// volume x G or weight x mass); dlogG = d log G / d alpha (0 if unused).
+// T is the replay's precision: the Gaussian factor exp(-|y - c|^2 / s^2) of
+// d/dQ is formed in it.
+template <typename T>
 __device__ __forceinline__ void add_source_der(WalkDer &D, const EqView &e, int dim, double base,
                                                double S, double yx, double yy, double yz,
                                                double dlogG) {
@@ -657,7 +685,7 @@ __device__ __forceinline__ void add_source_der(WalkDer &D, const EqView &e, int 
         const double dx = yx - e.cx, dy = yy - e.cy, dz = dim == 3 ? yz - e.cz : 0.0;
         const double s2 = e.src_radius * e.src_radius;
         const double d2 = dx * dx + dy * dy + dz * dz;
-        D.d_q += base * exp(-d2 / s2);
+        D.d_q += base * (double)Fx<T>::exp(-Fx<T>::div((T)d2, (T)s2));
         D.d_sigma += S * 2.0 * d2 / (s2 * e.src_radius);
         D.d_c[0] += S * 2.0 * dx / s2;
         D.d_c[1] += S * 2.0 * dy / s2;

@@ -541,8 +541,13 @@ void set_walks_per_point(RoundView &r, int cw) {
 // --------------------------------------------------------------------------
 // kernel 2a: FP64 plain walker, one lane per (point, walk)
 // --------------------------------------------------------------------------
+// At least 3 wavefronts per SIMD (at most 168 VGPRs) for the 3D forward
+// walks, which fit without spilling; 2D walks with a source term and adjoint
+// replays may use 256 rather than spill.
 template <int DIM, bool SRC, bool SCR, bool ADJ>
-__global__ __launch_bounds__(kBlock) void k_walk_fp64(MeshView m, EqView e, RoundView rv,
+__global__ __launch_bounds__(kBlock)
+__attribute__((amdgpu_waves_per_eu(ADJ || DIM == 2 ? 2 : 3, 8)))
+void k_walk_fp64(MeshView m, EqView e, RoundView rv,
                                                       double eps, int max_steps, double *vals,
                                                       int32_t *steps_out, AdjOut adj) {
     extern __shared__ int lds_i[];
@@ -723,7 +728,7 @@ __device__ double source_term_adj(const EqView &e, T px, T py, T pz, T R, T weig
         if (DIM == 2) mass = screened ? screened_mass_2d<T>(alpha, R) : R * R / (T)4;
         else mass = screened ? screened_mass_3d<T>(alpha, R) : R * R / (T)6;
         const double S = (double)(weight * mass * f);
-        add_source_der(D, e, DIM, (double)(weight * mass), S, sx, sy, sz, 0.0);
+        add_source_der<T>(D, e, DIM, (double)(weight * mass), S, sx, sy, sz, 0.0);
         return S;
     }
     // uniform ball point
@@ -783,8 +788,11 @@ __device__ double source_term_adj(const EqView &e, T px, T py, T pz, T R, T weig
         G = screened ? screened_green_3d<T>(alpha, R, dist) : poisson_green_3d<T>(R, dist);
     }
     const double S = (double)(weight * volume * f * G);
-    const double dlogG = (DIM == 3 && screened) ? dlog_g3_dalpha(e.alpha, (double)R, (double)dist) : 0.0;
-    add_source_der(D, e, DIM, (double)(weight * volume * G), S, sx, sy, sz, dlogG);
+    double dlogG = 0.0;
+    if (DIM == 3 && screened)
+        dlogG = F32 ? (double)dlog_g3_dalpha_f((float)alpha, (float)R, (float)dist)
+                    : dlog_g3_dalpha(e.alpha, (double)R, (double)dist);
+    add_source_der<T>(D, e, DIM, (double)(weight * volume * G), S, sx, sy, sz, dlogG);
     return S;
 }
 
@@ -1066,7 +1074,7 @@ void k_walk_fp32(MeshView m, EqView e, PersistArgs a, AdjOut adj) {
                                                         (uint32_t)steps, k0, k1);
             }
             if (SCR) {
-                if constexpr (ADJ && DIM == 3) D.L += dlog_w3_dalpha(e.alpha, (double)R);
+                if constexpr (ADJ && DIM == 3) D.L += (double)dlog_w3_dalpha_f(alpha, R);
                 weight *= DIM == 2 ? screened_weight_2d<float>(alpha, R)
                                    : screened_weight_3d<float>(alpha, R);
             }
