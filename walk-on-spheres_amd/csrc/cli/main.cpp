@@ -44,8 +44,9 @@ static void print_usage(const char *program) {
         "  -h, --help         show this help message\n"
         "\n"
         "GPU options:\n"
-        "      --precision P  fp64 (one lane per walk, default) or fp32 (persistent\n"
-        "                     walker, FP32 geometry, FP64 statistics)\n"
+        "      --precision P  fp64 (one lane per walk, default), fp32 (persistent\n"
+        "                     walker, FP32 geometry, FP64 statistics) or fp64-ref\n"
+        "                     (FP64 on the reference's own per-point random streams)\n"
         "      --target-se X  add walks per point until its standard error <= X\n"
         "      --max-walks N  cap of walks per point with --target-se\n"
         "      --problem NAME replace the equation's built-in data: annulus,\n"
@@ -166,9 +167,10 @@ int main(int argc, char **argv) {
         } else if (!std::strcmp(a, "--force")) {
             force_output = true;
         } else if (!std::strcmp(a, "--precision")) {
-            if (!need(i, "fp64 or fp32")) return 1;
+            if (!need(i, "fp64, fp32 or fp64-ref")) return 1;
             if (!std::strcmp(argv[i], "fp64")) opt.precision = WOS_FP64_PLAIN;
             else if (!std::strcmp(argv[i], "fp32")) opt.precision = WOS_FP32_PERSISTENT;
+            else if (!std::strcmp(argv[i], "fp64-ref")) opt.precision = WOS_FP64_REFERENCE;
             else {
                 std::fprintf(stderr, "Invalid --precision value: %s\n", argv[i]);
                 return 1;

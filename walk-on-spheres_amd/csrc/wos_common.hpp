@@ -17,6 +17,7 @@ constexpr double kPi = 3.141592653589793238462643383279502884;
 // Stream constants of the reference sweep (include/wos/runner.hpp:182, :208, :211).
 constexpr uint64_t kInsideStream = 0x243F6A8885A308D3ULL;
 constexpr uint64_t kWalkStream = 0x13198A2E03707344ULL;
+constexpr uint64_t kSourceStream = 0xA4093822299F31D0ULL;  // 2D only (runner.hpp:211)
 
 // include/wos/hash.hpp:7-12
 WOS_HD uint64_t splitmix64(uint64_t x) {
@@ -38,6 +39,7 @@ WOS_HD uint64_t rotl64(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
 // draws must equal the reference's to reproduce its location codes.
 struct Xoshiro {
     uint64_t s0, s1, s2, s3;
+    WOS_HD Xoshiro() : s0(0), s1(0), s2(0), s3(0) {}
     WOS_HD explicit Xoshiro(uint64_t seed) {
         s0 = splitmix64(seed);
         s1 = splitmix64(seed + 0x9E3779B97F4A7C15ULL);
@@ -57,6 +59,12 @@ struct Xoshiro {
         return result;
     }
     WOS_HD double unit() { return (double)(next() >> 11) * (1.0 / 9007199254740992.0); }
+    // include/wos/prng.hpp:31-37: redraw zero
+    WOS_HD double unit_open() {
+        double v;
+        do { v = unit(); } while (v == 0.0);
+        return v;
+    }
 };
 
 // Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as

@@ -112,6 +112,8 @@ struct StartInfo {
     uint32_t key0, key1;  // walk-stream key
     int32_t domain;       // the point's domain index
     int32_t pad;
+    uint32_t skey0, skey1;  // 2D source-stream seed of the reference streams
+    uint32_t pad1, pad2;
 };
 
 // the FP32 walker's copy of a start (two 16 B loads), one per inside point

@@ -284,6 +284,10 @@ __global__ __launch_bounds__(kBlock) void k_classify(MeshView m, EqView e, GridS
     S.bid = (DIM == 2 && q.prim >= 0) ? m.prim_bid[q.prim] : -1;
     S.key0 = (uint32_t)key;
     S.key1 = (uint32_t)(key >> 32);
+    const uint64_t skey = splitmix64(ps ^ kSourceStream);
+    S.skey0 = (uint32_t)skey;
+    S.skey1 = (uint32_t)(skey >> 32);
+    S.pad1 = S.pad2 = 0u;
     const int64_t o = dom.out_off + i;
     starts[o] = S;
     loc[o] = (uint8_t)code;
@@ -609,6 +613,95 @@ void k_walk_fp64(MeshView m, EqView e, RoundView rv,
         const double s = steps == 0 ? S->s : q.s, t = steps == 0 ? S->t : q.t;
         store_adjoint(adj, item, D, weight, g, prim, s, DIM == 3 ? t : 0.0);
     }
+}
+
+// --------------------------------------------------------------------------
+// kernel 2c: FP64 walker on the reference's streams, one lane per point
+// --------------------------------------------------------------------------
+// The reference draws a point's walks one after another from per-point
+// xoshiro256++ streams (include/wos/runner.hpp:208-221): 2D a walk stream and
+// a source stream, 3D one stream for both.  Walk i's draws start where walk
+// i-1 stopped, so one lane runs all of a point's walks of a round in order,
+// with the streams' state kept per inside point across rounds (8 words:
+// walk stream, source stream).  Steps, sources, weights and queries are those
+// of k_walk_fp64 (include/wos/wos.hpp:54-138), drawing in the reference's
+// order: source sample (any source kind but none, a zero source included),
+// then the step direction.
+
+// a stream as the samplers' RNG interface
+struct XoRef {
+    Xoshiro *x;
+    __device__ double unit() { return x->unit(); }
+    __device__ double unit_open() { return x->unit_open(); }
+};
+
+__global__ __launch_bounds__(kBlock) void k_ref_seed(const StartInfo *starts, const int32_t *inside,
+                                                     int64_t n, int dim, uint64_t *state) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const StartInfo &S = starts[inside[j]];
+    const Xoshiro w(((uint64_t)S.key1 << 32) | S.key0);
+    const Xoshiro src(dim == 2 ? (((uint64_t)S.skey1 << 32) | S.skey0) : 0ULL);
+    uint64_t *o = state + 8 * j;
+    o[0] = w.s0; o[1] = w.s1; o[2] = w.s2; o[3] = w.s3;
+    o[4] = src.s0; o[5] = src.s1; o[6] = src.s2; o[7] = src.s3;
+}
+
+template <int DIM, bool SRC, bool SCR>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2, 8)))
+void k_walk_ref(MeshView m, EqView e, RoundView rv, double eps, int max_steps, double *vals,
+                int32_t *steps_out, uint64_t *state) {
+    extern __shared__ int lds_i[];
+    int *stk = lds_i + threadIdx.x;
+    const int stride = blockDim.x;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= rv.n) return;
+    const int j = round_inside_id(rv, k);
+    const int64_t base = rv.off ? rv.off[k] : (int64_t)k * rv.cw;
+    const int cnt = rv.off ? (int)(rv.off[k + 1] - rv.off[k]) : rv.cw;
+    const StartInfo *S = &rv.starts[rv.inside[j]];
+    uint64_t *st = state + 8 * (int64_t)j;
+    Xoshiro ws, ss;
+    ws.s0 = st[0]; ws.s1 = st[1]; ws.s2 = st[2]; ws.s3 = st[3];
+    ss.s0 = st[4]; ss.s1 = st[5]; ss.s2 = st[6]; ss.s3 = st[7];
+    XoRef walk{&ws};
+    XoRef source{DIM == 2 ? &ss : &ws};
+    const int root = S->root64;
+    const double alpha = e.alpha;
+    for (int i = 0; i < cnt; ++i) {
+        double px = S->x, py = S->y, pz = S->z, R = S->r;
+        NpqD q{};
+        double sample = 0.0, weight = 1.0;
+        int steps = 0;
+        while (R > eps && steps < max_steps) {
+            if (SRC) sample += source_term<DIM, double>(e, px, py, pz, R, weight, source);
+            if (SCR)
+                weight *= DIM == 2 ? screened_weight_2d<double>(alpha, R)
+                                   : screened_weight_3d<double>(alpha, R);
+            if (DIM == 2) {
+                const double theta = walk.unit() * 2 * kPi;
+                px = px + R * cos(theta);
+                py = py + R * sin(theta);
+            } else {
+                const double u = walk.unit(), v = walk.unit();
+                const double z = 2 * u - 1, phi = 2 * kPi * v;
+                const double rr = sqrt(1 - z * z);
+                px = px + R * rr * cos(phi);
+                py = py + R * rr * sin(phi);
+                pz = pz + R * z;
+            }
+            ++steps;
+            q = DIM == 2 ? npq_ref_2d(m, root, px, py, stk, stride)
+                         : npq_ref_3d(m, root, px, py, pz, stk, stride);
+            R = sqrt(q.d2);
+        }
+        const bool hit = R > eps;
+        const double g = steps == 0 ? S->g : eq_boundary(e, m, q.px, q.py, q.pz, q.prim, q.s, q.t);
+        vals[base + i] = sample + weight * g;
+        steps_out[base + i] = steps | (hit ? (int32_t)0x80000000 : 0);
+    }
+    st[0] = ws.s0; st[1] = ws.s1; st[2] = ws.s2; st[3] = ws.s3;
+    st[4] = ss.s0; st[5] = ss.s1; st[6] = ss.s2; st[7] = ss.s3;
 }
 
 // FP32 source term.  Same draws as source_term (the walk's tag-1 counter
@@ -1305,6 +1398,7 @@ struct wos_ctx {
     int64_t grid_list_len = 0;
     DevBuf doms;                      // DomainF per domain
     DevBuf starts_f, r_kblk;          // FP32 start copies; item block -> point (uneven rounds)
+    DevBuf ref_state;                 // reference-stream state per inside point (8 words)
     DevBuf plist_in, loc, starts, amb, bvalue, inside, vals, steps, mean, m2, count, ssum, hits,
         counter, vv, pid_all, r_pid, r_wbeg, r_off, coef, scal_sum, grad_v, grad_b, dl_du,
         adj_scal, adj_vw, adj_prim;
@@ -1447,6 +1541,19 @@ hipError_t launch_classify(wos_ctx *c, K kernel, const EqView &ev, const GridSpe
 template <int DIM, bool SRC, bool SCR, bool ADJ>
 hipError_t launch_walk(wos_ctx *c, const EqView &ev, int precision, const RoundView &rv,
                        double eps, int max_steps, const AdjOut &adj, int *blocks_out) {
+    if (precision == WOS_FP64_REFERENCE) {
+        if constexpr (ADJ) {
+            return hipErrorInvalidValue;
+        } else {
+            const int blocks = (rv.n + kBlock - 1) / kBlock;
+            const size_t lds = (size_t)kBlock * (c->L.depth + 2) * sizeof(int);
+            hipLaunchKernelGGL((k_walk_ref<DIM, SRC, SCR>), dim3(blocks), dim3(kBlock), lds,
+                               c->stream, c->view, ev, rv, eps, max_steps, c->vals.as<double>(),
+                               c->steps.as<int32_t>(), c->ref_state.as<uint64_t>());
+            *blocks_out = blocks;
+            return hipGetLastError();
+        }
+    }
     if (precision == WOS_FP64_PLAIN) {
         const int blocks = (int)((rv.total + kBlock - 1) / kBlock);
         const size_t lds = (size_t)kBlock * (c->L.depth + 2) * sizeof(int);
@@ -1593,6 +1700,8 @@ int prepare_solve(wos_ctx *c, const std::vector<Part> &parts, const wos_eq_desc 
     if (adjr && !adjr->dl_du) return fail(WOS_ERR_INPUT, "dl_du is required");
     if (adjr && prm->target_se > 0.0)
         return fail(WOS_ERR_INPUT, "the adjoint replays a fixed number of walks (target_se = 0)");
+    if (adjr && prm->precision == WOS_FP64_REFERENCE)
+        return fail(WOS_ERR_INPUT, "the adjoint replays counter streams (precision FP64 or FP32)");
     int rc = validate_eq(c, eq);
     if (rc) return rc;
     if (prm->n_walks < 1) return fail(WOS_ERR_INPUT, "n_walks must be >= 1");
@@ -1601,7 +1710,8 @@ int prepare_solve(wos_ctx *c, const std::vector<Part> &parts, const wos_eq_desc 
         return fail(WOS_ERR_INPUT, "max_ray_attempts must be >= 3");
     if (!(prm->epsilon > 0.0) || !std::isfinite(prm->epsilon))
         return fail(WOS_ERR_INPUT, "epsilon must be positive and finite");
-    if (prm->precision != WOS_FP64_PLAIN && prm->precision != WOS_FP32_PERSISTENT)
+    if (prm->precision != WOS_FP64_PLAIN && prm->precision != WOS_FP32_PERSISTENT &&
+        prm->precision != WOS_FP64_REFERENCE)
         return fail(WOS_ERR_INPUT, "unknown precision");
     if (n <= 0 || n > ((int64_t)1 << 31) - 1) return fail(WOS_ERR_INPUT, "point count out of range");
     if (!(prm->target_se >= 0.0) || !std::isfinite(prm->target_se))
@@ -1815,6 +1925,14 @@ int solve_once(wos_ctx *c, const std::vector<Part> &parts, const wos_eq_desc *eq
                            0, c->stream, c->starts.as<StartInfo>(), c->inside.as<int32_t>(),
                            (int64_t)n_in, nullptr, c->starts_f.as<StartF>());
         HIPX(hipGetLastError());
+        if (prm->precision == WOS_FP64_REFERENCE) {
+            HIPX(c->ref_state.reserve((size_t)n_in * 8 * sizeof(uint64_t)));
+            hipLaunchKernelGGL(k_ref_seed, dim3((unsigned)((n_in + kBlock - 1) / kBlock)),
+                               dim3(kBlock), 0, c->stream, c->starts.as<StartInfo>(),
+                               c->inside.as<int32_t>(), (int64_t)n_in, dim,
+                               c->ref_state.as<uint64_t>());
+            HIPX(hipGetLastError());
+        }
         HIPX(c->mean.reserve(n_in * sizeof(double)));
         HIPX(c->m2.reserve(n_in * sizeof(double)));
         HIPX(c->count.reserve(n_in * sizeof(double)));

@@ -27,7 +27,7 @@ WOS_SOURCE_UNIFORM, WOS_SOURCE_GREEN = range(2)
 (WOS_BND_X2Y2_ID0, WOS_BND_Y5, WOS_BND_ZERO, WOS_BND_ID_CONST, WOS_BND_VERTEX,
  WOS_BND_SIN3X_COS2Y) = range(6)
 # enum wos_precision
-WOS_FP64_PLAIN, WOS_FP32_PERSISTENT = range(2)
+WOS_FP64_PLAIN, WOS_FP32_PERSISTENT, WOS_FP64_REFERENCE = range(3)
 WOS_MAX_BND_VALUES = 16
 
 LOC_OUTSIDE, LOC_INSIDE, LOC_BOUNDARY, LOC_INDETERMINATE = range(4)
