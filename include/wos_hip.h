@@ -42,7 +42,9 @@ enum wos_status {
 enum wos_equation_kind {
     WOS_EQ_LAPLACE = 0,          /* Delta u = 0                                  */
     WOS_EQ_POISSON = 1,          /* Delta u = -f                                 */
-    WOS_EQ_SCREENED = 2          /* Delta u - alpha^2 u = -f                     */
+    WOS_EQ_SCREENED = 2,         /* Delta u - alpha^2 u = -f                     */
+    WOS_EQ_HELMHOLTZ = 3         /* Delta u + k^2 u = 0, k = alpha, no source: weights
+                                    1/J0(kR) in 2D, kR/sin(kR) in 3D (helmholtz.cpp:15-78) */
 };
 
 /* values equal the reference SourceMode enum (include/wos/source_mode.hpp:5-9) */
@@ -66,7 +68,8 @@ enum wos_boundary_kind {
     WOS_BND_ZERO = 2,            /* g = 0 (poisson.cpp:43-48)                                */
     WOS_BND_ID_CONST = 3,        /* g = bnd_values[boundary id] (2D)                         */
     WOS_BND_VERTEX = 4,          /* g = barycentric / linear blend of per-vertex values      */
-    WOS_BND_SIN3X_COS2Y = 5      /* g = sin(3x) cos(2y)                                      */
+    WOS_BND_SIN3X_COS2Y = 5,     /* g = sin(3x) cos(2y)                                      */
+    WOS_BND_ONE = 6              /* g = 1 (helmholtz.cpp:22-26, :55-58)                      */
 };
 
 #define WOS_MAX_BND_VALUES 16

@@ -54,8 +54,11 @@ class Oracle:
         for name in ("orc_i0_scaled", "orc_k0_scaled", "orc_log_i0", "orc_log_k0"):
             getattr(L, name).restype = C.c_double
             getattr(L, name).argtypes = [C.c_double]
+        L.orc_bessel_j0_series.restype = C.c_double
+        L.orc_bessel_j0_series.argtypes = [C.c_double]
         for name in ("orc_poisson_green_2d", "orc_poisson_green_3d", "orc_screened_weight_2d",
-                     "orc_screened_weight_3d", "orc_screened_mass_2d", "orc_screened_mass_3d"):
+                     "orc_screened_weight_3d", "orc_screened_mass_2d", "orc_screened_mass_3d",
+                     "orc_helmholtz_weight_2d", "orc_helmholtz_weight_3d"):
             getattr(L, name).restype = C.c_double
             getattr(L, name).argtypes = [C.c_double, C.c_double]
         for name in ("orc_screened_green_2d", "orc_screened_green_3d"):

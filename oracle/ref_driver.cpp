@@ -22,6 +22,7 @@
 #include <vector>
 
 #include "wos/bvh.hpp"
+#include "wos/fastmath.hpp"
 #include "wos/grid.hpp"
 #include "wos/hash.hpp"
 #include "wos/inside.hpp"
@@ -75,6 +76,7 @@ struct Desc2D {
         case WOS_BND_X2Y2_ID0: return bid == 0 ? p.x * p.x - p.y * p.y : 0.0;
         case WOS_BND_ID_CONST: return (bid >= 0 && bid < e->n_bnd_values) ? e->bnd_values[bid] : 0.0;
         case WOS_BND_SIN3X_COS2Y: return std::sin(3.0 * p.x) * std::cos(2.0 * p.y);
+        case WOS_BND_ONE: return 1.0;
         default: return 0.0;
         }
     }
@@ -82,7 +84,15 @@ struct Desc2D {
         const double r = dist(x, y);
         return Scr ? screened::green_2d(alpha, s.radius, r) : poisson_green::green_2d(s.radius, r);
     }
-    double screening_factor(double radius) const { return screened::weight_2d(alpha, radius); }
+    // helmholtz.cpp:36-41 with k = alpha, else screened::weight_2d
+    double screening_factor(double radius) const {
+        if (e->kind == WOS_EQ_HELMHOLTZ) {
+            const double prod = radius * alpha;
+            if (prod < 1e-8) return 1.0;
+            return 1.0 / bessel_J0(prod);
+        }
+        return screened::weight_2d(alpha, radius);
+    }
     double green_mass(double radius) const {
         return Scr ? screened::mass_2d(alpha, radius) : poisson_green::mass_2d(radius);
     }
@@ -113,6 +123,7 @@ struct Desc3D {
             return y50 + 4.0 * y5m3;
         }
         case WOS_BND_SIN3X_COS2Y: return std::sin(3.0 * p.x) * std::cos(2.0 * p.y);
+        case WOS_BND_ONE: return 1.0;
         default: return 0.0;
         }
     }
@@ -120,7 +131,15 @@ struct Desc3D {
         const double r = dist(x, y);
         return Scr ? screened::green_3d(alpha, s.radius, r) : poisson_green::green_3d(s.radius, r);
     }
-    double screening_factor(double radius) const { return screened::weight_3d(alpha, radius); }
+    // helmholtz.cpp:70-75 with k = alpha, else screened::weight_3d
+    double screening_factor(double radius) const {
+        if (e->kind == WOS_EQ_HELMHOLTZ) {
+            const double prod = radius * alpha;
+            if (prod < 1e-8) return 1.0;
+            return prod / std::sin(prod);
+        }
+        return screened::weight_3d(alpha, radius);
+    }
     double green_mass(double radius) const {
         return Scr ? screened::mass_3d(alpha, radius) : poisson_green::mass_3d(radius);
     }
@@ -226,7 +245,7 @@ template<int N, typename F>
 void with_eq(const wos_eq_desc *e, F &&f) {
     const SourceMode mode = static_cast<SourceMode>(e->source_mode);
     const bool src = e->source_kind != WOS_SRC_NONE;
-    const bool scr = e->kind == WOS_EQ_SCREENED;
+    const bool scr = e->kind == WOS_EQ_SCREENED || e->kind == WOS_EQ_HELMHOLTZ;
     if constexpr (N == 2) {
         if (src && scr) f(Desc2D<true, true>{e, mode, e->alpha});
         else if (src) f(Desc2D<true, false>{e, mode, e->alpha});

@@ -284,6 +284,30 @@ double orc_screened_green_3d(double alpha, double R, double r) {
     const double ratio = exp(-x) * (-expm1(-2.0 * (X - x))) / (-expm1(-2.0 * X));
     return ratio / (4.0 * ORC_PI * r);
 }
+/* include/wos/fastmath.hpp:15-29: the series of J0 stopped at the first term
+ * below 1e-8 (the reference's own truncation; pinned against scipy's j0 to
+ * that accuracy) */
+double orc_bessel_j0_series(double x) {
+    const double q = 0.25 * x * x;
+    double term = 1.0, sum = 1.0;
+    for (int k = 1; k < 200; ++k) {
+        term *= q / (double)(k * k);
+        if (term < 1e-8) return sum;
+        sum += (k % 2 == 0 ? term : -term);
+    }
+    return sum;
+}
+/* src/equations/helmholtz.cpp:36-41 (2D) and :70-75 (3D) */
+double orc_helmholtz_weight_2d(double k, double R) {
+    const double prod = R * k;
+    if (prod < 1e-8) return 1.0;
+    return 1.0 / orc_bessel_j0_series(prod);
+}
+double orc_helmholtz_weight_3d(double k, double R) {
+    const double prod = R * k;
+    if (prod < 1e-8) return 1.0;
+    return prod / sin(prod);
+}
 /* include/wos/screened.hpp:82-91 */
 double orc_screened_mass_3d(double alpha, double R) {
     const double x = alpha * R;
@@ -905,6 +929,7 @@ static double eq_boundary(const wos_eq_desc *eq, const orc_bvh *b, const orc_npq
         return (1.0 - q->s - q->t) * vv[pr[0]] + q->s * vv[pr[1]] + q->t * vv[pr[2]];
     }
     case WOS_BND_SIN3X_COS2Y: return sin(3.0 * p[0]) * cos(2.0 * p[1]);
+    case WOS_BND_ONE: return 1.0;
     default: return 0.0;
     }
 }
@@ -1039,6 +1064,9 @@ static orc_sample wos_sample(const orc_bvh *b, const wos_eq_desc *eq, const doub
         if (eq->kind == WOS_EQ_SCREENED)
             weight *= dim == 2 ? orc_screened_weight_2d(eq->alpha, R)
                                : orc_screened_weight_3d(eq->alpha, R);
+        else if (eq->kind == WOS_EQ_HELMHOLTZ)
+            weight *= dim == 2 ? orc_helmholtz_weight_2d(eq->alpha, R)
+                               : orc_helmholtz_weight_3d(eq->alpha, R);
         double np[3] = {0, 0, 0};
         step_on_sphere(dim, p, R, walk, np);
         memcpy(p, np, sizeof(p));

@@ -49,6 +49,10 @@ double orc_screened_green_2d(double alpha, double R, double r);
 double orc_screened_green_3d(double alpha, double R, double r);
 double orc_screened_mass_2d(double alpha, double R);
 double orc_screened_mass_3d(double alpha, double R);
+/* Helmholtz (src/equations/helmholtz.cpp, include/wos/fastmath.hpp) */
+double orc_bessel_j0_series(double x);
+double orc_helmholtz_weight_2d(double k, double R);
+double orc_helmholtz_weight_3d(double k, double R);
 
 /* ---------------- primitives ---------------- */
 double orc_npq_seg(const double s0[2], const double s1[2], const double p[2],

@@ -56,6 +56,8 @@ def eq_cases():
          (0.1, 0.2, -0.2), 64, 1e-4),
         ("screened3d_const_green_a60", "ico2", lambda: problems.sphere_screened_constant(60.0, 1.0, 1),
          (0.1, 0.2, -0.2), 32, 1e-4),
+        ("helmholtz2d", "circle48", lambda: problems.helmholtz_builtin(2), (0.3, 0.1), 64, 1e-3),
+        ("helmholtz3d", "ico2", lambda: problems.helmholtz_builtin(3), (0.2, -0.1, 0.3), 64, 1e-3),
     ]
 
 

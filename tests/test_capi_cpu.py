@@ -53,7 +53,8 @@ def test_run_library_exports_the_registry_entry_point():
     lib.wos_hip_equation_name.restype = C.c_char_p
     n = lib.wos_hip_equation_count()
     assert [lib.wos_hip_equation_name(i) for i in range(n)] == [b"laplace", b"poisson",
-                                                                  b"screened_poisson"]
+                                                                  b"screened_poisson",
+                                                                  b"helmholtz"]
     assert lib.wos_hip_equation_name(n) is None
     # argument errors return 1 before any device use
     lib.wos_hip_run.restype = C.c_int

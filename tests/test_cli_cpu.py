@@ -48,7 +48,7 @@ def test_unknown_equation_lists_registry():
     rc, _, err = run("helmholtz3", "4", "4", "m.obj", "--seed", "1")
     assert rc == 1
     assert "Unknown equation: 'helmholtz3'" in err
-    for name in ("laplace", "poisson", "screened_poisson"):
+    for name in ("laplace", "poisson", "screened_poisson", "helmholtz"):
         assert f"  - {name}" in err
 
 

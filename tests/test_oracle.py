@@ -192,6 +192,35 @@ def test_bessel_scaled_against_scipy(oracle, x):
     assert math.isclose(oracle.L.orc_k0_scaled(x), special.k0e(x), rel_tol=5e-14)
 
 
+@pytest.mark.parametrize("x", [0.0, 1e-4, 0.3, 1.0, 2.0, 2.404, 3.5, 6.0])
+def test_helmholtz_j0_series_and_weights(oracle, x):
+    # the reference's series stops at the first term below 1e-8, so it is
+    # within that of J0 (an alternating series with decreasing terms)
+    L = oracle.L
+    assert abs(L.orc_bessel_j0_series(x) - special.j0(x)) <= 1.1e-8
+    if x >= 1e-8:
+        assert L.orc_helmholtz_weight_2d(1.0, x) == 1.0 / L.orc_bessel_j0_series(x)
+        if x < math.pi:
+            assert math.isclose(L.orc_helmholtz_weight_3d(1.0, x), x / math.sin(x), rel_tol=1e-15)
+    assert L.orc_helmholtz_weight_2d(1.0, 1e-9) == 1.0 and L.orc_helmholtz_weight_3d(1.0, 1e-9) == 1.0
+
+
+def test_helmholtz_walks_match_the_solution(oracle):
+    # u = J0(k r) / J0(k) on the unit disk and sin(k r) / (r sin k) on the
+    # unit ball for g = 1 (k = 1): the estimator's mean within 4 SE
+    V, F, B = meshgen.circle(256)
+    r = oracle.mesh(V, F, B).walks(problems.helmholtz_builtin(2), np.array([0.3, 0.2]), 4000,
+                                   1e-4, 1000, 1, 7)
+    exact = special.j0(math.hypot(0.3, 0.2)) / special.j0(1.0)
+    assert abs(r["mean"] - exact) <= 4 * r["standard_error"] + 2e-4
+    V, F, B = meshgen.icosphere(4)
+    r = oracle.mesh(V, F, B).walks(problems.helmholtz_builtin(3), np.array([0.2, -0.1, 0.3]),
+                                   4000, 1e-4, 1000, 1, 7)
+    rr = math.sqrt(0.04 + 0.01 + 0.09)
+    exact = math.sin(rr) / (rr * math.sin(1.0))
+    assert abs(r["mean"] - exact) <= 4 * r["standard_error"] + 3e-3
+
+
 def test_green_functions(oracle):
     L = oracle.L
     R, r = 0.7, 0.2

@@ -233,11 +233,32 @@ int screened_3d(int, int, const char *mesh, const char *out, int Nx, int Ny, int
     return screened_run(3, mesh, out, Nx, Ny, Nz, N_walks, eps, max_steps, mra, seed, alpha, mode);
 }
 
+// ---- helmholtz (src/equations/helmholtz.cpp): k = 1, no source, g = 1; the
+// reference ignores --alpha and --source-mode here and writes no source
+// metadata (helmholtz.cpp:79-94)
+int helmholtz_run(int dim, const char *mesh, const char *out, int Nx, int Ny, int Nz,
+                  int N_walks, double eps, int max_steps, int mra, uint64_t seed) {
+    wos_eq_desc e = blank(WOS_EQ_HELMHOLTZ);
+    e.alpha = 1.0;
+    e.boundary_kind = WOS_BND_ONE;
+    return run(dim, mesh, out, Nx, Ny, Nz, N_walks, eps, max_steps, mra, seed, *override_or(e),
+               false, 0.0, 0);
+}
+int helmholtz_2d(int, int, const char *mesh, const char *out, int Nx, int Ny, int Nz, int N_walks,
+                 double eps, int max_steps, int mra, uint64_t seed, double, SourceMode) {
+    return helmholtz_run(2, mesh, out, Nx, Ny, Nz, N_walks, eps, max_steps, mra, seed);
+}
+int helmholtz_3d(int, int, const char *mesh, const char *out, int Nx, int Ny, int Nz, int N_walks,
+                 double eps, int max_steps, int mra, uint64_t seed, double, SourceMode) {
+    return helmholtz_run(3, mesh, out, Nx, Ny, Nz, N_walks, eps, max_steps, mra, seed);
+}
+
 }  // namespace
 
 const Equation laplace = {"laplace", laplace_2d, laplace_3d};
 const Equation poisson = {"poisson", poisson_2d, poisson_3d};
 const Equation screened_poisson = {"screened_poisson", screened_2d, screened_3d};
+const Equation helmholtz = {"helmholtz", helmholtz_2d, helmholtz_3d};
 
 const wos_eq_desc *named_problem(const char *name, int *kind) {
     static wos_eq_desc annulus, sphere, star, plate;

@@ -36,8 +36,10 @@ RunOptions &run_options();
 extern const Equation laplace;
 extern const Equation poisson;
 extern const Equation screened_poisson;
+extern const Equation helmholtz;
 
-inline const Equation *const equation_registry[] = {&laplace, &poisson, &screened_poisson};
+inline const Equation *const equation_registry[] = {&laplace, &poisson, &screened_poisson,
+                                                     &helmholtz};
 
 // Descriptors of named benchmark problems (--problem); nullptr if unknown.
 const wos_eq_desc *named_problem(const char *name, int *kind);

@@ -606,6 +606,29 @@ template <typename T> __device__ __forceinline__ T screened_green_3d(T alpha, T 
                                -Fx<T>::expm1((T)-2 * X));
     return Fx<T>::div(ratio, (T)4 * (T)kPi * r);
 }
+// include/wos/fastmath.hpp:15-29: the series of J0, stopped at the first
+// term below 1e-8 (the reference's own truncation)
+template <typename T> __device__ __forceinline__ T bessel_j0_series(T x) {
+    const T q = (T)0.25 * x * x;
+    T term = 1, sum = 1;
+    for (int k = 1; k < 200; ++k) {
+        term *= Fx<T>::div(q, (T)(k * k));
+        if (term < (T)1e-8) return sum;
+        sum += (k % 2 == 0 ? term : -term);
+    }
+    return sum;
+}
+// Helmholtz weights (src/equations/helmholtz.cpp:36-41, :70-75)
+template <typename T> __device__ __forceinline__ T helmholtz_weight_2d(T k, T R) {
+    const T prod = R * k;
+    if (prod < (T)1e-8) return (T)1;
+    return Fx<T>::div((T)1, bessel_j0_series<T>(prod));
+}
+template <typename T> __device__ __forceinline__ T helmholtz_weight_3d(T k, T R) {
+    const T prod = R * k;
+    if (prod < (T)1e-8) return (T)1;
+    return Fx<T>::div(prod, Fx<T>::sin(prod));
+}
 template <typename T> __device__ __forceinline__ T screened_mass_3d(T alpha, T R) {
     const T x = alpha * R;
     if (x < (T)1e-3) {
@@ -614,6 +637,13 @@ template <typename T> __device__ __forceinline__ T screened_mass_3d(T alpha, T R
                         Fx<T>::div((T)31 * x2 * x2, (T)15120));
     }
     return Fx<T>::div((T)1 - screened_weight_3d<T>(alpha, R), alpha * alpha);
+}
+
+// the walk weight of one step: screened Poisson or Helmholtz (alpha = k)
+template <int DIM, typename T> __device__ __forceinline__ T step_weight(int kind, T alpha, T R) {
+    if (kind == WOS_EQ_HELMHOLTZ)
+        return DIM == 2 ? helmholtz_weight_2d<T>(alpha, R) : helmholtz_weight_3d<T>(alpha, R);
+    return DIM == 2 ? screened_weight_2d<T>(alpha, R) : screened_weight_3d<T>(alpha, R);
 }
 
 // --------------------------------------------------------------------------
@@ -750,6 +780,7 @@ __device__ __forceinline__ double eq_boundary(const EqView &e, const MeshView &m
         return (1.0 - s - t) * vv[pr[0]] + s * vv[pr[1]] + t * vv[pr[2]];
     }
     case WOS_BND_SIN3X_COS2Y: return sin(3.0 * x) * cos(2.0 * y);
+    case WOS_BND_ONE: return 1.0;
     default: return 0.0;
     }
 }

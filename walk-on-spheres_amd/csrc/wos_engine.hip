@@ -582,9 +582,9 @@ void k_walk_fp64(MeshView m, EqView e, RoundView rv,
             }
         }
         if (SCR) {
-            if constexpr (ADJ && DIM == 3) D.L += dlog_w3_dalpha(alpha, R);
-            weight *= DIM == 2 ? screened_weight_2d<double>(alpha, R)
-                               : screened_weight_3d<double>(alpha, R);
+            if constexpr (ADJ && DIM == 3)
+                if (e.kind == WOS_EQ_SCREENED) D.L += dlog_w3_dalpha(alpha, R);
+            weight *= step_weight<DIM, double>(e.kind, alpha, R);
         }
         CursorD wk(w, (uint32_t)steps, 0, k0, k1);
         if (DIM == 2) {
@@ -675,9 +675,7 @@ void k_walk_ref(MeshView m, EqView e, RoundView rv, double eps, int max_steps, d
         int steps = 0;
         while (R > eps && steps < max_steps) {
             if (SRC) sample += source_term<DIM, double>(e, px, py, pz, R, weight, source);
-            if (SCR)
-                weight *= DIM == 2 ? screened_weight_2d<double>(alpha, R)
-                                   : screened_weight_3d<double>(alpha, R);
+            if (SCR) weight *= step_weight<DIM, double>(e.kind, alpha, R);
             if (DIM == 2) {
                 const double theta = walk.unit() * 2 * kPi;
                 px = px + R * cos(theta);
@@ -1167,9 +1165,9 @@ void k_walk_fp32(MeshView m, EqView e, PersistArgs a, AdjOut adj) {
                                                         (uint32_t)steps, k0, k1);
             }
             if (SCR) {
-                if constexpr (ADJ && DIM == 3) D.L += (double)dlog_w3_dalpha_f(alpha, R);
-                weight *= DIM == 2 ? screened_weight_2d<float>(alpha, R)
-                                   : screened_weight_3d<float>(alpha, R);
+                if constexpr (ADJ && DIM == 3)
+                    if (e.kind == WOS_EQ_SCREENED) D.L += (double)dlog_w3_dalpha_f(alpha, R);
+                weight *= step_weight<DIM, float>(e.kind, alpha, R);
             }
             Cursor wk(w, (uint32_t)steps, 0, k0, k1);
             if (DIM == 2) {
@@ -1648,14 +1646,18 @@ hipError_t dispatch_walk(wos_ctx *c, int dim, bool adjoint, const EqView &ev, bo
 
 int validate_eq(const wos_ctx *c, const wos_eq_desc *eq) {
     if (!eq) return fail(WOS_ERR_INPUT, "equation descriptor is NULL");
-    if (eq->kind < 0 || eq->kind > 2) return fail(WOS_ERR_INPUT, "unknown equation kind");
+    if (eq->kind < 0 || eq->kind > 3) return fail(WOS_ERR_INPUT, "unknown equation kind");
     if (eq->source_mode < 0 || eq->source_mode > 1) return fail(WOS_ERR_INPUT, "unknown source mode");
     if (eq->source_kind < 0 || eq->source_kind > 5) return fail(WOS_ERR_INPUT, "unknown source kind");
-    if (eq->boundary_kind < 0 || eq->boundary_kind > 5) return fail(WOS_ERR_INPUT, "unknown boundary kind");
+    if (eq->boundary_kind < 0 || eq->boundary_kind > 6) return fail(WOS_ERR_INPUT, "unknown boundary kind");
     if (eq->kind == WOS_EQ_LAPLACE && eq->source_kind != WOS_SRC_NONE)
         return fail(WOS_ERR_INPUT, "laplace takes no source");
     if (eq->kind == WOS_EQ_SCREENED && !(eq->alpha > 0.0 && std::isfinite(eq->alpha)))
         return fail(WOS_ERR_INPUT, "screened_poisson needs a positive finite alpha");
+    if (eq->kind == WOS_EQ_HELMHOLTZ && eq->source_kind != WOS_SRC_NONE)
+        return fail(WOS_ERR_INPUT, "helmholtz takes no source");
+    if (eq->kind == WOS_EQ_HELMHOLTZ && !(eq->alpha > 0.0 && std::isfinite(eq->alpha)))
+        return fail(WOS_ERR_INPUT, "helmholtz needs a positive finite wavenumber (alpha)");
     if (eq->source_kind == WOS_SRC_GAUSSIAN && !(eq->src_radius > 0.0))
         return fail(WOS_ERR_INPUT, "gaussian source needs a positive width");
     if (eq->n_bnd_values < 0 || eq->n_bnd_values > WOS_MAX_BND_VALUES)
@@ -1945,7 +1947,7 @@ int solve_once(wos_ctx *c, const std::vector<Part> &parts, const wos_eq_desc *eq
         HIPX(hipMemsetAsync(c->hits.p, 0, n_in * sizeof(int32_t), c->stream));
         HIPX(c->counter.reserve(16 * kMaxParts * sizeof(unsigned long long)));
         const bool src = eq->source_kind != WOS_SRC_NONE;
-        const bool scr = eq->kind == WOS_EQ_SCREENED;
+        const bool scr = eq->kind == WOS_EQ_SCREENED || eq->kind == WOS_EQ_HELMHOLTZ;
         const bool adaptive = prm->target_se > 0.0;
         const int64_t cap = adaptive ? std::max<int64_t>(prm->max_walks, prm->n_walks) : prm->n_walks;
         // staging: one FP64 value + one int32 per walk (plus the adjoint
@@ -2126,7 +2128,8 @@ int solve_once(wos_ctx *c, const std::vector<Part> &parts, const wos_eq_desc *eq
             // alpha: pathwise only where the walk's draws do not depend on it
             const bool alpha_ok = eq->kind == WOS_EQ_SCREENED && dim == 3 &&
                                   (eq->source_kind == WOS_SRC_NONE || eq->source_mode == WOS_SOURCE_UNIFORM);
-            gp[WOS_GRAD_ALPHA] = eq->kind != WOS_EQ_SCREENED ? 0.0 : (alpha_ok ? g6[0] : NAN);
+            gp[WOS_GRAD_ALPHA] = eq->kind == WOS_EQ_HELMHOLTZ ? NAN
+                                 : eq->kind != WOS_EQ_SCREENED ? 0.0 : (alpha_ok ? g6[0] : NAN);
             gp[WOS_GRAD_SRC_VALUE] = g6[1];
             gp[WOS_GRAD_SRC_RADIUS] = eq->source_kind == WOS_SRC_GAUSSIAN ? g6[2] : 0.0;
             for (int i = 0; i < 3; ++i)
@@ -2256,7 +2259,8 @@ int solve_graph(wos_ctx *c, const std::vector<Part> &parts, const wos_eq_desc *e
             rv.n_dev = n_in_dev;
             rv.total_dev = total_dev;
             HIPX(dispatch_walk(c, dim, false, ev, eq->source_kind != WOS_SRC_NONE,
-                               eq->kind == WOS_EQ_SCREENED, WOS_FP32_PERSISTENT, rv, prm->epsilon,
+                               eq->kind == WOS_EQ_SCREENED || eq->kind == WOS_EQ_HELMHOLTZ,
+                               WOS_FP32_PERSISTENT, rv, prm->epsilon,
                                prm->max_steps, AdjOut{}, &c->graph_blocks));
             hipLaunchKernelGGL(k_welford, dim3((unsigned)nb), dim3(kBlock), 0, c->stream, rv,
                                c->vals.as<double>(), c->steps.as<int32_t>(), c->mean.as<double>(),

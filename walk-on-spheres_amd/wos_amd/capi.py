@@ -17,7 +17,7 @@ LIB_PATH = os.path.join(os.path.dirname(PKG_DIR), "csrc", "libwos_hip.so")
 # enum wos_status
 WOS_OK, WOS_ERR_INPUT, WOS_ERR_MESH, WOS_ERR_DEVICE, WOS_ERR_IO, WOS_ERR_STATE = range(6)
 # enum wos_equation_kind
-WOS_EQ_LAPLACE, WOS_EQ_POISSON, WOS_EQ_SCREENED = range(3)
+WOS_EQ_LAPLACE, WOS_EQ_POISSON, WOS_EQ_SCREENED, WOS_EQ_HELMHOLTZ = range(4)
 # enum wos_source_mode (values of the reference SourceMode)
 WOS_SOURCE_UNIFORM, WOS_SOURCE_GREEN = range(2)
 # enum wos_source_kind
@@ -25,7 +25,7 @@ WOS_SOURCE_UNIFORM, WOS_SOURCE_GREEN = range(2)
  WOS_SRC_ZERO) = range(6)
 # enum wos_boundary_kind
 (WOS_BND_X2Y2_ID0, WOS_BND_Y5, WOS_BND_ZERO, WOS_BND_ID_CONST, WOS_BND_VERTEX,
- WOS_BND_SIN3X_COS2Y) = range(6)
+ WOS_BND_SIN3X_COS2Y, WOS_BND_ONE) = range(7)
 # enum wos_precision
 WOS_FP64_PLAIN, WOS_FP32_PERSISTENT, WOS_FP64_REFERENCE = range(3)
 WOS_MAX_BND_VALUES = 16

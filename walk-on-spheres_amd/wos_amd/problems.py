@@ -6,8 +6,9 @@ poisson.cpp:15-118 and screened_poisson.cpp:40-141 as wos_eq_desc values.
 """
 from __future__ import annotations
 
-from .capi import (WOS_BND_ID_CONST, WOS_BND_SIN3X_COS2Y, WOS_BND_VERTEX, WOS_BND_X2Y2_ID0,
-                   WOS_BND_Y5, WOS_BND_ZERO, WOS_EQ_LAPLACE, WOS_EQ_POISSON, WOS_EQ_SCREENED,
+from .capi import (WOS_BND_ID_CONST, WOS_BND_ONE, WOS_BND_SIN3X_COS2Y, WOS_BND_VERTEX,
+                   WOS_BND_X2Y2_ID0, WOS_BND_Y5, WOS_BND_ZERO, WOS_EQ_HELMHOLTZ,
+                   WOS_EQ_LAPLACE, WOS_EQ_POISSON, WOS_EQ_SCREENED,
                    WOS_SOURCE_GREEN, WOS_SOURCE_UNIFORM, WOS_SRC_CONSTANT, WOS_SRC_DISK_SQ,
                    WOS_SRC_DISK_SQRT, WOS_SRC_GAUSSIAN, WOS_SRC_NONE, WOS_SRC_ZERO, make_eq)
 
@@ -43,8 +44,15 @@ def screened_builtin(dim: int, alpha: float = 5.0, source_mode: int = WOS_SOURCE
                    source_mode=source_mode, alpha=alpha)
 
 
+def helmholtz_builtin(dim: int, k: float = 1.0):
+    """Helmholtz2D/3D: Delta u + k^2 u = 0 with k = 1, no source, g = 1
+    (helmholtz.cpp:15-78); k travels in the descriptor's alpha."""
+    del dim
+    return make_eq(WOS_EQ_HELMHOLTZ, source_kind=WOS_SRC_NONE, boundary_kind=WOS_BND_ONE, alpha=k)
+
+
 BUILTINS = {"laplace": laplace_builtin, "poisson": poisson_builtin,
-            "screened_poisson": screened_builtin}
+            "screened_poisson": screened_builtin, "helmholtz": helmholtz_builtin}
 
 
 # ---- benchmark problems
