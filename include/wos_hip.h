@@ -50,7 +50,11 @@ enum wos_equation_kind {
 /* values equal the reference SourceMode enum (include/wos/source_mode.hpp:5-9) */
 enum wos_source_mode {
     WOS_SOURCE_UNIFORM = 0,      /* uniform ball point x Green function          */
-    WOS_SOURCE_GREEN = 1         /* Green-density radius, mass x f               */
+    WOS_SOURCE_GREEN = 1,        /* Green-density radius, mass x f               */
+    WOS_SOURCE_GREEN_MIS = 2     /* balance-heuristic mixture of the Green density and the
+                                    source's own (Gaussian) density, chosen with probability
+                                    mis_green_probability (source_sampling.hpp:35-78); 2D
+                                    Gaussian sources of Poisson / screened Poisson only */
 };
 
 enum wos_source_kind {
@@ -88,6 +92,14 @@ typedef struct wos_eq_desc {
     int32_t reserved0;
     const double *vertex_values; /* WOS_BND_VERTEX: one value per mesh vertex (host) */
     int64_t n_vertex_values;
+    /* WOS_SOURCE_GREEN_MIS: the source proposal is the normalised Gaussian of the
+     * source (per-axis width src_radius / sqrt 2 about src_center), truncated to the
+     * box when proposal_box = {xmin, xmax, ymin, ymax} is non-empty (all four 0: the
+     * whole plane), as GaussianHeatSource2D (apps/welding/include/welding/
+     * gaussian_heat_source.hpp:45-74); beta in (0, 1) is the Green-draw probability
+     * (source_mis_green_probability, level1_stationary_heat.cpp:188-190) */
+    double mis_green_probability;
+    double proposal_box[4];
 } wos_eq_desc;
 
 /* ---- execution parameters ---------------------------------------------- */

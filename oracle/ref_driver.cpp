@@ -12,7 +12,9 @@
 // poisson.cpp:15-118 and screened_poisson.cpp:40-141 and driven by the
 // descriptor of include/wos_hip.h; everything they call (wos_run,
 // wos_sample, bvh_npq, eberly, classify_point, poisson_green, screened,
-// bessel, PRNG) is the reference's own code.
+// bessel, PRNG) is the reference's own code, and so is the Gaussian source
+// proposal of GreenSourceMIS (apps/welding/include/welding/
+// gaussian_heat_source.hpp).
 #include <atomic>
 #include <cmath>
 #include <cstdint>
@@ -32,6 +34,7 @@
 #include "wos/prng.hpp"
 #include "wos/screened.hpp"
 #include "wos/wos.hpp"
+#include "welding/gaussian_heat_source.hpp"
 
 #include "../include/wos_hip.h"
 
@@ -62,6 +65,15 @@ struct Desc2D {
     const wos_eq_desc *e;
     SourceMode source_mode;
     double alpha;
+    // GreenSourceMIS: the Gaussian's own density on the proposal box, the
+    // welding app's finite-plate source (level1_stationary_heat.cpp:180-190)
+    const welding::GaussianHeatSource2D *proposal;
+
+    Point2D sample_source_proposal(PRNG &rng) const {
+        return proposal->sample_spatial_distribution(rng);
+    }
+    double source_proposal_pdf(Point2D p) const { return proposal->spatial_probability_density(p); }
+    double source_mis_green_probability() const { return e->mis_green_probability; }
 
     bool source_may_intersect(Sphere2D s) const {
         if (e->source_kind != WOS_SRC_DISK_SQ && e->source_kind != WOS_SRC_DISK_SQRT) return true;
@@ -247,10 +259,23 @@ void with_eq(const wos_eq_desc *e, F &&f) {
     const bool src = e->source_kind != WOS_SRC_NONE;
     const bool scr = e->kind == WOS_EQ_SCREENED || e->kind == WOS_EQ_HELMHOLTZ;
     if constexpr (N == 2) {
-        if (src && scr) f(Desc2D<true, true>{e, mode, e->alpha});
-        else if (src) f(Desc2D<true, false>{e, mode, e->alpha});
-        else if (scr) f(Desc2D<false, true>{e, mode, e->alpha});
-        else f(Desc2D<false, false>{e, mode, e->alpha});
+        // the proposal of a Gaussian source: sigma = R / sqrt 2 per axis,
+        // unit power, the descriptor's box (+-inf limits without one)
+        std::unique_ptr<welding::GaussianHeatSource2D> hs;
+        if (src && mode == SourceMode::GreenSourceMIS && e->source_kind == WOS_SRC_GAUSSIAN) {
+            const double *b = e->proposal_box;
+            const bool boxed = !(b[0] == 0.0 && b[1] == 0.0 && b[2] == 0.0 && b[3] == 0.0);
+            const double inf = INFINITY;
+            const double sigma = e->src_radius / std::sqrt(2.0);
+            hs = std::make_unique<welding::GaussianHeatSource2D>(
+                boxed ? b[0] : -inf, boxed ? b[1] : inf, boxed ? b[2] : -inf, boxed ? b[3] : inf,
+                1.0, 1.0, 1.0, e->src_center[0], e->src_center[1], sigma, sigma);
+        }
+        const welding::GaussianHeatSource2D *pr = hs.get();
+        if (src && scr) f(Desc2D<true, true>{e, mode, e->alpha, pr});
+        else if (src) f(Desc2D<true, false>{e, mode, e->alpha, pr});
+        else if (scr) f(Desc2D<false, true>{e, mode, e->alpha, pr});
+        else f(Desc2D<false, false>{e, mode, e->alpha, pr});
     } else {
         if (src && scr) f(Desc3D<true, true>{e, mode, e->alpha});
         else if (src) f(Desc3D<true, false>{e, mode, e->alpha});

@@ -1015,6 +1015,74 @@ static double dist_nd(const double *a, const double *b, int dim) {
     return sqrt(acc);
 }
 
+/* Source proposal of the Green-source mixture: the source's Gaussian
+ * exp(-|x - c|^2 / R^2) as a normal of per-axis width sigma = R / sqrt 2,
+ * truncated to proposal_box (all four 0: no truncation, i.e. +-inf limits);
+ * its normalisation is the product over both axes of
+ * sigma sqrt(pi/2) (erf((hi - c) / (sqrt2 sigma)) - erf((lo - c) / (sqrt2 sigma)))
+ * (apps/welding/include/welding/gaussian_heat_source.hpp:116-126). */
+typedef struct orc_proposal { double sigma, norm, lo[2], hi[2]; } orc_proposal;
+
+static void mis_proposal(const wos_eq_desc *eq, orc_proposal *q) {
+    const double sqrt_two = 1.41421356237309504880168872420969808;
+    const double sqrt_pi_over_two = 1.25331413731550025120788264240552263;
+    const double *b = eq->proposal_box;
+    const int boxed = !(b[0] == 0.0 && b[1] == 0.0 && b[2] == 0.0 && b[3] == 0.0);
+    q->sigma = eq->src_radius / sqrt(2.0);
+    q->norm = 1.0;
+    for (int k = 0; k < 2; ++k) {
+        q->lo[k] = boxed ? b[2 * k] : -INFINITY;
+        q->hi[k] = boxed ? b[2 * k + 1] : INFINITY;
+        const double c = eq->src_center[k];
+        q->norm *= q->sigma * sqrt_pi_over_two *
+                   (erf((q->hi[k] - c) / (sqrt_two * q->sigma)) -
+                    erf((q->lo[k] - c) / (sqrt_two * q->sigma)));
+    }
+}
+
+static int in_box(const orc_proposal *q, const double *x) {
+    return x[0] >= q->lo[0] && x[0] <= q->hi[0] && x[1] >= q->lo[1] && x[1] <= q->hi[1];
+}
+
+/* Green-source mixture estimate on one ball, balance heuristic
+ * (include/wos/source_sampling.hpp:35-78) with the finite-box Gaussian
+ * proposal sampled by Box-Muller rejection and its density
+ * (gaussian_heat_source.hpp:45-74, :109-113).  NaN where the reference
+ * throws (rejection exhausted, mixture density not positive). */
+static double mis_term(const wos_eq_desc *eq, const orc_proposal *q, const double *c, double R,
+                       orc_rng *rng) {
+    const double beta = eq->mis_green_probability;
+    double sp[2];
+    if (rng_unit(rng) < beta) {
+        eq_sample_green(eq, 2, c, R, rng, sp);
+    } else {
+        int ok = 0;
+        for (int attempt = 0; attempt < 1000000; ++attempt) {
+            const double magnitude = sqrt(-2.0 * log(rng_unit_open(rng)));
+            const double angle = 2.0 * ORC_PI * rng_unit(rng);
+            sp[0] = eq->src_center[0] + q->sigma * magnitude * cos(angle);
+            sp[1] = eq->src_center[1] + q->sigma * magnitude * sin(angle);
+            if (in_box(q, sp)) { ok = 1; break; }
+        }
+        if (!ok) return NAN;
+    }
+    const double d = dist_nd(c, sp, 2);
+    if (!(d > 0.0 && d < R)) return 0.0;
+    const double mass = eq_green_mass(eq, 2, R);
+    const double G = eq_green(eq, 2, R, d);
+    const double green_pdf = G / mass;
+    double source_pdf = 0.0;
+    if (in_box(q, sp)) {
+        const double ux = (sp[0] - eq->src_center[0]) / q->sigma;
+        const double uy = (sp[1] - eq->src_center[1]) / q->sigma;
+        source_pdf = exp(-0.5 * (ux * ux + uy * uy)) / q->norm;
+    }
+    const double mixture = beta * green_pdf + (1.0 - beta) * source_pdf;
+    if (!(mixture > 0.0) || !isfinite(mixture)) return NAN;
+    const double f = eq_source(eq, sp, 2);
+    return f == 0.0 ? 0.0 : G * f / mixture;
+}
+
 typedef struct orc_sample { double value; int steps; int hit; } orc_sample;
 
 /* Opt-in tally of the algorithmic work of the walk-step queries (node
@@ -1041,6 +1109,9 @@ static orc_sample wos_sample(const orc_bvh *b, const wos_eq_desc *eq, const doub
     double sample = 0.0, weight = 1.0;
     int steps = 0;
     const int has_source = eq->source_kind != WOS_SRC_NONE;
+    const int mis = has_source && dim == 2 && eq->source_mode == WOS_SOURCE_GREEN_MIS;
+    orc_proposal prop = {0.0, 0.0, {0.0, 0.0}, {0.0, 0.0}};
+    if (mis) mis_proposal(eq, &prop);
     while (R > epsilon && steps < max_steps) {
         rng_begin_step(walk, (uint32_t)steps);
         rng_begin_step(source, (uint32_t)steps);
@@ -1048,7 +1119,9 @@ static orc_sample wos_sample(const orc_bvh *b, const wos_eq_desc *eq, const doub
             const int may = dim == 2 ? eq_source_may_intersect(eq, p, R) : 1;
             if (may) {
                 double sp[3] = {0, 0, 0};
-                if (eq->source_mode == WOS_SOURCE_GREEN) {
+                if (mis) {
+                    sample += weight * mis_term(eq, &prop, p, R, source);
+                } else if (eq->source_mode == WOS_SOURCE_GREEN) {
                     eq_sample_green(eq, dim, p, R, source, sp);
                     const double f = eq_source(eq, sp, dim);
                     if (f != 0.0) sample += weight * eq_green_mass(eq, dim, R) * f;

@@ -31,6 +31,7 @@ def meshes():
         "ico1": meshgen.icosphere(1),
         "ico2": meshgen.icosphere(2),
         "torus": meshgen.torus(1.0, 0.35, 24, 12),
+        "plate": meshgen.rectangle(0.0, 0.12, 0.0, 0.08),
     }
 
 
@@ -58,6 +59,13 @@ def eq_cases():
          (0.1, 0.2, -0.2), 32, 1e-4),
         ("helmholtz2d", "circle48", lambda: problems.helmholtz_builtin(2), (0.3, 0.1), 64, 1e-3),
         ("helmholtz3d", "ico2", lambda: problems.helmholtz_builtin(3), (0.2, -0.1, 0.3), 64, 1e-3),
+        # welding level 1 (Green baseline and Green-source MIS, beta = 0.5)
+        ("welding_green", "plate", lambda: problems.welding_gaussian(1), (0.05, 0.03), 64, 8e-6),
+        ("welding_mis_box", "plate", lambda: problems.welding_gaussian(2), (0.05, 0.03), 64, 8e-6),
+        ("welding_mis_plane", "plate", lambda: problems.welding_gaussian(2, 0.3, truncate=False),
+         (0.02, 0.07), 64, 8e-6),
+        ("screened2d_mis", "plate", lambda: problems.welding_gaussian(2, 0.7, alpha=40.0),
+         (0.065, 0.035), 64, 8e-6),
     ]
 
 

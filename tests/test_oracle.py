@@ -18,7 +18,8 @@ from scipy import integrate, special
 
 import pyoracle
 from wos_amd import meshgen, problems
-from wos_amd.capi import WOS_BND_X2Y2_ID0, WOS_EQ_LAPLACE, make_eq
+from wos_amd.capi import (WOS_BND_X2Y2_ID0, WOS_BND_ZERO, WOS_EQ_LAPLACE, WOS_EQ_POISSON,
+                          WOS_SRC_GAUSSIAN, make_eq)
 
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
@@ -219,6 +220,40 @@ def test_helmholtz_walks_match_the_solution(oracle):
     rr = math.sqrt(0.04 + 0.01 + 0.09)
     exact = math.sin(rr) / (rr * math.sin(1.0))
     assert abs(r["mean"] - exact) <= 4 * r["standard_error"] + 3e-3
+
+
+@pytest.mark.parametrize("mode,beta,box", [
+    (2, 0.5, (0.0, 0.0, 0.0, 0.0)),      # Gaussian proposal on the whole plane
+    (2, 0.2, (-0.5, 0.5, -0.5, 0.5)),    # truncated well inside the source's tails
+    (2, 0.8, (-1.0, 1.0, -1.0, 1.0)),    # box around the disk
+    (1, 0.0, (0.0, 0.0, 0.0, 0.0)),      # Green mode, the baseline
+])
+def test_green_source_mis_unbiased_on_the_disk(oracle, mode, beta, box):
+    # -Lap u = exp(-|x|^2 / s^2) on the unit disk, u = 0 on its boundary:
+    # u(0) = int_0^1 r ln(1/r) exp(-r^2 / s^2) dr.  A wrong proposal
+    # normalisation, a swapped mixture weight or a lost truncation biases
+    # the mixture estimate by far more than its 4 standard errors here.
+    s = 0.3
+    exact = integrate.quad(lambda r: r * math.log(1.0 / r) * math.exp(-r * r / (s * s)), 0.0, 1.0,
+                           limit=200)[0]
+    eq = make_eq(WOS_EQ_POISSON, source_kind=WOS_SRC_GAUSSIAN, boundary_kind=WOS_BND_ZERO,
+                 source_mode=mode, src_value=1.0, src_radius=s, mis_green_probability=beta,
+                 proposal_box=box)
+    V, F, B = meshgen.circle(256)
+    r = oracle.mesh(V, F, B).walks(eq, np.array([0.0, 0.0]), 40000, 1e-4, 1000, 1, 5)
+    assert abs(r["mean"] - exact) <= 4 * r["standard_error"] + 2e-4
+
+
+def test_green_source_mis_lowers_the_variance(oracle):
+    # welding level 1: the same field from both modes, MIS with the smaller
+    # spread at the source and far from it
+    V, F, B = meshgen.rectangle(0.0, 0.12, 0.0, 0.08)
+    om = oracle.mesh(V, F, B)
+    for p0 in ((0.06, 0.04), (0.02, 0.07)):
+        g = om.walks(problems.welding_gaussian(1), np.array(p0), 20000, 8e-6, 1000, 1, 99)
+        m = om.walks(problems.welding_gaussian(2), np.array(p0), 20000, 8e-6, 1000, 1, 99)
+        assert abs(g["mean"] - m["mean"]) <= 4 * math.hypot(g["standard_error"], m["standard_error"])
+        assert m["variance"] < 0.8 * g["variance"]
 
 
 def test_green_functions(oracle):

@@ -95,6 +95,12 @@ struct EqView {
     double bnd[WOS_MAX_BND_VALUES];
     int n_bnd;
     const double *vertex_values;
+    // GREEN_MIS source proposal: Green-draw probability, per-axis width,
+    // normalisation (integral of the unnormalised Gaussian over the box or
+    // the plane) and the truncation box (used when mis_box != 0)
+    double mis_beta, mis_sigma, mis_norm;
+    double box[4];
+    int mis_box;
 };
 
 // start of a walk, produced by the classifier for every point

@@ -84,7 +84,7 @@ template <typename T> __device__ __forceinline__ T clamp01(T v) {
 }
 // include/wos/screened.hpp:105-129
 template <typename T, typename RNG>
-__device__ T screened_radius_2d(T alpha, T R, RNG &rng) {
+__device__ __forceinline__ T screened_radius_2d(T alpha, T R, RNG &rng) {
     const T X = alpha * R;
     for (int guard = 0; guard < 100000; ++guard) {
         T r, acceptance;
@@ -106,7 +106,7 @@ __device__ T screened_radius_2d(T alpha, T R, RNG &rng) {
 }
 // include/wos/screened.hpp:131-156
 template <typename T, typename RNG>
-__device__ T screened_radius_3d(T alpha, T R, RNG &rng) {
+__device__ __forceinline__ T screened_radius_3d(T alpha, T R, RNG &rng) {
     const T X = alpha * R;
     for (int guard = 0; guard < 100000; ++guard) {
         T r, acceptance;
@@ -130,14 +130,74 @@ __device__ T screened_radius_3d(T alpha, T R, RNG &rng) {
     return (T)0.5 * R;
 }
 
+// attempts of the truncated-Gaussian rejection sampler before it gives up
+// (gaussian_heat_source.hpp:58); the term is then NaN, so the point's mean
+// shows the failure
+constexpr int kMisMaxAttempts = 1000000;
+
+// Green-source mixture term of a 2D Gaussian source (include/wos/
+// source_sampling.hpp:35-78, the weight applied as at wos.hpp:86-88).  A
+// first draw picks the Green sample of the Green mode (probability beta) or
+// a draw of the source's own density q: a normal of per-axis width sigma
+// about the centre by Box-Muller, rejected until it falls in the truncation
+// box (gaussian_heat_source.hpp:55-74).  A sample outside the open ball adds
+// nothing; otherwise the term is weight G f / (beta G / mass + (1 - beta) q)
+// with q = exp(-((x-cx)/sigma)^2/2 - ((y-cy)/sigma)^2/2) / norm inside the
+// box, 0 outside (gaussian_heat_source.hpp:45-51, :109-113).
+template <typename T, typename RNG>
+__device__ double source_term_mis(const EqView &e, T px, T py, T R, T weight, RNG &rng) {
+    const bool screened = e.kind == WOS_EQ_SCREENED;
+    const T alpha = (T)e.alpha, beta = (T)e.mis_beta;
+    const T cx = (T)e.cx, cy = (T)e.cy, sigma = (T)e.mis_sigma;
+    T sx, sy;
+    if (rng.unit() < beta) {
+        const T radius = screened ? screened_radius_2d<T>(alpha, R, rng) : poisson_radius_2d<T>(R, rng);
+        const T angle = (T)2 * (T)kPi * rng.unit();
+        sx = px + radius * Fx<T>::cos(angle);
+        sy = py + radius * Fx<T>::sin(angle);
+    } else {
+        for (int attempt = 0;; ++attempt) {
+            if (attempt == kMisMaxAttempts) return NAN;
+            const T magnitude = Fx<T>::sqrt((T)-2 * Fx<T>::log(rng.unit_open()));
+            const T angle = (T)2 * (T)kPi * rng.unit();
+            sx = cx + sigma * magnitude * Fx<T>::cos(angle);
+            sy = cy + sigma * magnitude * Fx<T>::sin(angle);
+            if (!e.mis_box || (sx >= (T)e.box[0] && sx <= (T)e.box[1] && sy >= (T)e.box[2] &&
+                               sy <= (T)e.box[3]))
+                break;
+        }
+    }
+    const T dx = px - sx, dy = py - sy;
+    const T dist = Fx<T>::sqrt(dx * dx + dy * dy);
+    if (!(dist > (T)0 && dist < R)) return 0.0;
+    const T mass = screened ? screened_mass_2d<T>(alpha, R) : R * R / (T)4;
+    const T G = screened ? screened_green_2d<T>(alpha, R, dist) : poisson_green_2d<T>(R, dist);
+    const T green_pdf = Fx<T>::div(G, mass);
+    T q = (T)0;
+    if (!e.mis_box || (sx >= (T)e.box[0] && sx <= (T)e.box[1] && sy >= (T)e.box[2] &&
+                       sy <= (T)e.box[3])) {
+        const T ux = Fx<T>::div(sx - cx, sigma), uy = Fx<T>::div(sy - cy, sigma);
+        q = Fx<T>::div(Fx<T>::exp((T)-0.5 * (ux * ux + uy * uy)), (T)e.mis_norm);
+    }
+    const T mixture = beta * green_pdf + ((T)1 - beta) * q;
+    if (!(mixture > (T)0) || !(mixture < (T)INFINITY)) return NAN;
+    const T f = eq_source<T>(e, sx, sy, (T)0, 2);
+    if (f == (T)0) return 0.0;
+    return (double)(weight * Fx<T>::div(G * f, mixture));
+}
+
 // One source term of wos_sample (include/wos/wos.hpp:69-110): returns the
-// value to add (already multiplied by the weight).  T is the geometry type.
-template <int DIM, typename T, typename RNG>
+// value to add (already multiplied by the weight).  T is the geometry type;
+// the GREEN_MIS branch is compiled only into the kernels launched for it
+// (MIS = true), so the other kernels keep their registers.
+template <int DIM, typename T, typename RNG, bool MIS = false>
 __device__ __forceinline__ double source_term(const EqView &e, T px, T py, T pz, T R, T weight,
                                               RNG &rng) {
     const bool screened = e.kind == WOS_EQ_SCREENED;
     const T alpha = (T)e.alpha;
     if (DIM == 2 && !eq_source_may_intersect<T>(e, px, py, R)) return 0.0;
+    if constexpr (MIS && DIM == 2)
+        if (e.source_mode == WOS_SOURCE_GREEN_MIS) return source_term_mis<T>(e, px, py, R, weight, rng);
     T sx, sy, sz = (T)0;
     if (e.source_mode == WOS_SOURCE_GREEN) {
         if (DIM == 2) {
@@ -548,7 +608,7 @@ void set_walks_per_point(RoundView &r, int cw) {
 // At least 3 wavefronts per SIMD (at most 168 VGPRs) for the 3D forward
 // walks, which fit without spilling; 2D walks with a source term and adjoint
 // replays may use 256 rather than spill.
-template <int DIM, bool SRC, bool SCR, bool ADJ>
+template <int DIM, bool SRC, bool SCR, bool ADJ, bool MIS = false>
 __global__ __launch_bounds__(kBlock)
 __attribute__((amdgpu_waves_per_eu(ADJ || DIM == 2 ? 2 : 3, 8)))
 void k_walk_fp64(MeshView m, EqView e, RoundView rv,
@@ -578,7 +638,7 @@ void k_walk_fp64(MeshView m, EqView e, RoundView rv,
                                                           (uint32_t)steps, k0, k1, D);
             } else {
                 CursorD src(w, (uint32_t)steps, 1, k0, k1);
-                sample += source_term<DIM, double>(e, px, py, pz, R, weight, src);
+                sample += source_term<DIM, double, CursorD, MIS>(e, px, py, pz, R, weight, src);
             }
         }
         if (SCR) {
@@ -647,7 +707,7 @@ __global__ __launch_bounds__(kBlock) void k_ref_seed(const StartInfo *starts, co
     o[4] = src.s0; o[5] = src.s1; o[6] = src.s2; o[7] = src.s3;
 }
 
-template <int DIM, bool SRC, bool SCR>
+template <int DIM, bool SRC, bool SCR, bool MIS = false>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2, 8)))
 void k_walk_ref(MeshView m, EqView e, RoundView rv, double eps, int max_steps, double *vals,
                 int32_t *steps_out, uint64_t *state) {
@@ -674,7 +734,7 @@ void k_walk_ref(MeshView m, EqView e, RoundView rv, double eps, int max_steps, d
         double sample = 0.0, weight = 1.0;
         int steps = 0;
         while (R > eps && steps < max_steps) {
-            if (SRC) sample += source_term<DIM, double>(e, px, py, pz, R, weight, source);
+            if (SRC) sample += source_term<DIM, double, XoRef, MIS>(e, px, py, pz, R, weight, source);
             if (SCR) weight *= step_weight<DIM, double>(e.kind, alpha, R);
             if (DIM == 2) {
                 const double theta = walk.unit() * 2 * kPi;
@@ -714,6 +774,10 @@ __device__ __forceinline__ double source_term_f(const EqView &e, float px, float
     if (SMODE == 2) {
         CursorF src(w, step, 1, k0, k1);
         return source_term<DIM, float>(e, px, py, pz, R, weight, src);
+    }
+    if constexpr (SMODE == 3 && DIM == 2) {
+        CursorF src(w, step, 1, k0, k1);
+        return source_term_mis<float>(e, px, py, R, weight, src);
     }
     if (DIM == 2 && !eq_source_may_intersect<float>(e, px, py, R)) return 0.0;
     const bool screened = e.kind == WOS_EQ_SCREENED;
@@ -1539,15 +1603,26 @@ hipError_t launch_classify(wos_ctx *c, K kernel, const EqView &ev, const GridSpe
 template <int DIM, bool SRC, bool SCR, bool ADJ>
 hipError_t launch_walk(wos_ctx *c, const EqView &ev, int precision, const RoundView &rv,
                        double eps, int max_steps, const AdjOut &adj, int *blocks_out) {
+    // GREEN_MIS: 2D forward solves only (the descriptor check and the
+    // adjoint's refusal keep every other case off its kernels)
+    constexpr bool kMis = SRC && DIM == 2 && !ADJ;
+    const bool mis = kMis && ev.source_mode == WOS_SOURCE_GREEN_MIS;
     if (precision == WOS_FP64_REFERENCE) {
         if constexpr (ADJ) {
             return hipErrorInvalidValue;
         } else {
             const int blocks = (rv.n + kBlock - 1) / kBlock;
             const size_t lds = (size_t)kBlock * (c->L.depth + 2) * sizeof(int);
-            hipLaunchKernelGGL((k_walk_ref<DIM, SRC, SCR>), dim3(blocks), dim3(kBlock), lds,
-                               c->stream, c->view, ev, rv, eps, max_steps, c->vals.as<double>(),
-                               c->steps.as<int32_t>(), c->ref_state.as<uint64_t>());
+            if (mis)
+                hipLaunchKernelGGL((k_walk_ref<DIM, SRC, SCR, kMis>), dim3(blocks), dim3(kBlock),
+                                   lds, c->stream, c->view, ev, rv, eps, max_steps,
+                                   c->vals.as<double>(), c->steps.as<int32_t>(),
+                                   c->ref_state.as<uint64_t>());
+            else
+                hipLaunchKernelGGL((k_walk_ref<DIM, SRC, SCR>), dim3(blocks), dim3(kBlock), lds,
+                                   c->stream, c->view, ev, rv, eps, max_steps,
+                                   c->vals.as<double>(), c->steps.as<int32_t>(),
+                                   c->ref_state.as<uint64_t>());
             *blocks_out = blocks;
             return hipGetLastError();
         }
@@ -1555,9 +1630,14 @@ hipError_t launch_walk(wos_ctx *c, const EqView &ev, int precision, const RoundV
     if (precision == WOS_FP64_PLAIN) {
         const int blocks = (int)((rv.total + kBlock - 1) / kBlock);
         const size_t lds = (size_t)kBlock * (c->L.depth + 2) * sizeof(int);
-        hipLaunchKernelGGL((k_walk_fp64<DIM, SRC, SCR, ADJ>), dim3(blocks), dim3(kBlock), lds,
-                           c->stream, c->view, ev, rv, eps, max_steps, c->vals.as<double>(),
-                           c->steps.as<int32_t>(), adj);
+        if (mis)
+            hipLaunchKernelGGL((k_walk_fp64<DIM, SRC, SCR, ADJ, kMis>), dim3(blocks), dim3(kBlock),
+                               lds, c->stream, c->view, ev, rv, eps, max_steps,
+                               c->vals.as<double>(), c->steps.as<int32_t>(), adj);
+        else
+            hipLaunchKernelGGL((k_walk_fp64<DIM, SRC, SCR, ADJ>), dim3(blocks), dim3(kBlock), lds,
+                               c->stream, c->view, ev, rv, eps, max_steps, c->vals.as<double>(),
+                               c->steps.as<int32_t>(), adj);
         *blocks_out = blocks;
         return hipGetLastError();
     }
@@ -1565,8 +1645,9 @@ hipError_t launch_walk(wos_ctx *c, const EqView &ev, int precision, const RoundV
     if (ge != hipSuccess) return ge;
     constexpr int SMODE = SRC ? 1 : 0;
     const bool green = ev.source_mode == WOS_SOURCE_GREEN && SRC;
-    const void *kfn = green ? (const void *)k_walk_fp32<DIM, 2, SCR, ADJ>
-                            : (const void *)k_walk_fp32<DIM, SMODE, SCR, ADJ>;
+    const void *kfn = mis     ? (const void *)k_walk_fp32<DIM, kMis ? 3 : SMODE, SCR, ADJ>
+                      : green ? (const void *)k_walk_fp32<DIM, 2, SCR, ADJ>
+                              : (const void *)k_walk_fp32<DIM, SMODE, SCR, ADJ>;
     const int threads = c->tune.block_threads;
     const size_t stack_bytes = (size_t)threads * (c->L.depth + 1) * sizeof(StackEntry);
     const int node_bytes = DIM == 3 ? (int)sizeof(NodeF3) : (int)sizeof(NodeF2);
@@ -1612,7 +1693,10 @@ hipError_t launch_walk(wos_ctx *c, const EqView &ev, int precision, const RoundV
                        (int64_t)rv.total, rv.total_dev);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
-    if (green)
+    if (mis)
+        hipLaunchKernelGGL((k_walk_fp32<DIM, kMis ? 3 : SMODE, SCR, ADJ>), dim3(blocks),
+                           dim3(threads), lds, c->stream, c->view, ev, a, adj);
+    else if (green)
         hipLaunchKernelGGL((k_walk_fp32<DIM, 2, SCR, ADJ>), dim3(blocks), dim3(threads), lds,
                            c->stream, c->view, ev, a, adj);
     else
@@ -1644,10 +1728,36 @@ hipError_t dispatch_walk(wos_ctx *c, int dim, bool adjoint, const EqView &ev, bo
                    : dispatch_walk_t<3, false>(c, ev, src, scr, precision, rv, eps, max_steps, adj, blocks);
 }
 
+// Width and normalisation of the GREEN_MIS proposal: the source
+// exp(-|x - c|^2 / R^2) is a normal of per-axis width sigma = R / sqrt 2, and
+// its integral over the box (limits +-inf without one) is the product over
+// both axes of sigma sqrt(pi / 2) (erf((hi - c) / (sqrt 2 sigma)) -
+// erf((lo - c) / (sqrt 2 sigma))), gaussian_heat_source.hpp:116-126.
+double proposal_sigma(const wos_eq_desc *eq) { return eq->src_radius / std::sqrt(2.0); }
+bool proposal_boxed(const wos_eq_desc *eq) {
+    const double *b = eq->proposal_box;
+    return !(b[0] == 0.0 && b[1] == 0.0 && b[2] == 0.0 && b[3] == 0.0);
+}
+double proposal_norm(const wos_eq_desc *eq) {
+    constexpr double sqrt_two = 1.41421356237309504880168872420969808;
+    constexpr double sqrt_pi_over_two = 1.25331413731550025120788264240552263;
+    const double sigma = proposal_sigma(eq);
+    const bool boxed = proposal_boxed(eq);
+    double norm = 1.0;
+    for (int k = 0; k < 2; ++k) {
+        const double lo = boxed ? eq->proposal_box[2 * k] : -INFINITY;
+        const double hi = boxed ? eq->proposal_box[2 * k + 1] : INFINITY;
+        const double c = eq->src_center[k];
+        norm *= sigma * sqrt_pi_over_two *
+                (std::erf((hi - c) / (sqrt_two * sigma)) - std::erf((lo - c) / (sqrt_two * sigma)));
+    }
+    return norm;
+}
+
 int validate_eq(const wos_ctx *c, const wos_eq_desc *eq) {
     if (!eq) return fail(WOS_ERR_INPUT, "equation descriptor is NULL");
     if (eq->kind < 0 || eq->kind > 3) return fail(WOS_ERR_INPUT, "unknown equation kind");
-    if (eq->source_mode < 0 || eq->source_mode > 1) return fail(WOS_ERR_INPUT, "unknown source mode");
+    if (eq->source_mode < 0 || eq->source_mode > 2) return fail(WOS_ERR_INPUT, "unknown source mode");
     if (eq->source_kind < 0 || eq->source_kind > 5) return fail(WOS_ERR_INPUT, "unknown source kind");
     if (eq->boundary_kind < 0 || eq->boundary_kind > 6) return fail(WOS_ERR_INPUT, "unknown boundary kind");
     if (eq->kind == WOS_EQ_LAPLACE && eq->source_kind != WOS_SRC_NONE)
@@ -1660,6 +1770,25 @@ int validate_eq(const wos_ctx *c, const wos_eq_desc *eq) {
         return fail(WOS_ERR_INPUT, "helmholtz needs a positive finite wavenumber (alpha)");
     if (eq->source_kind == WOS_SRC_GAUSSIAN && !(eq->src_radius > 0.0))
         return fail(WOS_ERR_INPUT, "gaussian source needs a positive width");
+    if (eq->source_mode == WOS_SOURCE_GREEN_MIS && eq->source_kind != WOS_SRC_NONE) {
+        // source_sampling.hpp:43-47 and wos.hpp:89-92; the proposal's own
+        // checks follow gaussian_heat_source.hpp:29-35
+        if (c->mesh.dim != 2 || eq->source_kind != WOS_SRC_GAUSSIAN ||
+            (eq->kind != WOS_EQ_POISSON && eq->kind != WOS_EQ_SCREENED))
+            return fail(WOS_ERR_INPUT, "GreenSourceMIS requires a source proposal sampler and PDF "
+                                       "(a 2D Gaussian source of poisson or screened_poisson)");
+        const double beta = eq->mis_green_probability;
+        if (!(beta > 0.0 && beta < 1.0) || !std::isfinite(beta))
+            return fail(WOS_ERR_INPUT, "Green-source MIS probability must be finite and in (0, 1)");
+        const double *b = eq->proposal_box;
+        const bool none = b[0] == 0.0 && b[1] == 0.0 && b[2] == 0.0 && b[3] == 0.0;
+        if (!none && !(b[1] > b[0] && b[3] > b[2] && eq->src_center[0] >= b[0] &&
+                       eq->src_center[0] <= b[1] && eq->src_center[1] >= b[2] &&
+                       eq->src_center[1] <= b[3]))
+            return fail(WOS_ERR_INPUT, "invalid Gaussian source proposal box");
+        if (!(proposal_norm(eq) > 0.0) || !std::isfinite(proposal_norm(eq)))
+            return fail(WOS_ERR_INPUT, "Gaussian source proposal has no mass in its box");
+    }
     if (eq->n_bnd_values < 0 || eq->n_bnd_values > WOS_MAX_BND_VALUES)
         return fail(WOS_ERR_INPUT, "n_bnd_values out of range");
     if (eq->boundary_kind == WOS_BND_VERTEX &&
@@ -1706,6 +1835,8 @@ int prepare_solve(wos_ctx *c, const std::vector<Part> &parts, const wos_eq_desc 
         return fail(WOS_ERR_INPUT, "the adjoint replays counter streams (precision FP64 or FP32)");
     int rc = validate_eq(c, eq);
     if (rc) return rc;
+    if (adjr && eq->source_mode == WOS_SOURCE_GREEN_MIS && eq->source_kind != WOS_SRC_NONE)
+        return fail(WOS_ERR_INPUT, "the adjoint supports the uniform and Green source modes");
     if (prm->n_walks < 1) return fail(WOS_ERR_INPUT, "n_walks must be >= 1");
     if (prm->max_steps < 1) return fail(WOS_ERR_INPUT, "max_steps must be >= 1");
     if (c->mesh.dim == 3 && prm->max_ray_attempts < 3)
@@ -1730,6 +1861,13 @@ int prepare_solve(wos_ctx *c, const std::vector<Part> &parts, const wos_eq_desc 
     ev.src_radius = eq->src_radius;
     for (int i = 0; i < WOS_MAX_BND_VALUES; ++i) ev.bnd[i] = eq->bnd_values[i];
     ev.n_bnd = eq->n_bnd_values;
+    if (eq->source_mode == WOS_SOURCE_GREEN_MIS && eq->source_kind != WOS_SRC_NONE) {
+        ev.mis_beta = eq->mis_green_probability;
+        ev.mis_sigma = proposal_sigma(eq);
+        ev.mis_norm = proposal_norm(eq);
+        ev.mis_box = proposal_boxed(eq) ? 1 : 0;
+        for (int k = 0; k < 4; ++k) ev.box[k] = eq->proposal_box[k];
+    }
     ev.vertex_values = nullptr;
     if (eq->boundary_kind == WOS_BND_VERTEX) {
         std::vector<double> vv(eq->vertex_values, eq->vertex_values + eq->n_vertex_values);

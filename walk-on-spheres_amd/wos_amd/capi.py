@@ -19,7 +19,7 @@ WOS_OK, WOS_ERR_INPUT, WOS_ERR_MESH, WOS_ERR_DEVICE, WOS_ERR_IO, WOS_ERR_STATE =
 # enum wos_equation_kind
 WOS_EQ_LAPLACE, WOS_EQ_POISSON, WOS_EQ_SCREENED, WOS_EQ_HELMHOLTZ = range(4)
 # enum wos_source_mode (values of the reference SourceMode)
-WOS_SOURCE_UNIFORM, WOS_SOURCE_GREEN = range(2)
+WOS_SOURCE_UNIFORM, WOS_SOURCE_GREEN, WOS_SOURCE_GREEN_MIS = range(3)
 # enum wos_source_kind
 (WOS_SRC_NONE, WOS_SRC_DISK_SQ, WOS_SRC_DISK_SQRT, WOS_SRC_CONSTANT, WOS_SRC_GAUSSIAN,
  WOS_SRC_ZERO) = range(6)
@@ -52,6 +52,8 @@ class EqDesc(C.Structure):
         ("reserved0", C.c_int32),
         ("vertex_values", C.POINTER(C.c_double)),
         ("n_vertex_values", C.c_int64),
+        ("mis_green_probability", C.c_double),
+        ("proposal_box", C.c_double * 4),
     ]
 
 
@@ -137,7 +139,7 @@ def ptr(a, ctype):
 def make_eq(kind, source_kind=WOS_SRC_NONE, boundary_kind=WOS_BND_ZERO,
             source_mode=WOS_SOURCE_UNIFORM, alpha=0.0, src_value=0.0,
             src_center=(0.0, 0.0, 0.0), src_radius=0.0, bnd_values=(),
-            vertex_values=None):
+            vertex_values=None, mis_green_probability=0.0, proposal_box=(0.0, 0.0, 0.0, 0.0)):
     """Build a wos_eq_desc.  `vertex_values` must stay alive while in use."""
     e = EqDesc()
     e.kind = kind
@@ -154,6 +156,9 @@ def make_eq(kind, source_kind=WOS_SRC_NONE, boundary_kind=WOS_BND_ZERO,
     for i, v in enumerate(bnd_values):
         e.bnd_values[i] = v
     e.n_bnd_values = len(bnd_values)
+    e.mis_green_probability = mis_green_probability
+    for k in range(4):
+        e.proposal_box[k] = proposal_box[k]
     if vertex_values is not None:
         vv = np.ascontiguousarray(vertex_values, dtype=np.float64)
         e.vertex_values = ptr(vv, C.c_double)

@@ -54,6 +54,16 @@ def circle(n: int = 256, r: float = 1.0):
     return loops_to_mesh([polygon_loop(n, lambda t: np.full_like(t, r))])
 
 
+def rectangle(xmin: float, xmax: float, ymin: float, ymax: float):
+    """The four counter-clockwise edges of a rectangle, boundary id 0
+    (make_rectangular_plate, apps/welding/include/welding/rectangular_plate.hpp:11-33)."""
+    if not (xmax > xmin and ymax > ymin):
+        raise ValueError("rectangle needs increasing coordinate bounds")
+    verts = np.array([[xmin, ymin], [xmax, ymin], [xmax, ymax], [xmin, ymax]], np.float64)
+    prims = np.array([[0, 1], [1, 2], [2, 3], [3, 0]], np.int32)
+    return verts, prims, np.zeros(4, np.int32)
+
+
 def star(n: int, a: float, k: int):
     """r(theta) = 1 + a cos(k theta), counter-clockwise."""
     return loops_to_mesh([polygon_loop(n, lambda t: 1.0 + a * np.cos(k * t))])

@@ -9,7 +9,7 @@ from __future__ import annotations
 from .capi import (WOS_BND_ID_CONST, WOS_BND_ONE, WOS_BND_SIN3X_COS2Y, WOS_BND_VERTEX,
                    WOS_BND_X2Y2_ID0, WOS_BND_Y5, WOS_BND_ZERO, WOS_EQ_HELMHOLTZ,
                    WOS_EQ_LAPLACE, WOS_EQ_POISSON, WOS_EQ_SCREENED,
-                   WOS_SOURCE_GREEN, WOS_SOURCE_UNIFORM, WOS_SRC_CONSTANT, WOS_SRC_DISK_SQ,
+                   WOS_SOURCE_GREEN, WOS_SOURCE_GREEN_MIS, WOS_SOURCE_UNIFORM, WOS_SRC_CONSTANT, WOS_SRC_DISK_SQ,
                    WOS_SRC_DISK_SQRT, WOS_SRC_GAUSSIAN, WOS_SRC_NONE, WOS_SRC_ZERO, make_eq)
 
 
@@ -88,3 +88,40 @@ def plate_gaussian(alpha: float = 0.05, q: float = 1.0, sigma: float = 3.0,
     return make_eq(WOS_EQ_SCREENED, source_kind=WOS_SRC_GAUSSIAN, boundary_kind=WOS_BND_ZERO,
                    source_mode=source_mode, alpha=alpha, src_value=q, src_center=tuple(x0),
                    src_radius=sigma)
+
+
+# ---- welding level 1: stationary Gaussian heat in a thin plate
+WELDING_L1 = dict(length_x=0.12, length_y=0.08, thickness=0.006, conductivity=35.0,
+                  electrical_power=800.0, efficiency=0.75, sigma=0.006, nx=49, ny=33,
+                  walks=6000, max_steps=1000, epsilon=8e-6)
+
+
+def welding_gaussian_integral(lo: float, hi: float, c: float, sigma: float) -> float:
+    """sigma sqrt(pi/2) (erf((hi - c)/(sqrt2 sigma)) - erf((lo - c)/(sqrt2 sigma)))
+    (apps/welding/include/welding/gaussian_heat_source.hpp:116-126)."""
+    from math import erf, sqrt
+    return sigma * 1.25331413731550025120788264240552263 * (
+        erf((hi - c) / (sqrt(2.0) * sigma)) - erf((lo - c) / (sqrt(2.0) * sigma)))
+
+
+def welding_gaussian(source_mode: int = WOS_SOURCE_GREEN, beta: float = 0.5, cfg=None,
+                     truncate: bool = True, alpha: float = 0.0):
+    """The level-1 welding heat problem (apps/welding/src/level1_stationary_heat.cpp:
+    42-62, :624-666): 2D Poisson on the plate [0, Lx] x [0, Ly], g = 0, and
+    f = q'''/k with the finite-plate Gaussian q''' = eta P exp(-|x - c|^2 / (2 sigma^2))
+    / (t Ix Iy) centred on the plate (gaussian_heat_source.hpp:38-41).  In
+    GREEN_MIS mode the proposal is that Gaussian truncated to the plate (or,
+    truncate=False, on the whole plane).  alpha > 0 makes it the screened
+    equation with the same source."""
+    c = dict(WELDING_L1, **(cfg or {}))
+    cx, cy, s = 0.5 * c["length_x"], 0.5 * c["length_y"], c["sigma"]
+    ix = welding_gaussian_integral(0.0, c["length_x"], cx, s)
+    iy = welding_gaussian_integral(0.0, c["length_y"], cy, s)
+    peak = c["electrical_power"] * c["efficiency"] / (c["thickness"] * ix * iy)
+    box = (0.0, c["length_x"], 0.0, c["length_y"]) if truncate else (0.0, 0.0, 0.0, 0.0)
+    return make_eq(WOS_EQ_SCREENED if alpha > 0.0 else WOS_EQ_POISSON,
+                   source_kind=WOS_SRC_GAUSSIAN, boundary_kind=WOS_BND_ZERO,
+                   source_mode=source_mode, alpha=alpha, src_value=peak / c["conductivity"],
+                   src_center=(cx, cy, 0.0), src_radius=s * 2.0 ** 0.5,
+                   mis_green_probability=beta if source_mode == WOS_SOURCE_GREEN_MIS else 0.0,
+                   proposal_box=box)
