@@ -63,8 +63,49 @@ enum wos_source_kind {
     WOS_SRC_DISK_SQRT = 2,       /* f = value if sqrt(|x-c|^2) <= R (screened_poisson.cpp:48-61); 2D prefilter */
     WOS_SRC_CONSTANT = 3,        /* f = value everywhere                          */
     WOS_SRC_GAUSSIAN = 4,        /* f = value * exp(-|x-c|^2 / R^2)               */
-    WOS_SRC_ZERO = 5             /* f = 0 but sampled (reference Poisson3D, poisson.cpp:79-82) */
+    WOS_SRC_ZERO = 5,            /* f = 0 but sampled (reference Poisson3D, poisson.cpp:79-82) */
+    WOS_SRC_FIELD = 6            /* 2D: f = history + heat from *field_source (below), the
+                                    sources of the welding levels 0-3 */
 };
+
+/* ---- field source (WOS_SRC_FIELD): the implicit-Euler heat sources of the
+ * welding app (level0_cooling.cpp:194-196, level1_stationary_heat.cpp:146-148,
+ * level2_transient_heat.cpp:255-264, level3_moving_heat.cpp:189-194):
+ *   f(x) = history(x) + heat(x)
+ *   history(x) = history_scale * v(x), v clamped below at 0 when
+ *                clamp_nonnegative, v = FieldSampler2D interpolation of the
+ *                previous field (field_sampler.hpp:51-117); no values: 0
+ *   heat(x)    = heat_scale * (0 + sum_c a_c exp(-(dx^2 + dy^2) / 2) / b_c) / heat_divisor,
+ *                dx = (x - cx_c) / sx_c, dy = (y - cy_c) / sy_c
+ *                (gaussian_heat_source.hpp:38-41, moving_gaussian_heat_source.hpp:75-88);
+ *                0 when heat_scale == 0 or there are no terms */
+enum wos_interpolation {
+    WOS_INTERP_BILINEAR = 0,     /* FieldInterpolation2D::Bilinear */
+    WOS_INTERP_CUBIC = 1         /* tensor cubic Lagrange on 4 x 4 nodes (needs nx, ny >= 4,
+                                    else bilinear, as the reference) */
+};
+
+#define WOS_MAX_HEAT_TERMS 4
+
+typedef struct wos_heat_term {
+    double cx, cy;               /* centre                                         */
+    double sx, sy;               /* per-axis width sigma                           */
+    double a, b;                 /* term = a * shape / b                           */
+} wos_heat_term;
+
+typedef struct wos_field_source {
+    const double *values;        /* previous field, nx * ny, index i * ny + j (host); may be NULL */
+    int32_t nx, ny;              /* grid nodes, >= 2 each                          */
+    double bounds[4];            /* xmin, xmax, ymin, ymax of the grid             */
+    int32_t interpolation;       /* enum wos_interpolation                         */
+    int32_t clamp_nonnegative;   /* 1: max(0, v) (levels 2, 3); 0: v (level 0)     */
+    double history_scale;        /* alpha^2 of the implicit-Euler step             */
+    double heat_scale;           /* power fraction of the step (level 2), else 1   */
+    double heat_divisor;         /* conductivity k                                 */
+    int32_t n_heat_terms;        /* 0 .. WOS_MAX_HEAT_TERMS                        */
+    int32_t reserved0;
+    wos_heat_term heat[WOS_MAX_HEAT_TERMS];
+} wos_field_source;
 
 enum wos_boundary_kind {
     WOS_BND_X2Y2_ID0 = 0,        /* 2D: x^2-y^2 on boundary id 0, else 0 (laplace.cpp:16-18) */
@@ -100,6 +141,7 @@ typedef struct wos_eq_desc {
      * (source_mis_green_probability, level1_stationary_heat.cpp:188-190) */
     double mis_green_probability;
     double proposal_box[4];
+    const wos_field_source *field_source;   /* WOS_SRC_FIELD: the source (host memory) */
 } wos_eq_desc;
 
 /* ---- execution parameters ---------------------------------------------- */

@@ -86,6 +86,9 @@ class Oracle:
         L.orc_solve_points.argtypes = [C.c_void_p, C.c_void_p, _D, C.c_int64, _I64, C.c_int,
                                        C.c_double, C.c_int, C.c_int, C.c_uint64, C.c_int,
                                        C.c_int, _D, _D, _D, _D, _U8, _I32, _I32, _I64]
+        for name in ("orc_field_sample", "orc_field_source_value"):
+            getattr(L, name).restype = C.c_double
+            getattr(L, name).argtypes = [C.c_void_p, C.c_double, C.c_double]
         L.orc_count_reset.argtypes = [C.c_int]
         L.orc_count_get.argtypes = [_I64]
         L.orc_solve_grid.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int,

@@ -34,6 +34,7 @@
 #include "wos/prng.hpp"
 #include "wos/screened.hpp"
 #include "wos/wos.hpp"
+#include "welding/field_sampler.hpp"
 #include "welding/gaussian_heat_source.hpp"
 
 #include "../include/wos_hip.h"
@@ -68,6 +69,8 @@ struct Desc2D {
     // GreenSourceMIS: the Gaussian's own density on the proposal box, the
     // welding app's finite-plate source (level1_stationary_heat.cpp:180-190)
     const welding::GaussianHeatSource2D *proposal;
+    // WOS_SRC_FIELD: the reference's FieldSampler2D over the previous field
+    const welding::FieldSampler2D *field;
 
     Point2D sample_source_proposal(PRNG &rng) const {
         return proposal->sample_spatial_distribution(rng);
@@ -82,7 +85,32 @@ struct Desc2D {
         const double reach = s.radius + e->src_radius;
         return dx * dx + dy * dy <= reach * reach;
     }
-    double source(Point2D p) const { const double x[2] = {p.x, p.y}; return src_value(*e, x, 2); }
+    double source(Point2D p) const {
+        if (e->source_kind == WOS_SRC_FIELD) return field_value(p);
+        const double x[2] = {p.x, p.y};
+        return src_value(*e, x, 2);
+    }
+    // the welding sources (level0_cooling.cpp:194-196,
+    // level2_transient_heat.cpp:255-264, level3_moving_heat.cpp:189-194)
+    double field_value(Point2D p) const {
+        const wos_field_source &f = *e->field_source;
+        double history = 0.0;
+        if (field) {
+            const double v = field->sample(p);
+            history = f.history_scale * (f.clamp_nonnegative ? std::max(0.0, v) : v);
+        }
+        double heat = 0.0;
+        if (f.heat_scale != 0.0 && f.n_heat_terms > 0) {
+            double sum = 0.0;
+            for (int c = 0; c < f.n_heat_terms; ++c) {
+                const wos_heat_term &h = f.heat[c];
+                const double dx = (p.x - h.cx) / h.sx, dy = (p.y - h.cy) / h.sy;
+                sum += h.a * std::exp(-0.5 * (dx * dx + dy * dy)) / h.b;
+            }
+            heat = f.heat_scale * sum / f.heat_divisor;
+        }
+        return history + heat;
+    }
     double boundary(Point2D p, int bid) const {
         switch (e->boundary_kind) {
         case WOS_BND_X2Y2_ID0: return bid == 0 ? p.x * p.x - p.y * p.y : 0.0;
@@ -272,10 +300,24 @@ void with_eq(const wos_eq_desc *e, F &&f) {
                 1.0, 1.0, 1.0, e->src_center[0], e->src_center[1], sigma, sigma);
         }
         const welding::GaussianHeatSource2D *pr = hs.get();
-        if (src && scr) f(Desc2D<true, true>{e, mode, e->alpha, pr});
-        else if (src) f(Desc2D<true, false>{e, mode, e->alpha, pr});
-        else if (scr) f(Desc2D<false, true>{e, mode, e->alpha, pr});
-        else f(Desc2D<false, false>{e, mode, e->alpha, pr});
+        // the previous field of a WOS_SRC_FIELD source
+        std::unique_ptr<Grid> fg;
+        std::vector<double> fv;
+        std::unique_ptr<welding::FieldSampler2D> fs;
+        const wos_field_source *fd = e->source_kind == WOS_SRC_FIELD ? e->field_source : nullptr;
+        if (fd && fd->values) {
+            fg = std::make_unique<Grid>(Grid{fd->nx, fd->ny, 1, fd->bounds[0], fd->bounds[1],
+                                             fd->bounds[2], fd->bounds[3], 0.0, 0.0});
+            fv.assign(fd->values, fd->values + (size_t)fd->nx * fd->ny);
+            fs = std::make_unique<welding::FieldSampler2D>(
+                *fg, fv, fd->interpolation == WOS_INTERP_CUBIC ? welding::FieldInterpolation2D::Cubic
+                                                               : welding::FieldInterpolation2D::Bilinear);
+        }
+        const welding::FieldSampler2D *fp = fs.get();
+        if (src && scr) f(Desc2D<true, true>{e, mode, e->alpha, pr, fp});
+        else if (src) f(Desc2D<true, false>{e, mode, e->alpha, pr, fp});
+        else if (scr) f(Desc2D<false, true>{e, mode, e->alpha, pr, fp});
+        else f(Desc2D<false, false>{e, mode, e->alpha, pr, fp});
     } else {
         if (src && scr) f(Desc3D<true, true>{e, mode, e->alpha});
         else if (src) f(Desc3D<true, false>{e, mode, e->alpha});

@@ -865,10 +865,79 @@ int orc_classify(const orc_bvh *b, const double *p, double distance, double tol,
 /* equation functors                                                      */
 /* ===================================================================== */
 
+/* FieldSampler2D::sample (apps/welding/include/welding/field_sampler.hpp:51-117):
+ * grid coordinates clamped to [0, n-1], then bilinear on the cell, or the
+ * tensor cubic Lagrange interpolant on the 4 x 4 nodes starting one node
+ * below the cell (shifted inside at the edges) when both n >= 4. */
+static double clampd(double v, double lo, double hi) { return v < lo ? lo : (hi < v ? hi : v); }
+
+static void cubic_weights(double c, double w[4]) {
+    w[0] = -(c - 1.0) * (c - 2.0) * (c - 3.0) / 6.0;
+    w[1] = c * (c - 2.0) * (c - 3.0) / 2.0;
+    w[2] = -c * (c - 1.0) * (c - 3.0) / 2.0;
+    w[3] = c * (c - 1.0) * (c - 2.0) / 6.0;
+}
+
+double orc_field_sample(const wos_field_source *f, double x, double y) {
+    const int nx = f->nx, ny = f->ny;
+    const double *v = f->values;
+    const double gx = clampd((x - f->bounds[0]) / (f->bounds[1] - f->bounds[0]) * (double)(nx - 1),
+                             0.0, (double)(nx - 1));
+    const double gy = clampd((y - f->bounds[2]) / (f->bounds[3] - f->bounds[2]) * (double)(ny - 1),
+                             0.0, (double)(ny - 1));
+    if (f->interpolation == WOS_INTERP_CUBIC && nx >= 4 && ny >= 4) {
+        int ix = (int)floor(gx) - 1, iy = (int)floor(gy) - 1;
+        ix = ix < 0 ? 0 : (ix > nx - 4 ? nx - 4 : ix);
+        iy = iy < 0 ? 0 : (iy > ny - 4 ? ny - 4 : iy);
+        double wx[4], wy[4];
+        cubic_weights(gx - (double)ix, wx);
+        cubic_weights(gy - (double)iy, wy);
+        double value = 0.0;
+        for (int di = 0; di < 4; ++di) {
+            double row = 0.0;
+            for (int dj = 0; dj < 4; ++dj) row += wy[dj] * v[(size_t)(ix + di) * ny + iy + dj];
+            value += wx[di] * row;
+        }
+        return value;
+    }
+    const int i0 = (int)floor(gx), j0 = (int)floor(gy);
+    const int i1 = i0 + 1 < nx - 1 ? i0 + 1 : nx - 1, j1 = j0 + 1 < ny - 1 ? j0 + 1 : ny - 1;
+    const double tx = gx - (double)i0, ty = gy - (double)j0;
+    const double v00 = v[(size_t)i0 * ny + j0], v10 = v[(size_t)i1 * ny + j0];
+    const double v01 = v[(size_t)i0 * ny + j1], v11 = v[(size_t)i1 * ny + j1];
+    const double lower = (1.0 - tx) * v00 + tx * v10;
+    const double upper = (1.0 - tx) * v01 + tx * v11;
+    return (1.0 - ty) * lower + ty * upper;
+}
+
+/* the welding sources, f = history + heat (level0_cooling.cpp:194-196,
+ * level2_transient_heat.cpp:255-264, level3_moving_heat.cpp:189-194; heat
+ * terms as gaussian_heat_source.hpp:38-41, :109-113 and
+ * moving_gaussian_heat_source.hpp:75-88) */
+double orc_field_source_value(const wos_field_source *f, double x, double y) {
+    double history = 0.0;
+    if (f->values) {
+        const double v = orc_field_sample(f, x, y);
+        history = f->history_scale * (f->clamp_nonnegative ? (v < 0.0 ? 0.0 : v) : v);
+    }
+    double heat = 0.0;
+    if (f->heat_scale != 0.0 && f->n_heat_terms > 0) {
+        double sum = 0.0;
+        for (int c = 0; c < f->n_heat_terms; ++c) {
+            const wos_heat_term *h = &f->heat[c];
+            const double dx = (x - h->cx) / h->sx, dy = (y - h->cy) / h->sy;
+            sum += h->a * exp(-0.5 * (dx * dx + dy * dy)) / h->b;
+        }
+        heat = f->heat_scale * sum / f->heat_divisor;
+    }
+    return history + heat;
+}
+
 /* source f(x): poisson.cpp:34-41, screened_poisson.cpp:58-61, plus the
  * descriptor kinds the benchmark problems add */
 static double eq_source(const wos_eq_desc *eq, const double *x, int dim) {
     switch (eq->source_kind) {
+    case WOS_SRC_FIELD: return orc_field_source_value(eq->field_source, x[0], x[1]);
     case WOS_SRC_DISK_SQ: {
         double acc = 0.0;
         for (int k = 0; k < dim; ++k) {

@@ -112,6 +112,10 @@ enum { ORC_RNG_REFERENCE = 0, ORC_RNG_COUNTER = 1 };
 void orc_count_reset(int on);
 void orc_count_get(int64_t out[3]);
 
+/* FieldSampler2D interpolation and the welding source f = history + heat */
+double orc_field_sample(const wos_field_source *f, double x, double y);
+double orc_field_source_value(const wos_field_source *f, double x, double y);
+
 void orc_wos_run(const orc_bvh *bvh, const wos_eq_desc *eq, const double *p0,
                  int n_walks, double epsilon, int max_steps, int rng_mode,
                  uint64_t point_seed, double stats[4], int *max_steps_hits,

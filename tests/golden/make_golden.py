@@ -32,7 +32,54 @@ def meshes():
         "ico2": meshgen.icosphere(2),
         "torus": meshgen.torus(1.0, 0.35, 24, 12),
         "plate": meshgen.rectangle(0.0, 0.12, 0.0, 0.08),
+        "square": meshgen.rectangle(0.0, 1.0, 0.0, 1.0),
+        "plate_c": meshgen.rectangle(-0.06, 0.06, -0.04, 0.04),
     }
+
+
+def _field_cases():
+    """The welding transient sources (WOS_SRC_FIELD) on synthetic previous
+    fields: level 0 (bilinear, no clamp), level 2 (cubic, clamped, a
+    switched Gaussian) and level 3 (cubic, clamped, two moving terms)."""
+    from wos_amd import welding
+
+    def level0():
+        n = 17
+        xs = np.array([welding.grid_coordinate(0.0, 1.0, i, n) for i in range(n)])
+        prev = np.zeros((n, n))
+        prev[1:-1, 1:-1] = np.sin(np.pi * xs[1:-1, None]) * np.sin(np.pi * xs[None, 1:-1])
+        alpha = 1.0 / np.sqrt(0.01)
+        fs = welding.field_source(prev, (n, n, 0.0, 1.0, 0.0, 1.0), 0, False, alpha * alpha, 0.0)
+        return welding.field_equation(alpha, fs)
+
+    def plate_prev(nx, ny):
+        xs = np.array([welding.grid_coordinate(-0.06, 0.06, i, nx) for i in range(nx)])
+        ys = np.array([welding.grid_coordinate(-0.04, 0.04, j, ny) for j in range(ny)])
+        # negative near the edges, so the clamp acts
+        return 400.0 * np.cos(np.pi * xs[:, None] / 0.12) * np.cos(np.pi * ys[None, :] / 0.08) - 40.0
+
+    c = welding.Level2Config()
+    alpha = 1.0 / np.sqrt(c.conductivity / (c.density * c.specific_heat) * c.dt)
+
+    def level2():
+        ix = welding.gaussian_integral(-0.06, 0.06, 0.0, c.sigma_x)
+        iy = welding.gaussian_integral(-0.04, 0.04, 0.0, c.sigma_y)
+        term = (0.0, 0.0, c.sigma_x, c.sigma_y, 600.0, c.thickness * ix * iy)
+        fs = welding.field_source(plate_prev(13, 9), (13, 9, -0.06, 0.06, -0.04, 0.04), 1, True,
+                                  alpha * alpha, 0.4, c.conductivity, [term])
+        return welding.field_equation(alpha, fs)
+
+    def level3():
+        _, _, terms = welding.moving_step_terms(welding.Level3Config(), 2.0, 2.1)
+        fs = welding.field_source(plate_prev(25, 17), (25, 17, -0.06, 0.06, -0.04, 0.04), 1, True,
+                                  alpha * alpha, 1.0, c.conductivity, terms)
+        return welding.field_equation(alpha, fs)
+
+    return [
+        ("screened2d_field_level0", "square", level0, (0.3, 0.6), 64, 1e-3),
+        ("screened2d_field_level2", "plate_c", level2, (0.01, -0.005), 64, 8e-6),
+        ("screened2d_field_level3", "plate_c", level3, (-0.01, 0.002), 64, 8e-6),
+    ]
 
 
 def eq_cases():
@@ -66,7 +113,7 @@ def eq_cases():
          (0.02, 0.07), 64, 8e-6),
         ("screened2d_mis", "plate", lambda: problems.welding_gaussian(2, 0.7, alpha=40.0),
          (0.065, 0.035), 64, 8e-6),
-    ]
+    ] + _field_cases()
 
 
 def main():

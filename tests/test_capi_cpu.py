@@ -72,7 +72,8 @@ def test_version_and_error_string_without_gpu():
 
 STRUCTS = [("wos_eq_desc", capi.EqDesc), ("wos_run_params", capi.RunParams),
            ("wos_point_stats", capi.PointStats), ("wos_run_summary", capi.RunSummary),
-           ("wos_tuning", capi.Tuning)]
+           ("wos_tuning", capi.Tuning), ("wos_field_source", capi.FieldSource),
+           ("wos_heat_term", capi.HeatTerm)]
 
 
 def test_struct_layouts(tmp_path):

@@ -256,6 +256,83 @@ def test_green_source_mis_lowers_the_variance(oracle):
         assert m["variance"] < 0.8 * g["variance"]
 
 
+def _field(values, grid, interp, clamp=False, hist=1.0, heat_scale=0.0, div=1.0, terms=()):
+    from wos_amd import welding
+    return welding.field_source(values, grid, interp, clamp, hist, heat_scale, div, terms)
+
+
+@pytest.mark.parametrize("interp,deg", [(0, 1), (1, 3)])
+def test_field_sampler_reproduces_polynomials(oracle, interp, deg):
+    # bilinear interpolation is exact for a + bx + cy + dxy, the tensor cubic
+    # Lagrange interpolant for every polynomial of degree <= 3 in each
+    # variable; outside the grid the coordinates clamp to its edge
+    import ctypes as C
+    from wos_amd import welding
+    nx, ny, x0, x1, y0, y1 = 9, 7, -0.3, 0.5, 0.1, 0.9
+    xs = np.array([welding.grid_coordinate(x0, x1, i, nx) for i in range(nx)])
+    ys = np.array([welding.grid_coordinate(y0, y1, j, ny) for j in range(ny)])
+
+    def f(x, y):
+        v = 1 + 2 * x - 3 * y + 0.5 * x * y
+        return v + (x ** 3 * y ** 3 - 2 * x ** 2 * y + x ** 2 * y ** 3 if deg == 3 else 0.0)
+
+    fs = _field(f(xs[:, None], ys[None, :]), (nx, ny, x0, x1, y0, y1), interp)
+    pts = np.random.default_rng(1).uniform([x0, y0], [x1, y1], size=(300, 2))
+    for x, y in pts:
+        assert abs(oracle.L.orc_field_sample(C.addressof(fs), x, y) - f(x, y)) <= 1e-13
+    assert abs(oracle.L.orc_field_sample(C.addressof(fs), x0 - 1.0, y1 + 2.0) - f(x0, y1)) <= 1e-13
+    # a cubic request on a 3-node axis falls back to bilinear
+    fs3 = _field(f(xs[:3, None], ys[None, :]), (3, ny, x0, xs[2], y0, y1), 1)
+    x, y = 0.5 * (xs[0] + xs[1]), ys[2]
+    bil = 0.5 * (f(xs[0], y) + f(xs[1], y))
+    assert abs(oracle.L.orc_field_sample(C.addressof(fs3), x, y) - bil) <= 1e-13
+
+
+def test_field_source_heat_is_normalised_on_the_plate(oracle):
+    # thickness x midpoint integral of the volumetric heat over the plate =
+    # absorbed power x the step's power fraction (gaussian_heat_source.hpp:
+    # 76-92), for the stationary source and the two moving Gauss-Legendre
+    # components
+    import ctypes as C
+    from wos_amd import welding
+    c2, c3 = welding.Level2Config(), welding.Level3Config()
+    ix = welding.gaussian_integral(-0.06, 0.06, 0.0, c2.sigma_x)
+    iy = welding.gaussian_integral(-0.04, 0.04, 0.0, c2.sigma_y)
+    absorbed = c2.electrical_power * c2.efficiency
+    frac, _, terms3 = welding.moving_step_terms(c3, 4.95, 5.05)
+    assert frac == pytest.approx(0.5)
+    n = 400
+    xs = -0.06 + (np.arange(n) + 0.5) * 0.12 / n
+    ys = -0.04 + (np.arange(n) + 0.5) * 0.08 / n
+    for scale, terms, expect in (
+            (0.4, [(0.0, 0.0, c2.sigma_x, c2.sigma_y, absorbed, c2.thickness * ix * iy)], 0.4 * absorbed),
+            (1.0, terms3, frac * absorbed)):
+        fs = _field(None, (2, 2, -0.06, 0.06, -0.04, 0.04), 0, heat_scale=scale,
+                    div=c2.conductivity, terms=terms)
+        tot = sum(oracle.L.orc_field_source_value(C.addressof(fs), x, y) for x in xs for y in ys)
+        power = c2.thickness * c2.conductivity * tot * (0.12 / n) * (0.08 / n)
+        assert power == pytest.approx(expect, rel=1e-4)
+
+
+@pytest.mark.parametrize("interp,bias", [(0, 8e-3), (1, 1e-3)])
+def test_field_source_implicit_euler_step(oracle, interp, bias):
+    # one implicit-Euler step of the sin(pi x) sin(pi y) mode on the unit
+    # square: u_1(1/2, 1/2) = 1 / (1 + 2 pi^2 kappa dt) (level0_cooling.cpp:
+    # 286-288); the bilinear history loses up to 1 % of it, the cubic 0.1 %
+    from wos_amd import welding
+    n = 17
+    xs = np.array([welding.grid_coordinate(0.0, 1.0, i, n) for i in range(n)])
+    prev = np.zeros((n, n))
+    prev[1:-1, 1:-1] = np.sin(np.pi * xs[1:-1, None]) * np.sin(np.pi * xs[None, 1:-1])
+    alpha = 1.0 / math.sqrt(0.01)
+    fs = _field(prev, (n, n, 0.0, 1.0, 0.0, 1.0), interp, hist=alpha * alpha)
+    eq = welding.field_equation(alpha, fs)
+    V, F, B = meshgen.rectangle(0.0, 1.0, 0.0, 1.0)
+    r = oracle.mesh(V, F, B).walks(eq, np.array([0.5, 0.5]), 40000, 1e-3, 1000, 1, 3)
+    exact = 1.0 / (1.0 + 2.0 * math.pi ** 2 * 0.01)
+    assert abs(r["mean"] - exact) <= 4 * r["standard_error"] + bias
+
+
 def test_green_functions(oracle):
     L = oracle.L
     R, r = 0.7, 0.2

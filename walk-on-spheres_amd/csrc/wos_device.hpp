@@ -101,6 +101,14 @@ struct EqView {
     double mis_beta, mis_sigma, mis_norm;
     double box[4];
     int mis_box;
+    // WOS_SRC_FIELD: the previous field (device copy, or null), its grid and
+    // interpolation, and the heat terms (wos_field_source of the header)
+    const double *field;
+    int fnx, fny, finterp, fclamp;
+    double fb[4];
+    double hist_scale, heat_scale, heat_div;
+    int n_heat;
+    wos_heat_term heat[WOS_MAX_HEAT_TERMS];
 };
 
 // start of a walk, produced by the classifier for every point
@@ -735,9 +743,84 @@ __device__ __forceinline__ void add_source_der(WalkDer &D, const EqView &e, int 
 }
 
 // --------------------------------------------------------------------------
+// WOS_SRC_FIELD: interpolation of the previous field and the heat terms
+// --------------------------------------------------------------------------
+template <typename T> __device__ __forceinline__ T clampt(T v, T lo, T hi) {
+    return v < lo ? lo : (hi < v ? hi : v);
+}
+
+// FieldSampler2D::sample (apps/welding/include/welding/field_sampler.hpp:51-117):
+// grid coordinates clamped to the grid, then bilinear on the cell or tensor
+// cubic Lagrange on the 4 x 4 nodes around it (shifted inside at the edges)
+template <typename T>
+__device__ T field_sample(const EqView &e, T x, T y) {
+    const int nx = e.fnx, ny = e.fny;
+    const T gx = clampt<T>((x - (T)e.fb[0]) / ((T)e.fb[1] - (T)e.fb[0]) * (T)(nx - 1), (T)0,
+                           (T)(nx - 1));
+    const T gy = clampt<T>((y - (T)e.fb[2]) / ((T)e.fb[3] - (T)e.fb[2]) * (T)(ny - 1), (T)0,
+                           (T)(ny - 1));
+    const double *v = e.field;
+    if (e.finterp == WOS_INTERP_CUBIC && nx >= 4 && ny >= 4) {
+        const int ix = min(max((int)floor(gx) - 1, 0), nx - 4);
+        const int iy = min(max((int)floor(gy) - 1, 0), ny - 4);
+        T wx[4], wy[4];
+        const T cxs[2] = {gx - (T)ix, gy - (T)iy};
+        for (int k = 0; k < 2; ++k) {
+            const T c = cxs[k];
+            T *w = k == 0 ? wx : wy;
+            w[0] = -(c - (T)1) * (c - (T)2) * (c - (T)3) / (T)6;
+            w[1] = c * (c - (T)2) * (c - (T)3) / (T)2;
+            w[2] = -c * (c - (T)1) * (c - (T)3) / (T)2;
+            w[3] = c * (c - (T)1) * (c - (T)2) / (T)6;
+        }
+        T value = (T)0;
+        for (int di = 0; di < 4; ++di) {
+            const double *r = v + (size_t)(ix + di) * ny + iy;
+            T row = (T)0;
+            for (int dj = 0; dj < 4; ++dj) row += wy[dj] * (T)r[dj];
+            value += wx[di] * row;
+        }
+        return value;
+    }
+    const int i0 = (int)floor(gx), j0 = (int)floor(gy);
+    const int i1 = min(i0 + 1, nx - 1), j1 = min(j0 + 1, ny - 1);
+    const T tx = gx - (T)i0, ty = gy - (T)j0;
+    const T v00 = (T)v[(size_t)i0 * ny + j0], v10 = (T)v[(size_t)i1 * ny + j0];
+    const T v01 = (T)v[(size_t)i0 * ny + j1], v11 = (T)v[(size_t)i1 * ny + j1];
+    const T lower = ((T)1 - tx) * v00 + tx * v10;
+    const T upper = ((T)1 - tx) * v01 + tx * v11;
+    return ((T)1 - ty) * lower + ty * upper;
+}
+
+// f = history + heat (level2_transient_heat.cpp:255-264,
+// level3_moving_heat.cpp:189-194, level0_cooling.cpp:194-196); each heat
+// term a exp(-(dx^2 + dy^2) / 2) / b as GaussianHeatSource2D (:38-41, :109-113)
+// and MovingGaussianStepSource2D (:75-88) evaluate it
+template <typename T>
+__device__ T field_source_value(const EqView &e, T x, T y) {
+    T history = (T)0;
+    if (e.field) {
+        const T v = field_sample<T>(e, x, y);
+        history = (T)e.hist_scale * (e.fclamp ? (v < (T)0 ? (T)0 : v) : v);
+    }
+    T heat = (T)0;
+    if (e.heat_scale != 0.0 && e.n_heat > 0) {
+        T sum = (T)0;
+        for (int c = 0; c < e.n_heat; ++c) {
+            const wos_heat_term &h = e.heat[c];
+            const T dx = (x - (T)h.cx) / (T)h.sx, dy = (y - (T)h.cy) / (T)h.sy;
+            sum += (T)h.a * Fx<T>::exp((T)-0.5 * (dx * dx + dy * dy)) / (T)h.b;
+        }
+        heat = (T)e.heat_scale * sum / (T)e.heat_div;
+    }
+    return history + heat;
+}
+
+// --------------------------------------------------------------------------
 // equation functors (src/equations/*.cpp) driven by the descriptor
 // --------------------------------------------------------------------------
-template <typename T>
+// FIELD = true compiles the WOS_SRC_FIELD case (kernels launched for it only)
+template <typename T, bool FIELD = false>
 __device__ __forceinline__ T eq_source(const EqView &e, T x, T y, T z, int dim) {
     const T dx = x - (T)e.cx, dy = y - (T)e.cy;
     T acc = dx * dx + dy * dy;
@@ -748,6 +831,9 @@ __device__ __forceinline__ T eq_source(const EqView &e, T x, T y, T z, int dim) 
     case WOS_SRC_CONSTANT: return (T)e.src_value;
     case WOS_SRC_GAUSSIAN:
         return (T)e.src_value * Fx<T>::exp(-Fx<T>::div(acc, (T)e.src_radius * (T)e.src_radius));
+    case WOS_SRC_FIELD:
+        if constexpr (FIELD) return field_source_value<T>(e, x, y);
+        else return (T)0;
     default: return (T)0;
     }
 }

@@ -188,16 +188,18 @@ __device__ double source_term_mis(const EqView &e, T px, T py, T R, T weight, RN
 
 // One source term of wos_sample (include/wos/wos.hpp:69-110): returns the
 // value to add (already multiplied by the weight).  T is the geometry type;
-// the GREEN_MIS branch is compiled only into the kernels launched for it
-// (MIS = true), so the other kernels keep their registers.
-template <int DIM, typename T, typename RNG, bool MIS = false>
+// SX selects the source extensions compiled in: 0 none, 1 the GREEN_MIS
+// branch, 2 the WOS_SRC_FIELD source.  Each goes only into the kernels
+// launched for it, so the other kernels keep their registers.
+template <int DIM, typename T, typename RNG, int SX = 0>
 __device__ __forceinline__ double source_term(const EqView &e, T px, T py, T pz, T R, T weight,
                                               RNG &rng) {
     const bool screened = e.kind == WOS_EQ_SCREENED;
     const T alpha = (T)e.alpha;
     if (DIM == 2 && !eq_source_may_intersect<T>(e, px, py, R)) return 0.0;
-    if constexpr (MIS && DIM == 2)
+    if constexpr (SX == 1 && DIM == 2)
         if (e.source_mode == WOS_SOURCE_GREEN_MIS) return source_term_mis<T>(e, px, py, R, weight, rng);
+    constexpr bool FLD = SX == 2;
     T sx, sy, sz = (T)0;
     if (e.source_mode == WOS_SOURCE_GREEN) {
         if (DIM == 2) {
@@ -214,7 +216,7 @@ __device__ __forceinline__ double source_term(const EqView &e, T px, T py, T pz,
             sy = py + radius * xy * Fx<T>::sin(angle);
             sz = pz + radius * z;
         }
-        const T f = eq_source<T>(e, sx, sy, sz, DIM);
+        const T f = eq_source<T, FLD>(e, sx, sy, sz, DIM);
         if (f == (T)0) return 0.0;
         T mass;
         if (DIM == 2) mass = screened ? screened_mass_2d<T>(alpha, R) : R * R / (T)4;
@@ -240,7 +242,7 @@ __device__ __forceinline__ double source_term(const EqView &e, T px, T py, T pz,
         const T dx = px - sx, dy = py - sy, dz = pz - sz;
         dist = sqrt(dx * dx + dy * dy + dz * dz);
     }
-    const T f = eq_source<T>(e, sx, sy, sz, DIM);
+    const T f = eq_source<T, FLD>(e, sx, sy, sz, DIM);
     if (f == (T)0) return 0.0;
     T volume;
     if (DIM == 2) volume = (T)kPi * R * R;
@@ -608,7 +610,7 @@ void set_walks_per_point(RoundView &r, int cw) {
 // At least 3 wavefronts per SIMD (at most 168 VGPRs) for the 3D forward
 // walks, which fit without spilling; 2D walks with a source term and adjoint
 // replays may use 256 rather than spill.
-template <int DIM, bool SRC, bool SCR, bool ADJ, bool MIS = false>
+template <int DIM, bool SRC, bool SCR, bool ADJ, int SX = 0>
 __global__ __launch_bounds__(kBlock)
 __attribute__((amdgpu_waves_per_eu(ADJ || DIM == 2 ? 2 : 3, 8)))
 void k_walk_fp64(MeshView m, EqView e, RoundView rv,
@@ -638,7 +640,7 @@ void k_walk_fp64(MeshView m, EqView e, RoundView rv,
                                                           (uint32_t)steps, k0, k1, D);
             } else {
                 CursorD src(w, (uint32_t)steps, 1, k0, k1);
-                sample += source_term<DIM, double, CursorD, MIS>(e, px, py, pz, R, weight, src);
+                sample += source_term<DIM, double, CursorD, SX>(e, px, py, pz, R, weight, src);
             }
         }
         if (SCR) {
@@ -707,7 +709,7 @@ __global__ __launch_bounds__(kBlock) void k_ref_seed(const StartInfo *starts, co
     o[4] = src.s0; o[5] = src.s1; o[6] = src.s2; o[7] = src.s3;
 }
 
-template <int DIM, bool SRC, bool SCR, bool MIS = false>
+template <int DIM, bool SRC, bool SCR, int SX = 0>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2, 8)))
 void k_walk_ref(MeshView m, EqView e, RoundView rv, double eps, int max_steps, double *vals,
                 int32_t *steps_out, uint64_t *state) {
@@ -734,7 +736,7 @@ void k_walk_ref(MeshView m, EqView e, RoundView rv, double eps, int max_steps, d
         double sample = 0.0, weight = 1.0;
         int steps = 0;
         while (R > eps && steps < max_steps) {
-            if (SRC) sample += source_term<DIM, double, XoRef, MIS>(e, px, py, pz, R, weight, source);
+            if (SRC) sample += source_term<DIM, double, XoRef, SX>(e, px, py, pz, R, weight, source);
             if (SCR) weight *= step_weight<DIM, double>(e.kind, alpha, R);
             if (DIM == 2) {
                 const double theta = walk.unit() * 2 * kPi;
@@ -767,13 +769,20 @@ void k_walk_ref(MeshView m, EqView e, RoundView rv, double eps, int max_steps, d
 // centre in closed form (R sqrt(u) in 2D, R cbrt(w) in 3D) rather than from
 // rounded coordinates, and a constant source skips the direction entirely:
 // only the radius draw is made (block 1 word 0 in 3D, block 0 word 1 in 2D).
+// SMODE: 1 uniform, 2 Green, 3 GREEN_MIS, 4 Green with WOS_SRC_FIELD,
+// 5 uniform with WOS_SRC_FIELD (the field cases 2D only).
 template <int DIM, int SMODE>
 __device__ __forceinline__ double source_term_f(const EqView &e, float px, float py, float pz,
                                                 float R, float weight, uint32_t w, uint32_t step,
                                                 uint32_t k0, uint32_t k1) {
+    constexpr bool FLD = SMODE == 5;
     if (SMODE == 2) {
         CursorF src(w, step, 1, k0, k1);
         return source_term<DIM, float>(e, px, py, pz, R, weight, src);
+    }
+    if constexpr (SMODE == 4) {
+        CursorF src(w, step, 1, k0, k1);
+        return source_term<DIM, float, CursorF, 2>(e, px, py, pz, R, weight, src);
     }
     if constexpr (SMODE == 3 && DIM == 2) {
         CursorF src(w, step, 1, k0, k1);
@@ -813,7 +822,7 @@ __device__ __forceinline__ double source_term_f(const EqView &e, float px, float
             sy = __builtin_fmaf(dist * surf, sn, py);
             sz = __builtin_fmaf(dist, z, pz);
         }
-        f = eq_source<float>(e, sx, sy, sz, DIM);
+        f = eq_source<float, FLD>(e, sx, sy, sz, DIM);
     }
     if (f == 0.0f || !(dist > 0.0f)) return 0.0;
     float volume, G;
@@ -1464,6 +1473,7 @@ struct wos_ctx {
     DevBuf plist_in, loc, starts, amb, bvalue, inside, vals, steps, mean, m2, count, ssum, hits,
         counter, vv, pid_all, r_pid, r_wbeg, r_off, coef, scal_sum, grad_v, grad_b, dl_du,
         adj_scal, adj_vw, adj_prim;
+    DevBuf field;                     // WOS_SRC_FIELD: the previous field
     int last_rounds = 0;
     MeshView view{};
     hipEvent_t ev[6]{};
@@ -1600,29 +1610,43 @@ hipError_t launch_classify(wos_ctx *c, K kernel, const EqView &ev, const GridSpe
     return hipGetLastError();
 }
 
+template <int DIM, bool SRC, bool SCR, int SX>
+void launch_ref_sx(wos_ctx *c, const EqView &ev, const RoundView &rv, double eps, int max_steps,
+                   int blocks, size_t lds) {
+    hipLaunchKernelGGL((k_walk_ref<DIM, SRC, SCR, SX>), dim3(blocks), dim3(kBlock), lds, c->stream,
+                       c->view, ev, rv, eps, max_steps, c->vals.as<double>(),
+                       c->steps.as<int32_t>(), c->ref_state.as<uint64_t>());
+}
+
+template <int DIM, bool SRC, bool SCR, bool ADJ, int SX>
+void launch_fp64_sx(wos_ctx *c, const EqView &ev, const RoundView &rv, double eps, int max_steps,
+                    const AdjOut &adj, int blocks, size_t lds) {
+    hipLaunchKernelGGL((k_walk_fp64<DIM, SRC, SCR, ADJ, SX>), dim3(blocks), dim3(kBlock), lds,
+                       c->stream, c->view, ev, rv, eps, max_steps, c->vals.as<double>(),
+                       c->steps.as<int32_t>(), adj);
+}
+
 template <int DIM, bool SRC, bool SCR, bool ADJ>
 hipError_t launch_walk(wos_ctx *c, const EqView &ev, int precision, const RoundView &rv,
                        double eps, int max_steps, const AdjOut &adj, int *blocks_out) {
-    // GREEN_MIS: 2D forward solves only (the descriptor check and the
-    // adjoint's refusal keep every other case off its kernels)
-    constexpr bool kMis = SRC && DIM == 2 && !ADJ;
-    const bool mis = kMis && ev.source_mode == WOS_SOURCE_GREEN_MIS;
+    // source extensions, 2D forward solves only (the descriptor checks and
+    // the adjoint's refusals keep every other case off their kernels):
+    // 1 GREEN_MIS, 2 WOS_SRC_FIELD
+    constexpr bool kExt = SRC && DIM == 2 && !ADJ;
+    constexpr int kMis = kExt ? 1 : 0, kFld = kExt ? 2 : 0;
+    const int sx = !kExt ? 0
+                   : ev.source_mode == WOS_SOURCE_GREEN_MIS ? 1
+                   : ev.source_kind == WOS_SRC_FIELD      ? 2
+                                                          : 0;
     if (precision == WOS_FP64_REFERENCE) {
         if constexpr (ADJ) {
             return hipErrorInvalidValue;
         } else {
             const int blocks = (rv.n + kBlock - 1) / kBlock;
             const size_t lds = (size_t)kBlock * (c->L.depth + 2) * sizeof(int);
-            if (mis)
-                hipLaunchKernelGGL((k_walk_ref<DIM, SRC, SCR, kMis>), dim3(blocks), dim3(kBlock),
-                                   lds, c->stream, c->view, ev, rv, eps, max_steps,
-                                   c->vals.as<double>(), c->steps.as<int32_t>(),
-                                   c->ref_state.as<uint64_t>());
-            else
-                hipLaunchKernelGGL((k_walk_ref<DIM, SRC, SCR>), dim3(blocks), dim3(kBlock), lds,
-                                   c->stream, c->view, ev, rv, eps, max_steps,
-                                   c->vals.as<double>(), c->steps.as<int32_t>(),
-                                   c->ref_state.as<uint64_t>());
+            if (sx == 1) launch_ref_sx<DIM, SRC, SCR, kMis>(c, ev, rv, eps, max_steps, blocks, lds);
+            else if (sx == 2) launch_ref_sx<DIM, SRC, SCR, kFld>(c, ev, rv, eps, max_steps, blocks, lds);
+            else launch_ref_sx<DIM, SRC, SCR, 0>(c, ev, rv, eps, max_steps, blocks, lds);
             *blocks_out = blocks;
             return hipGetLastError();
         }
@@ -1630,24 +1654,25 @@ hipError_t launch_walk(wos_ctx *c, const EqView &ev, int precision, const RoundV
     if (precision == WOS_FP64_PLAIN) {
         const int blocks = (int)((rv.total + kBlock - 1) / kBlock);
         const size_t lds = (size_t)kBlock * (c->L.depth + 2) * sizeof(int);
-        if (mis)
-            hipLaunchKernelGGL((k_walk_fp64<DIM, SRC, SCR, ADJ, kMis>), dim3(blocks), dim3(kBlock),
-                               lds, c->stream, c->view, ev, rv, eps, max_steps,
-                               c->vals.as<double>(), c->steps.as<int32_t>(), adj);
-        else
-            hipLaunchKernelGGL((k_walk_fp64<DIM, SRC, SCR, ADJ>), dim3(blocks), dim3(kBlock), lds,
-                               c->stream, c->view, ev, rv, eps, max_steps, c->vals.as<double>(),
-                               c->steps.as<int32_t>(), adj);
+        if (sx == 1) launch_fp64_sx<DIM, SRC, SCR, ADJ, kMis>(c, ev, rv, eps, max_steps, adj, blocks, lds);
+        else if (sx == 2) launch_fp64_sx<DIM, SRC, SCR, ADJ, kFld>(c, ev, rv, eps, max_steps, adj, blocks, lds);
+        else launch_fp64_sx<DIM, SRC, SCR, ADJ, 0>(c, ev, rv, eps, max_steps, adj, blocks, lds);
         *blocks_out = blocks;
         return hipGetLastError();
     }
     hipError_t ge = ensure_grid(c);
     if (ge != hipSuccess) return ge;
+    // FP32 source modes: 0 none, 1 uniform, 2 Green, 3 GREEN_MIS, 4 Green
+    // with the field source, 5 uniform with the field source
     constexpr int SMODE = SRC ? 1 : 0;
+    constexpr int kM3 = kExt ? 3 : SMODE, kM4 = kExt ? 4 : SMODE, kM5 = kExt ? 5 : SMODE;
     const bool green = ev.source_mode == WOS_SOURCE_GREEN && SRC;
-    const void *kfn = mis     ? (const void *)k_walk_fp32<DIM, kMis ? 3 : SMODE, SCR, ADJ>
-                      : green ? (const void *)k_walk_fp32<DIM, 2, SCR, ADJ>
-                              : (const void *)k_walk_fp32<DIM, SMODE, SCR, ADJ>;
+    const int mode = sx == 1 ? 3 : sx == 2 ? (green ? 4 : 5) : green ? 2 : SMODE;
+    const void *kfn = mode == 3   ? (const void *)k_walk_fp32<DIM, kM3, SCR, ADJ>
+                      : mode == 4 ? (const void *)k_walk_fp32<DIM, kM4, SCR, ADJ>
+                      : mode == 5 ? (const void *)k_walk_fp32<DIM, kM5, SCR, ADJ>
+                      : mode == 2 ? (const void *)k_walk_fp32<DIM, 2, SCR, ADJ>
+                                  : (const void *)k_walk_fp32<DIM, SMODE, SCR, ADJ>;
     const int threads = c->tune.block_threads;
     const size_t stack_bytes = (size_t)threads * (c->L.depth + 1) * sizeof(StackEntry);
     const int node_bytes = DIM == 3 ? (int)sizeof(NodeF3) : (int)sizeof(NodeF2);
@@ -1693,10 +1718,16 @@ hipError_t launch_walk(wos_ctx *c, const EqView &ev, int precision, const RoundV
                        (int64_t)rv.total, rv.total_dev);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
-    if (mis)
-        hipLaunchKernelGGL((k_walk_fp32<DIM, kMis ? 3 : SMODE, SCR, ADJ>), dim3(blocks),
-                           dim3(threads), lds, c->stream, c->view, ev, a, adj);
-    else if (green)
+    if (mode == 3)
+        hipLaunchKernelGGL((k_walk_fp32<DIM, kM3, SCR, ADJ>), dim3(blocks), dim3(threads), lds,
+                           c->stream, c->view, ev, a, adj);
+    else if (mode == 4)
+        hipLaunchKernelGGL((k_walk_fp32<DIM, kM4, SCR, ADJ>), dim3(blocks), dim3(threads), lds,
+                           c->stream, c->view, ev, a, adj);
+    else if (mode == 5)
+        hipLaunchKernelGGL((k_walk_fp32<DIM, kM5, SCR, ADJ>), dim3(blocks), dim3(threads), lds,
+                           c->stream, c->view, ev, a, adj);
+    else if (mode == 2)
         hipLaunchKernelGGL((k_walk_fp32<DIM, 2, SCR, ADJ>), dim3(blocks), dim3(threads), lds,
                            c->stream, c->view, ev, a, adj);
     else
@@ -1758,7 +1789,7 @@ int validate_eq(const wos_ctx *c, const wos_eq_desc *eq) {
     if (!eq) return fail(WOS_ERR_INPUT, "equation descriptor is NULL");
     if (eq->kind < 0 || eq->kind > 3) return fail(WOS_ERR_INPUT, "unknown equation kind");
     if (eq->source_mode < 0 || eq->source_mode > 2) return fail(WOS_ERR_INPUT, "unknown source mode");
-    if (eq->source_kind < 0 || eq->source_kind > 5) return fail(WOS_ERR_INPUT, "unknown source kind");
+    if (eq->source_kind < 0 || eq->source_kind > 6) return fail(WOS_ERR_INPUT, "unknown source kind");
     if (eq->boundary_kind < 0 || eq->boundary_kind > 6) return fail(WOS_ERR_INPUT, "unknown boundary kind");
     if (eq->kind == WOS_EQ_LAPLACE && eq->source_kind != WOS_SRC_NONE)
         return fail(WOS_ERR_INPUT, "laplace takes no source");
@@ -1788,6 +1819,37 @@ int validate_eq(const wos_ctx *c, const wos_eq_desc *eq) {
             return fail(WOS_ERR_INPUT, "invalid Gaussian source proposal box");
         if (!(proposal_norm(eq) > 0.0) || !std::isfinite(proposal_norm(eq)))
             return fail(WOS_ERR_INPUT, "Gaussian source proposal has no mass in its box");
+    }
+    if (eq->source_kind == WOS_SRC_FIELD) {
+        const wos_field_source *f = eq->field_source;
+        if (c->mesh.dim != 2) return fail(WOS_ERR_INPUT, "the field source is 2D");
+        if (!f) return fail(WOS_ERR_INPUT, "WOS_SRC_FIELD needs field_source");
+        if (eq->source_mode == WOS_SOURCE_GREEN_MIS)
+            return fail(WOS_ERR_INPUT, "GreenSourceMIS requires a source proposal sampler and PDF "
+                                       "(a 2D Gaussian source of poisson or screened_poisson)");
+        if (f->values) {
+            // field_sampler.hpp:38-49
+            if (f->nx < 2 || f->ny < 2)
+                return fail(WOS_ERR_INPUT, "FieldSampler2D requires a complete grid of at least 2 x 2");
+            if (!(f->bounds[1] > f->bounds[0]) || !(f->bounds[3] > f->bounds[2]))
+                return fail(WOS_ERR_INPUT, "FieldSampler2D requires non-degenerate grid bounds");
+            if (f->interpolation != WOS_INTERP_BILINEAR && f->interpolation != WOS_INTERP_CUBIC)
+                return fail(WOS_ERR_INPUT, "unknown field interpolation");
+            if ((int64_t)f->nx * f->ny > ((int64_t)1 << 28))
+                return fail(WOS_ERR_INPUT, "field grid too large");
+        }
+        if (!std::isfinite(f->history_scale) || !std::isfinite(f->heat_scale))
+            return fail(WOS_ERR_INPUT, "field source scales must be finite");
+        if (f->n_heat_terms < 0 || f->n_heat_terms > WOS_MAX_HEAT_TERMS)
+            return fail(WOS_ERR_INPUT, "n_heat_terms out of range");
+        if (f->n_heat_terms > 0 && !(f->heat_divisor != 0.0 && std::isfinite(f->heat_divisor)))
+            return fail(WOS_ERR_INPUT, "heat_divisor must be finite and nonzero");
+        for (int k = 0; k < f->n_heat_terms; ++k) {
+            const wos_heat_term &h = f->heat[k];
+            if (!(h.sx > 0.0) || !(h.sy > 0.0) || !(h.b != 0.0) || !std::isfinite(h.a) ||
+                !std::isfinite(h.b) || !std::isfinite(h.cx) || !std::isfinite(h.cy))
+                return fail(WOS_ERR_INPUT, "invalid heat term");
+        }
     }
     if (eq->n_bnd_values < 0 || eq->n_bnd_values > WOS_MAX_BND_VALUES)
         return fail(WOS_ERR_INPUT, "n_bnd_values out of range");
@@ -1837,6 +1899,8 @@ int prepare_solve(wos_ctx *c, const std::vector<Part> &parts, const wos_eq_desc 
     if (rc) return rc;
     if (adjr && eq->source_mode == WOS_SOURCE_GREEN_MIS && eq->source_kind != WOS_SRC_NONE)
         return fail(WOS_ERR_INPUT, "the adjoint supports the uniform and Green source modes");
+    if (adjr && eq->source_kind == WOS_SRC_FIELD)
+        return fail(WOS_ERR_INPUT, "the adjoint does not differentiate field sources");
     if (prm->n_walks < 1) return fail(WOS_ERR_INPUT, "n_walks must be >= 1");
     if (prm->max_steps < 1) return fail(WOS_ERR_INPUT, "max_steps must be >= 1");
     if (c->mesh.dim == 3 && prm->max_ray_attempts < 3)
@@ -1867,6 +1931,23 @@ int prepare_solve(wos_ctx *c, const std::vector<Part> &parts, const wos_eq_desc 
         ev.mis_norm = proposal_norm(eq);
         ev.mis_box = proposal_boxed(eq) ? 1 : 0;
         for (int k = 0; k < 4; ++k) ev.box[k] = eq->proposal_box[k];
+    }
+    if (eq->source_kind == WOS_SRC_FIELD) {
+        const wos_field_source *f = eq->field_source;
+        ev.field = nullptr;
+        if (f->values) {
+            std::vector<double> fv(f->values, f->values + (size_t)f->nx * f->ny);
+            HIPX(upload(c->field, fv, c->stream));
+            ev.field = c->field.as<double>();
+        }
+        ev.fnx = f->nx; ev.fny = f->ny;
+        ev.finterp = f->interpolation; ev.fclamp = f->clamp_nonnegative ? 1 : 0;
+        for (int k = 0; k < 4; ++k) ev.fb[k] = f->bounds[k];
+        ev.hist_scale = f->history_scale;
+        ev.heat_scale = f->heat_scale;
+        ev.heat_div = f->heat_divisor;
+        ev.n_heat = f->n_heat_terms;
+        for (int k = 0; k < f->n_heat_terms; ++k) ev.heat[k] = f->heat[k];
     }
     ev.vertex_values = nullptr;
     if (eq->boundary_kind == WOS_BND_VERTEX) {
@@ -2359,7 +2440,8 @@ int solve_graph(wos_ctx *c, const std::vector<Part> &parts, const wos_eq_desc *e
     key_put(key, c->view);
     for (const DevBuf *b : {&c->loc, &c->starts, &c->starts_f, &c->amb, &c->bvalue, &c->inside, &c->g_bcount,
                             &c->g_boff, &c->g_misc, &c->mean, &c->m2, &c->count, &c->ssum,
-                            &c->hits, &c->vals, &c->steps, &c->counter, &c->pid_all, &c->vv})
+                            &c->hits, &c->vals, &c->steps, &c->counter, &c->pid_all, &c->vv,
+                            &c->field})
         key_put(key, b->p);
     if (key != c->graph_key) {
         drop_graph(c);

@@ -22,7 +22,7 @@ WOS_EQ_LAPLACE, WOS_EQ_POISSON, WOS_EQ_SCREENED, WOS_EQ_HELMHOLTZ = range(4)
 WOS_SOURCE_UNIFORM, WOS_SOURCE_GREEN, WOS_SOURCE_GREEN_MIS = range(3)
 # enum wos_source_kind
 (WOS_SRC_NONE, WOS_SRC_DISK_SQ, WOS_SRC_DISK_SQRT, WOS_SRC_CONSTANT, WOS_SRC_GAUSSIAN,
- WOS_SRC_ZERO) = range(6)
+ WOS_SRC_ZERO, WOS_SRC_FIELD) = range(7)
 # enum wos_boundary_kind
 (WOS_BND_X2Y2_ID0, WOS_BND_Y5, WOS_BND_ZERO, WOS_BND_ID_CONST, WOS_BND_VERTEX,
  WOS_BND_SIN3X_COS2Y, WOS_BND_ONE) = range(7)
@@ -35,6 +35,32 @@ LOC_OUTSIDE, LOC_INSIDE, LOC_BOUNDARY, LOC_INDETERMINATE = range(4)
 WOS_GRAD_ALPHA, WOS_GRAD_SRC_VALUE, WOS_GRAD_SRC_RADIUS, WOS_GRAD_SRC_CENTER = 0, 1, 2, 3
 WOS_GRAD_BND_VALUES = 6
 WOS_N_GRAD = 6 + WOS_MAX_BND_VALUES
+
+
+WOS_INTERP_BILINEAR, WOS_INTERP_CUBIC = 0, 1
+WOS_MAX_HEAT_TERMS = 4
+
+
+class HeatTerm(C.Structure):
+    _fields_ = [("cx", C.c_double), ("cy", C.c_double), ("sx", C.c_double),
+                ("sy", C.c_double), ("a", C.c_double), ("b", C.c_double)]
+
+
+class FieldSource(C.Structure):
+    _fields_ = [
+        ("values", C.POINTER(C.c_double)),
+        ("nx", C.c_int32),
+        ("ny", C.c_int32),
+        ("bounds", C.c_double * 4),
+        ("interpolation", C.c_int32),
+        ("clamp_nonnegative", C.c_int32),
+        ("history_scale", C.c_double),
+        ("heat_scale", C.c_double),
+        ("heat_divisor", C.c_double),
+        ("n_heat_terms", C.c_int32),
+        ("reserved0", C.c_int32),
+        ("heat", HeatTerm * WOS_MAX_HEAT_TERMS),
+    ]
 
 
 class EqDesc(C.Structure):
@@ -54,6 +80,7 @@ class EqDesc(C.Structure):
         ("n_vertex_values", C.c_int64),
         ("mis_green_probability", C.c_double),
         ("proposal_box", C.c_double * 4),
+        ("field_source", C.POINTER(FieldSource)),
     ]
 
 
