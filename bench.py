@@ -14,7 +14,8 @@ t_walk,p = (T / total steps) * mean_steps_p of the form being measured.
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-adjoint]
        python bench.py --fp32-only --steps 1 --warmup 0   (walker launches only,
                                                           for counter runs)
-       python bench.py --workload stars | --workload plate  (batched stars, plate)
+       python bench.py --workload stars | plate | welding  (batched stars, plate,
+       welding transients)
        python bench.py --copy-calibration 1024            (one 1024 MiB fp32
                                                           read+write kernel, the
                                                           scale for HBM counters)
@@ -257,6 +258,72 @@ def workload_plate(Context, meshgen, problems, caps):
                       "alpha": gp[caps.WOS_GRAD_ALPHA]})
 
 
+def workload_welding(Context, caps):
+    """Welding transients, one screened WoS solve per implicit-Euler step:
+    level 2 (plate 120 x 80 mm, 49 x 33 grid, 500 walks, 100 steps of 0.1 s,
+    cubic history, stationary Gaussian for 3 s, seed 20260819) in FP32 and
+    FP64, level 3 (97 x 65 grid, 500 walks, 150 steps, source moving for 5 s)
+    in FP32; the compiled reference on the host cores times step 21 of level 2
+    on a sample of its points (the reference's own seeds), scaled to the run:
+    whole-run seconds"""
+    from wos_amd import welding as W
+    ctx = Context(0)
+    out = {}
+    for label, prec in (("fp32", caps.WOS_FP32_PERSISTENT), ("fp64", caps.WOS_FP64_PLAIN)):
+        W.run_level2(ctx, W.Level2Config(), prec, steps=2)   # warm-up
+        t0 = time.perf_counter()
+        rec, _ = W.run_level2(ctx, W.Level2Config(), prec)
+        out[f"level2_{label}_s"] = time.perf_counter() - t0
+        out[f"level2_{label}_ms_per_step"] = 1e3 * out[f"level2_{label}_s"] / len(rec)
+        out[f"level2_{label}_peak_rise"] = max(r["peak_rise"] for r in rec)
+    t0 = time.perf_counter()
+    rec3, _, _ = W.run_level3(ctx, W.Level3Config(), caps.WOS_FP32_PERSISTENT)
+    out["level3_fp32_s"] = time.perf_counter() - t0
+    out["level3_fp32_ms_per_step"] = 1e3 * out["level3_fp32_s"] / len(rec3)
+    out["level3_fp32_peak_rise"] = max(r["peak_rise"] for r in rec3)
+    # the reference on the host: step 21 of level 2 from the GPU's step-20 field
+    c = W.Level2Config()
+    _, prev = W.run_level2(ctx, c, caps.WOS_FP32_PERSISTENT, steps=20)
+    cpu = None
+    try:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle
+        from wos_amd import meshgen
+        ref = pyoracle.Reference()
+        x0, x1, y0, y1 = -0.06, 0.06, -0.04, 0.04
+        kappa = c.conductivity / (c.density * c.specific_heat)
+        alpha = 1.0 / np.sqrt(kappa * c.dt)
+        ix = W.gaussian_integral(x0, x1, 0.0, c.sigma_x)
+        iy = W.gaussian_integral(y0, y1, 0.0, c.sigma_y)
+        term = (0.0, 0.0, c.sigma_x, c.sigma_y, c.electrical_power * c.efficiency,
+                c.thickness * ix * iy)
+        frac = W.heating_fraction(2.0, 2.1, c.heat_on_time)
+        fs = W.field_source(prev, (c.nx, c.ny, x0, x1, y0, y1), W.WOS_INTERP_CUBIC, True,
+                            alpha * alpha, frac, c.conductivity, [term])
+        eq = W.field_equation(alpha, fs)
+        idx = np.array([i * c.ny + j for i in range(1, c.nx - 1) for j in range(1, c.ny - 1)])
+        pick = idx[::4]
+        pts = np.array([(W.grid_coordinate(x0, x1, k // c.ny, c.nx),
+                         W.grid_coordinate(y0, y1, k % c.ny, c.ny)) for k in pick])
+        cores = min(os.cpu_count() or 1, 16)
+        rm = ref.mesh(*meshgen.rectangle(x0, x1, y0, y1))
+        t0 = time.perf_counter()
+        rm.solve_points(eq, pts, c.walks, c.epsilon, seed=W.step_seed(c.seed, 21),
+                        n_threads=cores, seed_index=pick)
+        dt = time.perf_counter() - t0
+        run_s = dt * len(idx) / len(pick) * 100
+        cpu = dict(value=run_s, unit="s (level-2 run, estimated)", cores=cores, kind="reference",
+                   sample=f"step 21 of level 2, {len(pick)} of {len(idx)} interior points, "
+                          f"{c.walks} walks each, {dt:.2f} s on {cores} threads, x 100 steps")
+        out["level2_fp32_speedup_vs_cpu"] = run_s / out["level2_fp32_s"]
+    except (FileNotFoundError, OSError):
+        pass
+    return dict(workload="welding", description=workload_welding.__doc__.split(":")[0].strip(),
+                metric="whole-run seconds of the level-2 and level-3 transients",
+                value=out["level2_fp32_s"], unit="s", higher_is_better=False, **out,
+                cpu_baseline=cpu)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -267,9 +334,11 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--fp32-only", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB")
-    ap.add_argument("--workload", default="icosphere", choices=("icosphere", "stars", "plate"),
+    ap.add_argument("--workload", default="icosphere",
+                    choices=("icosphere", "stars", "plate", "welding"),
                     help="icosphere: the headline line (default); stars: 16 batched star "
-                         "domains, adaptive; plate: the 200k-triangle plate with gradients")
+                         "domains, adaptive; plate: the 200k-triangle plate with gradients; "
+                         "welding: the level-2/3 transient time loops")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -288,6 +357,9 @@ def main():
         return
     if args.workload == "plate":
         print(json.dumps(workload_plate(Context, meshgen, problems, capi)))
+        return
+    if args.workload == "welding":
+        print(json.dumps(workload_welding(Context, capi)))
         return
 
     V, F, _ = meshgen.icosphere(4)
