@@ -83,9 +83,13 @@ template <typename T> __device__ __forceinline__ T clamp01(T v) {
     return v < (T)0 ? (T)0 : (v > (T)1 ? (T)1 : v);
 }
 // include/wos/screened.hpp:105-129
+// (the sphere's log K0(X) and log I0(X) are taken once, outside the
+// rejection loop: the same values in the same expression)
 template <typename T, typename RNG>
 __device__ __forceinline__ T screened_radius_2d(T alpha, T R, RNG &rng) {
     const T X = alpha * R;
+    const T lk_X = X < (T)2 ? (T)0 : log_k0<T>(X);
+    const T li_X = X < (T)2 ? (T)0 : log_i0<T>(X);
     for (int guard = 0; guard < 100000; ++guard) {
         T r, acceptance;
         if (X < (T)2) {
@@ -97,7 +101,7 @@ __device__ __forceinline__ T screened_radius_2d(T alpha, T R, RNG &rng) {
             r = sqrt((T)-4 * time * log(rng.unit_open()));
             if (r >= R) continue;
             const T x = alpha * r;
-            const T log_ratio = log_i0<T>(x) + log_k0<T>(X) - log_i0<T>(X) - log_k0<T>(x);
+            const T log_ratio = log_i0<T>(x) + lk_X - li_X - log_k0<T>(x);
             acceptance = -expm1(log_ratio < (T)0 ? log_ratio : (T)0);
         }
         if (rng.unit() < clamp01<T>(acceptance)) return r;
