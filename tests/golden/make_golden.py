@@ -240,7 +240,56 @@ def main():
         for k in ("mean", "variance", "standard_error", "mean_steps", "location", "hits"):
             sw[f"{name}__{k}"] = r[k]
     np.savez(os.path.join(HERE, "sweeps.npz"), **sw)
+
+    # ---- welding time loops: the compiled reference solves every step with
+    # its own FieldSampler2D history and the apps' per-step seeds
+    np.savez(os.path.join(HERE, "transient.npz"), **transient_fixtures(ref))
     print("wrote", sorted(f for f in os.listdir(HERE) if f.endswith(".npz")))
+
+
+def transient_cases():
+    """(name, config, steps) of the small transient runs; the same settings
+    drive the GPU's reference-stream runs."""
+    from wos_amd import welding
+    return [
+        ("level0", welding.Level0Config(walks=64), 2),
+        ("level2", welding.Level2Config(nx=13, ny=9, walks=32), 3),
+        ("level3", welding.Level3Config(nx=25, ny=17, walks=16), 3),
+    ]
+
+
+def transient_fixtures(ref):
+    """Fields after each step of the small level-0/2/3 runs: every step one
+    reference solve of all grid points (boundary nodes give g = 0, as the
+    apps leave them) with seed_index = grid index and the step's seed."""
+    from wos_amd import welding
+    out = {}
+    for name, c, steps in transient_cases():
+        if name == "level0":
+            grid = (c.grid_size, c.grid_size, 0.0, 1.0, 0.0, 1.0)
+            prev = welding.level0_initial(c)
+        else:
+            grid = welding.plate_grid(c)
+            prev = np.zeros((c.nx, c.ny))
+        nx, ny, x0, x1, y0, y1 = grid
+        rm = ref.mesh(*meshgen.rectangle(x0, x1, y0, y1))
+        pts = np.array([(welding.grid_coordinate(x0, x1, i, nx), welding.grid_coordinate(y0, y1, j, ny))
+                        for i in range(nx) for j in range(ny)])
+        for k in range(1, steps + 1):
+            if name == "level0":
+                eq = welding.level0_equation(c, prev)
+            elif name == "level2":
+                eq, _ = welding.level2_equation(c, prev, k)
+            else:
+                eq, _ = welding.level3_equation(c, prev, k)
+            r = ref.mesh(*meshgen.rectangle(x0, x1, y0, y1)).solve_points(
+                eq, pts, c.walks, c.epsilon, max_steps=c.max_steps,
+                seed=welding.step_seed(c.seed, k), n_threads=8, seed_index=np.arange(len(pts)))
+            prev = np.where(r["location"] == 1, r["mean"], 0.0).reshape(nx, ny)
+            out[f"{name}__step{k}"] = prev
+            out[f"{name}__location"] = r["location"]
+        del rm
+    return out
 
 
 if __name__ == "__main__":

@@ -161,3 +161,24 @@ def test_level3_moving_source_matches_the_modal_solution(gpu_ctx):
     print(f"peak at x = {s['peak_x']:.4f}, modal {px:.4f}")
     assert px < -0.01 and abs(s["peak_x"] - px) <= 2.5 * 0.12 / (c.nx - 1)
     assert abs(s["peak_y"]) <= 0.0015
+
+
+@pytest.mark.parametrize("case", ["level0", "level2", "level3"])
+def test_transient_reference_streams_equal_reference_golden(gpu_ctx, case):
+    """Small level-0/2/3 runs on the reference's streams: every step's
+    field equals the one the compiled reference wrote (screened 2D: its
+    libstdc++ Bessel functions and last-bit libm differences, 1e-11)."""
+    from wos_amd.capi import WOS_FP64_REFERENCE
+    mg = _golden()
+    g = np.load(os.path.join(GOLD, "transient.npz"))
+    name, c, steps = [t for t in mg.transient_cases() if t[0] == case][0]
+    run = {"level0": W.run_level0, "level2": W.run_level2, "level3": W.run_level3}[case]
+    fields = []
+    run(gpu_ctx, c, WOS_FP64_REFERENCE, steps=steps, fields=fields)
+    assert len(fields) == steps
+    for k, f in enumerate(fields, 1):
+        ref = g[f"{case}__step{k}"]
+        d = np.abs(f - ref)
+        print(f"{case} step {k}: max |d| {d.max():.3e} of {np.abs(ref).max():.4g}, "
+              f"bit-identical {np.mean(d == 0):.3f}")
+        assert np.allclose(f, ref, rtol=1e-11, atol=1e-13), (case, k)

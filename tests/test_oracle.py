@@ -333,6 +333,36 @@ def test_field_source_implicit_euler_step(oracle, interp, bias):
     assert abs(r["mean"] - exact) <= 4 * r["standard_error"] + bias
 
 
+def test_transient_loops_golden_reference_streams(oracle):
+    # the small level-0/2/3 time loops, every step solved by the oracle on
+    # the reference's streams, against the fields the compiled reference
+    # wrote step by step (screened 2D: its Bessel functions, 1e-12)
+    import importlib.util
+
+    class RefStreams:
+        """The oracle behind the reference driver's solve_points signature."""
+        def mesh(self, V, F, B=None):
+            om = oracle.mesh(V, F, B)
+
+            class M:
+                def solve_points(self, eq, pts, n_walks, eps, max_steps=1000, seed=0, n_threads=8,
+                                 seed_index=None):
+                    return om.solve_points(eq, pts, n_walks, eps, max_steps=max_steps, seed=seed,
+                                           rng_mode=0, n_threads=n_threads, seed_index=seed_index)
+            return M()
+
+    spec = importlib.util.spec_from_file_location("make_golden", os.path.join(GOLD, "make_golden.py"))
+    mg = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mg)
+    g = load("transient")
+    got = mg.transient_fixtures(RefStreams())
+    for k in g.files:
+        if k.endswith("__location"):
+            assert np.array_equal(got[k], g[k]), k
+        else:
+            assert np.allclose(got[k], g[k], rtol=1e-11, atol=1e-13), k
+
+
 def test_green_functions(oracle):
     L = oracle.L
     R, r = 0.7, 0.2

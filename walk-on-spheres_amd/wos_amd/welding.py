@@ -25,7 +25,7 @@ from __future__ import annotations
 import ctypes as C
 import math
 import time
-from dataclasses import dataclass, field
+from dataclasses import dataclass
 
 import numpy as np
 
@@ -248,25 +248,73 @@ def _step(ctx, nx, ny, eq, walks, eps, seed, step, max_steps, precision):
     return mean, se, int(r["max_steps_hits"].sum()), dt, r
 
 
-def run_level0(ctx: Context, cfg: Level0Config = None, precision=WOS_FP64_PLAIN, steps=None):
+def level0_equation(c: Level0Config, prev):
+    """Step equation of level 0 (level0_cooling.cpp:181-196, :361-362)."""
+    alpha = 1.0 / math.sqrt(c.diffusivity * c.dt)
+    n = c.grid_size
+    fs = field_source(prev, (n, n, 0.0, 1.0, 0.0, 1.0), WOS_INTERP_BILINEAR, False, alpha * alpha, 0.0)
+    return field_equation(alpha, fs)
+
+
+def level0_initial(c: Level0Config):
+    """sin(pi x) sin(pi y) on the interior nodes, 0 on the edges (:272-284)."""
+    n = c.grid_size
+    xs = np.array([grid_coordinate(0.0, 1.0, i, n) for i in range(n)])
+    prev = np.zeros((n, n))
+    prev[1:-1, 1:-1] = np.sin(math.pi * xs[1:-1, None]) * np.sin(math.pi * xs[None, 1:-1])
+    return prev
+
+
+def plate_grid(c):
+    """The centred plate and its grid (nx, ny, xmin, xmax, ymin, ymax)."""
+    return (c.nx, c.ny, -0.5 * c.length_x, 0.5 * c.length_x, -0.5 * c.length_y, 0.5 * c.length_y)
+
+
+def level2_equation(c: Level2Config, prev, step: int):
+    """Step equation of level 2 (level2_transient_heat.cpp:237-291, :529-545):
+    returns (equation, power fraction of the step)."""
+    grid = plate_grid(c)
+    _, _, x0, x1, y0, y1 = grid
+    kappa = c.conductivity / (c.density * c.specific_heat)
+    alpha = 1.0 / math.sqrt(kappa * c.dt)
+    ix = gaussian_integral(x0, x1, 0.0, c.sigma_x)
+    iy = gaussian_integral(y0, y1, 0.0, c.sigma_y)
+    term = (0.0, 0.0, c.sigma_x, c.sigma_y, c.electrical_power * c.efficiency,
+            c.thickness * ix * iy)
+    frac = heating_fraction((step - 1) * c.dt, step * c.dt, c.heat_on_time)
+    fs = field_source(prev, grid, c.history_interpolation, True, alpha * alpha, frac,
+                      c.conductivity, [term])
+    return field_equation(alpha, fs), frac
+
+
+def level3_equation(c: "Level3Config", prev, step: int):
+    """Step equation of level 3 (level3_moving_heat.cpp:174-224, :496-515):
+    returns (equation, active fraction of the step)."""
+    grid = plate_grid(c)
+    kappa = c.conductivity / (c.density * c.specific_heat)
+    alpha = 1.0 / math.sqrt(kappa * c.dt)
+    frac, _, terms = moving_step_terms(c, (step - 1) * c.dt, step * c.dt)
+    fs = field_source(prev, grid, WOS_INTERP_CUBIC, True, alpha * alpha, 1.0, c.conductivity, terms)
+    return field_equation(alpha, fs), frac
+
+
+def run_level0(ctx: Context, cfg: Level0Config = None, precision=WOS_FP64_PLAIN, steps=None,
+               fields=None):
     """Cooling of the sin(pi x) sin(pi y) mode on the unit square with the
     discrete decay 1 / (1 + 2 pi^2 kappa dt) per step (level0_cooling.cpp:
-    286-288, :394-404).  Returns per-step records and the final field."""
+    286-288, :394-404).  Returns per-step records and the final field; a
+    list passed as `fields` receives every step's field."""
     c = cfg or Level0Config()
     n = c.grid_size
     ctx.set_mesh(*meshgen.rectangle(0.0, 1.0, 0.0, 1.0))
-    grid = (n, n, 0.0, 1.0, 0.0, 1.0)
-    prev = np.zeros((n, n))
+    prev = level0_initial(c)
     xs = np.array([grid_coordinate(0.0, 1.0, i, n) for i in range(n)])
-    prev[1:-1, 1:-1] = np.sin(math.pi * xs[1:-1, None]) * np.sin(math.pi * xs[None, 1:-1])
-    alpha = 1.0 / math.sqrt(c.diffusivity * c.dt)
     decay = 1.0 / (1.0 + 2.0 * math.pi * math.pi * c.diffusivity * c.dt)
     exact_mode = np.sin(math.pi * xs[:, None]) * np.sin(math.pi * xs[None, :])
     ci = n // 2
     out = []
     for step in range(1, (steps or c.steps) + 1):
-        fs = field_source(prev, grid, WOS_INTERP_BILINEAR, False, alpha * alpha, 0.0)
-        mean, se, hits, secs, _ = _step(ctx, n, n, field_equation(alpha, fs), c.walks, c.epsilon,
+        mean, se, hits, secs, _ = _step(ctx, n, n, level0_equation(c, prev), c.walks, c.epsilon,
                                         c.seed, step, c.max_steps, precision)
         amp = decay ** step
         err = (mean - amp * exact_mode)[1:-1, 1:-1]
@@ -276,56 +324,52 @@ def run_level0(ctx: Context, cfg: Level0Config = None, precision=WOS_FP64_PLAIN,
                         center_error=abs(float(mean[ci, ci]) - amp),
                         rmse=float(np.sqrt(np.mean(err ** 2))), max_abs_error=float(np.abs(err).max()),
                         max_steps_hits=hits, step_seconds=secs))
+        if fields is not None:
+            fields.append(mean)
         prev = mean
     return out, prev
 
 
-def run_level2(ctx: Context, cfg: Level2Config = None, precision=WOS_FP64_PLAIN, steps=None):
+def run_level2(ctx: Context, cfg: Level2Config = None, precision=WOS_FP64_PLAIN, steps=None,
+               fields=None):
     """Stationary Gaussian heating for heat_on_time, then cooling
     (level2_transient_heat.cpp:398-640).  Returns per-step summaries (the
     columns of the app's summary CSV) and the final field."""
     c = cfg or Level2Config()
-    x0, x1, y0, y1 = -0.5 * c.length_x, 0.5 * c.length_x, -0.5 * c.length_y, 0.5 * c.length_y
-    ctx.set_mesh(*meshgen.rectangle(x0, x1, y0, y1))
-    grid = (c.nx, c.ny, x0, x1, y0, y1)
+    grid = plate_grid(c)
+    ctx.set_mesh(*meshgen.rectangle(*grid[2:]))
     n_steps = steps or int(round(c.total_time / c.dt))
-    kappa = c.conductivity / (c.density * c.specific_heat)
-    alpha = 1.0 / math.sqrt(kappa * c.dt)
     absorbed = c.electrical_power * c.efficiency
-    ix = gaussian_integral(x0, x1, 0.0, c.sigma_x)
-    iy = gaussian_integral(y0, y1, 0.0, c.sigma_y)
-    term = (0.0, 0.0, c.sigma_x, c.sigma_y, absorbed, c.thickness * ix * iy)
     prev = np.zeros((c.nx, c.ny))
     ci, cj = c.nx // 2, c.ny // 2
     energy_in = 0.0
     out = []
     for step in range(1, n_steps + 1):
-        tb, te = (step - 1) * c.dt, step * c.dt
-        frac = heating_fraction(tb, te, c.heat_on_time)
+        te = step * c.dt
+        eq, frac = level2_equation(c, prev, step)
         energy_in += frac * absorbed * c.dt
-        fs = field_source(prev, grid, c.history_interpolation, True, alpha * alpha, frac,
-                          c.conductivity, [term])
-        mean, se, hits, secs, _ = _step(ctx, c.nx, c.ny, field_equation(alpha, fs), c.walks,
-                                        c.epsilon, c.seed, step, c.max_steps, precision)
+        mean, se, hits, secs, _ = _step(ctx, c.nx, c.ny, eq, c.walks, c.epsilon, c.seed, step,
+                                        c.max_steps, precision)
         s = field_summary(mean, se, grid, c.density * c.specific_heat * c.thickness)
         s.update(step=step, time=te, source_fraction=frac, absorbed_power=frac * absorbed,
                  cumulative_input_energy=energy_in, center_temperature_rise=float(mean[ci, cj]),
                  center_standard_error=float(se[ci, cj]), max_steps_hits=hits, step_seconds=secs)
         out.append(s)
+        if fields is not None:
+            fields.append(mean)
         prev = mean
     return out, prev
 
 
-def run_level3(ctx: Context, cfg: Level3Config = None, precision=WOS_FP64_PLAIN, steps=None):
+def run_level3(ctx: Context, cfg: Level3Config = None, precision=WOS_FP64_PLAIN, steps=None,
+               fields=None):
     """Moving Gaussian source, then cooling (level3_moving_heat.cpp:335-600).
     Returns per-step summaries, per-step probe values and the final field."""
     c = cfg or Level3Config()
-    x0, x1, y0, y1 = -0.5 * c.length_x, 0.5 * c.length_x, -0.5 * c.length_y, 0.5 * c.length_y
+    grid = plate_grid(c)
+    _, _, x0, x1, y0, y1 = grid
     ctx.set_mesh(*meshgen.rectangle(x0, x1, y0, y1))
-    grid = (c.nx, c.ny, x0, x1, y0, y1)
     n_steps = steps or int(round(c.total_time / c.dt))
-    kappa = c.conductivity / (c.density * c.specific_heat)
-    alpha = 1.0 / math.sqrt(kappa * c.dt)
     probe_idx = []
     for name, px, py, _ in c.probes:
         i = int(round((px - x0) / (x1 - x0) * (c.nx - 1)))
@@ -339,13 +383,11 @@ def run_level3(ctx: Context, cfg: Level3Config = None, precision=WOS_FP64_PLAIN,
     energy_in = 0.0
     out, probes = [], []
     for step in range(1, n_steps + 1):
-        tb, te = (step - 1) * c.dt, step * c.dt
-        frac, _, terms = moving_step_terms(c, tb, te)
+        te = step * c.dt
+        eq, frac = level3_equation(c, prev, step)
         energy_in += frac * c.electrical_power * c.efficiency * c.dt
-        fs = field_source(prev, grid, WOS_INTERP_CUBIC, True, alpha * alpha, 1.0,
-                          c.conductivity, terms)
-        mean, se, hits, secs, _ = _step(ctx, c.nx, c.ny, field_equation(alpha, fs), c.walks,
-                                        c.epsilon, c.seed, step, c.max_steps, precision)
+        mean, se, hits, secs, _ = _step(ctx, c.nx, c.ny, eq, c.walks, c.epsilon, c.seed, step,
+                                        c.max_steps, precision)
         s = field_summary(mean, se, grid, c.density * c.specific_heat * c.thickness)
         s.update(step=step, time=te, phase=1 if te <= c.heat_off_time else 2,
                  source_x=c.source_start_x + c.weld_speed * min(max(te, 0.0), c.heat_off_time),
@@ -354,10 +396,13 @@ def run_level3(ctx: Context, cfg: Level3Config = None, precision=WOS_FP64_PLAIN,
                  center_standard_error=float(se[ci, cj]), max_steps_hits=hits, step_seconds=secs)
         out.append(s)
         probes.append({name: (float(mean[i, j]), float(se[i, j])) for name, i, j in probe_idx})
+        if fields is not None:
+            fields.append(mean)
         prev = mean
     return out, probes, prev
 
 
 __all__ = ["Level0Config", "Level2Config", "Level3Config", "run_level0", "run_level2",
-           "run_level3", "field_source", "field_equation", "moving_step_terms",
+           "run_level3", "level0_equation", "level2_equation", "level3_equation",
+           "level0_initial", "plate_grid", "field_source", "field_equation", "moving_step_terms",
            "heating_fraction", "gaussian_integral", "splitmix64", "step_seed"]
