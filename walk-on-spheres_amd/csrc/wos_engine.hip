@@ -1087,6 +1087,11 @@ struct PersistArgs {
     double *vals;
     int32_t *steps_out;
     unsigned long long *units;  // 7 query-unit counters, or nullptr
+    // forward walks with g != 0: the terminal projection is left to
+    // k_terminal, which reads the end point and weight (term_pw) and the
+    // nearest primitive's slot (term_slot, -1: value final) of every walk
+    float4 *term_pw;
+    int32_t *term_slot;
 };
 
 // Forward walks: at least 4 wavefronts per SIMD (at most 128 VGPRs), which
@@ -1152,21 +1157,31 @@ void k_walk_fp32(MeshView m, EqView e, PersistArgs a, AdjOut adj) {
                 searching = (qb & kQsMore) != 0u;
                 began = true;
             } else if (stop) {
-                // terminal value in FP64: nearest point on the nearest primitive
                 const bool hit = R > a.eps_f;
-                double g = 0.0, s = 0.0, t = 0.0;
-                int prim = -1;
-                const int slot = q.best_slot;
-                if (slot >= 0 && (ADJ || e.boundary_kind != WOS_BND_ZERO)) {
-                    prim = m.slot_prim[slot];
-                    double nx, ny, nz = 0.0;
-                    if (DIM == 2) npq_seg_d(m.segd[slot], (double)px, (double)py, nx, ny, s);
-                    else eberly_d(m.trid[slot], (double)px, (double)py, (double)pz, nx, ny, nz, s, t);
-                    g = eq_boundary(e, m, nx, ny, nz, prim, s, t);
-                }
-                a.vals[out_index] = sample + (double)weight * g;
                 a.steps_out[out_index] = steps | (hit ? (int32_t)0x80000000 : 0);
-                if constexpr (ADJ) store_adjoint(adj, (int64_t)out_index, D, (double)weight, g, prim, s, t);
+                if constexpr (ADJ) {
+                    // terminal value in FP64: nearest point on the nearest primitive
+                    double g = 0.0, s = 0.0, t = 0.0;
+                    int prim = -1;
+                    const int slot = q.best_slot;
+                    if (slot >= 0) {
+                        prim = m.slot_prim[slot];
+                        double nx, ny, nz = 0.0;
+                        if (DIM == 2) npq_seg_d(m.segd[slot], (double)px, (double)py, nx, ny, s);
+                        else eberly_d(m.trid[slot], (double)px, (double)py, (double)pz, nx, ny, nz, s, t);
+                        g = eq_boundary(e, m, nx, ny, nz, prim, s, t);
+                    }
+                    a.vals[out_index] = sample + (double)weight * g;
+                    store_adjoint(adj, (int64_t)out_index, D, (double)weight, g, prim, s, t);
+                } else {
+                    // g = 0: the value is the source sum; else k_terminal adds
+                    // weight x g at the FP64 nearest point, one lane per walk
+                    a.vals[out_index] = sample;
+                    if (a.term_slot) {
+                        a.term_pw[out_index] = make_float4(px, py, pz, weight);
+                        a.term_slot[out_index] = q.best_slot;
+                    }
+                }
                 active = false;
             }
         }
@@ -1219,6 +1234,7 @@ void k_walk_fp32(MeshView m, EqView e, PersistArgs a, AdjOut adj) {
                         const StartInfo &F = a.rv.starts[a.rv.inside[j]];
                         a.vals[out_index] = F.g;
                         a.steps_out[out_index] = 0;
+                        if (!ADJ && a.term_slot) a.term_slot[out_index] = -1;
                         if constexpr (ADJ)
                             store_adjoint(adj, (int64_t)item, D, 1.0, F.g, F.prim, F.s,
                                           DIM == 3 ? F.t : 0.0);
@@ -1309,6 +1325,31 @@ void k_walk_fp32(MeshView m, EqView e, PersistArgs a, AdjOut adj) {
         atomicAdd(a.units + 5, (unsigned long long)u_pr);
         atomicAdd(a.units + 6, (unsigned long long)u_lprim);
     }
+}
+
+// --------------------------------------------------------------------------
+// terminal values of the FP32 forward walks when g != 0: the FP64 nearest
+// point on the walk's nearest primitive (from the FP32 end point) and
+// value = sum of the source terms + weight x g there, one lane per walk (in
+// the persistent walker only the few ending lanes of a wavefront would run
+// this FP64 code)
+// --------------------------------------------------------------------------
+template <int DIM>
+__global__ __launch_bounds__(kBlock) void k_terminal(MeshView m, EqView e, RoundView rv,
+                                                     const float4 *pw, const int32_t *slots,
+                                                     double *vals) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t total = rv.total_dev ? *rv.total_dev : rv.total;
+    if (i >= total) return;
+    const int slot = slots[i];
+    if (slot < 0) return;
+    const float4 p = pw[i];
+    const int prim = m.slot_prim[slot];
+    double nx, ny, nz = 0.0, s = 0.0, t = 0.0;
+    if (DIM == 2) npq_seg_d(m.segd[slot], (double)p.x, (double)p.y, nx, ny, s);
+    else eberly_d(m.trid[slot], (double)p.x, (double)p.y, (double)p.z, nx, ny, nz, s, t);
+    const double g = eq_boundary(e, m, nx, ny, nz, prim, s, t);
+    vals[i] = vals[i] + (double)p.w * g;
 }
 
 // --------------------------------------------------------------------------
@@ -1480,6 +1521,7 @@ struct wos_ctx {
         counter, vv, pid_all, r_pid, r_wbeg, r_off, coef, scal_sum, grad_v, grad_b, dl_du,
         adj_scal, adj_vw, adj_prim;
     DevBuf field;                     // WOS_SRC_FIELD: the previous field
+    DevBuf term_pw, term_slot;        // FP32 walks' end points and slots (g != 0)
     int last_rounds = 0;
     MeshView view{};
     hipEvent_t ev[6]{};
@@ -1616,6 +1658,17 @@ hipError_t launch_classify(wos_ctx *c, K kernel, const EqView &ev, const GridSpe
     return hipGetLastError();
 }
 
+// end-point buffers of the deferred terminal projection (FP32 forward walks
+// with g != 0), one entry per walk of a round
+hipError_t reserve_terminal(wos_ctx *c, const EqView &ev, int precision, bool adjoint,
+                                   int64_t total) {
+    if (precision != WOS_FP32_PERSISTENT || adjoint || ev.boundary_kind == WOS_BND_ZERO)
+        return hipSuccess;
+    hipError_t e = c->term_pw.reserve((size_t)total * sizeof(float4));
+    if (e != hipSuccess) return e;
+    return c->term_slot.reserve((size_t)total * sizeof(int32_t));
+}
+
 template <int DIM, bool SRC, bool SCR, int SX>
 void launch_ref_sx(wos_ctx *c, const EqView &ev, const RoundView &rv, double eps, int max_steps,
                    int blocks, size_t lds) {
@@ -1724,6 +1777,9 @@ hipError_t launch_walk(wos_ctx *c, const EqView &ev, int precision, const RoundV
     a.vals = c->vals.as<double>();
     a.steps_out = c->steps.as<int32_t>();
     a.units = c->count_units ? c->units.as<unsigned long long>() : nullptr;
+    const bool defer = !ADJ && ev.boundary_kind != WOS_BND_ZERO;
+    a.term_pw = defer ? c->term_pw.as<float4>() : nullptr;
+    a.term_slot = defer ? c->term_slot.as<int32_t>() : nullptr;
     a.parts = c->tune.xcd_parts;
     hipLaunchKernelGGL(k_init_parts, dim3(1), dim3(64), 0, c->stream, a.counter, a.parts,
                        (int64_t)rv.total, rv.total_dev);
@@ -1744,6 +1800,13 @@ hipError_t launch_walk(wos_ctx *c, const EqView &ev, int precision, const RoundV
     else
         hipLaunchKernelGGL((k_walk_fp32<DIM, SMODE, SCR, ADJ>), dim3(blocks), dim3(threads), lds,
                            c->stream, c->view, ev, a, adj);
+    if (defer) {
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(k_terminal<DIM>, dim3((unsigned)((rv.total + kBlock - 1) / kBlock)),
+                           dim3(kBlock), 0, c->stream, c->view, ev, rv, a.term_pw, a.term_slot,
+                           a.vals);
+    }
     *blocks_out = blocks;
     c->last_threads = threads;
     c->last_cached = n_cached;
@@ -2279,6 +2342,7 @@ int solve_once(wos_ctx *c, const std::vector<Part> &parts, const wos_eq_desc *eq
             }
             HIPX(c->vals.reserve((size_t)rv.total * sizeof(double)));
             HIPX(c->steps.reserve((size_t)rv.total * sizeof(int32_t)));
+            HIPX(reserve_terminal(c, ev, prm->precision, adjr != nullptr, rv.total));
             if (adjr) {
                 HIPX(c->adj_scal.reserve((size_t)rv.total * 6 * sizeof(double)));
                 HIPX(c->adj_vw.reserve((size_t)rv.total * 3 * sizeof(double)));
@@ -2431,6 +2495,7 @@ int solve_graph(wos_ctx *c, const std::vector<Part> &parts, const wos_eq_desc *e
     HIPX(c->hits.reserve((size_t)n * sizeof(int32_t)));
     HIPX(c->vals.reserve((size_t)(n * nw) * sizeof(double)));
     HIPX(c->steps.reserve((size_t)(n * nw) * sizeof(int32_t)));
+    HIPX(reserve_terminal(c, ev, WOS_FP32_PERSISTENT, false, n * nw));
     HIPX(c->counter.reserve(16 * kMaxParts * sizeof(unsigned long long)));
     if (c->pid_all.bytes < (size_t)n * sizeof(int32_t)) {
         std::vector<int32_t> ids((size_t)n);
@@ -2452,7 +2517,7 @@ int solve_graph(wos_ctx *c, const std::vector<Part> &parts, const wos_eq_desc *e
     for (const DevBuf *b : {&c->loc, &c->starts, &c->starts_f, &c->amb, &c->bvalue, &c->inside, &c->g_bcount,
                             &c->g_boff, &c->g_misc, &c->mean, &c->m2, &c->count, &c->ssum,
                             &c->hits, &c->vals, &c->steps, &c->counter, &c->pid_all, &c->vv,
-                            &c->field})
+                            &c->field, &c->term_pw, &c->term_slot})
         key_put(key, b->p);
     if (key != c->graph_key) {
         drop_graph(c);
