@@ -193,3 +193,25 @@ def test_fp32_bitwise_reproducible(gpu_ctx):
     b = gpu_ctx.solve_grid(12, 12, 12, eq, p)
     for k in ("mean", "variance", "mean_steps", "location"):
         assert np.array_equal(a[k], b[k], equal_nan=True), k
+
+
+@pytest.mark.parametrize("shape", ["tetrahedron", "octahedron", "box12"])
+def test_classification_on_shallow_trees(gpu_ctx, oracle, shape):
+    """Meshes whose BVH has leaves above the start query's split depth (a
+    root leaf, leaves at depth 1 and 2): location codes, start distances and
+    FP64 means equal the oracle's."""
+    if shape == "tetrahedron":
+        V = np.array([[1, 1, 1], [1, -1, -1], [-1, 1, -1], [-1, -1, 1]], float)
+        F = np.array([[0, 1, 2], [0, 3, 1], [0, 2, 3], [1, 3, 2]], np.int32)
+    elif shape == "octahedron":
+        V = np.array([[1, 0, 0], [-1, 0, 0], [0, 1, 0], [0, -1, 0], [0, 0, 1], [0, 0, -1]], float)
+        F = np.array([[0, 2, 4], [2, 1, 4], [1, 3, 4], [3, 0, 4], [2, 0, 5], [1, 2, 5], [3, 1, 5],
+                      [0, 3, 5]], np.int32)
+    else:
+        V, F, _ = M.box(1.0, 0.7, 0.4, 1, 1, 1)
+    mesh = (V, F, None)
+    om = oracle.mesh(*mesh)
+    eq = Pr.laplace_builtin(3)
+    g, o = _run_both(gpu_ctx, om, mesh, eq, (9, 9, 9), 32, 1e-3, 17)
+    _compare_fp64(g, o, shape, exact_frac=0.9)
+    assert np.array_equal(g["ambiguous_ray_retries"], o["ambiguous"])

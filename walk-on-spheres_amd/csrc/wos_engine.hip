@@ -281,18 +281,8 @@ __device__ __forceinline__ double grid_coordinate(double lo, double hi, int inde
 }
 
 template <int DIM>
-__global__ __launch_bounds__(kBlock) void k_classify(MeshView m, EqView e, GridSpec g,
-                                                     DomainView dom, const double *plist,
-                                                     int64_t n, uint64_t seed,
-                                                     int max_ray_attempts, uint8_t *loc,
-                                                     StartInfo *starts, int32_t *amb,
-                                                     double *bvalue) {
-    extern __shared__ int lds_i[];
-    int *stk = lds_i + threadIdx.x;
-    const int stride = blockDim.x;
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    double x, y, z = 0.0;
+__device__ __forceinline__ void point_coords(const GridSpec &g, const double *plist, int64_t i,
+                                             double &x, double &y, double &z) {
     if (plist) {
         x = plist[i * DIM];
         y = plist[i * DIM + 1];
@@ -305,10 +295,19 @@ __global__ __launch_bounds__(kBlock) void k_classify(MeshView m, EqView e, GridS
         y = grid_coordinate(g.b[2], g.b[3], gj, g.ny);
         z = grid_coordinate(g.b[4], g.b[5], gk, g.nz);
     }
-    const uint64_t ps = point_seed(seed, (uint64_t)i);
-    NpqD q;
-    if (DIM == 2) q = npq_ref_2d(m, dom.root64, x, y, stk, stride);
-    else q = npq_ref_3d(m, dom.root64, x, y, z, stk, stride);
+}
+
+// Location code, walk keys and start record of point i from its start query
+// q (src/inside.cpp:199-260, runner.hpp:182-221).  first_rays (3D): -1, or
+// the first three ray casts made beforehand, bit k the hit parity of ray k
+// and bit 3 + k set when ray k was ambiguous.
+template <int DIM>
+__device__ __forceinline__ void classify_tail(const MeshView &m, const EqView &e,
+                                              const DomainView &dom, int64_t i, double x,
+                                              double y, double z, const NpqD &q, uint64_t ps,
+                                              int first_rays, int max_ray_attempts, int *stk,
+                                              int stride, uint8_t *loc, StartInfo *starts,
+                                              int32_t *amb, double *bvalue) {
     const double r = sqrt(q.d2);
     int code;
     int retries = 0;
@@ -317,9 +316,15 @@ __global__ __launch_bounds__(kBlock) void k_classify(MeshView m, EqView e, GridS
     } else if (DIM == 2) {
         code = winding_number_d(m, dom.s0, dom.s1, x, y) != 0 ? 1 : 0;
     } else {
-        // src/inside.cpp:228-260 with the inside stream of runner.hpp:182-183
+        // src/inside.cpp:228-260 with the inside stream of runner.hpp:182-183;
+        // when none of the first three casts was ambiguous they decide
+        // (three valid votes, no retries), else the loop runs from the start
         Xoshiro rng(splitmix64(ps ^ kInsideStream));
         int valid = 0, votes = 0;
+        if (first_rays >= 0 && (first_rays >> 3) == 0) {
+            valid = 3;
+            votes = (first_rays & 1) + ((first_rays >> 1) & 1) + ((first_rays >> 2) & 1);
+        }
         for (int attempts = 0; attempts < max_ray_attempts && valid < 3; ++attempts) {
             const double u = rng.unit(), v = rng.unit();
             const double zz = 2 * u - 1;
@@ -359,6 +364,120 @@ __global__ __launch_bounds__(kBlock) void k_classify(MeshView m, EqView e, GridS
     loc[o] = (uint8_t)code;
     amb[o] = retries;
     bvalue[o] = gval;
+}
+
+template <int DIM>
+__global__ __launch_bounds__(kBlock) void k_classify(MeshView m, EqView e, GridSpec g,
+                                                     DomainView dom, const double *plist,
+                                                     int64_t n, uint64_t seed,
+                                                     int max_ray_attempts, uint8_t *loc,
+                                                     StartInfo *starts, int32_t *amb,
+                                                     double *bvalue) {
+    extern __shared__ int lds_i[];
+    int *stk = lds_i + threadIdx.x;
+    const int stride = blockDim.x;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double x = 0.0, y = 0.0, z = 0.0;
+    point_coords<DIM>(g, plist, i, x, y, z);
+    const uint64_t ps = point_seed(seed, (uint64_t)i);
+    NpqD q;
+    if (DIM == 2) q = npq_ref_2d(m, dom.root64, x, y, stk, stride);
+    else q = npq_ref_3d(m, dom.root64, x, y, z, stk, stride);
+    classify_tail<DIM>(m, e, dom, i, x, y, z, q, ps, -1, max_ray_attempts, stk, stride, loc, starts,
+                       amb, bvalue);
+}
+
+// The start query on one of S = 2^D subtrees: subtree j is reached by
+// taking, at each of the top D levels, the near child (bit 0) or the far
+// child (bit 1) in the reference's near-first order, most significant bit
+// first, so j is the subtree's rank in the reference's traversal.  A leaf
+// above depth D is searched by the lane whose remaining bits are 0.  Within
+// the subtree the search is the reference's, so it returns the subtree's
+// first primitive (in that order) at its least distance; over the subtrees
+// the least distance with the lowest j is the reference's own result.
+template <int S>
+__device__ __forceinline__ NpqD npq_ref_3d_sub(const MeshView &m, int root, double px, double py,
+                                               double pz, int j, int *stk, int stride) {
+    constexpr int D = S == 8 ? 3 : (S == 4 ? 2 : (S == 2 ? 1 : 0));
+    static_assert((1 << D) == S, "S must be 1, 2, 4 or 8");
+    int curr = root;
+    for (int l = 0; l < D; ++l) {
+        const NodeD3 n = m.nd3[curr];
+        if (n.count > 0) {
+            if ((j & ((1 << (D - l)) - 1)) != 0) {
+                NpqD r;
+                r.d2 = INFINITY; r.px = r.py = r.pz = 0.0; r.s = r.t = 0.0; r.prim = -1; r.slot = -1;
+                return r;
+            }
+            break;
+        }
+        const int left = curr + 1, right = n.idx;
+        const bool left_near = box_d2_3(m.nd3[left], px, py, pz) <= box_d2_3(m.nd3[right], px, py, pz);
+        const bool far = ((j >> (D - 1 - l)) & 1) != 0;
+        curr = far == left_near ? right : left;
+    }
+    return npq_ref_3d(m, curr, px, py, pz, stk, stride);
+}
+
+// 3D classification with its two searches spread over a block: 64 points
+// per block, wavefront j < S runs subtree j of their start queries and
+// wavefront S + k their ray cast k (k < 3; the inside stream's draws 2k and
+// 2k + 1), so a start query that must visit much of the tree (points deep
+// inside a closed surface) runs on S lanes, and no wavefront mixes searches.
+// Wavefront 0 then merges the subtrees' results and finishes the points.
+template <int S>
+__global__ __launch_bounds__((S + 3) * 64) void k_classify3(MeshView m, EqView e, GridSpec g,
+                                                          DomainView dom, const double *plist,
+                                                          int64_t n, uint64_t seed,
+                                                          int max_ray_attempts, uint8_t *loc,
+                                                          StartInfo *starts, int32_t *amb,
+                                                          double *bvalue) {
+    extern __shared__ int lds_i[];
+    __shared__ NpqD s_npq[S * 64];
+    __shared__ int s_ray[3 * 64];
+    int *stk = lds_i + threadIdx.x;
+    const int stride = blockDim.x;
+    const int lane = (int)(threadIdx.x & 63u), role = (int)(threadIdx.x >> 6);
+    const int64_t i = (int64_t)blockIdx.x * 64 + lane;
+    double x = 0.0, y = 0.0, z = 0.0;
+    if (i < n) point_coords<3>(g, plist, i, x, y, z);
+    const uint64_t ps = point_seed(seed, (uint64_t)i);
+    if (role < S) {
+        NpqD q;
+        q.d2 = INFINITY; q.px = q.py = q.pz = 0.0; q.s = q.t = 0.0; q.prim = -1; q.slot = -1;
+        if (i < n) q = npq_ref_3d_sub<S>(m, dom.root64, x, y, z, role, stk, stride);
+        s_npq[role * 64 + lane] = q;
+    } else {
+        int res = 0;
+        if (i < n) {
+            Xoshiro rng(splitmix64(ps ^ kInsideStream));
+            for (int k = S; k < role; ++k) { rng.unit(); rng.unit(); }
+            const double u = rng.unit(), v = rng.unit();
+            const double zz = 2 * u - 1;
+            const double phi = 2 * kPi * v;
+            const double rr = sqrt(1 - zz * zz);
+            bool ambiguous;
+            const int hits = ray_cast_d(m, dom.root64, x, y, z, rr * cos(phi), rr * sin(phi), zz,
+                                        ambiguous, stk, stride);
+            res = ambiguous ? 2 : (hits & 1);
+        }
+        s_ray[(role - S) * 64 + lane] = res;
+    }
+    __syncthreads();
+    if (role != 0 || i >= n) return;
+    NpqD q = s_npq[lane];
+    for (int j = 1; j < S; ++j) {
+        const NpqD o = s_npq[j * 64 + lane];
+        if (o.d2 < q.d2) q = o;
+    }
+    int first_rays = 0;
+    for (int k = 0; k < 3; ++k) {
+        const int res = s_ray[k * 64 + lane];
+        first_rays |= ((res & 1) << k) | (((res >> 1) & 1) << (3 + k));
+    }
+    classify_tail<3>(m, e, dom, i, x, y, z, q, ps, first_rays, max_ray_attempts, stk, stride,
+                     loc, starts, amb, bvalue);
 }
 
 // --------------------------------------------------------------------------
@@ -1649,10 +1768,10 @@ hipError_t ensure_grid(wos_ctx *c) {
 template <typename K>
 hipError_t launch_classify(wos_ctx *c, K kernel, const EqView &ev, const GridSpec &g,
                            const DomainView &dv, const double *plist, int64_t n, uint64_t seed,
-                           int mra) {
-    const int blocks = (int)((n + kBlock - 1) / kBlock);
-    const size_t lds = (size_t)kBlock * (c->L.depth + 2) * sizeof(int);
-    hipLaunchKernelGGL(kernel, dim3(blocks), dim3(kBlock), lds, c->stream, c->view, ev, g, dv,
+                           int mra, int per_block, int threads) {
+    const int blocks = (int)((n + per_block - 1) / per_block);
+    const size_t lds = (size_t)threads * (c->L.depth + 2) * sizeof(int);
+    hipLaunchKernelGGL(kernel, dim3(blocks), dim3(threads), lds, c->stream, c->view, ev, g, dv,
                        plist, n, seed, mra, c->loc.as<uint8_t>(), c->starts.as<StartInfo>(),
                        c->amb.as<int32_t>(), c->bvalue.as<double>());
     return hipGetLastError();
@@ -2072,10 +2191,30 @@ int classify_parts(wos_ctx *c, const std::vector<Part> &parts, const EqView &ev,
             DomainView dv{D.node_off, D.root_ref, (int)D.prim_off, (int)(D.prim_off + D.n_prims),
                           part_tol[pi], off, fp32_margin(D), p.domain};
             const double *dp = p.points ? c->plist_in.as<double>() + poff * dim : nullptr;
-            hipError_t le = dim == 2 ? launch_classify(c, k_classify<2>, ev, g, dv, dp, p.count,
-                                                       prm->seed, prm->max_ray_attempts)
-                                     : launch_classify(c, k_classify<3>, ev, g, dv, dp, p.count,
-                                                       prm->seed, prm->max_ray_attempts);
+            hipError_t le;
+            if (dim == 2) {
+                le = launch_classify(c, k_classify<2>, ev, g, dv, dp, p.count, prm->seed,
+                                     prm->max_ray_attempts, kBlock, kBlock);
+            } else {
+                // the start query on 8 (or 4) subtrees where the block's
+                // traversal stacks and result slots fit the LDS, else one
+                // lane per point
+                const size_t per_thread = (size_t)(c->L.depth + 2) * sizeof(int);
+                const size_t fixed = 3 * 64 * sizeof(int);
+                auto fits = [&](int split) {
+                    return (size_t)(split + 3) * 64 * per_thread + fixed +
+                               (size_t)split * 64 * sizeof(NpqD) <= c->lds_per_block;
+                };
+                if (fits(8))
+                    le = launch_classify(c, k_classify3<8>, ev, g, dv, dp, p.count, prm->seed,
+                                         prm->max_ray_attempts, 64, 11 * 64);
+                else if (fits(4))
+                    le = launch_classify(c, k_classify3<4>, ev, g, dv, dp, p.count, prm->seed,
+                                         prm->max_ray_attempts, 64, 7 * 64);
+                else
+                    le = launch_classify(c, k_classify<3>, ev, g, dv, dp, p.count, prm->seed,
+                                         prm->max_ray_attempts, kBlock, kBlock);
+            }
             HIPX(le);
             off += p.count;
             if (p.points) poff += p.count;
