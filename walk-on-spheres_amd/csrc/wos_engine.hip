@@ -2494,7 +2494,9 @@ int solve_once(wos_ctx *c, const std::vector<Part> &parts, const wos_eq_desc *eq
             HIPX(dispatch_walk(c, dim, adjr != nullptr, ev, src, scr, prm->precision, rv,
                                prm->epsilon, prm->max_steps, adj, &blocks));
             HIPX(hipEventRecord(c->ev[3], c->stream));
-            hipLaunchKernelGGL(k_welford, dim3((rv.n + kBlock - 1) / kBlock), dim3(kBlock), 0,
+            // one wavefront per block: a point's merge is a serial chain, so
+            // the kernel wants its few wavefronts on as many CUs as possible
+            hipLaunchKernelGGL(k_welford, dim3((rv.n + 63) / 64), dim3(64), 0,
                                c->stream, rv, c->vals.as<double>(), c->steps.as<int32_t>(),
                                c->mean.as<double>(), c->m2.as<double>(), c->count.as<double>(),
                                c->ssum.as<double>(), c->hits.as<int32_t>());
@@ -2697,7 +2699,7 @@ int solve_graph(wos_ctx *c, const std::vector<Part> &parts, const wos_eq_desc *e
                                eq->kind == WOS_EQ_SCREENED || eq->kind == WOS_EQ_HELMHOLTZ,
                                WOS_FP32_PERSISTENT, rv, prm->epsilon,
                                prm->max_steps, AdjOut{}, &c->graph_blocks));
-            hipLaunchKernelGGL(k_welford, dim3((unsigned)nb), dim3(kBlock), 0, c->stream, rv,
+            hipLaunchKernelGGL(k_welford, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, c->stream, rv,
                                c->vals.as<double>(), c->steps.as<int32_t>(), c->mean.as<double>(),
                                c->m2.as<double>(), c->count.as<double>(), c->ssum.as<double>(),
                                c->hits.as<int32_t>());
