@@ -2607,7 +2607,7 @@ void drop_graph(wos_ctx *c) {
 
 int solve_graph(wos_ctx *c, const std::vector<Part> &parts, const wos_eq_desc *eq,
                 const wos_run_params *prm, wos_point_stats *out, wos_run_summary *summary,
-                wos_run_summary *part_summaries) {
+                wos_run_summary *part_summaries, bool capture = true) {
     if (!out || !out->mean) return fail(WOS_ERR_INPUT, "out->mean is required");
     if (!prm) return fail(WOS_ERR_INPUT, "params are required");
     if (prm->precision != WOS_FP32_PERSISTENT || prm->target_se > 0.0)
@@ -2660,10 +2660,7 @@ int solve_graph(wos_ctx *c, const std::vector<Part> &parts, const wos_eq_desc *e
                             &c->hits, &c->vals, &c->steps, &c->counter, &c->pid_all, &c->vv,
                             &c->field, &c->term_pw, &c->term_slot})
         key_put(key, b->p);
-    if (key != c->graph_key) {
-        drop_graph(c);
-        HIPX(hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
-        auto body = [&]() -> int {
+    auto body = [&]() -> int {
             int r = classify_parts(c, parts, ev, prm, c->graph_tol);
             if (r != WOS_OK) return r;
             hipLaunchKernelGGL(k_inside_count, dim3((unsigned)nb), dim3(kBlock), 0, c->stream,
@@ -2695,10 +2692,12 @@ int solve_graph(wos_ctx *c, const std::vector<Part> &parts, const wos_eq_desc *e
             rv.total = n * nw;
             rv.n_dev = n_in_dev;
             rv.total_dev = total_dev;
+            if (!capture) HIPX(hipEventRecord(c->ev[2], c->stream));
             HIPX(dispatch_walk(c, dim, false, ev, eq->source_kind != WOS_SRC_NONE,
                                eq->kind == WOS_EQ_SCREENED || eq->kind == WOS_EQ_HELMHOLTZ,
                                WOS_FP32_PERSISTENT, rv, prm->epsilon,
                                prm->max_steps, AdjOut{}, &c->graph_blocks));
+            if (!capture) HIPX(hipEventRecord(c->ev[3], c->stream));
             hipLaunchKernelGGL(k_welford, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, c->stream, rv,
                                c->vals.as<double>(), c->steps.as<int32_t>(), c->mean.as<double>(),
                                c->m2.as<double>(), c->count.as<double>(), c->ssum.as<double>(),
@@ -2706,20 +2705,39 @@ int solve_graph(wos_ctx *c, const std::vector<Part> &parts, const wos_eq_desc *e
             HIPX(hipGetLastError());
             return WOS_OK;
         };
+    double classify_s = 0.0, walk_s, total_walk_s;
+    if (!capture) {
+        // the same device sequence launched directly: no host round trip
+        // between classification and walks
+        HIPX(hipEventRecord(c->ev[0], c->stream));
         const int brc = body();
-        hipGraph_t g = nullptr;
-        const hipError_t ec = hipStreamEndCapture(c->stream, &g);
-        if (brc != WOS_OK) { if (g) (void)hipGraphDestroy(g); return brc; }
-        HIPX(ec);
-        c->graph = g;
-        HIPX(hipGraphInstantiate(&c->graph_exec, c->graph, nullptr, nullptr, 0));
-        c->graph_key = key;
+        if (brc != WOS_OK) return brc;
+        HIPX(hipEventRecord(c->ev[1], c->stream));
+        HIPX(hipStreamSynchronize(c->stream));
+        classify_s = elapsed(c->ev[0], c->ev[2]);
+        walk_s = elapsed(c->ev[2], c->ev[3]);
+        total_walk_s = elapsed(c->ev[2], c->ev[1]);
+    } else {
+        if (key != c->graph_key) {
+            drop_graph(c);
+            HIPX(hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
+            const int brc = body();
+            hipGraph_t g = nullptr;
+            const hipError_t ec = hipStreamEndCapture(c->stream, &g);
+            if (brc != WOS_OK) { if (g) (void)hipGraphDestroy(g); return brc; }
+            HIPX(ec);
+            c->graph = g;
+            HIPX(hipGraphInstantiate(&c->graph_exec, c->graph, nullptr, nullptr, 0));
+            c->graph_key = key;
+        }
+        HIPX(hipEventRecord(c->ev[0], c->stream));
+        HIPX(hipGraphLaunch(c->graph_exec, c->stream));
+        HIPX(hipEventRecord(c->ev[1], c->stream));
+        HIPX(hipStreamSynchronize(c->stream));
+        // the graph runs classification and walks back to back: its time is
+        // reported as both the solve and the walker time
+        walk_s = total_walk_s = elapsed(c->ev[0], c->ev[1]);
     }
-    HIPX(hipEventRecord(c->ev[0], c->stream));
-    HIPX(hipGraphLaunch(c->graph_exec, c->stream));
-    HIPX(hipEventRecord(c->ev[1], c->stream));
-    HIPX(hipStreamSynchronize(c->stream));
-    const double graph_s = elapsed(c->ev[0], c->ev[1]);
 
     int32_t n_in = 0;
     HIPX(hipMemcpy(&n_in, n_in_dev, sizeof(int32_t), hipMemcpyDeviceToHost));
@@ -2739,11 +2757,9 @@ int solve_graph(wos_ctx *c, const std::vector<Part> &parts, const wos_eq_desc *e
     }
     HIPX(hipStreamSynchronize(c->stream));
     c->last_rounds = 1;
-    // the graph runs classification and walks back to back: its time is
-    // reported as both the solve and the walker time
     return finish_solve(c, parts, c->graph_tol, n, loc, amb, bval, h_mean, h_m2, h_cnt, h_ss,
-                        h_hits, 0.0, graph_s, graph_s, c->graph_blocks, prm, out, summary,
-                        part_summaries);
+                        h_hits, classify_s, walk_s, total_walk_s, c->graph_blocks, prm, out,
+                        summary, part_summaries);
 }
 
 // ---------------------------------------------------------------------------
@@ -2833,6 +2849,22 @@ int autotune(wos_ctx *c, const std::vector<Part> &parts, const wos_eq_desc *eq,
 
 // solve_once with the autotuned launch shape when params->tune = 1 (FP32):
 // the shape cached for this problem, found by autotune() on first use.
+// A fixed-walk FP32 forward grid solve that fits one round runs the graph's
+// device sequence directly (device-side compaction of the inside points);
+// everything else takes the general path of solve_once.
+bool device_compaction_ok(const wos_ctx *c, const std::vector<Part> &parts,
+                          const wos_run_params *prm, const AdjRequest *adjr) {
+    if (!c || !c->has_mesh || !prm || adjr || prm->precision != WOS_FP32_PERSISTENT ||
+        prm->target_se != 0.0 || prm->count_units || prm->n_walks < 1)
+        return false;
+    int64_t n = 0;
+    for (const Part &p : parts) {
+        if (!p.grid) return false;
+        n += p.count;
+    }
+    return n > 0 && n * (int64_t)prm->n_walks <= ((int64_t)4 << 30) / 12;
+}
+
 int solve(wos_ctx *c, const std::vector<Part> &parts, const wos_eq_desc *eq,
           const wos_run_params *prm, wos_point_stats *out, wos_run_summary *summary,
           const AdjRequest *adjr = nullptr, wos_run_summary *part_summaries = nullptr) {
@@ -2841,9 +2873,10 @@ int solve(wos_ctx *c, const std::vector<Part> &parts, const wos_eq_desc *eq,
         if (prm->use_graph != 1) return fail(WOS_ERR_INPUT, "use_graph must be 0 or 1");
     }
     auto run = [&]() {
-        return prm && prm->use_graph ? solve_graph(c, parts, eq, prm, out, summary, part_summaries)
-                                     : solve_once(c, parts, eq, prm, out, summary, adjr,
-                                                  part_summaries);
+        if (prm && prm->use_graph) return solve_graph(c, parts, eq, prm, out, summary, part_summaries);
+        if (device_compaction_ok(c, parts, prm, adjr))
+            return solve_graph(c, parts, eq, prm, out, summary, part_summaries, false);
+        return solve_once(c, parts, eq, prm, out, summary, adjr, part_summaries);
     };
     if (!c || !prm || prm->tune == 0 || prm->precision != WOS_FP32_PERSISTENT || !c->has_mesh)
         return run();
