@@ -219,7 +219,8 @@ typedef struct wos_run_summary {
     int64_t q_leaf_prims;        /* primitives tested from BVH leaves              */
 } wos_run_summary;
 
-/* Launch shape of the FP32 persistent walker (set by the autotuner). */
+/* Launch shape of the FP32 persistent walker (set by the autotuner; a new
+ * context starts from {128, -1, 64, 0, 40, 0.2, -1, 127, 8}). */
 typedef struct wos_tuning {
     int32_t block_threads;       /* 64, 128, 192 or 256 walkers per block          */
     int32_t lds_nodes;           /* BVH nodes cached in LDS per block; -1 = all that

@@ -98,8 +98,8 @@ class Context:
             res.append(r)
         return res, self._summary(summ)
 
-    def set_tuning(self, block_threads=256, lds_nodes=-1, chunk=64, blocks_per_cu=0,
-                   min_ready=48, relax=0.2, grid_cells=-1, grid_list_max=127,
+    def set_tuning(self, block_threads=128, lds_nodes=-1, chunk=64, blocks_per_cu=0,
+                   min_ready=40, relax=0.2, grid_cells=-1, grid_list_max=127,
                    xcd_parts=8):
         t = capi.Tuning(block_threads, lds_nodes, chunk, blocks_per_cu, min_ready, relax,
                         grid_cells, grid_list_max, xcd_parts)
