@@ -244,7 +244,33 @@ def main():
     # ---- welding time loops: the compiled reference solves every step with
     # its own FieldSampler2D history and the apps' per-step seeds
     np.savez(os.path.join(HERE, "transient.npz"), **transient_fixtures(ref))
+    # the whole level-2 run (100 steps, about a minute on 8 threads)
+    np.savez(os.path.join(HERE, "transient_level2_full.npz"), **level2_full_fixture(ref))
     print("wrote", sorted(f for f in os.listdir(HERE) if f.endswith(".npz")))
+
+
+def level2_full_fixture(ref):
+    """The level-2 run (49 x 33 grid, 100 steps; 100 walks instead of 500
+    to keep the GPU's one-lane-per-point replay short), every step solved by
+    the reference with its per-step seeds: the final field and the centre
+    rise and peak of every step."""
+    from wos_amd import welding
+    c = welding.Level2Config(walks=100)
+    nx, ny, x0, x1, y0, y1 = welding.plate_grid(c)
+    rm = ref.mesh(*meshgen.rectangle(x0, x1, y0, y1))
+    pts = np.array([(welding.grid_coordinate(x0, x1, i, nx), welding.grid_coordinate(y0, y1, j, ny))
+                    for i in range(nx) for j in range(ny)])
+    prev = np.zeros((nx, ny))
+    centre, peak = [], []
+    for k in range(1, int(round(c.total_time / c.dt)) + 1):
+        eq, _ = welding.level2_equation(c, prev, k)
+        r = rm.solve_points(eq, pts, c.walks, c.epsilon, max_steps=c.max_steps,
+                            seed=welding.step_seed(c.seed, k), n_threads=8,
+                            seed_index=np.arange(len(pts)))
+        prev = np.where(r["location"] == 1, r["mean"], 0.0).reshape(nx, ny)
+        centre.append(prev[nx // 2, ny // 2])
+        peak.append(prev.max())
+    return dict(final=prev, centre=np.array(centre), peak=np.array(peak))
 
 
 def transient_cases():

@@ -199,3 +199,24 @@ def test_transient_reference_streams_equal_reference_golden(gpu_ctx, case):
         print(f"{case} step {k}: max |d| {d.max():.3e} of {np.abs(ref).max():.4g}, "
               f"bit-identical {np.mean(d == 0):.3f}")
         assert np.allclose(f, ref, rtol=1e-11, atol=1e-13), (case, k)
+
+
+def test_level2_run_on_reference_streams_equals_reference(gpu_ctx):
+    """The whole level-2 run (100 steps, 1457 points; 100 walks) on the
+    reference's streams: every step's centre and peak rise and the final
+    field equal the run the compiled reference made step by step
+    (tests/golden/transient_level2_full.npz), to last-bit rounding carried
+    through 100 steps."""
+    from wos_amd.capi import WOS_FP64_REFERENCE
+    g = np.load(os.path.join(GOLD, "transient_level2_full.npz"))
+    fields = []
+    rec, final = W.run_level2(gpu_ctx, W.Level2Config(walks=100), WOS_FP64_REFERENCE,
+                              fields=fields)
+    centre = np.array([r["center_temperature_rise"] for r in rec])
+    peak = np.array([f.max() for f in fields])
+    d = np.abs(final - g["final"])
+    print(f"final field: max |d| {d.max():.3e} of {g['final'].max():.4g}, "
+          f"bit-identical {np.mean(d == 0):.3f}; centre max |d| {np.abs(centre - g['centre']).max():.3e}")
+    assert np.allclose(final, g["final"], rtol=1e-10, atol=1e-12)
+    assert np.allclose(centre, g["centre"], rtol=1e-10, atol=1e-12)
+    assert np.allclose(peak, g["peak"], rtol=1e-10, atol=1e-12)
