@@ -458,6 +458,28 @@ def test_estimator_harmonic_and_screened(oracle):
         assert abs(r["mean"] - exact) <= 4 * r["standard_error"] + 3e-4
 
 
+def test_uniform_source_screened_ball(oracle):
+    """The uniform-ball source estimator (src/wos.cpp:33-56), the default
+    mode, on the screened unit-ball problem against its closed form.  A
+    walk's value |B| G(x, y) f carries the 1/(4 pi |x - y|) singularity of
+    the 3D Green function: P(value > v) falls like v^-3, so the variance is
+    finite but the per-walk values are strongly right-skewed (skewness about
+    15 at r = 0.81).  Enough walks (20000) make the mean near-Gaussian;
+    3e-5 allows for the icosphere's faces lying inside the unit sphere."""
+    alpha = 10.0
+    V, F, _ = meshgen.icosphere(3)
+    om = oracle.mesh(V, F)
+    eq = problems.sphere_screened_constant(alpha, 1.0)
+    for p in ([0.0, 0.0, 0.0], [0.5, 0.1, -0.2], [0.0, 0.7, 0.3]):
+        r = om.walks(eq, p, 20000, 1e-4, 1000, 1, 81)
+        rr = np.linalg.norm(p)
+        exact = (1 - (alpha / math.sinh(alpha) if rr == 0 else math.sinh(alpha * rr) / (rr * math.sinh(alpha)))) / alpha ** 2
+        assert abs(r["mean"] - exact) <= 4 * r["standard_error"] + 3e-5, p
+        v = r["values"]
+        skew = np.mean((v - v.mean()) ** 3) / v.std() ** 3
+        assert skew > 3.0, p
+
+
 # ------------------------------------------------------------------ live reference
 def test_live_reference_random_meshes(oracle, reference):
     rng = np.random.default_rng(99)
