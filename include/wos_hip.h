@@ -341,8 +341,10 @@ int wos_solve_grid(wos_ctx *ctx, int nx, int ny, int nz, const wos_eq_desc *eq,
                    wos_run_summary *summary);
 
 /* Same estimator on an explicit point list (n x dim doubles).  Point i uses
- * the seed splitmix64(seed ^ splitmix64(i)).  Replaces the direct
- * wos::wos_run calls of the welding apps (level1_stationary_heat.cpp:257). */
+ * the seed splitmix64(seed ^ splitmix64(i)).  n must be in [1, 2^31 - 1]
+ * (WOS_ERR_INPUT otherwise: the reference's grids have at least one point,
+ * src/main.cpp:233).  Replaces the direct wos::wos_run calls of the welding
+ * apps (level1_stationary_heat.cpp:257). */
 int wos_solve_points(wos_ctx *ctx, const double *points, int64_t n,
                      const wos_eq_desc *eq, const wos_run_params *params,
                      wos_point_stats *out, wos_run_summary *summary);

@@ -215,3 +215,36 @@ def test_classification_on_shallow_trees(gpu_ctx, oracle, shape):
     g, o = _run_both(gpu_ctx, om, mesh, eq, (9, 9, 9), 32, 1e-3, 17)
     _compare_fp64(g, o, shape, exact_frac=0.9)
     assert np.array_equal(g["ambiguous_ray_retries"], o["ambiguous"])
+
+
+def test_empty_outside_and_single_axis_inputs(gpu_ctx, oracle):
+    """An empty point list is refused (the reference's grids have at least
+    one point, src/main.cpp:233); a list with no inside point returns its
+    location codes and NaN statistics without walking; a grid with a single
+    point along an axis places it at the axis minimum
+    (include/wos/grid.hpp:13-18) and equals the oracle."""
+    V, F, B = M.annulus(32)
+    gpu_ctx.set_mesh(V, F, B)
+    eq = Pr.annulus_laplace()
+    for prec in (WOS_FP64_PLAIN, WOS_FP32_PERSISTENT):
+        with pytest.raises(RuntimeError):
+            gpu_ctx.solve_points(np.zeros((0, 2)), eq, Context.params(8, 1e-3, 5, precision=prec))
+        r = gpu_ctx.solve_points(np.array([[5.0, 5.0], [0.0, 0.0], [-3.0, 0.1]]), eq,
+                                 Context.params(8, 1e-3, 5, precision=prec))
+        assert list(r["location"]) == [0, 0, 0]
+        assert np.all(np.isnan(r["mean"])) and np.all(np.isnan(r["mean_steps"]))
+    om = oracle.mesh(V, F, B)
+    for grid in ((1, 9, 1), (9, 1, 1), (1, 1, 1)):
+        g = gpu_ctx.solve_grid(*grid, eq, Context.params(64, 1e-3, 5))
+        o = om.solve_grid(eq, *grid, 64, 1e-3, seed=5, rng_mode=1, n_threads=1)
+        assert np.array_equal(g["location"], o["location"]), grid
+        assert np.array_equal(g["mean"], o["mean"], equal_nan=True) or np.allclose(
+            g["mean"], o["mean"], rtol=0, atol=1e-12, equal_nan=True), grid
+    V3, F3, _ = M.icosphere(2)
+    gpu_ctx.set_mesh(V3, F3)
+    eq3 = Pr.sphere_screened_constant()
+    om3 = oracle.mesh(V3, F3)
+    g = gpu_ctx.solve_grid(1, 5, 3, eq3, Context.params(32, 1e-4, 9))
+    o = om3.solve_grid(eq3, 1, 5, 3, 32, 1e-4, seed=9, rng_mode=1, n_threads=1)
+    assert np.array_equal(g["location"], o["location"])
+    assert np.allclose(g["mean"], o["mean"], rtol=0, atol=1e-12, equal_nan=True)
