@@ -1,4 +1,5 @@
-"""The 3D screened-Poisson grid workload against the unit ball's closed form.
+"""Unit-ball closed forms for the 3D walkers: the screened-Poisson grid
+workload, and Poisson with a constant source in both source modes.
 
 Screened Poisson with alpha = 10, f = 1 and g = 0 on an icosphere of
 subdivision 4, the default uniform-ball source sampling (src/wos.cpp:33-56),
@@ -17,7 +18,8 @@ import pytest
 
 from wos_amd import meshgen as M
 from wos_amd import problems as Pr
-from wos_amd.capi import WOS_FP32_PERSISTENT, WOS_FP64_PLAIN
+from wos_amd.capi import (WOS_EQ_POISSON, WOS_FP32_PERSISTENT, WOS_FP64_PLAIN, WOS_SOURCE_GREEN,
+                          WOS_SOURCE_UNIFORM, WOS_SRC_CONSTANT, make_eq)
 from wos_amd.engine import Context
 
 pytestmark = pytest.mark.gpu
@@ -51,3 +53,27 @@ def test_screened_ball_matches_closed_form(gpu_ctx, precision):
     assert np.all(np.abs(res["mean"] - exact) <= 4.5 * se + 3e-5)
     assert np.mean(z ** 2) < 1.8
     assert abs(z.mean()) < 0.6
+
+
+@pytest.mark.parametrize("mode", [WOS_SOURCE_UNIFORM, WOS_SOURCE_GREEN])
+@pytest.mark.parametrize("precision", [WOS_FP64_PLAIN, WOS_FP32_PERSISTENT])
+def test_poisson_ball_matches_closed_form(gpu_ctx, precision, mode):
+    """Poisson with f = 1, g = 0 on the unit ball: u = (1 - r^2) / 6.  The
+    uniform mode uses the 3D Green function (R - r)/(4 pi r R) over the ball
+    (src/wos.cpp:33-56), the Green mode the mass R^2/6 and the Beta(2, 2)
+    radius (include/wos/poisson.hpp:12-51).  32768 walks per point; 3e-4
+    allows for the icosphere's faces (u' = -1/3 at the surface)."""
+    gpu_ctx.set_mesh(*M.icosphere(4))
+    rng = np.random.default_rng(7)
+    d = rng.normal(size=(64, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    r = np.sqrt(rng.uniform(0.0, 0.9 ** 2, size=64))
+    eq = make_eq(WOS_EQ_POISSON, source_kind=WOS_SRC_CONSTANT, source_mode=mode, src_value=1.0)
+    res = gpu_ctx.solve_points(d * r[:, None], eq, Context.params(32768, 1e-4, 3, precision=precision))
+    assert np.all(res["location"] == 1)
+    exact = (1.0 - r * r) / 6.0
+    se = res["standard_error"]
+    z = (res["mean"] - exact) / se
+    print(f"poisson mode {mode} precision {precision}: max|z| {np.abs(z).max():.3f} "
+          f"mean z^2 {np.mean(z ** 2):.3f} max|err| {np.abs(res['mean'] - exact).max():.3e}")
+    assert np.all(np.abs(res["mean"] - exact) <= 4.5 * se + 3e-4)
