@@ -217,6 +217,14 @@ typedef struct wos_run_summary {
     int64_t q_list_prims;        /* primitives tested from cell lists              */
     int64_t q_nodes;             /* BVH internal nodes opened                      */
     int64_t q_leaf_prims;        /* primitives tested from BVH leaves              */
+    int64_t failed_walks;        /* walks whose value is NaN: a rejection sampler
+                                    gave up (screened radius samplers after 10^5
+                                    trials, the GREEN_MIS proposal after 10^6; the
+                                    reference loops until acceptance); their points'
+                                    means are NaN                                   */
+    double grid_build_seconds;   /* FP32: seconds spent building the distance grid
+                                    in this call (0 when it was already built)      */
+    int64_t grid_bytes;          /* FP32: device bytes of the distance grid         */
 } wos_run_summary;
 
 /* Launch shape of the FP32 persistent walker (set by the autotuner; a new
@@ -238,8 +246,11 @@ typedef struct wos_tuning {
                                     with an exact search in every case.            */
     int32_t grid_cells;          /* cells per domain of the FP32 distance grid
                                     (cell lists near the boundary, cell distance
-                                    bounds inside); 0 = no grid, -1 = 2^24 for
-                                    triangle meshes and 2^22 for segment meshes    */
+                                    bounds inside); 0 = no grid, -1 = sized per
+                                    domain from its primitive count: 512 cells per
+                                    primitive rounded up to a power of two, at
+                                    least 2^16 and at most 2^24 (triangles) or
+                                    2^22 (segments)                                 */
     int32_t grid_list_max;       /* longest cell list kept (1..127); a cell with
                                     more candidate primitives keeps none           */
     int32_t xcd_parts;           /* the walk items are split into this many
@@ -348,6 +359,17 @@ int wos_solve_grid(wos_ctx *ctx, int nx, int ny, int nz, const wos_eq_desc *eq,
 int wos_solve_points(wos_ctx *ctx, const double *points, int64_t n,
                      const wos_eq_desc *eq, const wos_run_params *params,
                      wos_point_stats *out, wos_run_summary *summary);
+
+/* Point list with explicit seed indices: point i uses the seed
+ * splitmix64(seed ^ splitmix64(seed_index[i])), the reference sweep's seed of
+ * the grid point whose global linear index (i Ny + j) Nz + k is seed_index[i]
+ * (include/wos/runner.hpp:163-168).  A subset of a grid solved this way gets
+ * the grid solve's walks; this is how a rank of the reference's Cartesian
+ * block decomposition (runner.hpp:103-134) seeds its block.  Otherwise as
+ * wos_solve_points. */
+int wos_solve_points_indexed(wos_ctx *ctx, const double *points, const int64_t *seed_index,
+                             int64_t n, const wos_eq_desc *eq, const wos_run_params *params,
+                             wos_point_stats *out, wos_run_summary *summary);
 
 /* ---- adjoint (no reference counterpart: the reference has no gradients) --
  * Gradient of L with respect to the equation data, given dL/du for every

@@ -34,15 +34,21 @@ typedef struct wos_exec_opts {
 } wos_exec_opts;
 
 /* Registry size and the name of entry `equation_id` ("laplace", "poisson",
- * "screened_poisson"), NULL when out of range (src/equations/equations.hpp:26-31
- * without helmholtz). */
+ * "screened_poisson", "helmholtz"), NULL when out of range
+ * (src/equations/equations.hpp:26-31). */
 int wos_hip_equation_count(void);
 const char *wos_hip_equation_name(int equation_id);
 
 /* Equation::run_2D / run_3D (src/equations/equations.hpp:7-17) of registry
  * entry `equation_id` with dim = 2 or 3.  source_mode: 0 uniform, 1 green
  * (include/wos/source_mode.hpp:5-9).  problem: NULL for the reference's
- * built-in equation data. */
+ * built-in equation data.
+ *
+ * Threading: reentrant.  Every call creates and destroys its own GPU context
+ * and keeps its execution options per thread, so calls may run concurrently
+ * on several threads (on one GPU or several); their stdout lines interleave.
+ * Two calls must not write the same output file.  The reference calls run
+ * once per MPI rank, single-threaded. */
 int wos_hip_run(int dim, const char *mesh_filename, const char *output_filename, int Nx, int Ny,
                 int Nz, int N_walks, double epsilon, int max_steps, int max_ray_attempts,
                 uint64_t seed, double alpha, int source_mode, int equation_id,

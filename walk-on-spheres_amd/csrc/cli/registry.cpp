@@ -19,8 +19,11 @@
 
 namespace woscli {
 
+// one set of options per thread: wos_hip_run sets them on the calling
+// thread for the duration of its call, so calls on several threads (each
+// with its own context) do not see each other's precision or device
 RunOptions &run_options() {
-    static RunOptions opts;
+    thread_local RunOptions opts;
     return opts;
 }
 
@@ -261,37 +264,38 @@ const Equation screened_poisson = {"screened_poisson", screened_2d, screened_3d}
 const Equation helmholtz = {"helmholtz", helmholtz_2d, helmholtz_3d};
 
 const wos_eq_desc *named_problem(const char *name, int *kind) {
-    static wos_eq_desc annulus, sphere, star, plate;
-    static bool init = false;
-    if (!init) {
-        annulus = blank(WOS_EQ_LAPLACE);           // g = 1 on boundary id 0, 0 on id 1
-        annulus.boundary_kind = WOS_BND_ID_CONST;
-        annulus.bnd_values[0] = 1.0;
-        annulus.n_bnd_values = 2;
-        sphere = blank(WOS_EQ_SCREENED);           // f = 1, g = 0
-        sphere.source_kind = WOS_SRC_CONSTANT;
-        sphere.src_value = 1.0;
-        sphere.boundary_kind = WOS_BND_ZERO;
-        star = blank(WOS_EQ_POISSON);              // f = exp(-|x|^2 / 0.1), g = sin 3x cos 2y
-        star.source_kind = WOS_SRC_GAUSSIAN;
-        star.src_value = 1.0;
-        star.src_radius = std::sqrt(0.1);
-        star.boundary_kind = WOS_BND_SIN3X_COS2Y;
-        plate = blank(WOS_EQ_SCREENED);            // f = exp(-|x - x0|^2 / 9), g = 0
-        plate.source_kind = WOS_SRC_GAUSSIAN;
-        plate.src_value = 1.0;
-        plate.src_radius = 3.0;
-        plate.src_center[0] = 40.0;
-        plate.src_center[1] = 30.0;
-        plate.src_center[2] = 10.0;
-        plate.boundary_kind = WOS_BND_ZERO;
-        init = true;
-    }
+    struct Named { wos_eq_desc annulus, sphere, star, plate; };
+    // built once (thread-safe static initialisation)
+    static const Named P = [] {
+        Named n;
+        n.annulus = blank(WOS_EQ_LAPLACE);         // g = 1 on boundary id 0, 0 on id 1
+        n.annulus.boundary_kind = WOS_BND_ID_CONST;
+        n.annulus.bnd_values[0] = 1.0;
+        n.annulus.n_bnd_values = 2;
+        n.sphere = blank(WOS_EQ_SCREENED);         // f = 1, g = 0
+        n.sphere.source_kind = WOS_SRC_CONSTANT;
+        n.sphere.src_value = 1.0;
+        n.sphere.boundary_kind = WOS_BND_ZERO;
+        n.star = blank(WOS_EQ_POISSON);            // f = exp(-|x|^2 / 0.1), g = sin 3x cos 2y
+        n.star.source_kind = WOS_SRC_GAUSSIAN;
+        n.star.src_value = 1.0;
+        n.star.src_radius = std::sqrt(0.1);
+        n.star.boundary_kind = WOS_BND_SIN3X_COS2Y;
+        n.plate = blank(WOS_EQ_SCREENED);          // f = exp(-|x - x0|^2 / 9), g = 0
+        n.plate.source_kind = WOS_SRC_GAUSSIAN;
+        n.plate.src_value = 1.0;
+        n.plate.src_radius = 3.0;
+        n.plate.src_center[0] = 40.0;
+        n.plate.src_center[1] = 30.0;
+        n.plate.src_center[2] = 10.0;
+        n.plate.boundary_kind = WOS_BND_ZERO;
+        return n;
+    }();
     const wos_eq_desc *p = nullptr;
-    if (!std::strcmp(name, "annulus")) p = &annulus;
-    else if (!std::strcmp(name, "sphere-screened")) p = &sphere;
-    else if (!std::strcmp(name, "star-gaussian")) p = &star;
-    else if (!std::strcmp(name, "plate-gaussian")) p = &plate;
+    if (!std::strcmp(name, "annulus")) p = &P.annulus;
+    else if (!std::strcmp(name, "sphere-screened")) p = &P.sphere;
+    else if (!std::strcmp(name, "star-gaussian")) p = &P.star;
+    else if (!std::strcmp(name, "plate-gaussian")) p = &P.plate;
     if (p && kind) *kind = p->kind;
     return p;
 }

@@ -3,7 +3,8 @@
 // Same plugin table as the reference (src/equations/equations.hpp:7-31):
 // name + run_2D + run_3D with the reference's parameter list, looked up by
 // exact name.  The GPU-specific choices (precision, adaptive target, problem
-// data, device) travel in RunOptions, set once by main before dispatch.
+// data, device) travel in RunOptions, set by main (or wos_hip_run) before
+// dispatch on the thread that runs the equation; each thread has its own.
 #pragma once
 #include <cstdint>
 

@@ -82,6 +82,14 @@ __device__ __forceinline__ T normal_draw(RNG &rng) {
 template <typename T> __device__ __forceinline__ T clamp01(T v) {
     return v < (T)0 ? (T)0 : (v > (T)1 ? (T)1 : v);
 }
+// The reference's rejection loops (include/wos/screened.hpp:105-156) run
+// until a proposal is accepted.  Here a loop stops after kRadiusMaxTrials
+// and returns NaN: the walk's value becomes NaN, the point's mean shows it
+// and k_welford counts the walk into the summary's failed_walks, instead of
+// a biased radius being used silently.  (The acceptance rate is bounded away
+// from zero for every alpha R, so no walk is expected to stop here.)
+constexpr int kRadiusMaxTrials = 100000;
+
 // include/wos/screened.hpp:105-129
 // (the sphere's log K0(X) and log I0(X) are taken once, outside the
 // rejection loop: the same values in the same expression)
@@ -90,7 +98,7 @@ __device__ __forceinline__ T screened_radius_2d(T alpha, T R, RNG &rng) {
     const T X = alpha * R;
     const T lk_X = X < (T)2 ? (T)0 : log_k0<T>(X);
     const T li_X = X < (T)2 ? (T)0 : log_i0<T>(X);
-    for (int guard = 0; guard < 100000; ++guard) {
+    for (int guard = 0; guard < kRadiusMaxTrials; ++guard) {
         T r, acceptance;
         if (X < (T)2) {
             r = poisson_radius_2d<T>(R, rng);
@@ -106,13 +114,13 @@ __device__ __forceinline__ T screened_radius_2d(T alpha, T R, RNG &rng) {
         }
         if (rng.unit() < clamp01<T>(acceptance)) return r;
     }
-    return (T)0.5 * R;
+    return (T)NAN;
 }
 // include/wos/screened.hpp:131-156
 template <typename T, typename RNG>
 __device__ __forceinline__ T screened_radius_3d(T alpha, T R, RNG &rng) {
     const T X = alpha * R;
-    for (int guard = 0; guard < 100000; ++guard) {
+    for (int guard = 0; guard < kRadiusMaxTrials; ++guard) {
         T r, acceptance;
         if (X < (T)2) {
             r = poisson_radius_3d<T>(R, rng);
@@ -131,7 +139,7 @@ __device__ __forceinline__ T screened_radius_3d(T alpha, T R, RNG &rng) {
         }
         if (rng.unit() < clamp01<T>(acceptance)) return r;
     }
-    return (T)0.5 * R;
+    return (T)NAN;
 }
 
 // attempts of the truncated-Gaussian rejection sampler before it gives up
@@ -369,7 +377,7 @@ __device__ __forceinline__ void classify_tail(const MeshView &m, const EqView &e
 template <int DIM>
 __global__ __launch_bounds__(kBlock) void k_classify(MeshView m, EqView e, GridSpec g,
                                                      DomainView dom, const double *plist,
-                                                     int64_t n, uint64_t seed,
+                                                     const int64_t *sidx, int64_t n, uint64_t seed,
                                                      int max_ray_attempts, uint8_t *loc,
                                                      StartInfo *starts, int32_t *amb,
                                                      double *bvalue) {
@@ -380,7 +388,7 @@ __global__ __launch_bounds__(kBlock) void k_classify(MeshView m, EqView e, GridS
     if (i >= n) return;
     double x = 0.0, y = 0.0, z = 0.0;
     point_coords<DIM>(g, plist, i, x, y, z);
-    const uint64_t ps = point_seed(seed, (uint64_t)i);
+    const uint64_t ps = point_seed(seed, sidx ? (uint64_t)sidx[i] : (uint64_t)i);
     NpqD q;
     if (DIM == 2) q = npq_ref_2d(m, dom.root64, x, y, stk, stride);
     else q = npq_ref_3d(m, dom.root64, x, y, z, stk, stride);
@@ -429,7 +437,8 @@ __device__ __forceinline__ NpqD npq_ref_3d_sub(const MeshView &m, int root, doub
 template <int S>
 __global__ __launch_bounds__((S + 3) * 64) void k_classify3(MeshView m, EqView e, GridSpec g,
                                                           DomainView dom, const double *plist,
-                                                          int64_t n, uint64_t seed,
+                                                          const int64_t *sidx, int64_t n,
+                                                          uint64_t seed,
                                                           int max_ray_attempts, uint8_t *loc,
                                                           StartInfo *starts, int32_t *amb,
                                                           double *bvalue) {
@@ -442,7 +451,7 @@ __global__ __launch_bounds__((S + 3) * 64) void k_classify3(MeshView m, EqView e
     const int64_t i = (int64_t)blockIdx.x * 64 + lane;
     double x = 0.0, y = 0.0, z = 0.0;
     if (i < n) point_coords<3>(g, plist, i, x, y, z);
-    const uint64_t ps = point_seed(seed, (uint64_t)i);
+    const uint64_t ps = point_seed(seed, (sidx && i < n) ? (uint64_t)sidx[i] : (uint64_t)i);
     if (role < S) {
         NpqD q;
         q.d2 = INFINITY; q.px = q.py = q.pz = 0.0; q.s = q.t = 0.0; q.prim = -1; q.slot = -1;
@@ -1474,10 +1483,12 @@ __global__ __launch_bounds__(kBlock) void k_terminal(MeshView m, EqView e, Round
 // --------------------------------------------------------------------------
 // kernel 3: sequential Welford per point (include/wos/wos.hpp:169-180)
 // --------------------------------------------------------------------------
+// Walks whose value is NaN (a rejection sampler that gave up) are counted
+// into *failed; they still enter the statistics, so the point's mean is NaN.
 __global__ __launch_bounds__(kBlock) void k_welford(RoundView rv, const double *vals,
                                                     const int32_t *steps, double *mean, double *m2,
                                                     double *count, double *steps_sum,
-                                                    int32_t *hits) {
+                                                    int32_t *hits, unsigned long long *failed) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= (rv.n_dev ? *rv.n_dev : rv.n)) return;
     const int p = rv.pid[k];
@@ -1485,10 +1496,12 @@ __global__ __launch_bounds__(kBlock) void k_welford(RoundView rv, const double *
     const int cnt = rv.off ? (int)(rv.off[k + 1] - rv.off[k]) : rv.cw;
     double mu = mean[p], M2 = m2[p], n = count[p], ts = steps_sum[p];
     int h = hits[p];
+    unsigned nan_walks = 0u;
     for (int j = 0; j < cnt; ++j) {
         const size_t o = (size_t)(base + j);
         const double x = vals[o];
         const int32_t s = steps[o];
+        nan_walks += x != x ? 1u : 0u;
         n = n + 1.0;
         const double delta_before = x - mu;
         mu += delta_before / n;
@@ -1498,6 +1511,7 @@ __global__ __launch_bounds__(kBlock) void k_welford(RoundView rv, const double *
         h += (s < 0) ? 1 : 0;
     }
     mean[p] = mu; m2[p] = M2; count[p] = n; steps_sum[p] = ts; hits[p] = h;
+    if (nan_walks) atomicAdd(failed, (unsigned long long)nan_walks);
 }
 
 // --------------------------------------------------------------------------
@@ -1619,6 +1633,7 @@ struct wos_ctx {
     DevBuf nd2, nd3, segd, trid, slot_prim, prim_bid, prims, verts, nf2, nf3, segf, trif;
     DevBuf cq_out, cq_prim;
     DevBuf units;                     // FP32 query-unit counters
+    DevBuf failed;                    // walks with a NaN value (one counter)
     std::vector<std::pair<TuneKey, wos_tuning>> tuned;  // autotuned shapes
     // graph replay: the captured solve, what it was captured for, its tolerances
     hipGraph_t graph = nullptr;
@@ -1629,13 +1644,17 @@ struct wos_ctx {
     DevBuf g_bcount, g_boff, g_misc;
     bool count_units = false;
     DevBuf g_info, g_cells, g_list;   // distance grid
-    int grid_built_cells = 0;         // target cells per domain of the built grid (0: none)
+    bool grid_valid = false;          // the grid below belongs to the current mesh
+    int grid_built_cells = 0;         // the tuning's grid_cells it was built for
     int grid_built_cap = 0;           // its list cap
     float grid_built_relax = -1.0f;   // and the relax it was built for
     int64_t grid_list_len = 0;
+    int64_t grid_bytes = 0;           // device bytes of cells, lists and grid records
+    double grid_build_pending_s = 0.0;  // last build's seconds, until a summary reports them
     DevBuf doms;                      // DomainF per domain
     DevBuf starts_f, r_kblk;          // FP32 start copies; item block -> point (uneven rounds)
     DevBuf ref_state;                 // reference-stream state per inside point (8 words)
+    DevBuf sidx;                      // per-point seed indices of an indexed point list
     DevBuf plist_in, loc, starts, amb, bvalue, inside, vals, steps, mean, m2, count, ssum, hits,
         counter, vv, pid_all, r_pid, r_wbeg, r_off, coef, scal_sum, grad_v, grad_b, dl_du,
         adj_scal, adj_vw, adj_prim;
@@ -1656,26 +1675,45 @@ struct wos_ctx {
 
 namespace {
 
+// Cells of a domain's distance grid.  grid_cells > 0: that many.  -1 (the
+// default): sized from the domain's primitive count, kGridCellsPerPrim cells
+// per primitive rounded up to a power of two, clamped to
+// [kGridMinCells, kGridMaxCells(dim)] -- enough that the cells near the
+// boundary are several times smaller than a primitive (short lists) without
+// the 2^24 cells of a fine grid for a mesh of a few thousand triangles.
+constexpr int64_t kGridCellsPerPrim = 512;
+constexpr int64_t kGridMinCells = (int64_t)1 << 16;
+int64_t grid_cells_for(int tune_cells, int dim, int64_t n_prims) {
+    if (tune_cells >= 0) return tune_cells;
+    const int64_t hi = dim == 3 ? (int64_t)1 << 24 : (int64_t)1 << 22;
+    int64_t cells = kGridMinCells;
+    while (cells < hi && cells < kGridCellsPerPrim * n_prims) cells <<= 1;
+    return cells;
+}
+
 // Build the distance grid of every domain for the FP32 walker (once per mesh
-// and grid size) and point the kernel view at it; grid_cells == 0 turns it
-// off, -1 takes 2^24 cells per domain in 3D and 2^22 in 2D.  Cubic cells of
-// size h = (bbox volume / cells)^(1/dim) over the domain's bounding box
-// padded by half a cell.
+// and grid setting) and point the kernel view at it; grid_cells == 0 turns
+// it off (see grid_cells_for for the sizes).  Cubic cells of size
+// h = (bbox volume / cells)^(1/dim) over the domain's bounding box padded by
+// half a cell.  The build's seconds are kept for the next solve summary.
 hipError_t ensure_grid(wos_ctx *c) {
-    const int target = c->tune.grid_cells >= 0 ? c->tune.grid_cells
-                                               : (c->mesh.dim == 3 ? 1 << 24 : 1 << 22);
-    if (target <= 0) {
+    if (c->tune.grid_cells == 0) {
         c->view.grids = nullptr; c->view.cells = nullptr; c->view.glist = nullptr;
         return hipSuccess;
     }
-    if (c->grid_built_cells != target || c->grid_built_cap != c->tune.grid_list_max ||
-        c->grid_built_relax != c->tune.relax) {
+    if (!c->grid_valid || c->grid_built_cells != c->tune.grid_cells ||
+        c->grid_built_cap != c->tune.grid_list_max || c->grid_built_relax != c->tune.relax) {
+        hipError_t e0 = hipStreamSynchronize(c->stream);
+        if (e0 != hipSuccess) return e0;
+        const auto t0 = std::chrono::steady_clock::now();
+        c->grid_valid = false;
         const int dim = c->mesh.dim;
         const int nd = (int)c->L.domains.size();
         std::vector<GridInfo> info(nd);
         int64_t total = 0;
         for (int d = 0; d < nd; ++d) {
             const DomainInfo &D = c->L.domains[d];
+            const int64_t target = grid_cells_for(c->tune.grid_cells, dim, D.n_prims);
             double ext[3], emax = 0.0;
             for (int k = 0; k < dim; ++k) {
                 ext[k] = D.bounds[2 * k + 1] - D.bounds[2 * k];
@@ -1758,9 +1796,16 @@ hipError_t ensure_grid(wos_ctx *c) {
             }
         }
         if ((e = upload(c->g_info, info, c->stream)) != hipSuccess) return e;
-        c->grid_built_cells = target;
+        if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) return e;
+        c->grid_built_cells = c->tune.grid_cells;
         c->grid_built_cap = c->tune.grid_list_max;
         c->grid_built_relax = c->tune.relax;
+        c->grid_bytes = total * (int64_t)sizeof(GridCellF) +
+                        c->grid_list_len * (int64_t)sizeof(GridEntry) +
+                        nd * (int64_t)sizeof(GridInfo);
+        c->grid_build_pending_s =
+            std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        c->grid_valid = true;
     }
     c->view.grids = c->g_info.as<GridInfo>();
     c->view.cells = c->g_cells.as<GridCellF>();
@@ -1770,12 +1815,12 @@ hipError_t ensure_grid(wos_ctx *c) {
 
 template <typename K>
 hipError_t launch_classify(wos_ctx *c, K kernel, const EqView &ev, const GridSpec &g,
-                           const DomainView &dv, const double *plist, int64_t n, uint64_t seed,
-                           int mra, int per_block, int threads) {
+                           const DomainView &dv, const double *plist, const int64_t *sidx,
+                           int64_t n, uint64_t seed, int mra, int per_block, int threads) {
     const int blocks = (int)((n + per_block - 1) / per_block);
     const size_t lds = (size_t)threads * (c->L.depth + 2) * sizeof(int);
     hipLaunchKernelGGL(kernel, dim3(blocks), dim3(threads), lds, c->stream, c->view, ev, g, dv,
-                       plist, n, seed, mra, c->loc.as<uint8_t>(), c->starts.as<StartInfo>(),
+                       plist, sidx, n, seed, mra, c->loc.as<uint8_t>(), c->starts.as<StartInfo>(),
                        c->amb.as<int32_t>(), c->bvalue.as<double>());
     return hipGetLastError();
 }
@@ -2075,6 +2120,7 @@ struct Part {
     const GridSpec *grid;
     const double *points;
     int64_t count;
+    const int64_t *seed_index = nullptr;  // point list: per-point seed indices (else the list index)
 };
 
 // Validate a solve, build its equation view and stage its inputs.
@@ -2152,6 +2198,7 @@ int prepare_solve(wos_ctx *c, const std::vector<Part> &parts, const wos_eq_desc 
         ev.vertex_values = c->vv.as<double>();
     }
 
+    HIPX(c->failed.reserve(sizeof(unsigned long long)));
     HIPX(c->loc.reserve(n));
     HIPX(c->starts.reserve(n * sizeof(StartInfo)));
     HIPX(c->amb.reserve(n * sizeof(int32_t)));
@@ -2161,6 +2208,10 @@ int prepare_solve(wos_ctx *c, const std::vector<Part> &parts, const wos_eq_desc 
         for (const Part &p : parts)
             if (p.points) pts.insert(pts.end(), p.points, p.points + p.count * dim);
         if (!pts.empty()) HIPX(upload(c->plist_in, pts, c->stream));
+        std::vector<int64_t> sidx;
+        for (const Part &p : parts)
+            if (p.points && p.seed_index) sidx.insert(sidx.end(), p.seed_index, p.seed_index + p.count);
+        if (!sidx.empty()) HIPX(upload(c->sidx, sidx, c->stream));
     }
 
     n_out = n;
@@ -2194,9 +2245,10 @@ int classify_parts(wos_ctx *c, const std::vector<Part> &parts, const EqView &ev,
             DomainView dv{D.node_off, D.root_ref, (int)D.prim_off, (int)(D.prim_off + D.n_prims),
                           part_tol[pi], off, fp32_margin(D), p.domain};
             const double *dp = p.points ? c->plist_in.as<double>() + poff * dim : nullptr;
+            const int64_t *ds = p.seed_index ? c->sidx.as<int64_t>() + poff : nullptr;
             hipError_t le;
             if (dim == 2) {
-                le = launch_classify(c, k_classify<2>, ev, g, dv, dp, p.count, prm->seed,
+                le = launch_classify(c, k_classify<2>, ev, g, dv, dp, ds, p.count, prm->seed,
                                      prm->max_ray_attempts, kBlock, kBlock);
             } else {
                 // the start query on 8 (or 4) subtrees where the block's
@@ -2209,13 +2261,13 @@ int classify_parts(wos_ctx *c, const std::vector<Part> &parts, const EqView &ev,
                                (size_t)split * 64 * sizeof(NpqD) <= c->lds_per_block;
                 };
                 if (fits(8))
-                    le = launch_classify(c, k_classify3<8>, ev, g, dv, dp, p.count, prm->seed,
+                    le = launch_classify(c, k_classify3<8>, ev, g, dv, dp, ds, p.count, prm->seed,
                                          prm->max_ray_attempts, 64, 11 * 64);
                 else if (fits(4))
-                    le = launch_classify(c, k_classify3<4>, ev, g, dv, dp, p.count, prm->seed,
+                    le = launch_classify(c, k_classify3<4>, ev, g, dv, dp, ds, p.count, prm->seed,
                                          prm->max_ray_attempts, 64, 7 * 64);
                 else
-                    le = launch_classify(c, k_classify<3>, ev, g, dv, dp, p.count, prm->seed,
+                    le = launch_classify(c, k_classify<3>, ev, g, dv, dp, ds, p.count, prm->seed,
                                          prm->max_ray_attempts, kBlock, kBlock);
             }
             HIPX(le);
@@ -2301,6 +2353,15 @@ int finish_solve(wos_ctx *c, const std::vector<Part> &parts,
     S.launch_blocks = blocks;
     S.launch_threads = prm->precision == WOS_FP32_PERSISTENT ? c->last_threads : kBlock;
     S.lds_cached_nodes = prm->precision == WOS_FP32_PERSISTENT ? c->last_cached : 0;
+    {
+        unsigned long long nf = 0;
+        HIPX(hipMemcpyAsync(&nf, c->failed.p, sizeof(nf), hipMemcpyDeviceToHost, c->stream));
+        HIPX(hipStreamSynchronize(c->stream));
+        S.failed_walks = (int64_t)nf;
+    }
+    S.grid_build_seconds = c->grid_build_pending_s;
+    c->grid_build_pending_s = 0.0;
+    S.grid_bytes = c->grid_valid ? c->grid_bytes : 0;
     if (c->count_units) {
         unsigned long long u[7];
         HIPX(hipMemcpyAsync(u, c->units.p, sizeof(u), hipMemcpyDeviceToHost, c->stream));
@@ -2323,6 +2384,8 @@ int solve_once(wos_ctx *c, const std::vector<Part> &parts, const wos_eq_desc *eq
     int rc = prepare_solve(c, parts, eq, prm, adjr, ev, n);
     if (rc != WOS_OK) return rc;
     const int dim = c->mesh.dim;
+    // the distance grid is built (once per mesh) outside the timed walks
+    if (prm->precision == WOS_FP32_PERSISTENT) HIPX(ensure_grid(c));
     HIPX(hipEventRecord(c->ev[0], c->stream));
     std::vector<double> part_tol;
     rc = classify_parts(c, parts, ev, prm, part_tol);
@@ -2351,6 +2414,9 @@ int solve_once(wos_ctx *c, const std::vector<Part> &parts, const wos_eq_desc *eq
                            c->units.as<unsigned long long>(), 7);
         HIPX(hipGetLastError());
     }
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, c->stream,
+                       c->failed.as<unsigned long long>(), 1);
+    HIPX(hipGetLastError());
     std::vector<double> h_mean(n_in), h_m2(n_in), h_cnt(n_in), h_ss(n_in);
     std::vector<int32_t> h_hits(n_in);
     double walk_s = 0.0, total_walk_s = 0.0;
@@ -2502,7 +2568,8 @@ int solve_once(wos_ctx *c, const std::vector<Part> &parts, const wos_eq_desc *eq
             hipLaunchKernelGGL(k_welford, dim3((rv.n + 63) / 64), dim3(64), 0,
                                c->stream, rv, c->vals.as<double>(), c->steps.as<int32_t>(),
                                c->mean.as<double>(), c->m2.as<double>(), c->count.as<double>(),
-                               c->ssum.as<double>(), c->hits.as<int32_t>());
+                               c->ssum.as<double>(), c->hits.as<int32_t>(),
+                               c->failed.as<unsigned long long>());
             HIPX(hipGetLastError());
             if (adjr) {
                 hipLaunchKernelGGL(k_adj_reduce, dim3((rv.n + kBlock - 1) / kBlock), dim3(kBlock), 0,
@@ -2661,6 +2728,7 @@ int solve_graph(wos_ctx *c, const std::vector<Part> &parts, const wos_eq_desc *e
     for (const DevBuf *b : {&c->loc, &c->starts, &c->starts_f, &c->amb, &c->bvalue, &c->inside, &c->g_bcount,
                             &c->g_boff, &c->g_misc, &c->mean, &c->m2, &c->count, &c->ssum,
                             &c->hits, &c->vals, &c->steps, &c->counter, &c->pid_all, &c->vv,
+                            &c->failed,
                             &c->field, &c->term_pw, &c->term_slot})
         key_put(key, b->p);
     auto body = [&]() -> int {
@@ -2682,6 +2750,8 @@ int solve_graph(wos_ctx *c, const std::vector<Part> &parts, const wos_eq_desc *e
                                    b->as<unsigned long long>(), n);
             hipLaunchKernelGGL(k_zero_u32, dim3(256), dim3(256), 0, c->stream,
                                c->hits.as<uint32_t>(), n);
+            hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, c->stream,
+                               c->failed.as<unsigned long long>(), 1);
             HIPX(hipGetLastError());
             RoundView rv{};
             rv.starts = c->starts.as<StartInfo>();
@@ -2704,7 +2774,7 @@ int solve_graph(wos_ctx *c, const std::vector<Part> &parts, const wos_eq_desc *e
             hipLaunchKernelGGL(k_welford, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, c->stream, rv,
                                c->vals.as<double>(), c->steps.as<int32_t>(), c->mean.as<double>(),
                                c->m2.as<double>(), c->count.as<double>(), c->ssum.as<double>(),
-                               c->hits.as<int32_t>());
+                               c->hits.as<int32_t>(), c->failed.as<unsigned long long>());
             HIPX(hipGetLastError());
             return WOS_OK;
         };
@@ -3000,6 +3070,12 @@ int wos_ctx_set_domains(wos_ctx *c, int dim, int n_domains, const double *const 
     std::vector<const HostBVH *> bp;
     for (int d = 0; d < n_domains; ++d) { mp.push_back(&meshes[d]); bp.push_back(&bvhs[d]); }
     DeviceLayouts L = make_layouts(mp, bp);
+    // From here on the old mesh's buffers are replaced in place: until every
+    // upload has succeeded the context holds no mesh, so a failed upload
+    // leaves it refusing solves instead of reading freed buffers.
+    c->has_mesh = false;
+    c->grid_valid = false;
+    drop_graph(c);
     HIPX(hipSetDevice(c->device));
     hipStream_t s = c->stream;
     if (dim == 2) {
@@ -3036,9 +3112,7 @@ int wos_ctx_set_domains(wos_ctx *c, int dim, int n_domains, const double *const 
     c->mesh = std::move(all);
     c->L = std::move(L);
     c->has_mesh = true;
-    c->grid_built_cells = 0;
     c->tuned.clear();
-    drop_graph(c);
     return WOS_OK;
 }
 
@@ -3157,6 +3231,16 @@ int wos_solve_points(wos_ctx *c, const double *points, int64_t n, const wos_eq_d
                      wos_run_summary *summary) {
     if (!points) return fail(WOS_ERR_INPUT, "points is NULL");
     return solve(c, {Part{0, nullptr, points, n}}, eq, params, out, summary);
+}
+
+int wos_solve_points_indexed(wos_ctx *c, const double *points, const int64_t *seed_index,
+                             int64_t n, const wos_eq_desc *eq, const wos_run_params *params,
+                             wos_point_stats *out, wos_run_summary *summary) {
+    if (!points) return fail(WOS_ERR_INPUT, "points is NULL");
+    if (!seed_index) return fail(WOS_ERR_INPUT, "seed_index is NULL");
+    Part p{0, nullptr, points, n};
+    p.seed_index = seed_index;
+    return solve(c, {p}, eq, params, out, summary);
 }
 
 int wos_closest_points(wos_ctx *c, int domain, const double *points, int64_t n, int precision,

@@ -139,6 +139,9 @@ class RunSummary(C.Structure):
         ("q_list_prims", C.c_int64),
         ("q_nodes", C.c_int64),
         ("q_leaf_prims", C.c_int64),
+        ("failed_walks", C.c_int64),
+        ("grid_build_seconds", C.c_double),
+        ("grid_bytes", C.c_int64),
     ]
 
 
@@ -219,6 +222,10 @@ def load(path: str | None = None):
     lib.wos_solve_points.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.c_int64,
                                      C.POINTER(EqDesc), C.POINTER(RunParams),
                                      C.POINTER(PointStats), C.POINTER(RunSummary)]
+    lib.wos_solve_points_indexed.argtypes = [C.c_void_p, C.POINTER(C.c_double),
+                                             C.POINTER(C.c_int64), C.c_int64,
+                                             C.POINTER(EqDesc), C.POINTER(RunParams),
+                                             C.POINTER(PointStats), C.POINTER(RunSummary)]
     for name in ("wos_adjoint_points",):
         getattr(lib, name).argtypes = [C.c_void_p, C.POINTER(C.c_double), C.c_int64,
                                        C.POINTER(EqDesc), C.POINTER(RunParams),

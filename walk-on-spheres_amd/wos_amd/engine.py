@@ -228,10 +228,18 @@ class Context:
             nz = 1
         return self._solve(self.lib.wos_solve_grid, nx * ny * nz, eq, prm, nx, ny, nz)
 
-    def solve_points(self, points, eq: EqDesc, prm: RunParams):
+    def solve_points(self, points, eq: EqDesc, prm: RunParams, seed_index=None):
+        """Point i is seeded by its list index, or by seed_index[i] (the
+        global grid index whose seed it takes, wos_solve_points_indexed)."""
         pts = np.ascontiguousarray(points, np.float64)
-        return self._solve(self.lib.wos_solve_points, len(pts), eq, prm,
-                           ptr(pts, C.c_double), len(pts))
+        if seed_index is None:
+            return self._solve(self.lib.wos_solve_points, len(pts), eq, prm,
+                               ptr(pts, C.c_double), len(pts))
+        si = np.ascontiguousarray(seed_index, np.int64)
+        if si.shape != (len(pts),):
+            raise ValueError("seed_index must have one entry per point")
+        return self._solve(self.lib.wos_solve_points_indexed, len(pts), eq, prm,
+                           ptr(pts, C.c_double), ptr(si, C.c_int64), len(pts))
 
 
 __all__ = ["Context", "screened_epsilon", "WOS_FP64_PLAIN", "WOS_FP32_PERSISTENT"]
