@@ -198,28 +198,35 @@ def _cpu_run(kind, rmesh, eq, pts, walks, eps, seed, threads, seed_index):
     return dt, r
 
 
-def cpu_sample(mesh, eq, pts, seed_index, eps, seed, budget_s, max_walks):
-    """Time the reference on `pts` with walks sized to about budget_s."""
+def cpu_sample(mesh, eq, pts, seed_index, eps, seed, budget_s, walks, min_points=None):
+    """Time the reference on a prefix of `pts` (every point at the workload's
+    own `walks`), as many points as fit about budget_s (at least
+    `min_points`, all of them if they fit); a short pilot sizes it."""
     pyoracle, lib, kind = _ref_lib()
     cpus = host_cpus()
     rmesh = lib.mesh(*mesh)
-    walks = 16
-    dt, r = _cpu_run(kind, rmesh, eq, pts, walks, eps, seed, cpus["threads"], seed_index)
-    walks = int(min(max_walks, max(16, walks * budget_s / max(dt, 1e-3))))
-    dt, r = _cpu_run(kind, rmesh, eq, pts, walks, eps, seed, cpus["threads"], seed_index)
+    n = len(pts)
+    pilot = min(n, 4 * cpus["threads"])
+    pw = max(1, min(walks, 64))
+    dt, _ = _cpu_run(kind, rmesh, eq, pts[:pilot], pw, eps, seed, cpus["threads"],
+                     None if seed_index is None else seed_index[:pilot])
+    per_point = max(dt, 1e-4) / pilot * walks / pw
+    m = int(min(n, max(min_points or cpus["threads"], budget_s / per_point)))
+    si = None if seed_index is None else seed_index[:m]
+    dt, r = _cpu_run(kind, rmesh, eq, pts[:m], walks, eps, seed, cpus["threads"], si)
     ins = r["location"] == 1
     steps = float(np.sum(r["mean_steps"][ins]) * walks)
     return dict(kind=kind, cpus=cpus, walks=walks, seconds=dt, steps=steps, result=r,
-                inside=int(ins.sum()), t_step=dt / max(steps, 1.0))
+                points=m, inside=int(ins.sum()), t_step=dt / max(steps, 1.0))
 
 
-def cpu_baseline_grid(mesh, eq, pts_all, inside_idx, var_ref, eps, seed, budget_s=10.0,
-                      max_walks=4096, n_sample=256):
+def cpu_baseline_grid(mesh, eq, pts_all, inside_idx, var_ref, eps, seed, walks, budget_s=12.0):
     """Grid pts/s at SE <= 1e-3 of the reference on a sample of the inside
-    points (every k-th), by the same cost model as the GPU forms."""
-    stride = max(1, len(inside_idx) // n_sample)
-    pick = inside_idx[::stride]
-    s = cpu_sample(mesh, eq, pts_all[pick], pick, eps, seed, budget_s, max_walks)
+    points (spread over the grid: a fixed random order), at the workload's
+    walks per point, by the same cost model as the GPU forms."""
+    cand = inside_idx[np.random.default_rng(0).permutation(len(inside_idx))]
+    s = cpu_sample(mesh, eq, pts_all[cand], cand, eps, seed, budget_s, walks)
+    pick = cand[:s["points"]]
     ms = s["result"]["mean_steps"]
     n_p = np.maximum(1.0, np.ceil(var_ref[np.searchsorted(inside_idx, pick)] / SE_TARGET ** 2))
     pts_s = len(pick) / float(np.sum(n_p * s["t_step"] * ms))
@@ -229,7 +236,7 @@ def cpu_baseline_grid(mesh, eq, pts_all, inside_idx, var_ref, eps, seed, budget_
                 cgroup_cpu_quota=c["cgroup_quota"],
                 steps_per_s=s["steps"] / s["seconds"],
                 walks_per_s=len(pick) * s["walks"] / s["seconds"],
-                sample=f"{len(pick)} of {len(inside_idx)} inside points (every {stride}th), "
+                sample=f"{len(pick)} of {len(inside_idx)} inside points (random), "
                        f"{s['walks']} walks each, {s['seconds']:.1f} s on {c['threads']} threads")
 
 
@@ -317,7 +324,7 @@ def grid_workload(key, Context, capi, mesh, eq, grid, nw, eps, seed, steps, warm
             inside_idx = np.nonzero(ins)[0]
             pts_all = _grid_points(s32, grid, dim)
             entry["cpu_baseline"] = cpu_baseline_grid((V, F, B), eq, pts_all, inside_idx, var_ref,
-                                                      eps, seed)
+                                                      eps, seed, nw)
             entry["vs_cpu"] = v32 / entry["cpu_baseline"]["value"]
     ctx.close()
     return entry, r32
@@ -449,7 +456,7 @@ def workload_torus(Context, meshgen, problems, capi, args, want_cpu, walks=2048)
     f64 = ctx.solve_points(pts, eq, p64)
     e["fp64_forward_walks_per_s"] = f64["summary"]["total_walks"] / (time.perf_counter() - t0)
     if want_cpu:
-        s = cpu_sample((V, F, None), eq, pts[:256], None, 1e-3, 3, 10.0, walks)
+        s = cpu_sample((V, F, None), eq, pts, None, 1e-3, 3, 10.0, walks)
         c = s["cpus"]
         e["cpu_baseline"] = dict(
             value=s["inside"] * s["walks"] / s["seconds"], unit="forward walks/s",
@@ -458,7 +465,7 @@ def workload_torus(Context, meshgen, problems, capi, args, want_cpu, walks=2048)
             steps_per_s=s["steps"] / s["seconds"],
             note="the reference has neither an adjoint nor per-vertex boundary data: its "
                  "forward walks/s on the same mesh, probes and walks (g = 0 at the exit)",
-            sample=f"256 of 10000 probes ({s['inside']} inside), {s['walks']} walks each, "
+            sample=f"{s['points']} of 10000 probes ({s['inside']} inside), {s['walks']} walks each, "
                    f"{s['seconds']:.1f} s on {c['threads']} threads")
         e["vs_cpu_forward"] = e["forward_walks_per_s"] / e["cpu_baseline"]["value"]
         e["vs_cpu_adjoint"] = e["value"] / e["cpu_baseline"]["value"]
@@ -512,28 +519,30 @@ def workload_stars(Context, meshgen, problems, capi, args, want_cpu):
                                tot["walk_kernel_seconds"], units, 2, True),
              domains=per)
     if want_cpu:
-        # the reference, one domain after the other, on 32 inside points of
-        # each: its time per walk step there, times the walks the adaptive
-        # run took per domain and the reference's own mean steps
-        est, walks_used, secs = 0.0, None, 0.0
+        # the reference, one domain after the other, on random inside points
+        # of each at 1024 walks: its time per walk step there, times the
+        # walks the adaptive run took per domain and the reference's own
+        # mean steps
+        est, walks_used, secs, npts = 0.0, None, 0.0, 0
         c = None
         for f, r, d in zip(fam, res, per):
             ins = np.nonzero(r["location"] == 1)[0]
-            pick = ins[:: max(1, len(ins) // 32)][:32]
+            pick = ins[np.random.default_rng(0).permutation(len(ins))]
             pts = _grid_points(r["summary"], (256, 256, 1), 2)[pick]
-            s = cpu_sample(f["mesh"], eq, pts, pick, 1e-3, 11, 0.6, 4096)
+            s = cpu_sample(f["mesh"], eq, pts, pick, 1e-3, 11, 0.75, 1024)
             c = s["cpus"]
             ms = float(np.mean(s["result"]["mean_steps"][s["result"]["location"] == 1]))
             est += s["t_step"] * ms * d["total_walks"]
             secs += s["seconds"]
             walks_used = s["walks"]
+            npts += s["points"]
         e["cpu_baseline"] = dict(
             value=reached / est, unit="pts/s", cores=c["threads"], kind="reference",
             host_cores_affinity=c["affinity"], host_cores_total=c["total"],
             cgroup_cpu_quota=c["cgroup_quota"], estimated_seconds=est,
-            sample=f"32 inside points per domain (16 domains), about {walks_used} walks each, "
-                   f"{secs:.1f} s on {c['threads']} threads; time per step x the adaptive "
-                   f"run's walks per domain x the reference's mean steps")
+            sample=f"{npts} inside points over the 16 domains (random), {walks_used} walks "
+                   f"each, {secs:.1f} s on {c['threads']} threads; time per step x the "
+                   f"adaptive run's walks per domain x the reference's mean steps")
         e["vs_cpu"] = e["value"] / e["cpu_baseline"]["value"]
     ctx.close()
     return e

@@ -247,10 +247,10 @@ typedef struct wos_tuning {
     int32_t grid_cells;          /* cells per domain of the FP32 distance grid
                                     (cell lists near the boundary, cell distance
                                     bounds inside); 0 = no grid, -1 = sized per
-                                    domain from its primitive count: 512 cells per
-                                    primitive rounded up to a power of two, at
-                                    least 2^16 and at most 2^24 (triangles) or
-                                    2^22 (segments)                                 */
+                                    domain from its primitive count: 4096 cells
+                                    per triangle or 512 per segment, rounded up to
+                                    a power of two, at least 2^16 and at most 2^24
+                                    (triangles) or 2^22 (segments)                  */
     int32_t grid_list_max;       /* longest cell list kept (1..127); a cell with
                                     more candidate primitives keeps none           */
     int32_t xcd_parts;           /* the walk items are split into this many

@@ -49,6 +49,7 @@ class Oracle:
         L.orc_xoshiro_seed.argtypes = [_U64, C.c_uint64]
         L.orc_xoshiro_next.argtypes = [_U64]
         L.orc_philox4x32_10.argtypes = [_U32, _U32, _U32]
+        L.orc_counter_draws.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int, _D]
         L.orc_point_seed.restype = C.c_uint64
         L.orc_point_seed.argtypes = [C.c_uint64, C.c_uint64]
         for name in ("orc_i0_scaled", "orc_k0_scaled", "orc_log_i0", "orc_log_k0"):
@@ -111,6 +112,11 @@ class Oracle:
         o = np.zeros(4, np.uint32)
         self.L.orc_philox4x32_10(_p(c, _U32), _p(k, _U32), _p(o, _U32))
         return o
+
+    def counter_draws(self, stream_seed, walk, step, first, n):
+        out = np.zeros(n)
+        self.L.orc_counter_draws(stream_seed, walk, step, first, n, _p(out, _D))
+        return out
 
     # ---- geometry
     def npq_seg(self, s0, s1, p):

@@ -83,51 +83,70 @@ static const uint64_t WOS_STREAM = 0x13198A2E03707344ULL;     /* runner.hpp:208 
 static const uint64_t SOURCE_STREAM = 0xA4093822299F31D0ULL;  /* runner.hpp:211 */
 
 /* A draw stream: either a reference xoshiro256++ stream (prng.hpp) or the
- * counter stream.  Counter layout: Philox(ctr = {walk, step, block, tag},
- * key = stream seed), each call yielding two u64 (x0|x1<<32, x2|x3<<32).
- * tag 0 = the walk step, tag 1 = source sampling; the block index advances
- * within one step for rejection samplers. */
+ * counter stream of one walk step.  Counter layout (key = the point's walk
+ * stream seed): uniform k = 0, 1, 2, ... of step s of walk w takes its top
+ * 32 bits from word k mod 4 of Philox(ctr = {w, s, k / 4, 0}) and its next
+ * 21 bits from the top of word k mod 4 of Philox(ctr = {w, s, k / 4, 1}).
+ * Slots 0 .. dim-2 are the step direction, the source term draws from slot
+ * dim-1 on; a uniform ball point draws its radius first. */
 typedef struct orc_rng {
     int mode;
     uint64_t *xo;
     uint32_t key[2];
-    uint32_t walk, step, block, tag;
-    uint64_t pending;
-    int has_pending;
+    uint32_t walk, step, next, held;  /* held: block index + 1 in hi/lo (0: none) */
+    uint32_t hi[4], lo[4];
 } orc_rng;
 
-static uint64_t rng_u64(orc_rng *r) {
-    if (r->mode == ORC_RNG_REFERENCE) return orc_xoshiro_next(r->xo);
-    if (r->has_pending) { r->has_pending = 0; return r->pending; }
-    uint32_t ctr[4] = {r->walk, r->step, r->block, r->tag}, o[4];
-    r->block++;
-    orc_philox4x32_10(ctr, r->key, o);
-    r->pending = (uint64_t)o[2] | ((uint64_t)o[3] << 32);
-    r->has_pending = 1;
-    return (uint64_t)o[0] | ((uint64_t)o[1] << 32);
+/* the 53 bits of the next counter-stream uniform */
+static uint64_t rng_bits53(orc_rng *r) {
+    const uint32_t k = r->next++, blk = k >> 2;
+    if (r->held != blk + 1) {
+        uint32_t ctr[4] = {r->walk, r->step, blk, 0};
+        orc_philox4x32_10(ctr, r->key, r->hi);
+        ctr[3] = 1;
+        orc_philox4x32_10(ctr, r->key, r->lo);
+        r->held = blk + 1;
+    }
+    return ((uint64_t)r->hi[k & 3] << 21) | (r->lo[k & 3] >> 11);
 }
 
-/* include/wos/prng.hpp:25-27 */
+/* include/wos/prng.hpp:25-27; the counter stream: k 2^-53 of its 53 bits */
 static double rng_unit(orc_rng *r) {
-    return (double)(rng_u64(r) >> 11) * (1.0 / (double)(1ULL << 53));
+    if (r->mode == ORC_RNG_REFERENCE)
+        return (double)(orc_xoshiro_next(r->xo) >> 11) * (1.0 / (double)(1ULL << 53));
+    return (double)rng_bits53(r) * (1.0 / (double)(1ULL << 53));
 }
 
 /* include/wos/prng.hpp:31-37 (reference: reject zero).  The counter stream
- * maps to the cell midpoint instead, (2k+1) 2^-53 with k the top 52 bits. */
+ * maps to the cell midpoint instead, (2j+1) 2^-53 with j its top 52 bits. */
 static double rng_unit_open(orc_rng *r) {
     if (r->mode == ORC_RNG_REFERENCE) {
         double v;
         do { v = rng_unit(r); } while (v == 0.0);
         return v;
     }
-    const uint64_t x = rng_u64(r);
-    return (double)(((x >> 12) << 1) | 1ULL) * (1.0 / (double)(1ULL << 53));
+    return (double)(rng_bits53(r) | 1ULL) * (1.0 / (double)(1ULL << 53));
 }
 
-static void rng_begin_step(orc_rng *r, uint32_t step) {
+/* counter stream of step `step` from slot `first` on */
+static void rng_begin_step(orc_rng *r, uint32_t step, uint32_t first) {
     r->step = step;
-    r->block = 0;
-    r->has_pending = 0;
+    r->next = first;
+    r->held = 0;
+}
+
+/* The first n uniforms of a counter step stream from slot `first` on (FP64
+ * values; exported for the tests of the stream layout). */
+void orc_counter_draws(uint64_t stream_seed, uint32_t walk, uint32_t step, uint32_t first,
+                       int n, double *out) {
+    orc_rng r;
+    memset(&r, 0, sizeof r);
+    r.mode = ORC_RNG_COUNTER;
+    r.key[0] = (uint32_t)stream_seed;
+    r.key[1] = (uint32_t)(stream_seed >> 32);
+    r.walk = walk;
+    rng_begin_step(&r, step, first);
+    for (int i = 0; i < n; ++i) out[i] = rng_unit(&r);
 }
 
 /* ===================================================================== */
@@ -1035,15 +1054,32 @@ static void eq_sample_green(const wos_eq_desc *eq, int dim, const double *c, dou
     }
 }
 
-/* src/wos.cpp:33-56 (uniform point in the ball) */
+/* src/wos.cpp:33-56 (uniform point in the ball); the counter stream draws
+ * the radius first, the reference's stream the direction first */
 static void sphere_sample(int dim, const double *c, double R, orc_rng *rng, double *out) {
+    const int radius_first = rng->mode != ORC_RNG_REFERENCE;
     if (dim == 2) {
-        const double theta = rng_unit(rng) * 2 * ORC_PI;
-        const double r = R * sqrt(rng_unit_open(rng));
+        double theta, r;
+        if (radius_first) {
+            r = R * sqrt(rng_unit_open(rng));
+            theta = rng_unit(rng) * 2 * ORC_PI;
+        } else {
+            theta = rng_unit(rng) * 2 * ORC_PI;
+            r = R * sqrt(rng_unit_open(rng));
+        }
         out[0] = c[0] + r * cos(theta);
         out[1] = c[1] + r * sin(theta);
     } else {
-        const double u = rng_unit(rng), v = rng_unit(rng), w = rng_unit_open(rng);
+        double u, v, w;
+        if (radius_first) {
+            w = rng_unit_open(rng);
+            u = rng_unit(rng);
+            v = rng_unit(rng);
+        } else {
+            u = rng_unit(rng);
+            v = rng_unit(rng);
+            w = rng_unit_open(rng);
+        }
         const double z = 2 * u - 1, phi = 2 * ORC_PI * v;
         const double surf = sqrt(1 - z * z), vol = R * cbrt(w);
         out[0] = c[0] + vol * surf * cos(phi);
@@ -1182,8 +1218,8 @@ static orc_sample wos_sample(const orc_bvh *b, const wos_eq_desc *eq, const doub
     orc_proposal prop = {0.0, 0.0, {0.0, 0.0}, {0.0, 0.0}};
     if (mis) mis_proposal(eq, &prop);
     while (R > epsilon && steps < max_steps) {
-        rng_begin_step(walk, (uint32_t)steps);
-        rng_begin_step(source, (uint32_t)steps);
+        rng_begin_step(walk, (uint32_t)steps, 0);
+        rng_begin_step(source, (uint32_t)steps, (uint32_t)(dim - 1));
         if (has_source) {
             const int may = dim == 2 ? eq_source_may_intersect(eq, p, R) : 1;
             if (may) {
@@ -1257,8 +1293,6 @@ void orc_wos_run(const orc_bvh *b, const wos_eq_desc *eq, const double *p0, int 
     } else {
         walk.key[0] = source.key[0] = (uint32_t)walk_seed;
         walk.key[1] = source.key[1] = (uint32_t)(walk_seed >> 32);
-        walk.tag = 0;
-        source.tag = 1;
     }
     double mean = 0.0, m2 = 0.0, total_steps = 0.0;
     int hits = 0;

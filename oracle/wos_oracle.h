@@ -106,6 +106,10 @@ int orc_classify(const orc_bvh *bvh, const double *p, double distance, double to
 
 /* ---------------- estimator ---------------- */
 enum { ORC_RNG_REFERENCE = 0, ORC_RNG_COUNTER = 1 };
+/* The first n uniforms (FP64) of the counter stream of step `step` of walk
+ * `walk` under key `stream_seed`, from slot `first` on. */
+void orc_counter_draws(uint64_t stream_seed, uint32_t walk, uint32_t step, uint32_t first,
+                       int n, double *out);
 
 /* opt-in tally of walk-step query work: {node records read, primitive
  * tests, queries} */

@@ -116,3 +116,83 @@ def test_registry_entry_point_matches_library(tmp_path):
         want = ctx.solve_grid(10, 9, 1, eq if eq is not None else Pr.laplace_builtin(2),
                               Context.params(24, 1e-3, 77, precision=prec))
         assert np.array_equal(mean, want["mean"], equal_nan=True), label
+
+
+def test_registry_entry_point_is_reentrant(tmp_path):
+    """Two threads call wos_hip_run at once with different execution options
+    (FP64 and FP32, different seeds and meshes); each writes what it writes
+    when it runs alone (the options are per thread, each call has its own
+    context)."""
+    import threading
+    C = ctypes
+    from wos_amd.capi import WOS_FP32_PERSISTENT, WOS_FP64_PLAIN
+    if not os.path.exists(H5DUMP):
+        pytest.skip("h5dump not available")
+    lib = C.CDLL(os.path.join(ROOT, "walk-on-spheres_amd", "csrc", "libwos_run.so"))
+    lib.wos_hip_run.restype = C.c_int
+    jobs = []
+    for k, (mesh_fn, prec, seed) in enumerate(((lambda: M.annulus(64), WOS_FP64_PLAIN, 11),
+                                               (lambda: M.star(256, 0.3, 5), WOS_FP32_PERSISTENT, 12))):
+        V, F, B = mesh_fn()
+        path = tmp_path / f"m{k}.obj"
+        M.write_obj(str(path), V, F, B)
+        jobs.append((str(path), ExecOpts(prec, 0, 0.0, 0, 0), seed))
+
+    def run(k, tag, rcs):
+        path, opts, seed = jobs[k]
+        out = str(tmp_path / f"{tag}_{k}.h5")
+        rcs[(tag, k)] = lib.wos_hip_run(2, path.encode(), out.encode(), 20, 18, 1, 64,
+                                        C.c_double(1e-3), 1000, 12, C.c_uint64(seed),
+                                        C.c_double(5.0), 0, 1, None, C.byref(opts))
+
+    rcs = {}
+    for k in range(2):
+        run(k, "alone", rcs)
+    for rep in range(3):
+        th = [threading.Thread(target=run, args=(k, f"both{rep}", rcs)) for k in range(2)]
+        for t in th:
+            t.start()
+        for t in th:
+            t.join(timeout=120)
+        assert not any(t.is_alive() for t in th)
+    assert all(rc == 0 for rc in rcs.values()), rcs
+    for k in range(2):
+        want = h5_array(str(tmp_path / f"alone_{k}.h5"), "/mean", "<f8", str(tmp_path))
+        for rep in range(3):
+            got = h5_array(str(tmp_path / f"both{rep}_{k}.h5"), "/mean", "<f8", str(tmp_path))
+            assert np.array_equal(got, want, equal_nan=True), (k, rep)
+
+
+def test_two_contexts_on_two_threads():
+    """Two contexts on one GPU, driven from two threads at once: each
+    context's solves equal its solves run alone."""
+    import threading
+    from wos_amd.capi import WOS_FP32_PERSISTENT, WOS_FP64_PLAIN
+    probs = [(M.icosphere(3), Pr.sphere_screened_constant(), (10, 10, 10), WOS_FP32_PERSISTENT),
+             (M.annulus(128), Pr.annulus_laplace(), (40, 40, 1), WOS_FP64_PLAIN)]
+    ctxs = []
+    for mesh, _, _, _ in probs:
+        c = Context(0)
+        c.set_mesh(*mesh)
+        ctxs.append(c)
+    alone = [c.solve_grid(*g, eq, Context.params(128, 1e-3, 5, precision=p))["mean"]
+             for c, (_, eq, g, p) in zip(ctxs, probs)]
+    got = {}
+
+    def run(k):
+        _, eq, g, p = probs[k]
+        got[k] = [ctxs[k].solve_grid(*g, eq, Context.params(128, 1e-3, 5, precision=p))["mean"]
+                  for _ in range(4)]
+
+    th = [threading.Thread(target=run, args=(k,)) for k in range(2)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join(timeout=120)
+    assert not any(t.is_alive() for t in th)
+    for k in range(2):
+        assert len(got[k]) == 4
+        for m in got[k]:
+            assert np.array_equal(m, alone[k], equal_nan=True), k
+    for c in ctxs:
+        c.close()

@@ -100,9 +100,24 @@ def test_annulus_fp32_tracks_fp64_and_analytic(gpu_ctx):
     X, Y = np.meshgrid(xs, xs, indexing="ij")
     r = np.sqrt(X ** 2 + Y ** 2).ravel()
     exact = np.log(r[ins]) / np.log(0.25)
-    err = np.abs(b["mean"][ins] - exact)
-    print(f"fp32 vs exact: max err={err.max():.3e} mean err={err.mean():.3e}")
-    assert np.all(err <= 4.0 * b["standard_error"][ins] + 6e-3)
+    # 2904 points at 1024 walks: a distributional check on the z-scores of
+    # FP32 and FP64 against u, with a bias budget of 2e-3 (epsilon shell
+    # and 128-gon sagitta, both below 1e-3 here) taken off each error
+    for label, res in (("fp32", b), ("fp64", a)):
+        err = res["mean"][ins] - exact
+        se = res["standard_error"][ins]
+        live = se > 0
+        zz = np.maximum(np.abs(err[live]) - 2e-3, 0.0) / se[live]
+        k = int(np.argmax(np.abs(err)))
+        print(f"{label} vs exact: max err={np.abs(err).max():.3e} (se {se[k]:.3e}, "
+              f"u {exact[k]:.4f}) mean err={np.abs(err).mean():.3e} max z={zz.max():.2f} "
+              f"mean z^2={np.mean(zz ** 2):.3f} |z|>3: {np.mean(zz > 3):.4f}")
+        # SE = 0: every walk ended at once (start within epsilon: g at the
+        # nearest point, off u by at most |grad u| eps = eps / (0.25 ln 4)),
+        # or all n walks exited on one circle (likely only where u or 1 - u
+        # is below a few / n: P = (1 - p)^n, e^-8 at p = 8 / n)
+        assert np.all(np.abs(err[~live]) <= 4e-3 + 8.0 / n)
+        assert zz.max() <= 5.0 and np.mean(zz ** 2) <= 1.2 and np.mean(zz > 3) <= 0.01
 
 
 def test_icosphere_codes_and_fp64(gpu_ctx, oracle):

@@ -71,6 +71,29 @@ def test_philox_stream_is_uniform(oracle):
     assert stats.kstest(u, "uniform").pvalue > 1e-3
 
 
+def test_counter_stream_layout_and_uniformity(oracle):
+    """Uniform k of a step stream: top 32 bits from word k % 4 of the block
+    {walk, step, k // 4, 0}, next 21 bits from the top of the same word of
+    block {walk, step, k // 4, 1}; the FP32 draw (top 24 bits) is the FP64
+    draw rounded down to 2^-24.  The draws are uniform (KS) and slots that
+    share a block are uncorrelated."""
+    from scipy import stats
+    seed = 0x0123456789ABCDEF
+    key = [seed & 0xFFFFFFFF, seed >> 32]
+    d = oracle.counter_draws(seed, 7, 3, 0, 9)
+    for k in range(9):
+        hi = int(oracle.philox([7, 3, k // 4, 0], key)[k % 4])
+        lo = int(oracle.philox([7, 3, k // 4, 1], key)[k % 4])
+        assert d[k] == ((hi << 21) | (lo >> 11)) / 2.0 ** 53
+        assert np.floor(d[k] * 2.0 ** 24) / 2.0 ** 24 == (hi >> 8) / 2.0 ** 24
+    # a stream started at slot 2 continues the same sequence
+    assert np.array_equal(oracle.counter_draws(seed, 7, 3, 2, 7), d[2:])
+    u = np.array([oracle.counter_draws(seed, w, 5, 0, 3) for w in range(4000)])
+    for j in range(3):
+        assert stats.kstest(u[:, j], "uniform").pvalue > 1e-3
+    assert abs(np.corrcoef(u[:, 0], u[:, 2])[0, 1]) < 0.05
+
+
 # ------------------------------------------------------------------ geometry
 def test_bvh_matches_reference_nodes(oracle):
     g = load("bvh")

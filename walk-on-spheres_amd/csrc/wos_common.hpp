@@ -68,11 +68,11 @@ struct Xoshiro {
 };
 
 // Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as
-// easy as 1, 2, 3", SC'11).  Counter {walk, step, block, tag}, key = the
-// point's walk-stream seed.  Two u64 per call.
-struct U64x2 { uint64_t a, b; };
+// easy as 1, 2, 3", SC'11): the four 32-bit words of the block at counter
+// {c0, c1, c2, c3} under key {k0, k1}.
+struct Words4 { uint32_t w0, w1, w2, w3; };
 
-WOS_HD U64x2 philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0, uint32_t k1) {
+WOS_HD Words4 philox4(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0, uint32_t k1) {
 #pragma unroll
     for (int r = 0; r < 10; ++r) {
         if (r > 0) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
@@ -85,40 +85,73 @@ WOS_HD U64x2 philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t
         c0 = n0;
         c2 = n2;
     }
-    return U64x2{(uint64_t)c0 | ((uint64_t)c1 << 32), (uint64_t)c2 | ((uint64_t)c3 << 32)};
+    return Words4{c0, c1, c2, c3};
 }
 
-// uniform in [0, 1) from the top 53 bits (include/wos/prng.hpp:25-27)
-WOS_HD double u01(uint64_t x) { return (double)(x >> 11) * (1.0 / 9007199254740992.0); }
-// uniform in (0, 1): midpoint of a 2^-52 cell, (2k+1) 2^-53
-WOS_HD double u01_open(uint64_t x) {
-    return (double)(((x >> 12) << 1) | 1ULL) * (1.0 / 9007199254740992.0);
-}
-// FP32 forms from the top 24 bits of the same word (common random numbers
-// between the FP32 and FP64 walkers)
-WOS_HD float u01f(uint64_t x) { return (float)(uint32_t)(x >> 40) * (1.0f / 16777216.0f); }
-WOS_HD float u01f_open(uint64_t x) {
-    return (float)(uint32_t)(((x >> 41) << 1) | 1ULL) * (1.0f / 16777216.0f);
+WOS_HD uint32_t word_at(const Words4 &b, uint32_t k) {
+    return k == 0 ? b.w0 : k == 1 ? b.w1 : k == 2 ? b.w2 : b.w3;
 }
 
-// Draw cursor over the counter stream of one (walk, step, tag).
-struct Cursor {
-    uint32_t walk, step, block, tag, k0, k1;
-    uint64_t pending;
-    bool has;
-    WOS_HD Cursor(uint32_t w, uint32_t s, uint32_t t, uint32_t key0, uint32_t key1)
-        : walk(w), step(s), block(0), tag(t), k0(key0), k1(key1), pending(0), has(false) {}
-    WOS_HD uint64_t next() {
-        if (has) { has = false; return pending; }
-        U64x2 r = philox(walk, step, block++, tag, k0, k1);
-        pending = r.b;
-        has = true;
-        return r.a;
+// The counter stream of one walk step (walk w, step s, key = the point's
+// walk-stream seed).  Its uniforms are numbered by slot k = 0, 1, 2, ...:
+// uniform k takes its top 32 bits from word k mod 4 of the block at counter
+// {w, s, k / 4, 0} and, in FP64, its next 21 bits from the top of the same
+// word of the block at {w, s, k / 4, 1}.  So an FP32 draw is exactly the
+// top 24 bits of the FP64 draw of the same slot (common random numbers
+// between the walkers), and one block serves four FP32 uniforms.
+// Slots 0 .. dim-2 are the step direction; the source term draws from slot
+// dim-1 on (its rejection samplers continue into later blocks).
+WOS_HD uint64_t bits53(uint32_t hi, uint32_t lo) { return ((uint64_t)hi << 21) | (lo >> 11); }
+// uniform in [0, 1): k 2^-53 (include/wos/prng.hpp:25-27 scaling)
+WOS_HD double u53(uint64_t x) { return (double)x * (1.0 / 9007199254740992.0); }
+// uniform in (0, 1): the odd multiple (2j+1) 2^-53 of the cell midpoint
+WOS_HD double u53_open(uint64_t x) { return (double)(x | 1ULL) * (1.0 / 9007199254740992.0); }
+// FP32 forms from the top 24 bits of the slot's high word
+WOS_HD float u24(uint32_t hi) { return (float)(hi >> 8) * (1.0f / 16777216.0f); }
+WOS_HD float u24_open(uint32_t hi) { return (float)((hi >> 8) | 1u) * (1.0f / 16777216.0f); }
+
+// Sequential FP64 draws of a step stream from slot `first` on.
+struct StepStreamD {
+    static constexpr bool radius_first = true;  // a ball point draws its radius first
+    uint32_t walk, step, k0, k1, next, held;  // held: block index + 1 in hi/lo (0: none)
+    Words4 hi, lo;
+    WOS_HD StepStreamD(uint32_t w, uint32_t s, uint32_t key0, uint32_t key1, uint32_t first)
+        : walk(w), step(s), k0(key0), k1(key1), next(first), held(0), hi{}, lo{} {}
+    WOS_HD uint64_t bits() {
+        const uint32_t k = next++, blk = k >> 2;
+        if (held != blk + 1) {
+            hi = philox4(walk, step, blk, 0, k0, k1);
+            lo = philox4(walk, step, blk, 1, k0, k1);
+            held = blk + 1;
+        }
+        return bits53(word_at(hi, k & 3), word_at(lo, k & 3));
     }
-    WOS_HD double unit() { return u01(next()); }
-    WOS_HD double unit_open() { return u01_open(next()); }
-    WOS_HD float unitf() { return u01f(next()); }
-    WOS_HD float unitf_open() { return u01f_open(next()); }
+    WOS_HD double unit() { return u53(bits()); }
+    WOS_HD double unit_open() { return u53_open(bits()); }
+    WOS_HD void seek(uint32_t slot) { next = slot; }  // the held block stays valid
+};
+
+// Sequential FP32 draws of a step stream from slot `first` on; `b0` may
+// hand over block 0, already formed by the caller.
+struct StepStreamF {
+    static constexpr bool radius_first = true;
+    uint32_t walk, step, k0, k1, next, held;
+    Words4 hi;
+    WOS_HD StepStreamF(uint32_t w, uint32_t s, uint32_t key0, uint32_t key1, uint32_t first)
+        : walk(w), step(s), k0(key0), k1(key1), next(first), held(0), hi{} {}
+    WOS_HD StepStreamF(uint32_t w, uint32_t s, uint32_t key0, uint32_t key1, uint32_t first,
+                       const Words4 &b0)
+        : walk(w), step(s), k0(key0), k1(key1), next(first), held(1), hi(b0) {}
+    WOS_HD uint32_t word() {
+        const uint32_t k = next++, blk = k >> 2;
+        if (held != blk + 1) {
+            hi = philox4(walk, step, blk, 0, k0, k1);
+            held = blk + 1;
+        }
+        return word_at(hi, k & 3);
+    }
+    WOS_HD float unit() { return u24(word()); }
+    WOS_HD float unit_open() { return u24_open(word()); }
 };
 
 }  // namespace wosx

@@ -39,20 +39,13 @@ constexpr int kMaxParts = 64;  // largest xcd_parts
 // --------------------------------------------------------------------------
 // walk-step helpers (FP64): src/wos.cpp:9-56, include/wos/wos.hpp:54-138
 // --------------------------------------------------------------------------
-struct CursorD {
-    Cursor c;
-    __device__ CursorD(uint32_t w, uint32_t s, uint32_t tag, uint32_t k0, uint32_t k1)
-        : c(w, s, tag, k0, k1) {}
-    __device__ double unit() { return c.unit(); }
-    __device__ double unit_open() { return c.unit_open(); }
-};
-struct CursorF {
-    Cursor c;
-    __device__ CursorF(uint32_t w, uint32_t s, uint32_t tag, uint32_t k0, uint32_t k1)
-        : c(w, s, tag, k0, k1) {}
-    __device__ float unit() { return c.unitf(); }
-    __device__ float unit_open() { return c.unitf_open(); }
-};
+// The samplers take their uniforms from an RNG object with unit() and
+// unit_open(): the counter streams of one walk step (StepStreamD/F,
+// wos_common.hpp) or a reference xoshiro stream (XoRef).  RNG::radius_first
+// orders the draws of a uniform ball point: the counter streams draw its
+// radius first (so an FP32 constant-source step needs one Philox block: the
+// direction's and the radius's slots share it), the reference draws the
+// direction first (src/wos.cpp:33-56).
 
 template <typename T> __device__ __forceinline__ T tsin(T x) { return sin(x); }
 template <typename T> __device__ __forceinline__ T tcos(T x) { return cos(x); }
@@ -238,14 +231,29 @@ __device__ __forceinline__ double source_term(const EqView &e, T px, T py, T pz,
     // uniform ball point (src/wos.cpp:33-56)
     T dist;
     if (DIM == 2) {
-        const T theta = rng.unit() * (T)2 * (T)kPi;
-        const T r = R * sqrt(rng.unit_open());
+        T theta, r;
+        if constexpr (RNG::radius_first) {
+            r = R * sqrt(rng.unit_open());
+            theta = rng.unit() * (T)2 * (T)kPi;
+        } else {
+            theta = rng.unit() * (T)2 * (T)kPi;
+            r = R * sqrt(rng.unit_open());
+        }
         sx = px + r * cos(theta);
         sy = py + r * sin(theta);
         const T dx = px - sx, dy = py - sy;
         dist = sqrt(dx * dx + dy * dy);
     } else {
-        const T u = rng.unit(), v = rng.unit(), w = rng.unit_open();
+        T u, v, w;
+        if constexpr (RNG::radius_first) {
+            w = rng.unit_open();
+            u = rng.unit();
+            v = rng.unit();
+        } else {
+            u = rng.unit();
+            v = rng.unit();
+            w = rng.unit_open();
+        }
         const T z = (T)2 * u - (T)1, phi = (T)2 * (T)kPi * v;
         const T surf = sqrt((T)1 - z * z), vol = R * cbrt(w);
         sx = px + vol * surf * cos(phi);
@@ -664,9 +672,9 @@ __device__ __forceinline__ void store_adjoint(const AdjOut &a, int64_t item, Wal
     a.prim[item] = prim;
 }
 
-template <int DIM, typename T, int SMODE>
-__device__ double source_term_adj(const EqView &e, T px, T py, T pz, T R, T weight, uint32_t w,
-                                  uint32_t step, uint32_t k0, uint32_t k1, WalkDer &D);
+template <int DIM, typename T, int SMODE, typename RNG>
+__device__ double source_term_adj(const EqView &e, T px, T py, T pz, T R, T weight, RNG &rng,
+                                  WalkDer &D);
 
 // --------------------------------------------------------------------------
 // one round of walks: active point k runs walks [wbeg[k], wbeg[k] + count_k);
@@ -766,27 +774,30 @@ void k_walk_fp64(MeshView m, EqView e, RoundView rv,
     const double alpha = e.alpha;
     WalkDer D{};
     while (R > eps && steps < max_steps) {
+        // the step's counter stream: the source term draws from slot DIM-1
+        // on, then the direction takes slots 0 .. DIM-2 (block 0 is formed
+        // once when the source's draws stayed in it)
+        StepStreamD rng(w, (uint32_t)steps, k0, k1, DIM - 1);
         if (SRC && e.source_kind != WOS_SRC_ZERO) {
-            if constexpr (ADJ) {
-                sample += source_term_adj<DIM, double, 1>(e, px, py, pz, R, weight, w,
-                                                          (uint32_t)steps, k0, k1, D);
-            } else {
-                CursorD src(w, (uint32_t)steps, 1, k0, k1);
-                sample += source_term<DIM, double, CursorD, SX>(e, px, py, pz, R, weight, src);
-            }
+            if constexpr (ADJ)
+                sample += source_term_adj<DIM, double, 1>(e, px, py, pz, R, weight, rng, D);
+            else
+                sample += source_term<DIM, double, StepStreamD, SX>(e, px, py, pz, R, weight, rng);
         }
         if (SCR) {
             if constexpr (ADJ && DIM == 3)
                 if (e.kind == WOS_EQ_SCREENED) D.L += dlog_w3_dalpha(alpha, R);
             weight *= step_weight<DIM, double>(e.kind, alpha, R);
         }
-        CursorD wk(w, (uint32_t)steps, 0, k0, k1);
+        rng.seek(0);
+        const double du = rng.unit();
+        const double dv = DIM == 3 ? rng.unit() : 0.0;
         if (DIM == 2) {
-            const double theta = wk.unit() * 2 * kPi;
+            const double theta = du * 2 * kPi;
             px = px + R * cos(theta);
             py = py + R * sin(theta);
         } else {
-            const double u = wk.unit(), v = wk.unit();
+            const double u = du, v = dv;
             const double z = 2 * u - 1, phi = 2 * kPi * v;
             const double rr = sqrt(1 - z * z);
             px = px + R * rr * cos(phi);
@@ -824,6 +835,7 @@ void k_walk_fp64(MeshView m, EqView e, RoundView rv,
 
 // a stream as the samplers' RNG interface
 struct XoRef {
+    static constexpr bool radius_first = false;
     Xoshiro *x;
     __device__ double unit() { return x->unit(); }
     __device__ double unit_open() { return x->unit_open(); }
@@ -896,55 +908,40 @@ void k_walk_ref(MeshView m, EqView e, RoundView rv, double eps, int max_steps, d
     st[4] = ss.s0; st[5] = ss.s1; st[6] = ss.s2; st[7] = ss.s3;
 }
 
-// FP32 source term.  Same draws as source_term (the walk's tag-1 counter
-// block), but the Green function takes the sample's distance from the
-// centre in closed form (R sqrt(u) in 2D, R cbrt(w) in 3D) rather than from
-// rounded coordinates, and a constant source skips the direction entirely:
-// only the radius draw is made (block 1 word 0 in 3D, block 0 word 1 in 2D).
+// FP32 source term.  Same draws as source_term (the step stream from slot
+// DIM-1 on, `rng`), but the Green function takes the sample's distance from
+// the centre in closed form (R sqrt(u) in 2D, R cbrt(w) in 3D) rather than
+// from rounded coordinates, and a constant source skips the direction
+// entirely: only the radius draw is made (the first source slot, which
+// shares its Philox block with the step direction).
 // SMODE: 1 uniform, 2 Green, 3 GREEN_MIS, 4 Green with WOS_SRC_FIELD,
 // 5 uniform with WOS_SRC_FIELD (the field cases 2D only).
 template <int DIM, int SMODE>
 __device__ __forceinline__ double source_term_f(const EqView &e, float px, float py, float pz,
-                                                float R, float weight, uint32_t w, uint32_t step,
-                                                uint32_t k0, uint32_t k1) {
+                                                float R, float weight, StepStreamF &rng) {
     constexpr bool FLD = SMODE == 5;
-    if (SMODE == 2) {
-        CursorF src(w, step, 1, k0, k1);
-        return source_term<DIM, float>(e, px, py, pz, R, weight, src);
-    }
-    if constexpr (SMODE == 4) {
-        CursorF src(w, step, 1, k0, k1);
-        return source_term<DIM, float, CursorF, 2>(e, px, py, pz, R, weight, src);
-    }
-    if constexpr (SMODE == 3 && DIM == 2) {
-        CursorF src(w, step, 1, k0, k1);
-        return source_term_mis<float>(e, px, py, R, weight, src);
-    }
+    if (SMODE == 2) return source_term<DIM, float>(e, px, py, pz, R, weight, rng);
+    if constexpr (SMODE == 4) return source_term<DIM, float, StepStreamF, 2>(e, px, py, pz, R, weight, rng);
+    if constexpr (SMODE == 3 && DIM == 2) return source_term_mis<float>(e, px, py, R, weight, rng);
     if (DIM == 2 && !eq_source_may_intersect<float>(e, px, py, R)) return 0.0;
     const bool screened = e.kind == WOS_EQ_SCREENED;
     const float alpha = (float)e.alpha;
     float f, dist;
     if (e.source_kind == WOS_SRC_CONSTANT) {
-        if (DIM == 2) {
-            const U64x2 r = philox(w, step, 0, 1, k0, k1);
-            dist = R * Fx<float>::sqrt(u01f_open(r.b));
-        } else {
-            const U64x2 r = philox(w, step, 1, 1, k0, k1);
-            dist = R * Fx<float>::cbrt(u01f_open(r.a));
-        }
+        const float ro = rng.unit_open();
+        dist = DIM == 2 ? R * Fx<float>::sqrt(ro) : R * Fx<float>::cbrt(ro);
         f = (float)e.src_value;
     } else {
-        CursorF rng(w, step, 1, k0, k1);
         float sx, sy, sz = 0.0f;
         if (DIM == 2) {
-            const float theta = rng.unit();
             dist = R * Fx<float>::sqrt(rng.unit_open());
+            const float theta = rng.unit();
             float sn, cs;
             __sincosf(theta * (float)(2.0 * kPi), &sn, &cs);
             sx = __builtin_fmaf(dist, cs, px);
             sy = __builtin_fmaf(dist, sn, py);
         } else {
-            const float u = rng.unit(), v = rng.unit(), ww = rng.unit_open();
+            const float ww = rng.unit_open(), u = rng.unit(), v = rng.unit();
             const float z = 2.0f * u - 1.0f;
             const float surf = Fx<float>::sqrt(fmaxf(0.0f, 1.0f - z * z));
             dist = R * Fx<float>::cbrt(ww);
@@ -971,9 +968,9 @@ __device__ __forceinline__ double source_term_f(const EqView &e, float px, float
 // Source term plus its parameter derivatives (adjoint replay).  Same draws
 // and same value as source_term (T = double) / source_term_f (T = float);
 // the derivatives are formed in FP64.
-template <int DIM, typename T, int SMODE>
-__device__ double source_term_adj(const EqView &e, T px, T py, T pz, T R, T weight, uint32_t w,
-                                  uint32_t step, uint32_t k0, uint32_t k1, WalkDer &D) {
+template <int DIM, typename T, int SMODE, typename RNG>
+__device__ double source_term_adj(const EqView &e, T px, T py, T pz, T R, T weight, RNG &rng,
+                                  WalkDer &D) {
     constexpr bool F32 = sizeof(T) == 4;
     const bool screened = e.kind == WOS_EQ_SCREENED;
     const T alpha = (T)e.alpha;
@@ -981,42 +978,19 @@ __device__ double source_term_adj(const EqView &e, T px, T py, T pz, T R, T weig
     T sx = px, sy = py, sz = pz, dist = (T)0;
     if (SMODE == 2 || e.source_mode == WOS_SOURCE_GREEN) {
         T radius;
-        if (F32) {
-            CursorF rng(w, step, 1, k0, k1);
-            if (DIM == 2) {
-                radius = screened ? (T)screened_radius_2d<float>((float)alpha, (float)R, rng)
-                                  : (T)poisson_radius_2d<float>((float)R, rng);
-                const T angle = (T)2 * (T)kPi * (T)rng.unit();
-                sx = px + radius * Fx<T>::cos(angle);
-                sy = py + radius * Fx<T>::sin(angle);
-            } else {
-                radius = screened ? (T)screened_radius_3d<float>((float)alpha, (float)R, rng)
-                                  : (T)poisson_radius_3d<float>((float)R, rng);
-                const T z = (T)2 * (T)rng.unit() - (T)1;
-                const T angle = (T)2 * (T)kPi * (T)rng.unit();
-                const T xy = Fx<T>::sqrt((T)1 - z * z);
-                sx = px + radius * xy * Fx<T>::cos(angle);
-                sy = py + radius * xy * Fx<T>::sin(angle);
-                sz = pz + radius * z;
-            }
+        if (DIM == 2) {
+            radius = screened ? screened_radius_2d<T>(alpha, R, rng) : poisson_radius_2d<T>(R, rng);
+            const T angle = (T)2 * (T)kPi * rng.unit();
+            sx = px + radius * Fx<T>::cos(angle);
+            sy = py + radius * Fx<T>::sin(angle);
         } else {
-            CursorD rng(w, step, 1, k0, k1);
-            if (DIM == 2) {
-                radius = screened ? (T)screened_radius_2d<double>((double)alpha, (double)R, rng)
-                                  : (T)poisson_radius_2d<double>((double)R, rng);
-                const T angle = (T)2 * (T)kPi * (T)rng.unit();
-                sx = px + radius * cos(angle);
-                sy = py + radius * sin(angle);
-            } else {
-                radius = screened ? (T)screened_radius_3d<double>((double)alpha, (double)R, rng)
-                                  : (T)poisson_radius_3d<double>((double)R, rng);
-                const T z = (T)2 * (T)rng.unit() - (T)1;
-                const T angle = (T)2 * (T)kPi * (T)rng.unit();
-                const T xy = sqrt((T)1 - z * z);
-                sx = px + radius * xy * cos(angle);
-                sy = py + radius * xy * sin(angle);
-                sz = pz + radius * z;
-            }
+            radius = screened ? screened_radius_3d<T>(alpha, R, rng) : poisson_radius_3d<T>(R, rng);
+            const T z = (T)2 * rng.unit() - (T)1;
+            const T angle = (T)2 * (T)kPi * rng.unit();
+            const T xy = Fx<T>::sqrt((T)1 - z * z);
+            sx = px + radius * xy * Fx<T>::cos(angle);
+            sy = py + radius * xy * Fx<T>::sin(angle);
+            sz = pz + radius * z;
         }
         const T f = eq_source<T>(e, sx, sy, sz, DIM);
         if (f == (T)0) return 0.0;
@@ -1027,43 +1001,39 @@ __device__ double source_term_adj(const EqView &e, T px, T py, T pz, T R, T weig
         add_source_der<T>(D, e, DIM, (double)(weight * mass), S, sx, sy, sz, 0.0);
         return S;
     }
-    // uniform ball point
+    // uniform ball point, radius first (the counter streams' order)
     if constexpr (F32) {
         if (e.source_kind == WOS_SRC_CONSTANT) {
-            if (DIM == 2) dist = R * Fx<float>::sqrt(u01f_open(philox(w, step, 0, 1, k0, k1).b));
-            else dist = R * Fx<float>::cbrt(u01f_open(philox(w, step, 1, 1, k0, k1).a));
+            const float ro = rng.unit_open();
+            dist = DIM == 2 ? R * Fx<float>::sqrt(ro) : R * Fx<float>::cbrt(ro);
+        } else if (DIM == 2) {
+            dist = R * Fx<float>::sqrt(rng.unit_open());
+            const float theta = rng.unit();
+            float sn, cs;
+            __sincosf(theta * (float)(2.0 * kPi), &sn, &cs);
+            sx = __builtin_fmaf(dist, cs, px);
+            sy = __builtin_fmaf(dist, sn, py);
         } else {
-            CursorF rng(w, step, 1, k0, k1);
-            if (DIM == 2) {
-                const float theta = rng.unit();
-                dist = R * Fx<float>::sqrt(rng.unit_open());
-                float sn, cs;
-                __sincosf(theta * (float)(2.0 * kPi), &sn, &cs);
-                sx = __builtin_fmaf(dist, cs, px);
-                sy = __builtin_fmaf(dist, sn, py);
-            } else {
-                const float u = rng.unit(), v = rng.unit(), ww = rng.unit_open();
-                const float z = 2.0f * u - 1.0f;
-                const float surf = Fx<float>::sqrt(fmaxf(0.0f, 1.0f - z * z));
-                dist = R * Fx<float>::cbrt(ww);
-                float sn, cs;
-                __sincosf(v * (float)(2.0 * kPi), &sn, &cs);
-                sx = __builtin_fmaf(dist * surf, cs, px);
-                sy = __builtin_fmaf(dist * surf, sn, py);
-                sz = __builtin_fmaf(dist, z, pz);
-            }
+            const float ww = rng.unit_open(), u = rng.unit(), v = rng.unit();
+            const float z = 2.0f * u - 1.0f;
+            const float surf = Fx<float>::sqrt(fmaxf(0.0f, 1.0f - z * z));
+            dist = R * Fx<float>::cbrt(ww);
+            float sn, cs;
+            __sincosf(v * (float)(2.0 * kPi), &sn, &cs);
+            sx = __builtin_fmaf(dist * surf, cs, px);
+            sy = __builtin_fmaf(dist * surf, sn, py);
+            sz = __builtin_fmaf(dist, z, pz);
         }
     } else {
-        CursorD rng(w, step, 1, k0, k1);
         if (DIM == 2) {
-            const T theta = (T)rng.unit() * (T)2 * (T)kPi;
-            const T r = R * sqrt((T)rng.unit_open());
+            const T r = R * sqrt(rng.unit_open());
+            const T theta = rng.unit() * (T)2 * (T)kPi;
             sx = px + r * cos(theta);
             sy = py + r * sin(theta);
             const T dx = px - sx, dy = py - sy;
             dist = sqrt(dx * dx + dy * dy);
         } else {
-            const T u = (T)rng.unit(), v = (T)rng.unit(), ww = (T)rng.unit_open();
+            const T ww = rng.unit_open(), u = rng.unit(), v = rng.unit();
             const T z = (T)2 * u - (T)1, phi = (T)2 * (T)kPi * v;
             const T surf = sqrt((T)1 - z * z), vol = R * cbrt(ww);
             sx = px + vol * surf * cos(phi);
@@ -1379,28 +1349,29 @@ void k_walk_fp32(MeshView m, EqView e, PersistArgs a, AdjOut adj) {
         // radius is the relaxed one, and the exact result (which query_begin
         // may already have completed) decides the walk's end next iteration.
         if (active && !searching && !began) {
+            // block 0 of the step's counter stream: the direction's slots and
+            // the first source slots (a constant source needs no other block)
+            const Words4 b0 = philox4(w, (uint32_t)steps, 0, 0, k0, k1);
             if (SMODE != 0 && e.source_kind != WOS_SRC_ZERO) {
+                StepStreamF src(w, (uint32_t)steps, k0, k1, DIM - 1, b0);
                 if constexpr (ADJ)
-                    sample += source_term_adj<DIM, float, SMODE>(e, px, py, pz, R, weight, w,
-                                                                 (uint32_t)steps, k0, k1, D);
+                    sample += source_term_adj<DIM, float, SMODE>(e, px, py, pz, R, weight, src, D);
                 else
-                    sample += source_term_f<DIM, SMODE>(e, px, py, pz, R, weight, w,
-                                                        (uint32_t)steps, k0, k1);
+                    sample += source_term_f<DIM, SMODE>(e, px, py, pz, R, weight, src);
             }
             if (SCR) {
                 if constexpr (ADJ && DIM == 3)
                     if (e.kind == WOS_EQ_SCREENED) D.L += (double)dlog_w3_dalpha_f(alpha, R);
                 weight *= step_weight<DIM, float>(e.kind, alpha, R);
             }
-            Cursor wk(w, (uint32_t)steps, 0, k0, k1);
             if (DIM == 2) {
-                const float u = wk.unitf();
+                const float u = u24(b0.w0);
                 float sn, cs;
                 __sincosf(u * (float)(2.0 * kPi), &sn, &cs);
                 px = __builtin_fmaf(R, cs, px);
                 py = __builtin_fmaf(R, sn, py);
             } else {
-                const float u = wk.unitf(), v = wk.unitf();
+                const float u = u24(b0.w0), v = u24(b0.w1);
                 const float z = 2.0f * u - 1.0f;
                 const float rr = Fx<float>::sqrt(fmaxf(0.0f, 1.0f - z * z));
                 float sn, cs;
@@ -1676,18 +1647,21 @@ struct wos_ctx {
 namespace {
 
 // Cells of a domain's distance grid.  grid_cells > 0: that many.  -1 (the
-// default): sized from the domain's primitive count, kGridCellsPerPrim cells
-// per primitive rounded up to a power of two, clamped to
-// [kGridMinCells, kGridMaxCells(dim)] -- enough that the cells near the
-// boundary are several times smaller than a primitive (short lists) without
-// the 2^24 cells of a fine grid for a mesh of a few thousand triangles.
-constexpr int64_t kGridCellsPerPrim = 512;
+// default): sized from the domain's primitive count, 512 cells per segment
+// (2D) or 4096 per triangle (3D) rounded up to a power of two, at least
+// 2^16 and at most 2^22 (2D) or 2^24 (3D).  Measured: in 3D the walker runs
+// faster with every doubling up to 2^24 cells (icosphere s4: 42.8 ms at
+// 2^18, 31.0 at 2^22, 29.0 at 2^24; the 200k-triangle plate 420, 181, 147
+// ms at 2^20, 2^22, 2^24) against a build of 17-150 ms once per mesh; in 2D
+// per-domain sizing is fastest (16 star domains of 512-16384 segments:
+// 187 ms against 197 ms at 2^22 cells each, with half the memory).
 constexpr int64_t kGridMinCells = (int64_t)1 << 16;
 int64_t grid_cells_for(int tune_cells, int dim, int64_t n_prims) {
     if (tune_cells >= 0) return tune_cells;
+    const int64_t per_prim = dim == 3 ? 4096 : 512;
     const int64_t hi = dim == 3 ? (int64_t)1 << 24 : (int64_t)1 << 22;
     int64_t cells = kGridMinCells;
-    while (cells < hi && cells < kGridCellsPerPrim * n_prims) cells <<= 1;
+    while (cells < hi && cells < per_prim * n_prims) cells <<= 1;
     return cells;
 }
 
