@@ -679,6 +679,33 @@ template <typename T> __device__ __forceinline__ T screened_green_3d(T alpha, T 
                                -Fx<T>::expm1((T)-2 * X));
     return Fx<T>::div(ratio, (T)4 * (T)kPi * r);
 }
+// The 3D screened kernel's terms of one walk step at X = alpha R (FP32
+// walker): e^-X and q = 1 - e^-2X, shared by the step's weight X / sinh X =
+// 2 X e^-X / q and its source term's Green function, whose ratio
+// sinh(alpha (R - r)) / sinh X = e^-(alpha r) (1 - e^-2 alpha (R - r)) / q.
+// The same formulas as screened_weight_3d / screened_green_3d.
+struct Scr3 { float X, eX, q; };
+__device__ __forceinline__ Scr3 scr3_terms(float alpha, float R) {
+    const float X = alpha * R;
+    return Scr3{X, Fx<float>::exp(-X), -Fx<float>::expm1(-2.0f * X)};
+}
+__device__ __forceinline__ float scr3_weight(const Scr3 &S) {
+    if (S.X < 1e-3f) {
+        const float x2 = S.X * S.X;
+        return __builtin_fmaf(x2, __builtin_fmaf(x2, 7.0f / 360.0f, -1.0f / 6.0f), 1.0f);
+    }
+    return 2.0f * S.X * S.eX * __builtin_amdgcn_rcpf(S.q);
+}
+// G at distance 0 < r <= R from the centre (the walker's source samples)
+__device__ __forceinline__ float scr3_green(const Scr3 &S, float alpha, float R, float r) {
+    if (!(r < R)) return 0.0f;
+    if (S.X < 1e-5f) return poisson_green_3d<float>(R, r);
+    const float x = alpha * r;
+    const float ratio = Fx<float>::exp(-x) * (-Fx<float>::expm1(-2.0f * (S.X - x))) *
+                        __builtin_amdgcn_rcpf(S.q);
+    return ratio * __builtin_amdgcn_rcpf(4.0f * (float)kPi * r);
+}
+
 // include/wos/fastmath.hpp:15-29: the series of J0, stopped at the first
 // term below 1e-8 (the reference's own truncation)
 template <typename T> __device__ __forceinline__ T bessel_j0_series(T x) {

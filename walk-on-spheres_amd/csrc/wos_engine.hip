@@ -918,7 +918,8 @@ void k_walk_ref(MeshView m, EqView e, RoundView rv, double eps, int max_steps, d
 // 5 uniform with WOS_SRC_FIELD (the field cases 2D only).
 template <int DIM, int SMODE>
 __device__ __forceinline__ double source_term_f(const EqView &e, float px, float py, float pz,
-                                                float R, float weight, StepStreamF &rng) {
+                                                float R, float weight, StepStreamF &rng,
+                                                bool use_s3 = false, Scr3 s3 = Scr3{}) {
     constexpr bool FLD = SMODE == 5;
     if (SMODE == 2) return source_term<DIM, float>(e, px, py, pz, R, weight, rng);
     if constexpr (SMODE == 4) return source_term<DIM, float, StepStreamF, 2>(e, px, py, pz, R, weight, rng);
@@ -960,7 +961,8 @@ __device__ __forceinline__ double source_term_f(const EqView &e, float px, float
         G = screened ? screened_green_2d<float>(alpha, R, dist) : poisson_green_2d<float>(R, dist);
     } else {
         volume = (float)(4.0 * kPi / 3.0) * R * R * R;
-        G = screened ? screened_green_3d<float>(alpha, R, dist) : poisson_green_3d<float>(R, dist);
+        if (use_s3) G = scr3_green(s3, alpha, R, dist);
+        else G = screened ? screened_green_3d<float>(alpha, R, dist) : poisson_green_3d<float>(R, dist);
     }
     return (double)(weight * volume * f * G);
 }
@@ -1352,17 +1354,23 @@ void k_walk_fp32(MeshView m, EqView e, PersistArgs a, AdjOut adj) {
             // block 0 of the step's counter stream: the direction's slots and
             // the first source slots (a constant source needs no other block)
             const Words4 b0 = philox4(w, (uint32_t)steps, 0, 0, k0, k1);
+            // 3D screened forward walks: the weight's and the Green
+            // function's exponentials of alpha R formed once
+            constexpr bool kScr3 = SCR && DIM == 3 && !ADJ;
+            const bool scr3 = kScr3 && e.kind == WOS_EQ_SCREENED;
+            Scr3 s3{0.0f, 0.0f, 0.0f};
+            if (scr3) s3 = scr3_terms(alpha, R);
             if (SMODE != 0 && e.source_kind != WOS_SRC_ZERO) {
                 StepStreamF src(w, (uint32_t)steps, k0, k1, DIM - 1, b0);
                 if constexpr (ADJ)
                     sample += source_term_adj<DIM, float, SMODE>(e, px, py, pz, R, weight, src, D);
                 else
-                    sample += source_term_f<DIM, SMODE>(e, px, py, pz, R, weight, src);
+                    sample += source_term_f<DIM, SMODE>(e, px, py, pz, R, weight, src, scr3, s3);
             }
             if (SCR) {
                 if constexpr (ADJ && DIM == 3)
                     if (e.kind == WOS_EQ_SCREENED) D.L += (double)dlog_w3_dalpha_f(alpha, R);
-                weight *= step_weight<DIM, float>(e.kind, alpha, R);
+                weight *= scr3 ? scr3_weight(s3) : step_weight<DIM, float>(e.kind, alpha, R);
             }
             if (DIM == 2) {
                 const float u = u24(b0.w0);
