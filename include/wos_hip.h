@@ -326,6 +326,20 @@ int wos_closest_points(wos_ctx *ctx, int domain, const double *points, int64_t n
                        int precision, double relax, double *dist, double *lower,
                        double *nearest, int32_t *prim);
 
+/* wos_closest_points without any device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments and results as wos_closest_points, bit for bit.  The
+ * points and results are staged in one fixed 16 MiB device array per GPU
+ * (held until the process ends), which limits n to 246 704 points in 3D and
+ * 322 614 in 2D (larger n: WOS_ERR_INPUT).  The array is shared by every
+ * call on that GPU, so calls that overlap on one GPU (two threads, two
+ * contexts) race; serialise them.  FP32 needs the context's distance grid
+ * already built (any earlier FP32 call builds it), else WOS_ERR_STATE. */
+int wos_closest_points_noalloc(wos_ctx *ctx, int domain, const double *points, int64_t n,
+                               int precision, double relax, double *dist, double *lower,
+                               double *nearest, int32_t *prim);
+
 /* Launch shape of the FP32 walker.  No reference counterpart (the reference
  * has one CPU code path); used by the autotuner. */
 int wos_ctx_set_tuning(wos_ctx *ctx, const wos_tuning *tuning);

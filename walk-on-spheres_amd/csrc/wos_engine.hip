@@ -21,6 +21,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1590,6 +1591,38 @@ hipError_t upload(DevBuf &b, const std::vector<T> &v, hipStream_t s) {
 
 }  // namespace
 
+// Staging memory of wos_closest_points_noalloc: one fixed array in each
+// GPU's copy of the code object, held until the process ends, so that the
+// entry point allocates nothing.  In host code `scratch` is only a symbol:
+// its device address, resolved per device by scratch_address(), is what
+// kernels and copies use.  Every call on a GPU shares that GPU's array.
+constexpr size_t kScratchBytes = (size_t)16 << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_addr[kMaxDevices] = {};
+
+// The device address of `scratch` on `device` (which must be current),
+// resolved once per device.
+hipError_t scratch_address(int device, void **out) {
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_addr[device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        if (!p) return hipErrorInvalidSymbol;
+        g_scratch_addr[device] = p;
+    }
+    *out = g_scratch_addr[device];
+    return hipSuccess;
+}
+
+}  // namespace
+
 // what the autotuner's cached shapes are keyed by
 struct TuneKey {
     int dim = 0, n_domains = 0;
@@ -1650,6 +1683,7 @@ struct wos_ctx {
     size_t lds_per_block = 65536;
     int last_threads = 0;
     int last_cached = 0;
+    void *scratch_dev = nullptr;      // this device's `scratch` array (no-allocation entry point)
 };
 
 namespace {
@@ -2996,6 +3030,13 @@ wos_ctx *wos_ctx_create(int device) {
             fail(WOS_ERR_DEVICE, "hipEventCreate failed");
             return nullptr;
         }
+    {
+        const hipError_t e = scratch_address(device, &c->scratch_dev);
+        if (e != hipSuccess) {
+            fail(WOS_ERR_DEVICE, std::string("resolving the scratch array: ") + hipGetErrorString(e));
+            return nullptr;
+        }
+    }
     return c.release();
 }
 
@@ -3271,6 +3312,86 @@ int wos_closest_points(wos_ctx *c, int domain, const double *points, int64_t n, 
                             c->stream));
     if (prim)
         HIPX(hipMemcpyAsync(prim, c->cq_prim.p, n * sizeof(int32_t), hipMemcpyDeviceToHost,
+                            c->stream));
+    HIPX(hipStreamSynchronize(c->stream));
+    return WOS_OK;
+}
+
+int wos_closest_points_noalloc(wos_ctx *c, int domain, const double *points, int64_t n,
+                               int precision, double relax, double *dist, double *lower,
+                               double *nearest, int32_t *prim) {
+    if (!c || !c->has_mesh) return fail(c ? WOS_ERR_STATE : WOS_ERR_INPUT, "no mesh set on this context");
+    if (domain < 0 || domain >= (int)c->L.domains.size()) return fail(WOS_ERR_INPUT, "domain out of range");
+    if (n < 0) return fail(WOS_ERR_INPUT, "n must be >= 0");
+    if (n > 0 && (!points || !dist)) return fail(WOS_ERR_INPUT, "points and dist must not be NULL");
+    if (precision != WOS_FP64_PLAIN && precision != WOS_FP32_PERSISTENT)
+        return fail(WOS_ERR_INPUT, "unknown precision");
+    if (!(relax >= 0.0 && relax < 0.5)) return fail(WOS_ERR_INPUT, "relax must be in [0, 0.5)");
+    if (precision == WOS_FP64_PLAIN && relax != 0.0)
+        return fail(WOS_ERR_INPUT, "the FP64 search is always exact (relax must be 0)");
+    if (n == 0) return WOS_OK;
+    const int dim = c->mesh.dim;
+    // staging in the scratch array: points, then dist, lower, nearest, prim,
+    // each part 256-byte aligned
+    auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const size_t b_pts = up((size_t)n * dim * sizeof(double)), b_d = up((size_t)n * sizeof(double));
+    const size_t b_near = up((size_t)n * dim * sizeof(double)), b_prim = up((size_t)n * sizeof(int32_t));
+    const int64_t per_point = (int64_t)(dim * 16 + 20);
+    if ((double)n * (double)per_point + 5.0 * 256.0 > (double)kScratchBytes ||
+        b_pts + 2 * b_d + b_near + b_prim > kScratchBytes)
+        return fail(WOS_ERR_INPUT, "wos_closest_points_noalloc stages at most " +
+                                       std::to_string((kScratchBytes - 5 * 256) / per_point) +
+                                       " points (a fixed " + std::to_string(kScratchBytes >> 20) +
+                                       " MiB device array); use wos_closest_points");
+    for (int64_t i = 0; i < n * dim; ++i)
+        if (!std::isfinite(points[i])) return fail(WOS_ERR_INPUT, "points must be finite");
+    const int fp32 = precision == WOS_FP32_PERSISTENT;
+    if (fp32) {
+        if (c->tune.grid_cells == 0) {
+            c->view.grids = nullptr; c->view.cells = nullptr; c->view.glist = nullptr;
+        } else if (!c->grid_valid || c->grid_built_cells != c->tune.grid_cells ||
+                   c->grid_built_cap != c->tune.grid_list_max ||
+                   c->grid_built_relax != c->tune.relax) {
+            return fail(WOS_ERR_STATE, "the FP32 distance grid of this mesh and tuning is not built "
+                                       "(building it allocates): make one FP32 call on this "
+                                       "context first, e.g. wos_closest_points");
+        } else {
+            c->view.grids = c->g_info.as<GridInfo>();
+            c->view.cells = c->g_cells.as<GridCellF>();
+            c->view.glist = c->g_list.as<GridEntry>();
+        }
+    }
+    HIPX(hipSetDevice(c->device));
+    unsigned char *base = static_cast<unsigned char *>(c->scratch_dev);
+    double *d_pts = reinterpret_cast<double *>(base);
+    double *d_dist = reinterpret_cast<double *>(base + b_pts);
+    double *d_low = reinterpret_cast<double *>(base + b_pts + b_d);
+    double *d_near = reinterpret_cast<double *>(base + b_pts + 2 * b_d);
+    int32_t *d_prim = reinterpret_cast<int32_t *>(base + b_pts + 2 * b_d + b_near);
+    HIPX(hipMemcpyAsync(d_pts, points, (size_t)n * dim * sizeof(double), hipMemcpyHostToDevice,
+                        c->stream));
+    const DomainInfo &D = c->L.domains[domain];
+    const float cf = (float)((1.0 - relax) * (1.0 - relax));
+    const int blocks = (int)((n + kBlock - 1) / kBlock);
+    const size_t lds = (size_t)kBlock * std::max((c->L.depth + 2) * sizeof(int),
+                                                 (c->L.depth + 1) * sizeof(StackEntry));
+    if (dim == 2)
+        hipLaunchKernelGGL(k_closest<2>, dim3(blocks), dim3(kBlock), lds, c->stream, c->view,
+                           D.node_off, D.root_ref, domain, fp32, cf, d_pts, n, d_dist, d_low,
+                           d_near, d_prim);
+    else
+        hipLaunchKernelGGL(k_closest<3>, dim3(blocks), dim3(kBlock), lds, c->stream, c->view,
+                           D.node_off, D.root_ref, domain, fp32, cf, d_pts, n, d_dist, d_low,
+                           d_near, d_prim);
+    HIPX(hipGetLastError());
+    HIPX(hipMemcpyAsync(dist, d_dist, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    if (lower)
+        HIPX(hipMemcpyAsync(lower, d_low, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    if (nearest)
+        HIPX(hipMemcpyAsync(nearest, d_near, (size_t)n * dim * sizeof(double), hipMemcpyDeviceToHost,
+                            c->stream));
+    if (prim)
+        HIPX(hipMemcpyAsync(prim, d_prim, (size_t)n * sizeof(int32_t), hipMemcpyDeviceToHost,
                             c->stream));
     HIPX(hipStreamSynchronize(c->stream));
     return WOS_OK;

@@ -253,6 +253,7 @@ def load(path: str | None = None):
                                        C.c_int, C.c_double, C.POINTER(C.c_double),
                                        C.POINTER(C.c_double), C.POINTER(C.c_double),
                                        C.POINTER(C.c_int32)]
+    lib.wos_closest_points_noalloc.argtypes = lib.wos_closest_points.argtypes
     lib.wos_ctx_autotune.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.POINTER(EqDesc),
                                      C.POINTER(RunParams), C.POINTER(Tuning)]
     lib.wos_ctx_set_tuning.argtypes = [C.c_void_p, C.POINTER(Tuning)]

@@ -138,9 +138,12 @@ class Context:
         p.use_graph = use_graph
         return p
 
-    def closest_points(self, points, precision=WOS_FP64_PLAIN, relax=0.0, domain=0):
+    def closest_points(self, points, precision=WOS_FP64_PLAIN, relax=0.0, domain=0,
+                       noalloc=False):
         """Closest boundary point per point: dict(dist, lower, nearest, prim)
-        (bvh_npq of the reference, include/wos/bvh.hpp:38-39)."""
+        (bvh_npq of the reference, include/wos/bvh.hpp:38-39).  noalloc:
+        the variant that stages in the library's fixed device array
+        (wos_closest_points_noalloc)."""
         dim = self.dim
         pts = np.ascontiguousarray(points, np.float64)
         n = len(pts)
@@ -148,7 +151,8 @@ class Context:
         dist, lower = np.zeros(n), np.zeros(n)
         nearest = np.zeros((n, dim))
         prim = np.zeros(n, np.int32)
-        rc = self.lib.wos_closest_points(self.h, domain, ptr(pts, C.c_double), n, precision, relax,
+        fn = self.lib.wos_closest_points_noalloc if noalloc else self.lib.wos_closest_points
+        rc = fn(self.h, domain, ptr(pts, C.c_double), n, precision, relax,
                                          ptr(dist, C.c_double), ptr(lower, C.c_double),
                                          ptr(nearest, C.c_double), ptr(prim, C.c_int32))
         capi.check(self.lib, rc)
