@@ -1072,7 +1072,36 @@ struct QueryF {
     int32_t best_slot;
     uint32_t ckey;    // 1 + index of the grid cell held in `cell`, 0 for none
     GridCellF cell;   // the last grid cell read (cells do not change during a launch)
+    int32_t tslot;    // slot of the triangle in the lane's LDS copy (-1: none)
 };
+
+// A lane's LDS copy of one triangle record, three 16 B rows `stride`
+// float4 apart (rows of consecutive lanes are adjacent: conflict-free).
+struct TriCache {
+    float4 *row;
+    int stride;
+};
+
+// the triangle of `slot` from the lane's LDS copy, or from global memory
+// (then copied there); the walker's steps near the boundary keep testing
+// the same cell's first list entry
+__device__ __forceinline__ TriF tri_cached(const MeshView &m, QueryF &q, const TriCache &tc,
+                                           int slot) {
+    float4 r0, r1, r2;
+    if (slot == q.tslot) {
+        r0 = tc.row[0]; r1 = tc.row[tc.stride]; r2 = tc.row[2 * tc.stride];
+    } else {
+        const float4 *g = reinterpret_cast<const float4 *>(m.trif + slot);
+        r0 = g[0]; r1 = g[1]; r2 = g[2];
+        tc.row[0] = r0; tc.row[tc.stride] = r1; tc.row[2 * tc.stride] = r2;
+        q.tslot = slot;
+    }
+    TriF T;
+    T.ox = r0.x; T.oy = r0.y; T.oz = r0.z; T.l1 = r0.w;
+    T.ux = r1.x; T.uy = r1.y; T.uz = r1.z; T.x2 = r1.w;
+    T.wx = r2.x; T.wy = r2.y; T.wz = r2.z; T.y2 = r2.w;
+    return T;
+}
 
 // Set the next unit from a child reference.
 __device__ __forceinline__ void query_enter(QueryF &q, int32_t ref) {
@@ -1128,7 +1157,7 @@ __device__ __forceinline__ T domain_record(const T *arr, int index) {
 template <int DIM>
 __device__ __forceinline__ uint32_t query_begin(QueryF &q, const MeshView &m, int32_t root,
                                                 int domain, float cf, float px, float py,
-                                                float pz) {
+                                                float pz, TriCache tc = TriCache{nullptr, 0}) {
     q.lb = INFINITY;
     q.cf = cf;
     q.sp = 0;
@@ -1166,7 +1195,12 @@ __device__ __forceinline__ uint32_t query_begin(QueryF &q, const MeshView &m, in
                 // entry 0 now; the list only if entry 1's bound is below cf x best
                 q.indirect = true;
                 q.t = t;
-                q.best = leaf_d2f<DIM>(m, C.slot0, px, py, pz);
+                if constexpr (DIM == 3) {
+                    if (tc.row) q.best = tri_d2f(tri_cached(m, q, tc, C.slot0), px, py, pz);
+                    else q.best = leaf_d2f<DIM>(m, C.slot0, px, py, pz);
+                } else {
+                    q.best = leaf_d2f<DIM>(m, C.slot0, px, py, pz);
+                }
                 q.best_slot = C.slot0;
                 q.lslot = (int32_t)C.first + 1;
                 q.lcount = (int32_t)len - 1;

@@ -1185,6 +1185,9 @@ struct PersistArgs {
     int chunk;
     int n_cached;        // internal BVH nodes staged in LDS (breadth-first prefix)
     int cache_bytes;     // LDS bytes reserved for them (16 B aligned)
+    int tri_off;         // 3D: LDS offset of the lanes' triangle copies (-1: none)
+    StackEntry *gstack;  // traversal stacks in global memory: entry k of lane g at
+    int gstride;         // gstack[k * gstride + g], gstride = the launch's lanes
     double *vals;
     int32_t *steps_out;
     unsigned long long *units;  // 7 query-unit counters, or nullptr
@@ -1213,8 +1216,15 @@ void k_walk_fp32(MeshView m, EqView e, PersistArgs a, AdjOut adj) {
         __syncthreads();
     }
     const void *cache = lds_raw;
-    StackEntry *stk = reinterpret_cast<StackEntry *>(lds_raw + a.cache_bytes) + threadIdx.x;
-    const int stride = blockDim.x;
+    // the BVH traversal stack (needed only where the distance grid cannot
+    // answer, rare with a grid) lives in global memory, so that the LDS does
+    // not limit occupancy for deep trees
+    StackEntry *stk = a.gstack + (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int stride = a.gstride;
+    const TriCache tcp{DIM == 3 && a.tri_off > 0
+                           ? reinterpret_cast<float4 *>(lds_raw + a.tri_off) + threadIdx.x
+                           : nullptr,
+                       (int)blockDim.x};
     const int lane = __lane_id();
 
     // lane state
@@ -1228,6 +1238,7 @@ void k_walk_fp32(MeshView m, EqView e, PersistArgs a, AdjOut adj) {
     size_t out_index = 0;
     QueryF q{};
     q.best_slot = -1;
+    q.tslot = -1;
     const float alpha = (float)e.alpha;
     WalkDer D{};
 
@@ -1254,7 +1265,7 @@ void k_walk_fp32(MeshView m, EqView e, PersistArgs a, AdjOut adj) {
             R = Fx<float>::sqrt(exact ? q.best : q.lb) * a.shrink_rel - margin;
             const bool stop = !(R > a.eps_f) || steps >= a.max_steps;
             if (stop && !exact) {
-                qb = query_begin<DIM>(q, m, root, dom, 1.0f, px, py, pz);
+                qb = query_begin<DIM>(q, m, root, dom, 1.0f, px, py, pz, tcp);
                 searching = (qb & kQsMore) != 0u;
                 began = true;
             } else if (stop) {
@@ -1390,7 +1401,7 @@ void k_walk_fp32(MeshView m, EqView e, PersistArgs a, AdjOut adj) {
                 pz = __builtin_fmaf(R, z, pz);
             }
             ++steps;
-            qb = query_begin<DIM>(q, m, root, dom, a.cf, px, py, pz);
+            qb = query_begin<DIM>(q, m, root, dom, a.cf, px, py, pz, tcp);
             searching = (qb & kQsMore) != 0u;
             began = true;
         }
@@ -1645,6 +1656,7 @@ struct wos_ctx {
     DevBuf nd2, nd3, segd, trid, slot_prim, prim_bid, prims, verts, nf2, nf3, segf, trif;
     DevBuf cq_out, cq_prim;
     DevBuf units;                     // FP32 query-unit counters
+    DevBuf wstack;                    // FP32 walker traversal stacks (every lane of a full GPU)
     DevBuf failed;                    // walks with a NaN value (one counter)
     std::vector<std::pair<TuneKey, wos_tuning>> tuned;  // autotuned shapes
     // graph replay: the captured solve, what it was captured for, its tolerances
@@ -1829,6 +1841,14 @@ hipError_t ensure_grid(wos_ctx *c) {
     return hipSuccess;
 }
 
+// Global traversal stacks of the FP32 walker: depth + 1 entries for every
+// lane the GPU can hold (2048 per CU), sized once per mesh, before any
+// graph capture.
+hipError_t ensure_walk_stack(wos_ctx *c) {
+    const size_t need = (size_t)c->n_cu * 2048 * (c->L.depth + 1) * sizeof(StackEntry);
+    return c->wstack.reserve(need);
+}
+
 template <typename K>
 hipError_t launch_classify(wos_ctx *c, K kernel, const EqView &ev, const GridSpec &g,
                            const DomainView &dv, const double *plist, const int64_t *sidx,
@@ -1916,7 +1936,9 @@ hipError_t launch_walk(wos_ctx *c, const EqView &ev, int precision, const RoundV
                       : mode == 2 ? (const void *)k_walk_fp32<DIM, 2, SCR, ADJ>
                                   : (const void *)k_walk_fp32<DIM, SMODE, SCR, ADJ>;
     const int threads = c->tune.block_threads;
-    const size_t stack_bytes = (size_t)threads * (c->L.depth + 1) * sizeof(StackEntry);
+    // LDS per block beyond the node cache: one triangle record per thread (3D)
+    const size_t tri_bytes = DIM == 3 ? (size_t)threads * 3 * 16 + 16 : 0;
+    const size_t stack_bytes = tri_bytes;
     const int node_bytes = DIM == 3 ? (int)sizeof(NodeF3) : (int)sizeof(NodeF2);
     const int n_internal = DIM == 3 ? (int)c->L.nf3.size() : (int)c->L.nf2.size();
     // occupancy allowed by registers alone, then the LDS each block may use
@@ -1940,9 +1962,15 @@ hipError_t launch_walk(wos_ctx *c, const EqView &ev, int precision, const RoundV
     if (e != hipSuccess) return e;
     per_cu2 = std::max(1, std::min(per_cu2, per_cu));
     const int blocks = c->n_cu * per_cu2;
+    if ((size_t)blocks * threads * (c->L.depth + 1) * sizeof(StackEntry) > c->wstack.bytes)
+        return hipErrorInvalidValue;  // ensure_walk_stack sizes it for any launch
     PersistArgs a;
+    a.gstack = c->wstack.as<StackEntry>();
+    a.gstride = blocks * threads;
     a.n_cached = n_cached;
     a.cache_bytes = cache_bytes;
+    // the triangle rows after the node cache and the stacks (16 B aligned)
+    a.tri_off = DIM == 3 ? (int)(cache_bytes + 15) / 16 * 16 : -1;
     a.rv = rv;
     a.max_steps = max_steps;
     a.eps = eps;
@@ -2401,7 +2429,10 @@ int solve_once(wos_ctx *c, const std::vector<Part> &parts, const wos_eq_desc *eq
     if (rc != WOS_OK) return rc;
     const int dim = c->mesh.dim;
     // the distance grid is built (once per mesh) outside the timed walks
-    if (prm->precision == WOS_FP32_PERSISTENT) HIPX(ensure_grid(c));
+    if (prm->precision == WOS_FP32_PERSISTENT) {
+        HIPX(ensure_grid(c));
+        HIPX(ensure_walk_stack(c));
+    }
     HIPX(hipEventRecord(c->ev[0], c->stream));
     std::vector<double> part_tol;
     rc = classify_parts(c, parts, ev, prm, part_tol);
@@ -2730,6 +2761,7 @@ int solve_graph(wos_ctx *c, const std::vector<Part> &parts, const wos_eq_desc *e
         HIPX(upload(c->pid_all, ids, c->stream));
     }
     HIPX(ensure_grid(c));
+    HIPX(ensure_walk_stack(c));
     c->count_units = false;
     int32_t *n_in_dev = c->g_misc.as<int32_t>();
     int64_t *total_dev = reinterpret_cast<int64_t *>(c->g_misc.as<uint8_t>() + 8);
@@ -2744,7 +2776,7 @@ int solve_graph(wos_ctx *c, const std::vector<Part> &parts, const wos_eq_desc *e
     for (const DevBuf *b : {&c->loc, &c->starts, &c->starts_f, &c->amb, &c->bvalue, &c->inside, &c->g_bcount,
                             &c->g_boff, &c->g_misc, &c->mean, &c->m2, &c->count, &c->ssum,
                             &c->hits, &c->vals, &c->steps, &c->counter, &c->pid_all, &c->vv,
-                            &c->failed,
+                            &c->failed, &c->wstack,
                             &c->field, &c->term_pw, &c->term_slot})
         key_put(key, b->p);
     auto body = [&]() -> int {
