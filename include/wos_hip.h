@@ -166,7 +166,10 @@ typedef struct wos_run_params {
     int32_t max_walks;           /* adaptive cap per point                         */
     int32_t tune;                /* 0: the context's launch shape; 1 (FP32): the
                                     autotuned one (wos_ctx_autotune), found on
-                                    first use of a problem; results are the same  */
+                                    first use of a problem; results are the same;
+                                    2: also the tuned BVH width (the walks where
+                                    the tree is searched may change, their
+                                    expectation does not)                          */
     int32_t count_units;         /* 1: count the FP32 walker's query units into the
                                     summary (wavefront ballots; off by default)    */
     int32_t use_graph;           /* 1: run a fixed-walk FP32 grid solve as one
@@ -228,7 +231,7 @@ typedef struct wos_run_summary {
 } wos_run_summary;
 
 /* Launch shape of the FP32 persistent walker (set by the autotuner; a new
- * context starts from {128, -1, 64, 0, 40, 0.2, -1, 127, 8}). */
+ * context starts from {128, -1, 64, 0, 40, 0.2, -1, 127, 8, 2, 0}). */
 typedef struct wos_tuning {
     int32_t block_threads;       /* 64, 128, 192 or 256 walkers per block          */
     int32_t lds_nodes;           /* BVH nodes cached in LDS per block; -1 = all that
@@ -258,6 +261,14 @@ typedef struct wos_tuning {
                                     workgroup b starts on part b % xcd_parts, which
                                     keeps each XCD's L2 on one region; 8 = the
                                     XCDs of an MI355X, 1 = one shared range       */
+    int32_t bvh_width;           /* 2 or 4: the FP32 walker's tree where the distance
+                                    grid has no list (the binary tree, or the same
+                                    tree with every second level folded into 4-wide
+                                    nodes); the order of a relaxed search, and so a
+                                    walk, can change with it, not its expectation  */
+    int32_t stack_lds;           /* 0..64: entries of each lane's traversal stack
+                                    kept in LDS, the rest in global memory; never
+                                    changes a result                               */
 } wos_tuning;
 
 typedef struct wos_ctx wos_ctx;
@@ -348,12 +359,14 @@ int wos_ctx_get_tuning(const wos_ctx *ctx, wos_tuning *tuning);
 /* Autotune the FP32 walker's launch shape on an Nx x Ny x Nz grid problem
  * over every domain of the context (the problem wos_solve_grid_batch would
  * solve): pilot solves of about 16 M walks, coordinate descent over walkers
- * per block, lanes per walk-step phase, refill chunk, LDS node cache and
- * xcd_parts.
+ * per block, lanes per walk-step phase, refill chunk, LDS node cache,
+ * xcd_parts and the LDS share of the traversal stack (stack_lds).
  * These fields never change a result, so a tuned solve is bit-identical to
- * an untuned one; grid_cells, grid_list_max and relax are kept as set.  The
+ * an untuned one; grid_cells, grid_list_max and relax are kept as set.  With
+ * params->tune = 2 the search also takes the BVH width (2 or 4), which can
+ * change walks where the tree is searched (not their expectation).  The
  * chosen shape becomes the context's tuning, is written to `chosen` (may be
- * NULL) and is cached for solves with params->tune = 1, which run this
+ * NULL) and is cached for solves with params->tune = 1 or 2, which run this
  * search themselves on first use of a problem.  No reference counterpart. */
 int wos_ctx_autotune(wos_ctx *ctx, int nx, int ny, int nz, const wos_eq_desc *eq,
                      const wos_run_params *params, wos_tuning *chosen);

@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <limits>
 
 namespace wosx {
 
@@ -162,6 +163,87 @@ static TriF triangle_frame(const HostMesh &m, int32_t i0, int32_t i1, int32_t i2
     return f;
 }
 
+namespace {
+
+// One child of a 4-wide node: its box (FP32, already rounded outward) and
+// its reference in the binary forest.
+struct WideChild { float lo[3], hi[3]; int32_t ref; };
+
+// The 4-wide forest from the binary FP32 forest: wide node of binary node b
+// holds b's children, each internal child replaced by its two children.
+// Numbered breadth-first over all domains (roots first), like the binary
+// forest, so the top levels form a prefix.  Leaf references keep their code.
+void build_wide(DeviceLayouts &L, int dim) {
+    auto children = [&](int32_t b, WideChild out[2]) {
+        if (dim == 3) {
+            const NodeF3 &n = L.nf3[(size_t)b];
+            out[0] = WideChild{{n.l_lox, n.l_loy, n.l_loz}, {n.l_hix, n.l_hiy, n.l_hiz}, n.left};
+            out[1] = WideChild{{n.r_lox, n.r_loy, n.r_loz}, {n.r_hix, n.r_hiy, n.r_hiz}, n.right};
+        } else {
+            const NodeF2 &n = L.nf2[(size_t)b];
+            out[0] = WideChild{{n.l_lox, n.l_loy, 0.0f}, {n.l_hix, n.l_hiy, 0.0f}, n.left};
+            out[1] = WideChild{{n.r_lox, n.r_loy, 0.0f}, {n.r_hix, n.r_hiy, 0.0f}, n.right};
+        }
+    };
+    std::vector<int32_t> queue, level;  // binary internal nodes of the wide nodes
+    std::vector<std::vector<WideChild>> kids;
+    for (DomainInfo &D : L.domains) {
+        if (D.root_ref >= 0) {
+            D.root_ref_w = (int32_t)queue.size();
+            queue.push_back(D.root_ref);
+            level.push_back(1);
+        } else {
+            D.root_ref_w = D.root_ref;  // a leaf root
+        }
+    }
+    L.depth_w = 0;
+    for (size_t h = 0; h < queue.size(); ++h) {
+        WideChild c2[2], g2[2];
+        children(queue[h], c2);
+        std::vector<WideChild> k;
+        for (const WideChild &c : c2) {
+            if (c.ref < 0) {
+                k.push_back(c);
+                continue;
+            }
+            children(c.ref, g2);
+            for (WideChild g : g2) {
+                if (g.ref >= 0) {
+                    const int32_t wi = (int32_t)queue.size();
+                    queue.push_back(g.ref);
+                    level.push_back(level[h] + 1);
+                    g.ref = wi;
+                }
+                k.push_back(g);
+            }
+        }
+        L.depth_w = std::max(L.depth_w, level[h]);
+        kids.push_back(std::move(k));
+    }
+    const float inf = std::numeric_limits<float>::infinity();
+    if (dim == 3) L.nw3.assign(queue.size(), NodeF3W{}); else L.nw2.assign(queue.size(), NodeF2W{});
+    for (size_t h = 0; h < queue.size(); ++h) {
+        for (int j = 0; j < 4; ++j) {
+            const bool used = j < (int)kids[h].size();
+            const WideChild c = used ? kids[h][(size_t)j]
+                                     : WideChild{{inf, inf, inf}, {-inf, -inf, -inf}, 0};
+            if (dim == 3) {
+                NodeF3W &n = L.nw3[h];
+                n.lox[j] = c.lo[0]; n.loy[j] = c.lo[1]; n.loz[j] = c.lo[2];
+                n.hix[j] = c.hi[0]; n.hiy[j] = c.hi[1]; n.hiz[j] = c.hi[2];
+                n.ref[j] = c.ref;
+            } else {
+                NodeF2W &n = L.nw2[h];
+                n.lox[j] = c.lo[0]; n.loy[j] = c.lo[1];
+                n.hix[j] = c.hi[0]; n.hiy[j] = c.hi[1];
+                n.ref[j] = c.ref;
+            }
+        }
+    }
+}
+
+}  // namespace
+
 DeviceLayouts make_layouts(const std::vector<const HostMesh *> &meshes,
                            const std::vector<const HostBVH *> &bvhs) {
     DeviceLayouts L;
@@ -295,6 +377,7 @@ DeviceLayouts make_layouts(const std::vector<const HostMesh *> &meshes,
             L.nf3[remap[i]] = n;
         }
     }
+    build_wide(L, dim);
     return L;
 }
 

@@ -67,6 +67,20 @@ struct alignas(16) NodeF2 {
     float r_lox, r_loy, r_hix, r_hiy;
     int32_t left, right, pad0, pad1;
 };
+// FP32 4-wide nodes: the binary FP32 tree with every second level folded in
+// (a node's children are its binary children, or their children where those
+// are internal).  Up to four children's boxes (rounded outward, each
+// coordinate a row of four) and references; an unused child has the empty
+// box lo = +inf, hi = -inf, which is never entered.
+struct alignas(16) NodeF3W {
+    float lox[4], loy[4], loz[4], hix[4], hiy[4], hiz[4];
+    int32_t ref[4];
+    int32_t pad[4];
+};
+struct alignas(16) NodeF2W {
+    float lox[4], loy[4], hix[4], hiy[4];
+    int32_t ref[4];
+};
 // FP32 leaf slots.  A triangle (48 B: three 16 B loads) in its own frame,
 // formed in FP64: origin o at one end of its longest edge, unit vector u
 // along that edge, unit vector w in the plane towards the third vertex
@@ -86,6 +100,7 @@ struct DomainInfo {
     int64_t prim_off = 0, n_prims = 0;   // primitives == leaf slots
     int32_t node_off = 0;                // FP64 root node (reference order)
     int32_t root_ref = 0;                // FP32 root reference
+    int32_t root_ref_w = 0;              // FP32 4-wide root reference
     int depth = 0;
     double bounds[6] = {0, 0, 0, 0, 0, 0};  // bbox of the domain's vertices (mesh.cpp:296-322)
     double coord_scale = 1.0;               // max |coordinate| of this domain, >= 1
@@ -98,11 +113,14 @@ struct DeviceLayouts {
     std::vector<TriD> trid;     // slot order
     std::vector<NodeF2> nf2;
     std::vector<NodeF3> nf3;
+    std::vector<NodeF2W> nw2;   // the 4-wide forest (breadth-first)
+    std::vector<NodeF3W> nw3;
     std::vector<SegF> segf;
     std::vector<TriF> trif;
     std::vector<int32_t> slot_prim; // global slot -> global primitive
     std::vector<DomainInfo> domains;
     int depth = 0;             // deepest domain
+    int depth_w = 0;           // deepest domain of the 4-wide forest (internal levels)
     double coord_scale = 1.0;  // max |coordinate| over all domains, >= 1
 };
 

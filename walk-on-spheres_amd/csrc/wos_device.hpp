@@ -77,9 +77,11 @@ struct MeshView {
     const int32_t *prim_bid;   // primitive -> boundary id (2D)
     const int32_t *prims;      // primitive -> vertex ids (dim per primitive)
     const double *verts;       // vertex coordinates (dim per vertex)
-    // FP32, child-box layout
+    // FP32, child-box layout (binary and 4-wide)
     const NodeF2 *nf2;
     const NodeF3 *nf3;
+    const NodeF2W *nw2;
+    const NodeF3W *nw3;
     const SegF *segf;
     const TriF *trif;
     // distance grid (nullptr when off)
@@ -1047,6 +1049,23 @@ __device__ __forceinline__ float leaf_d2f(const MeshView &m, int slot, float px,
 
 struct StackEntry { int32_t ref; float d2; };
 
+// A lane's traversal stack: entries 0 .. klds-1 in LDS (`lds`, `lstride`
+// entries apart), deeper ones in global memory (`glob`, `gstride` apart).
+struct StackRef {
+    StackEntry *lds;
+    int lstride;
+    StackEntry *glob;
+    int gstride;
+    int klds;
+    __device__ __forceinline__ void put(int i, StackEntry e) const {
+        if (i < klds) lds[i * lstride] = e;
+        else glob[i * gstride] = e;
+    }
+    __device__ __forceinline__ StackEntry get(int i) const {
+        return i < klds ? lds[i * lstride] : glob[i * gstride];
+    }
+};
+
 // Resumable closest-point query of one lane.  A query advances by one unit
 // per call of step(): one internal node (both child boxes) or one leaf
 // primitive, so lanes of a wave whose queries differ in length by orders of
@@ -1243,9 +1262,9 @@ __device__ __forceinline__ float leaf_dist(const LeafRec<DIM> &r, float px, floa
 // One unit: an internal node, a BVH leaf primitive, or the next two entries
 // of a cell list.  Returns the kQs* bits of what it did; kQsMore is clear
 // when the query is complete.
-template <int DIM>
+template <int DIM, int W = 2>
 __device__ __forceinline__ uint32_t query_step(QueryF &q, const MeshView &m, float px, float py,
-                                               float pz, StackEntry *stk, int stride,
+                                               float pz, const StackRef &stk,
                                                const void *cache, int n_cached) {
     bool pop;
     uint32_t did = 0u;
@@ -1300,6 +1319,51 @@ __device__ __forceinline__ uint32_t query_step(QueryF &q, const MeshView &m, flo
         --q.lcount;
         did |= kQsLeaf;
         pop = q.lcount == 0;
+    } else if constexpr (W == 4) {
+        // a 4-wide node: the children whose boxes are below cf x best, the
+        // nearest entered and the others pushed farthest first
+        float d[4];
+        int32_t c[4];
+        const int32_t ref = q.cur;
+        if constexpr (DIM == 3) {
+            const NodeF3W &n = ref < n_cached ? static_cast<const NodeF3W *>(cache)[ref] : m.nw3[ref];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                d[k] = box_d2f(n.lox[k], n.loy[k], n.loz[k], n.hix[k], n.hiy[k], n.hiz[k], px, py, pz);
+                c[k] = n.ref[k];
+            }
+        } else {
+            const NodeF2W &n = ref < n_cached ? static_cast<const NodeF2W *>(cache)[ref] : m.nw2[ref];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                d[k] = box_d2f2(n.lox[k], n.loy[k], n.hix[k], n.hiy[k], px, py);
+                c[k] = n.ref[k];
+            }
+        }
+        did |= kQsNode;
+        const float thr = q.best * q.cf;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const bool hit = d[k] < thr;
+            if (!hit && d[k] < q.best) q.lb = fminf(q.lb, d[k]);
+            if (!hit) d[k] = INFINITY;
+        }
+        // sorting network on (distance, reference): ascending
+        auto cx = [&](int i, int j) {
+            const bool sw = d[j] < d[i];
+            const float di = d[i], dj = d[j];
+            const int32_t ci = c[i], cj = c[j];
+            d[i] = sw ? dj : di; d[j] = sw ? di : dj;
+            c[i] = sw ? cj : ci; c[j] = sw ? ci : cj;
+        };
+        cx(0, 1); cx(2, 3); cx(0, 2); cx(1, 3); cx(1, 2);
+        pop = !(d[0] < INFINITY);
+        if (!pop) {
+            if (d[3] < INFINITY) stk.put(q.sp++, StackEntry{c[3], d[3]});
+            if (d[2] < INFINITY) stk.put(q.sp++, StackEntry{c[2], d[2]});
+            if (d[1] < INFINITY) stk.put(q.sp++, StackEntry{c[1], d[1]});
+            query_enter(q, c[0]);
+        }
     } else {
         float dl, dr;
         int32_t cl, cr;
@@ -1326,7 +1390,7 @@ __device__ __forceinline__ uint32_t query_step(QueryF &q, const MeshView &m, flo
             StackEntry far;
             far.ref = left_first ? cr : cl;
             far.d2 = left_first ? dr : dl;
-            stk[(q.sp++) * stride] = far;
+            stk.put(q.sp++, far);
             query_enter(q, left_first ? cl : cr);
         } else if (hl) {
             query_enter(q, cl);
@@ -1338,7 +1402,7 @@ __device__ __forceinline__ uint32_t query_step(QueryF &q, const MeshView &m, flo
     }
     if (pop) {
         while (q.sp > 0) {
-            const StackEntry e = stk[(--q.sp) * stride];
+            const StackEntry e = stk.get(--q.sp);
             if (e.d2 < q.best * q.cf) {
                 query_enter(q, e.ref);
                 return did | kQsMore;

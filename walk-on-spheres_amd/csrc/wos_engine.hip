@@ -603,7 +603,7 @@ __global__ __launch_bounds__(kBlock) void k_grid_build(MeshView m, GridInfo G, i
 // per lane.  FP64 runs the reference traversal; FP32 runs the walker's query
 // (pruning factor cf) and projects onto the primitive it found in FP64.
 // --------------------------------------------------------------------------
-template <int DIM>
+template <int DIM, int W>
 __global__ __launch_bounds__(kBlock) void k_closest(MeshView m, int32_t root64, int32_t root32,
                                                     int domain, int fp32, float cf, const double *pts,
                                                     int64_t n, double *dist, double *lower,
@@ -622,13 +622,14 @@ __global__ __launch_bounds__(kBlock) void k_closest(MeshView m, int32_t root64, 
         prim[i] = q.prim;
         nx = q.px; ny = q.py; nz = q.pz;
     } else {
-        StackEntry *stk = reinterpret_cast<StackEntry *>(lds_c) + threadIdx.x;
+        const StackRef stk{reinterpret_cast<StackEntry *>(lds_c) + threadIdx.x, (int)blockDim.x,
+                           nullptr, 0, 1 << 30};
         const float fx = (float)x, fy = (float)y, fz = (float)z;
         QueryF q{};
         q.best_slot = -1;
         bool more = (query_begin<DIM>(q, m, root32, domain, cf, fx, fy, fz) & kQsMore) != 0u;
         while (more)
-            more = (query_step<DIM>(q, m, fx, fy, fz, stk, blockDim.x, nullptr, 0) & kQsMore) != 0u;
+            more = (query_step<DIM, W>(q, m, fx, fy, fz, stk, nullptr, 0) & kQsMore) != 0u;
         dist[i] = sqrt((double)q.best);
         lower[i] = sqrt((double)fminf(q.best, q.lb));
         const int slot = q.best_slot;
@@ -1188,6 +1189,8 @@ struct PersistArgs {
     int tri_off;         // 3D: LDS offset of the lanes' triangle copies (-1: none)
     StackEntry *gstack;  // traversal stacks in global memory: entry k of lane g at
     int gstride;         // gstack[k * gstride + g], gstride = the launch's lanes
+    int klds;            // stack entries per lane held in LDS (the first ones)
+    int stack_off;       // LDS offset of those entries
     double *vals;
     int32_t *steps_out;
     unsigned long long *units;  // 7 query-unit counters, or nullptr
@@ -1198,20 +1201,26 @@ struct PersistArgs {
     int32_t *term_slot;
 };
 
-// Forward walks: at least 4 wavefronts per SIMD (at most 128 VGPRs), which
-// every forward variant fits without spilling (more wavefronts measured no
-// faster).  Adjoint replays carry the FP64 parameter derivatives and may use
-// up to 256 VGPRs rather than spill them.
-template <int DIM, int SMODE, bool SCR, bool ADJ>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(ADJ ? 2 : 4, 8)))
+// Forward walks: at least 5 wavefronts per SIMD (at most 96 VGPRs) in 3D
+// without Green-function sampling, which fit without spilling, else 4 (at
+// most 128 VGPRs; the 3D Green samplers would spill at 96).  Adjoint
+// replays carry the FP64 parameter derivatives and may use up to 256 VGPRs
+// rather than spill them.
+template <int DIM, int SMODE, bool SCR, bool ADJ, int W>
+__global__ __launch_bounds__(kBlock)
+__attribute__((amdgpu_waves_per_eu(ADJ ? 2 : (DIM == 3 && SMODE != 2 ? 5 : 4), 8)))
 void k_walk_fp32(MeshView m, EqView e, PersistArgs a, AdjOut adj) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     // stage the top of the tree once per block
     {
-        const int4 *src = DIM == 3 ? reinterpret_cast<const int4 *>(m.nf3)
-                                   : reinterpret_cast<const int4 *>(m.nf2);
+        const int4 *src = W == 4 ? (DIM == 3 ? reinterpret_cast<const int4 *>(m.nw3)
+                                             : reinterpret_cast<const int4 *>(m.nw2))
+                                 : (DIM == 3 ? reinterpret_cast<const int4 *>(m.nf3)
+                                             : reinterpret_cast<const int4 *>(m.nf2));
         int4 *dst = reinterpret_cast<int4 *>(lds_raw);
-        const int n16 = a.n_cached * (int)(DIM == 3 ? sizeof(NodeF3) : sizeof(NodeF2)) / 16;
+        constexpr int kNodeBytes = W == 4 ? (DIM == 3 ? (int)sizeof(NodeF3W) : (int)sizeof(NodeF2W))
+                                          : (DIM == 3 ? (int)sizeof(NodeF3) : (int)sizeof(NodeF2));
+        const int n16 = a.n_cached * kNodeBytes / 16;
         for (int i = threadIdx.x; i < n16; i += blockDim.x) dst[i] = src[i];
         __syncthreads();
     }
@@ -1219,8 +1228,9 @@ void k_walk_fp32(MeshView m, EqView e, PersistArgs a, AdjOut adj) {
     // the BVH traversal stack (needed only where the distance grid cannot
     // answer, rare with a grid) lives in global memory, so that the LDS does
     // not limit occupancy for deep trees
-    StackEntry *stk = a.gstack + (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int stride = a.gstride;
+    const StackRef stk{reinterpret_cast<StackEntry *>(lds_raw + a.stack_off) + threadIdx.x,
+                       (int)blockDim.x, a.gstack + (size_t)blockIdx.x * blockDim.x + threadIdx.x,
+                       a.gstride, a.klds};
     const TriCache tcp{DIM == 3 && a.tri_off > 0
                            ? reinterpret_cast<float4 *>(lds_raw + a.tri_off) + threadIdx.x
                            : nullptr,
@@ -1420,7 +1430,7 @@ void k_walk_fp32(MeshView m, EqView e, PersistArgs a, AdjOut adj) {
         for (;;) {
             uint32_t did = 0u;
             if (searching) {
-                did = query_step<DIM>(q, m, px, py, pz, stk, stride, cache, a.n_cached);
+                did = query_step<DIM, W>(q, m, px, py, pz, stk, cache, a.n_cached);
                 searching = (did & kQsMore) != 0u;
             }
             if (a.units) {
@@ -1638,9 +1648,9 @@ hipError_t scratch_address(int device, void **out) {
 struct TuneKey {
     int dim = 0, n_domains = 0;
     int64_t n_prims = 0, n_points = 0;
-    int kind = 0, source_mode = 0, source_kind = 0, grid_cells = 0;
+    int kind = 0, source_mode = 0, source_kind = 0, grid_cells = 0, mode = 0;
     bool operator==(const TuneKey &o) const {
-        return dim == o.dim && n_domains == o.n_domains && n_prims == o.n_prims &&
+        return dim == o.dim && n_domains == o.n_domains && n_prims == o.n_prims && mode == o.mode &&
                n_points == o.n_points && kind == o.kind && source_mode == o.source_mode &&
                source_kind == o.source_kind && grid_cells == o.grid_cells;
     }
@@ -1654,6 +1664,7 @@ struct wos_ctx {
     DeviceLayouts L;
     bool has_mesh = false;
     DevBuf nd2, nd3, segd, trid, slot_prim, prim_bid, prims, verts, nf2, nf3, segf, trif;
+    DevBuf nw2, nw3, doms_w;          // the 4-wide FP32 forest and its domain roots
     DevBuf cq_out, cq_prim;
     DevBuf units;                     // FP32 query-unit counters
     DevBuf wstack;                    // FP32 walker traversal stacks (every lane of a full GPU)
@@ -1690,7 +1701,7 @@ struct wos_ctx {
     // defaults: two wavefronts per block and 40 ready lanes per walk-step
     // phase measured 5.6 % faster than 256 and 48 on the icosphere benchmark
     // and no slower on the plate and star workloads
-    wos_tuning tune{128, -1, 64, 0, 40, 0.2f, -1, 127, 8};
+    wos_tuning tune{128, -1, 64, 0, 40, 0.2f, -1, 127, 8, 2, 0};
     size_t lds_per_cu = 65536;
     size_t lds_per_block = 65536;
     int last_threads = 0;
@@ -1844,8 +1855,15 @@ hipError_t ensure_grid(wos_ctx *c) {
 // Global traversal stacks of the FP32 walker: depth + 1 entries for every
 // lane the GPU can hold (2048 per CU), sized once per mesh, before any
 // graph capture.
+// Stack entries a traversal needs: a binary node pushes at most one entry
+// per level (depth + 1), a 4-wide node at most three (3 depth_w + 2).
+int stack_entries(const wos_ctx *c, int width) {
+    return width == 4 ? 3 * c->L.depth_w + 2 : c->L.depth + 1;
+}
+
 hipError_t ensure_walk_stack(wos_ctx *c) {
-    const size_t need = (size_t)c->n_cu * 2048 * (c->L.depth + 1) * sizeof(StackEntry);
+    const int per_lane = std::max(stack_entries(c, 2), stack_entries(c, 4));
+    const size_t need = (size_t)c->n_cu * 2048 * per_lane * sizeof(StackEntry);
     return c->wstack.reserve(need);
 }
 
@@ -1886,6 +1904,118 @@ void launch_fp64_sx(wos_ctx *c, const EqView &ev, const RoundView &rv, double ep
     hipLaunchKernelGGL((k_walk_fp64<DIM, SRC, SCR, ADJ, SX>), dim3(blocks), dim3(kBlock), lds,
                        c->stream, c->view, ev, rv, eps, max_steps, c->vals.as<double>(),
                        c->steps.as<int32_t>(), adj);
+}
+
+// The FP32 persistent walker on the binary (W = 2) or 4-wide (W = 4) FP32
+// forest: launch shape from the occupancy, LDS layout, arguments.
+template <int DIM, int SMODE, int kM3, int kM4, int kM5, bool SCR, bool ADJ, int W>
+hipError_t launch_fp32_w(wos_ctx *c, const EqView &ev, int mode, const RoundView &rv, double eps,
+                         int max_steps, const AdjOut &adj, int *blocks_out) {
+    const void *kfn = mode == 3   ? (const void *)k_walk_fp32<DIM, kM3, SCR, ADJ, W>
+                      : mode == 4 ? (const void *)k_walk_fp32<DIM, kM4, SCR, ADJ, W>
+                      : mode == 5 ? (const void *)k_walk_fp32<DIM, kM5, SCR, ADJ, W>
+                      : mode == 2 ? (const void *)k_walk_fp32<DIM, 2, SCR, ADJ, W>
+                                  : (const void *)k_walk_fp32<DIM, SMODE, SCR, ADJ, W>;
+    const int threads = c->tune.block_threads;
+    // LDS per block beyond the node cache: one triangle record per thread
+    // (3D) and the first klds entries of every thread's traversal stack
+    const int need = stack_entries(c, W);
+    const int klds = std::min(c->tune.stack_lds, need);
+    const size_t tri_bytes = DIM == 3 ? (size_t)threads * 3 * 16 + 16 : 0;
+    const size_t lstack_bytes = (size_t)threads * klds * sizeof(StackEntry);
+    const size_t stack_bytes = tri_bytes + lstack_bytes;
+    const int node_bytes = W == 4 ? (DIM == 3 ? (int)sizeof(NodeF3W) : (int)sizeof(NodeF2W))
+                                  : (DIM == 3 ? (int)sizeof(NodeF3) : (int)sizeof(NodeF2));
+    const int n_internal = W == 4 ? (DIM == 3 ? (int)c->L.nw3.size() : (int)c->L.nw2.size())
+                                  : (DIM == 3 ? (int)c->L.nf3.size() : (int)c->L.nf2.size());
+    // occupancy allowed by registers alone, then the LDS each block may use
+    // for the node cache without lowering it
+    int per_cu = 0;
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, threads, stack_bytes);
+    if (e != hipSuccess) return e;
+    per_cu = std::max(1, per_cu);
+    if (c->tune.blocks_per_cu > 0) per_cu = std::min(per_cu, c->tune.blocks_per_cu);
+    int n_cached = c->tune.lds_nodes;
+    if (n_cached < 0) {
+        const size_t per_block = std::min<size_t>(c->lds_per_cu / per_cu, c->lds_per_block);
+        n_cached = per_block > stack_bytes ? (int)((per_block - stack_bytes) / node_bytes) : 0;
+    }
+    n_cached = std::max(0, std::min(n_cached, n_internal));
+    const int cache_bytes = (n_cached * node_bytes + 15) / 16 * 16;
+    const size_t lds = stack_bytes + cache_bytes + 16;
+    if (lds > c->lds_per_block) return hipErrorInvalidValue;
+    int per_cu2 = 0;
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu2, kfn, threads, lds);
+    if (e != hipSuccess) return e;
+    per_cu2 = std::max(1, std::min(per_cu2, per_cu));
+    const int blocks = c->n_cu * per_cu2;
+    if ((size_t)blocks * threads * need * sizeof(StackEntry) > c->wstack.bytes)
+        return hipErrorInvalidValue;  // ensure_walk_stack sizes it for any launch
+    PersistArgs a;
+    a.gstack = c->wstack.as<StackEntry>();
+    a.gstride = blocks * threads;
+    a.klds = klds;
+    a.n_cached = n_cached;
+    a.cache_bytes = cache_bytes;
+    // after the node cache: the triangle rows (16 B aligned), then the
+    // LDS part of the stacks
+    a.tri_off = DIM == 3 ? (int)(cache_bytes + 15) / 16 * 16 : -1;
+    a.stack_off = (int)((cache_bytes + tri_bytes + 15) / 16 * 16);
+    if ((size_t)a.stack_off + lstack_bytes > lds) return hipErrorInvalidValue;
+    a.rv = rv;
+    a.max_steps = max_steps;
+    a.eps = eps;
+    a.eps_f = (float)eps;
+    a.shrink_rel = 1.0f - 1.0f / 131072.0f;
+    a.cf = (1.0f - c->tune.relax) * (1.0f - c->tune.relax);
+    a.min_ready = c->tune.min_ready;
+    // refilling once 8 lanes are free rather than at every free lane runs
+    // the refill code (item lookup, start record, domain record) about a
+    // third as often; measured 1.4-2 % faster walks than 1 or 16 on the
+    // icosphere, plate and star workloads, with identical results
+    a.refill_min = 8;
+    a.counter = c->counter.as<unsigned long long>();
+    a.chunk = c->tune.chunk;
+    a.vals = c->vals.as<double>();
+    a.steps_out = c->steps.as<int32_t>();
+    a.units = c->count_units ? c->units.as<unsigned long long>() : nullptr;
+    const bool defer = !ADJ && ev.boundary_kind != WOS_BND_ZERO;
+    a.term_pw = defer ? c->term_pw.as<float4>() : nullptr;
+    a.term_slot = defer ? c->term_slot.as<int32_t>() : nullptr;
+    a.parts = c->tune.xcd_parts;
+    // the 4-wide forest has its own domain roots
+    MeshView view = c->view;
+    if (W == 4) view.doms = c->doms_w.as<DomainF>();
+    hipLaunchKernelGGL(k_init_parts, dim3(1), dim3(64), 0, c->stream, a.counter, a.parts,
+                       (int64_t)rv.total, rv.total_dev);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    if (mode == 3)
+        hipLaunchKernelGGL((k_walk_fp32<DIM, kM3, SCR, ADJ, W>), dim3(blocks), dim3(threads), lds,
+                           c->stream, view, ev, a, adj);
+    else if (mode == 4)
+        hipLaunchKernelGGL((k_walk_fp32<DIM, kM4, SCR, ADJ, W>), dim3(blocks), dim3(threads), lds,
+                           c->stream, view, ev, a, adj);
+    else if (mode == 5)
+        hipLaunchKernelGGL((k_walk_fp32<DIM, kM5, SCR, ADJ, W>), dim3(blocks), dim3(threads), lds,
+                           c->stream, view, ev, a, adj);
+    else if (mode == 2)
+        hipLaunchKernelGGL((k_walk_fp32<DIM, 2, SCR, ADJ, W>), dim3(blocks), dim3(threads), lds,
+                           c->stream, view, ev, a, adj);
+    else
+        hipLaunchKernelGGL((k_walk_fp32<DIM, SMODE, SCR, ADJ, W>), dim3(blocks), dim3(threads), lds,
+                           c->stream, view, ev, a, adj);
+    if (defer) {
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(k_terminal<DIM>, dim3((unsigned)((rv.total + kBlock - 1) / kBlock)),
+                           dim3(kBlock), 0, c->stream, c->view, ev, rv, a.term_pw, a.term_slot,
+                           a.vals);
+    }
+    *blocks_out = blocks;
+    c->last_threads = threads;
+    c->last_cached = n_cached;
+    return hipGetLastError();
 }
 
 template <int DIM, bool SRC, bool SCR, bool ADJ>
@@ -1930,98 +2060,11 @@ hipError_t launch_walk(wos_ctx *c, const EqView &ev, int precision, const RoundV
     constexpr int kM3 = kExt ? 3 : SMODE, kM4 = kExt ? 4 : SMODE, kM5 = kExt ? 5 : SMODE;
     const bool green = ev.source_mode == WOS_SOURCE_GREEN && SRC;
     const int mode = sx == 1 ? 3 : sx == 2 ? (green ? 4 : 5) : green ? 2 : SMODE;
-    const void *kfn = mode == 3   ? (const void *)k_walk_fp32<DIM, kM3, SCR, ADJ>
-                      : mode == 4 ? (const void *)k_walk_fp32<DIM, kM4, SCR, ADJ>
-                      : mode == 5 ? (const void *)k_walk_fp32<DIM, kM5, SCR, ADJ>
-                      : mode == 2 ? (const void *)k_walk_fp32<DIM, 2, SCR, ADJ>
-                                  : (const void *)k_walk_fp32<DIM, SMODE, SCR, ADJ>;
-    const int threads = c->tune.block_threads;
-    // LDS per block beyond the node cache: one triangle record per thread (3D)
-    const size_t tri_bytes = DIM == 3 ? (size_t)threads * 3 * 16 + 16 : 0;
-    const size_t stack_bytes = tri_bytes;
-    const int node_bytes = DIM == 3 ? (int)sizeof(NodeF3) : (int)sizeof(NodeF2);
-    const int n_internal = DIM == 3 ? (int)c->L.nf3.size() : (int)c->L.nf2.size();
-    // occupancy allowed by registers alone, then the LDS each block may use
-    // for the node cache without lowering it
-    int per_cu = 0;
-    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, threads, stack_bytes);
-    if (e != hipSuccess) return e;
-    per_cu = std::max(1, per_cu);
-    if (c->tune.blocks_per_cu > 0) per_cu = std::min(per_cu, c->tune.blocks_per_cu);
-    int n_cached = c->tune.lds_nodes;
-    if (n_cached < 0) {
-        const size_t per_block = std::min<size_t>(c->lds_per_cu / per_cu, c->lds_per_block);
-        n_cached = per_block > stack_bytes ? (int)((per_block - stack_bytes) / node_bytes) : 0;
-    }
-    n_cached = std::max(0, std::min(n_cached, n_internal));
-    const int cache_bytes = (n_cached * node_bytes + 15) / 16 * 16;
-    const size_t lds = stack_bytes + cache_bytes;
-    if (lds > c->lds_per_block) return hipErrorInvalidValue;
-    int per_cu2 = 0;
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu2, kfn, threads, lds);
-    if (e != hipSuccess) return e;
-    per_cu2 = std::max(1, std::min(per_cu2, per_cu));
-    const int blocks = c->n_cu * per_cu2;
-    if ((size_t)blocks * threads * (c->L.depth + 1) * sizeof(StackEntry) > c->wstack.bytes)
-        return hipErrorInvalidValue;  // ensure_walk_stack sizes it for any launch
-    PersistArgs a;
-    a.gstack = c->wstack.as<StackEntry>();
-    a.gstride = blocks * threads;
-    a.n_cached = n_cached;
-    a.cache_bytes = cache_bytes;
-    // the triangle rows after the node cache and the stacks (16 B aligned)
-    a.tri_off = DIM == 3 ? (int)(cache_bytes + 15) / 16 * 16 : -1;
-    a.rv = rv;
-    a.max_steps = max_steps;
-    a.eps = eps;
-    a.eps_f = (float)eps;
-    a.shrink_rel = 1.0f - 1.0f / 131072.0f;
-    a.cf = (1.0f - c->tune.relax) * (1.0f - c->tune.relax);
-    a.min_ready = c->tune.min_ready;
-    // refilling once 8 lanes are free rather than at every free lane runs
-    // the refill code (item lookup, start record, domain record) about a
-    // third as often; measured 1.4-2 % faster walks than 1 or 16 on the
-    // icosphere, plate and star workloads, with identical results
-    a.refill_min = 8;
-    a.counter = c->counter.as<unsigned long long>();
-    a.chunk = c->tune.chunk;
-    a.vals = c->vals.as<double>();
-    a.steps_out = c->steps.as<int32_t>();
-    a.units = c->count_units ? c->units.as<unsigned long long>() : nullptr;
-    const bool defer = !ADJ && ev.boundary_kind != WOS_BND_ZERO;
-    a.term_pw = defer ? c->term_pw.as<float4>() : nullptr;
-    a.term_slot = defer ? c->term_slot.as<int32_t>() : nullptr;
-    a.parts = c->tune.xcd_parts;
-    hipLaunchKernelGGL(k_init_parts, dim3(1), dim3(64), 0, c->stream, a.counter, a.parts,
-                       (int64_t)rv.total, rv.total_dev);
-    e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    if (mode == 3)
-        hipLaunchKernelGGL((k_walk_fp32<DIM, kM3, SCR, ADJ>), dim3(blocks), dim3(threads), lds,
-                           c->stream, c->view, ev, a, adj);
-    else if (mode == 4)
-        hipLaunchKernelGGL((k_walk_fp32<DIM, kM4, SCR, ADJ>), dim3(blocks), dim3(threads), lds,
-                           c->stream, c->view, ev, a, adj);
-    else if (mode == 5)
-        hipLaunchKernelGGL((k_walk_fp32<DIM, kM5, SCR, ADJ>), dim3(blocks), dim3(threads), lds,
-                           c->stream, c->view, ev, a, adj);
-    else if (mode == 2)
-        hipLaunchKernelGGL((k_walk_fp32<DIM, 2, SCR, ADJ>), dim3(blocks), dim3(threads), lds,
-                           c->stream, c->view, ev, a, adj);
-    else
-        hipLaunchKernelGGL((k_walk_fp32<DIM, SMODE, SCR, ADJ>), dim3(blocks), dim3(threads), lds,
-                           c->stream, c->view, ev, a, adj);
-    if (defer) {
-        e = hipGetLastError();
-        if (e != hipSuccess) return e;
-        hipLaunchKernelGGL(k_terminal<DIM>, dim3((unsigned)((rv.total + kBlock - 1) / kBlock)),
-                           dim3(kBlock), 0, c->stream, c->view, ev, rv, a.term_pw, a.term_slot,
-                           a.vals);
-    }
-    *blocks_out = blocks;
-    c->last_threads = threads;
-    c->last_cached = n_cached;
-    return hipGetLastError();
+    if (c->tune.bvh_width == 4)
+        return launch_fp32_w<DIM, SMODE, kM3, kM4, kM5, SCR, ADJ, 4>(c, ev, mode, rv, eps, max_steps,
+                                                                      adj, blocks_out);
+    return launch_fp32_w<DIM, SMODE, kM3, kM4, kM5, SCR, ADJ, 2>(c, ev, mode, rv, eps, max_steps,
+                                                                  adj, blocks_out);
 }
 
 template <int DIM, bool ADJ>
@@ -2884,17 +2927,22 @@ int solve_graph(wos_ctx *c, const std::vector<Part> &parts, const wos_eq_desc *e
 }
 
 // ---------------------------------------------------------------------------
-// autotuner.  It searches only launch-shape fields that never change a
-// result (walkers per block, lanes per walk-step phase, refill chunk, LDS
-// node cache): every walk's draws and queries are independent of them, so a
-// tuned solve is bit-identical to an untuned one.  The distance grid, its
-// list cap and relax stay as set.  Coordinate descent from the current
+// autotuner.  It searches launch-shape fields that never change a result
+// (walkers per block, lanes per walk-step phase, refill chunk, LDS node
+// cache, xcd_parts, the LDS share of the traversal stack): every walk's
+// draws and queries are independent of them, so a tuned solve is
+// bit-identical to an untuned one.  With params->tune = 2 it also searches
+// the BVH width, which can change the order of a relaxed tree search (and
+// so a walk) where the grid has no list, not the estimator's expectation.
+// The distance grid, its list cap and relax stay as set.  Coordinate descent from the current
 // shape over a fixed candidate list, each candidate timed twice (walker
 // kernel time, the smaller counts) on a pilot solve of the same problem with
 // about 16 M walks.
 // ---------------------------------------------------------------------------
-TuneKey tune_key(const wos_ctx *c, const std::vector<Part> &parts, const wos_eq_desc *eq) {
+TuneKey tune_key(const wos_ctx *c, const std::vector<Part> &parts, const wos_eq_desc *eq,
+                 int mode) {
     TuneKey k{};
+    k.mode = mode;
     k.dim = c->mesh.dim;
     k.n_domains = (int)c->L.domains.size();
     k.n_prims = c->view.n_prims;
@@ -2944,6 +2992,8 @@ int autotune(wos_ctx *c, const std::vector<Part> &parts, const wos_eq_desc *eq,
     static const int kChunk[] = {32, 64, 256};
     static const int kLds[] = {0, -1};
     static const int kParts[] = {1, 8};
+    static const int kStack[] = {0, 8, 32};
+    static const int kWidth[] = {2, 4};
     auto descend = [&](int wos_tuning::*field, const int *vals, int nv) -> int {
         for (int i = 0; i < nv; ++i) {
             if (cur.*field == vals[i]) continue;
@@ -2959,12 +3009,14 @@ int autotune(wos_ctx *c, const std::vector<Part> &parts, const wos_eq_desc *eq,
     if ((rc = descend(&wos_tuning::min_ready, kReady, 6)) == WOS_OK &&
         (rc = descend(&wos_tuning::block_threads, kThreads, 3)) == WOS_OK &&
         (rc = descend(&wos_tuning::chunk, kChunk, 3)) == WOS_OK &&
-        (rc = descend(&wos_tuning::lds_nodes, kLds, 2)) == WOS_OK)
-        rc = descend(&wos_tuning::xcd_parts, kParts, 2);
+        (rc = descend(&wos_tuning::lds_nodes, kLds, 2)) == WOS_OK &&
+        (rc = descend(&wos_tuning::xcd_parts, kParts, 2)) == WOS_OK &&
+        (rc = descend(&wos_tuning::stack_lds, kStack, 3)) == WOS_OK && prm->tune == 2)
+        rc = descend(&wos_tuning::bvh_width, kWidth, 2);
     c->tune = saved;
     if (rc != WOS_OK) return rc;
     *chosen = cur;
-    c->tuned.emplace_back(tune_key(c, parts, eq), cur);
+    c->tuned.emplace_back(tune_key(c, parts, eq, prm->tune == 2 ? 2 : 1), cur);
     return WOS_OK;
 }
 
@@ -3001,8 +3053,8 @@ int solve(wos_ctx *c, const std::vector<Part> &parts, const wos_eq_desc *eq,
     };
     if (!c || !prm || prm->tune == 0 || prm->precision != WOS_FP32_PERSISTENT || !c->has_mesh)
         return run();
-    if (prm->tune != 1) return fail(WOS_ERR_INPUT, "tune must be 0 or 1");
-    const TuneKey key = tune_key(c, parts, eq);
+    if (prm->tune != 1 && prm->tune != 2) return fail(WOS_ERR_INPUT, "tune must be 0, 1 or 2");
+    const TuneKey key = tune_key(c, parts, eq, prm->tune);
     const wos_tuning *found = nullptr;
     for (const auto &e : c->tuned)
         if (e.first == key) found = &e.second;
@@ -3019,6 +3071,8 @@ int solve(wos_ctx *c, const std::vector<Part> &parts, const wos_eq_desc *eq,
     c->tune.chunk = shape.chunk;
     c->tune.lds_nodes = shape.lds_nodes;
     c->tune.xcd_parts = shape.xcd_parts;
+    c->tune.stack_lds = shape.stack_lds;
+    if (prm->tune == 2) c->tune.bvh_width = shape.bvh_width;
     const int rc = run();
     c->tune = saved;
     return rc;
@@ -3137,16 +3191,22 @@ int wos_ctx_set_domains(wos_ctx *c, int dim, int n_domains, const double *const 
         HIPX(upload(c->nd2, L.nd2, s));
         HIPX(upload(c->segd, L.segd, s));
         HIPX(upload(c->nf2, L.nf2, s));
+        HIPX(upload(c->nw2, L.nw2, s));
         HIPX(upload(c->segf, L.segf, s));
     } else {
         HIPX(upload(c->nd3, L.nd3, s));
         HIPX(upload(c->trid, L.trid, s));
         HIPX(upload(c->nf3, L.nf3, s));
+        HIPX(upload(c->nw3, L.nw3, s));
         HIPX(upload(c->trif, L.trif, s));
     }
-    std::vector<DomainF> doms;
-    for (const DomainInfo &D : L.domains) doms.push_back(DomainF{D.root_ref, fp32_margin(D)});
+    std::vector<DomainF> doms, doms_w;
+    for (const DomainInfo &D : L.domains) {
+        doms.push_back(DomainF{D.root_ref, fp32_margin(D)});
+        doms_w.push_back(DomainF{D.root_ref_w, fp32_margin(D)});
+    }
     HIPX(upload(c->doms, doms, s));
+    HIPX(upload(c->doms_w, doms_w, s));
     HIPX(upload(c->slot_prim, L.slot_prim, s));
     HIPX(upload(c->prim_bid, all.bids, s));
     HIPX(upload(c->prims, all.prims, s));
@@ -3161,6 +3221,7 @@ int wos_ctx_set_domains(wos_ctx *c, int dim, int n_domains, const double *const 
     v.slot_prim = c->slot_prim.as<int32_t>(); v.prim_bid = c->prim_bid.as<int32_t>();
     v.prims = c->prims.as<int32_t>(); v.verts = c->verts.as<double>();
     v.nf2 = c->nf2.as<NodeF2>(); v.nf3 = c->nf3.as<NodeF3>();
+    v.nw2 = c->nw2.as<NodeF2W>(); v.nw3 = c->nw3.as<NodeF3W>();
     v.segf = c->segf.as<SegF>(); v.trif = c->trif.as<TriF>();
     v.doms = c->doms.as<DomainF>();
     c->view = v;
@@ -3228,6 +3289,8 @@ int wos_ctx_set_tuning(wos_ctx *c, const wos_tuning *t) {
         return fail(WOS_ERR_INPUT, "grid_list_max must be in 1..127");
     if (t->xcd_parts < 1 || t->xcd_parts > kMaxParts)
         return fail(WOS_ERR_INPUT, "xcd_parts must be in 1..64");
+    if (t->bvh_width != 2 && t->bvh_width != 4) return fail(WOS_ERR_INPUT, "bvh_width must be 2 or 4");
+    if (t->stack_lds < 0 || t->stack_lds > 64) return fail(WOS_ERR_INPUT, "stack_lds must be in 0..64");
     c->tune = *t;
     return WOS_OK;
 }
@@ -3323,17 +3386,27 @@ int wos_closest_points(wos_ctx *c, int domain, const double *points, int64_t n, 
     const DomainInfo &D = c->L.domains[domain];
     const float cf = (float)((1.0 - relax) * (1.0 - relax));
     const int blocks = (int)((n + kBlock - 1) / kBlock);
+    const int width = c->tune.bvh_width;
     const size_t lds = (size_t)kBlock * std::max((c->L.depth + 2) * sizeof(int),
-                                                 (c->L.depth + 1) * sizeof(StackEntry));
+                                                 stack_entries(c, width) * sizeof(StackEntry));
     const int fp32 = precision == WOS_FP32_PERSISTENT;
     if (fp32) HIPX(ensure_grid(c));
-    if (dim == 2)
-        hipLaunchKernelGGL(k_closest<2>, dim3(blocks), dim3(kBlock), lds, c->stream, c->view,
-                           D.node_off, D.root_ref, domain, fp32, cf, c->plist_in.as<double>(), n, d_dist,
+    const int32_t root32 = width == 4 ? D.root_ref_w : D.root_ref;
+    if (dim == 2 && width == 4)
+        hipLaunchKernelGGL((k_closest<2, 4>), dim3(blocks), dim3(kBlock), lds, c->stream, c->view,
+                           D.node_off, root32, domain, fp32, cf, c->plist_in.as<double>(), n, d_dist,
+                           d_low, d_near, c->cq_prim.as<int32_t>());
+    else if (dim == 2)
+        hipLaunchKernelGGL((k_closest<2, 2>), dim3(blocks), dim3(kBlock), lds, c->stream, c->view,
+                           D.node_off, root32, domain, fp32, cf, c->plist_in.as<double>(), n, d_dist,
+                           d_low, d_near, c->cq_prim.as<int32_t>());
+    else if (width == 4)
+        hipLaunchKernelGGL((k_closest<3, 4>), dim3(blocks), dim3(kBlock), lds, c->stream, c->view,
+                           D.node_off, root32, domain, fp32, cf, c->plist_in.as<double>(), n, d_dist,
                            d_low, d_near, c->cq_prim.as<int32_t>());
     else
-        hipLaunchKernelGGL(k_closest<3>, dim3(blocks), dim3(kBlock), lds, c->stream, c->view,
-                           D.node_off, D.root_ref, domain, fp32, cf, c->plist_in.as<double>(), n, d_dist,
+        hipLaunchKernelGGL((k_closest<3, 2>), dim3(blocks), dim3(kBlock), lds, c->stream, c->view,
+                           D.node_off, root32, domain, fp32, cf, c->plist_in.as<double>(), n, d_dist,
                            d_low, d_near, c->cq_prim.as<int32_t>());
     HIPX(hipGetLastError());
     HIPX(hipMemcpyAsync(dist, d_dist, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -3405,16 +3478,26 @@ int wos_closest_points_noalloc(wos_ctx *c, int domain, const double *points, int
     const DomainInfo &D = c->L.domains[domain];
     const float cf = (float)((1.0 - relax) * (1.0 - relax));
     const int blocks = (int)((n + kBlock - 1) / kBlock);
+    const int width = c->tune.bvh_width;
     const size_t lds = (size_t)kBlock * std::max((c->L.depth + 2) * sizeof(int),
-                                                 (c->L.depth + 1) * sizeof(StackEntry));
-    if (dim == 2)
-        hipLaunchKernelGGL(k_closest<2>, dim3(blocks), dim3(kBlock), lds, c->stream, c->view,
-                           D.node_off, D.root_ref, domain, fp32, cf, d_pts, n, d_dist, d_low,
-                           d_near, d_prim);
+                                                 stack_entries(c, width) * sizeof(StackEntry));
+    const int32_t root32 = width == 4 ? D.root_ref_w : D.root_ref;
+    if (dim == 2 && width == 4)
+        hipLaunchKernelGGL((k_closest<2, 4>), dim3(blocks), dim3(kBlock), lds, c->stream, c->view,
+                           D.node_off, root32, domain, fp32, cf, d_pts, n, d_dist, d_low, d_near,
+                           d_prim);
+    else if (dim == 2)
+        hipLaunchKernelGGL((k_closest<2, 2>), dim3(blocks), dim3(kBlock), lds, c->stream, c->view,
+                           D.node_off, root32, domain, fp32, cf, d_pts, n, d_dist, d_low, d_near,
+                           d_prim);
+    else if (width == 4)
+        hipLaunchKernelGGL((k_closest<3, 4>), dim3(blocks), dim3(kBlock), lds, c->stream, c->view,
+                           D.node_off, root32, domain, fp32, cf, d_pts, n, d_dist, d_low, d_near,
+                           d_prim);
     else
-        hipLaunchKernelGGL(k_closest<3>, dim3(blocks), dim3(kBlock), lds, c->stream, c->view,
-                           D.node_off, D.root_ref, domain, fp32, cf, d_pts, n, d_dist, d_low,
-                           d_near, d_prim);
+        hipLaunchKernelGGL((k_closest<3, 2>), dim3(blocks), dim3(kBlock), lds, c->stream, c->view,
+                           D.node_off, root32, domain, fp32, cf, d_pts, n, d_dist, d_low, d_near,
+                           d_prim);
     HIPX(hipGetLastError());
     HIPX(hipMemcpyAsync(dist, d_dist, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     if (lower)

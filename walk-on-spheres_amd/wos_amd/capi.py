@@ -156,6 +156,8 @@ class Tuning(C.Structure):
         ("grid_cells", C.c_int32),
         ("grid_list_max", C.c_int32),
         ("xcd_parts", C.c_int32),
+        ("bvh_width", C.c_int32),
+        ("stack_lds", C.c_int32),
     ]
 
 

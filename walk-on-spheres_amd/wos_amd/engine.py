@@ -100,9 +100,9 @@ class Context:
 
     def set_tuning(self, block_threads=128, lds_nodes=-1, chunk=64, blocks_per_cu=0,
                    min_ready=40, relax=0.2, grid_cells=-1, grid_list_max=127,
-                   xcd_parts=8):
+                   xcd_parts=8, bvh_width=2, stack_lds=0):
         t = capi.Tuning(block_threads, lds_nodes, chunk, blocks_per_cu, min_ready, relax,
-                        grid_cells, grid_list_max, xcd_parts)
+                        grid_cells, grid_list_max, xcd_parts, bvh_width, stack_lds)
         capi.check(self.lib, self.lib.wos_ctx_set_tuning(self.h, C.byref(t)))
 
     def autotune(self, nx, ny, nz, eq: EqDesc, prm: RunParams):
