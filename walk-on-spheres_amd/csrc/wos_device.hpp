@@ -1293,8 +1293,9 @@ __device__ __forceinline__ uint32_t query_step(QueryF &q, const MeshView &m, flo
         } else {
             // only the records that are tested are loaded, both in flight
             const bool t1 = two && l12 < thr;
+            // (R1 is read only where t1 holds: no copy of R0 into it)
             const LeafRec<DIM> R0 = leaf_load<DIM>(m, E0.slot);
-            LeafRec<DIM> R1 = R0;
+            LeafRec<DIM> R1;
             if (t1) R1 = leaf_load<DIM>(m, E1.slot);
             const float d0 = leaf_dist<DIM>(R0, px, py, pz);
             const float d1 = t1 ? leaf_dist<DIM>(R1, px, py, pz) : INFINITY;

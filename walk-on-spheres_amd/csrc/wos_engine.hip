@@ -1251,6 +1251,20 @@ void k_walk_fp32(MeshView m, EqView e, PersistArgs a, AdjOut adj) {
     q.tslot = -1;
     const float alpha = (float)e.alpha;
     WalkDer D{};
+    // forward walks: the lane's walk has ended and its results are not yet
+    // stored (g = 0: the value is the source sum; else k_terminal adds
+    // weight x g at the FP64 nearest point, one lane per walk)
+    bool pending = false;
+    auto store_walk = [&]() {
+        const bool hit = R > a.eps_f;
+        a.steps_out[out_index] = steps | (hit ? (int32_t)0x80000000 : 0);
+        a.vals[out_index] = sample;
+        if (a.term_slot) {
+            a.term_pw[out_index] = make_float4(px, py, pz, weight);
+            a.term_slot[out_index] = q.best_slot;
+        }
+        pending = false;
+    };
 
     // wave-uniform pool of work items.  The item range is split into
     // a.parts contiguous parts (points are in sweep order, so a part is a
@@ -1268,6 +1282,9 @@ void k_walk_fp32(MeshView m, EqView e, PersistArgs a, AdjOut adj) {
 
     for (;;) {
         bool began = false;
+        // a walk that the relaxed bound would end searches again exactly
+        // (cf = 1) and takes no step in this iteration
+        bool requery = false;
         uint32_t qb = 0u;
         // ---- finished queries: radius, then end of walk or exact re-query
         if (active && !searching) {
@@ -1275,13 +1292,11 @@ void k_walk_fp32(MeshView m, EqView e, PersistArgs a, AdjOut adj) {
             R = Fx<float>::sqrt(exact ? q.best : q.lb) * a.shrink_rel - margin;
             const bool stop = !(R > a.eps_f) || steps >= a.max_steps;
             if (stop && !exact) {
-                qb = query_begin<DIM>(q, m, root, dom, 1.0f, px, py, pz, tcp);
-                searching = (qb & kQsMore) != 0u;
-                began = true;
+                requery = true;
             } else if (stop) {
-                const bool hit = R > a.eps_f;
-                a.steps_out[out_index] = steps | (hit ? (int32_t)0x80000000 : 0);
                 if constexpr (ADJ) {
+                    const bool hit = R > a.eps_f;
+                    a.steps_out[out_index] = steps | (hit ? (int32_t)0x80000000 : 0);
                     // terminal value in FP64: nearest point on the nearest primitive
                     double g = 0.0, s = 0.0, t = 0.0;
                     int prim = -1;
@@ -1296,13 +1311,10 @@ void k_walk_fp32(MeshView m, EqView e, PersistArgs a, AdjOut adj) {
                     a.vals[out_index] = sample + (double)weight * g;
                     store_adjoint(adj, (int64_t)out_index, D, (double)weight, g, prim, s, t);
                 } else {
-                    // g = 0: the value is the source sum; else k_terminal adds
-                    // weight x g at the FP64 nearest point, one lane per walk
-                    a.vals[out_index] = sample;
-                    if (a.term_slot) {
-                        a.term_pw[out_index] = make_float4(px, py, pz, weight);
-                        a.term_slot[out_index] = q.best_slot;
-                    }
+                    // stored when the lane is refilled (or the pool is empty):
+                    // a few lanes end per iteration, and storing them there
+                    // runs the stores about a third as often
+                    pending = true;
                 }
                 active = false;
             }
@@ -1333,6 +1345,7 @@ void k_walk_fp32(MeshView m, EqView e, PersistArgs a, AdjOut adj) {
                 const uint32_t lo = (uint32_t)free_mask, hi = (uint32_t)(free_mask >> 32);
                 const int rank = __builtin_amdgcn_mbcnt_hi(hi, __builtin_amdgcn_mbcnt_lo(lo, 0));
                 const uint64_t item = pool_next + (uint64_t)rank;
+                if (pending && item < pool_end) store_walk();
                 if (!active && item < pool_end) {
                     const int j = round_inside_id(a.rv, round_point(a.rv, (int64_t)item, w));
                     const StartF S = a.rv.sf[j];
@@ -1367,12 +1380,15 @@ void k_walk_fp32(MeshView m, EqView e, PersistArgs a, AdjOut adj) {
                 pool_next = pool_next + nfree < pool_end ? pool_next + nfree : pool_end;
             }
         }
+        if (exhausted && pending) store_walk();
 
         // ---- one walk step (include/wos/wos.hpp:68-127), then the next query.
-        // A lane that began its exact re-query above does not step: its
-        // radius is the relaxed one, and the exact result (which query_begin
-        // may already have completed) decides the walk's end next iteration.
-        if (active && !searching && !began) {
+        // A lane re-querying exactly does not step: its radius is the
+        // relaxed one, and the exact result (which query_begin may already
+        // complete) decides the walk's end next iteration.  Both kinds of
+        // lane then begin their query in one call.
+        const bool stepping = active && !searching && !requery;
+        if (stepping) {
             // block 0 of the step's counter stream: the direction's slots and
             // the first source slots (a constant source needs no other block)
             const Words4 b0 = philox4(w, (uint32_t)steps, 0, 0, k0, k1);
@@ -1411,7 +1427,9 @@ void k_walk_fp32(MeshView m, EqView e, PersistArgs a, AdjOut adj) {
                 pz = __builtin_fmaf(R, z, pz);
             }
             ++steps;
-            qb = query_begin<DIM>(q, m, root, dom, a.cf, px, py, pz, tcp);
+        }
+        if (stepping || requery) {
+            qb = query_begin<DIM>(q, m, root, dom, requery ? 1.0f : a.cf, px, py, pz, tcp);
             searching = (qb & kQsMore) != 0u;
             began = true;
         }
