@@ -1074,10 +1074,54 @@ __global__ void k_zero_u64(unsigned long long *p, int64_t n) {
         p[i] = 0ull;
 }
 
-// first item of every part of the walker's item range
+// Launch fields that only the refill and the end-of-walk stores read.  The
+// walker reads them from this record in device memory where it uses them
+// (scalar loads through a pointer the compiler cannot hoist out of the
+// loop), so they do not hold scalar registers across the loop; held as
+// kernel arguments they were loaded once and spilled.  k_init_parts writes
+// the record before each launch.
+struct WalkCold {
+    RoundView rv;
+    unsigned long long *counter;
+    double *vals;
+    int32_t *steps_out;
+    float4 *term_pw;
+    int32_t *term_slot;
+    int parts;
+    int chunk;
+};
+typedef __attribute__((address_space(4))) const WalkCold ConstWalkCold;
+static_assert(offsetof(WalkCold, rv) == 0, "cold_round reads the record's first bytes");
+// the walker's counter buffer: one counter per part, 16 words apart, then
+// the cold record
+constexpr size_t kCounterWords = 16 * kMaxParts;
+constexpr size_t kCounterBytes = kCounterWords * sizeof(unsigned long long) + sizeof(WalkCold);
+
+// the cold record through a pointer opaque to the compiler (an empty asm
+// that "changes" it): loads through it stay where they are written
+__device__ __forceinline__ ConstWalkCold *cold_record(const WalkCold *p) {
+    asm volatile("" : "+s"(p));
+    return (ConstWalkCold *)p;
+}
+// the record's RoundView, word by word (scalar loads of the words used)
+__device__ __forceinline__ RoundView cold_round(ConstWalkCold *K) {
+    static_assert(sizeof(RoundView) % 4 == 0, "whole 32-bit words");
+    typedef __attribute__((address_space(4))) const uint32_t ConstWord;
+    const ConstWord *src = (const ConstWord *)K;
+    uint32_t w[sizeof(RoundView) / 4];
+#pragma unroll
+    for (size_t k = 0; k < sizeof(RoundView) / 4; ++k) w[k] = src[k];
+    RoundView rv;
+    __builtin_memcpy(&rv, w, sizeof(rv));
+    return rv;
+}
+
+// first item of every part of the walker's item range; thread 0 also
+// writes the launch's cold record
 __global__ void k_init_parts(unsigned long long *counter, int parts, int64_t total,
-                             const int64_t *total_dev) {
+                             const int64_t *total_dev, WalkCold cold, WalkCold *cold_dst) {
     const int p = threadIdx.x;
+    if (p == 0) *cold_dst = cold;
     if (p >= parts) return;
     const uint64_t t = total_dev ? (uint64_t)*total_dev : (uint64_t)total;
     counter[16 * p] = t * (uint64_t)p / (uint64_t)parts;
@@ -1174,6 +1218,7 @@ __global__ __launch_bounds__(kBlock) void k_inside_scatter(const uint8_t *loc, i
 // primitive.
 struct PersistArgs {
     RoundView rv;
+    const WalkCold *cold;  // rv, counter, parts, chunk, vals, steps_out, term_*
     int max_steps;
     double eps;
     float eps_f;
@@ -1256,25 +1301,34 @@ void k_walk_fp32(MeshView m, EqView e, PersistArgs a, AdjOut adj) {
     // weight x g at the FP64 nearest point, one lane per walk)
     bool pending = false;
     auto store_walk = [&]() {
+        ConstWalkCold *K = cold_record(a.cold);
         const bool hit = R > a.eps_f;
-        a.steps_out[out_index] = steps | (hit ? (int32_t)0x80000000 : 0);
-        a.vals[out_index] = sample;
-        if (a.term_slot) {
-            a.term_pw[out_index] = make_float4(px, py, pz, weight);
-            a.term_slot[out_index] = q.best_slot;
+        K->steps_out[out_index] = steps | (hit ? (int32_t)0x80000000 : 0);
+        K->vals[out_index] = sample;
+        int32_t *const term_slot = K->term_slot;
+        if (term_slot) {
+            K->term_pw[out_index] = make_float4(px, py, pz, weight);
+            term_slot[out_index] = q.best_slot;
         }
         pending = false;
     };
 
     // wave-uniform pool of work items.  The item range is split into
-    // a.parts contiguous parts (points are in sweep order, so a part is a
+    // `parts` contiguous parts (points are in sweep order, so a part is a
     // region of the domain); the hardware places workgroups on the XCDs
     // round-robin, so taking part blockIdx.x % parts first keeps each XCD's
     // L2 on one region.  An exhausted part sends the wave to the next one.
     uint64_t pool_next = 0, pool_end = 0;
     bool exhausted = false;
-    const uint64_t total = a.rv.total_dev ? (uint64_t)*a.rv.total_dev : (uint64_t)a.rv.total;
-    int part = (int)(blockIdx.x % (unsigned)a.parts), parts_left = a.parts;
+    uint64_t total;
+    int part, parts_left;
+    {
+        ConstWalkCold *K = cold_record(a.cold);
+        const int64_t *total_dev = K->rv.total_dev;
+        total = total_dev ? (uint64_t)*total_dev : (uint64_t)K->rv.total;
+        parts_left = K->parts;
+        part = (int)(blockIdx.x % (unsigned)parts_left);
+    }
     // wave-uniform query-unit counts (a.units != nullptr): queries, grid
     // bounds, list scans, list entries read, nodes, leaf primitives, list
     // primitives tested
@@ -1295,8 +1349,9 @@ void k_walk_fp32(MeshView m, EqView e, PersistArgs a, AdjOut adj) {
                 requery = true;
             } else if (stop) {
                 if constexpr (ADJ) {
+                    ConstWalkCold *K = cold_record(a.cold);
                     const bool hit = R > a.eps_f;
-                    a.steps_out[out_index] = steps | (hit ? (int32_t)0x80000000 : 0);
+                    K->steps_out[out_index] = steps | (hit ? (int32_t)0x80000000 : 0);
                     // terminal value in FP64: nearest point on the nearest primitive
                     double g = 0.0, s = 0.0, t = 0.0;
                     int prim = -1;
@@ -1308,7 +1363,7 @@ void k_walk_fp32(MeshView m, EqView e, PersistArgs a, AdjOut adj) {
                         else eberly_d(m.trid[slot], (double)px, (double)py, (double)pz, nx, ny, nz, s, t);
                         g = eq_boundary(e, m, nx, ny, nz, prim, s, t);
                     }
-                    a.vals[out_index] = sample + (double)weight * g;
+                    K->vals[out_index] = sample + (double)weight * g;
                     store_adjoint(adj, (int64_t)out_index, D, (double)weight, g, prim, s, t);
                 } else {
                     // stored when the lane is refilled (or the pool is empty):
@@ -1324,21 +1379,24 @@ void k_walk_fp32(MeshView m, EqView e, PersistArgs a, AdjOut adj) {
         const uint64_t free_mask = __ballot(!active);
         if (free_mask != 0 && !exhausted &&
             (__popcll(free_mask) >= a.refill_min || free_mask == ~0ull)) {
+            ConstWalkCold *K = cold_record(a.cold);
+            const int parts = K->parts;
             while (pool_next >= pool_end) {
-                const uint64_t part_end = total * (uint64_t)(part + 1) / (uint64_t)a.parts;
+                const uint64_t part_end = total * (uint64_t)(part + 1) / (uint64_t)parts;
+                const int chunk = K->chunk;
                 unsigned long long base = 0;
                 if (lane == 0)
-                    base = atomicAdd(a.counter + 16 * part, (unsigned long long)a.chunk);
+                    base = atomicAdd(K->counter + 16 * part, (unsigned long long)chunk);
                 base = __shfl(base, 0);
                 if (base < part_end) {
                     pool_next = base;
-                    pool_end = base + (uint64_t)a.chunk;
+                    pool_end = base + (uint64_t)chunk;
                     if (pool_end > part_end) pool_end = part_end;
                 } else if (--parts_left == 0) {
                     exhausted = true;
                     break;
                 } else {
-                    part = part + 1 == a.parts ? 0 : part + 1;
+                    part = part + 1 == parts ? 0 : part + 1;
                 }
             }
             if (!exhausted) {
@@ -1347,8 +1405,9 @@ void k_walk_fp32(MeshView m, EqView e, PersistArgs a, AdjOut adj) {
                 const uint64_t item = pool_next + (uint64_t)rank;
                 if (pending && item < pool_end) store_walk();
                 if (!active && item < pool_end) {
-                    const int j = round_inside_id(a.rv, round_point(a.rv, (int64_t)item, w));
-                    const StartF S = a.rv.sf[j];
+                    const RoundView rv = cold_round(K);
+                    const int j = round_inside_id(rv, round_point(rv, (int64_t)item, w));
+                    const StartF S = rv.sf[j];
                     dom = S.domain;
                     const DomainF DF = domain_record(m.doms, dom);
                     px = S.x; py = S.y; pz = S.z;
@@ -1366,10 +1425,11 @@ void k_walk_fp32(MeshView m, EqView e, PersistArgs a, AdjOut adj) {
                     if constexpr (ADJ) D = WalkDer{};
                     if (!(R > a.eps_f)) {
                         // zero-step walk: g at the start's nearest point
-                        const StartInfo &F = a.rv.starts[a.rv.inside[j]];
-                        a.vals[out_index] = F.g;
-                        a.steps_out[out_index] = 0;
-                        if (!ADJ && a.term_slot) a.term_slot[out_index] = -1;
+                        const StartInfo &F = rv.starts[rv.inside[j]];
+                        K->vals[out_index] = F.g;
+                        K->steps_out[out_index] = 0;
+                        int32_t *const term_slot = K->term_slot;
+                        if (!ADJ && term_slot) term_slot[out_index] = -1;
                         if constexpr (ADJ)
                             store_adjoint(adj, (int64_t)item, D, 1.0, F.g, F.prim, F.s,
                                           DIM == 3 ? F.t : 0.0);
@@ -2004,8 +2064,12 @@ hipError_t launch_fp32_w(wos_ctx *c, const EqView &ev, int mode, const RoundView
     // the 4-wide forest has its own domain roots
     MeshView view = c->view;
     if (W == 4) view.doms = c->doms_w.as<DomainF>();
+    // the cold record sits after the part counters in the same buffer
+    WalkCold cold{a.rv, a.counter, a.vals, a.steps_out, a.term_pw, a.term_slot, a.parts, a.chunk};
+    WalkCold *cold_dst = reinterpret_cast<WalkCold *>(a.counter + kCounterWords);
+    a.cold = cold_dst;
     hipLaunchKernelGGL(k_init_parts, dim3(1), dim3(64), 0, c->stream, a.counter, a.parts,
-                       (int64_t)rv.total, rv.total_dev);
+                       (int64_t)rv.total, rv.total_dev, cold, cold_dst);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
     if (mode == 3)
@@ -2554,7 +2618,7 @@ int solve_once(wos_ctx *c, const std::vector<Part> &parts, const wos_eq_desc *eq
         HIPX(hipMemsetAsync(c->count.p, 0, n_in * sizeof(double), c->stream));
         HIPX(hipMemsetAsync(c->ssum.p, 0, n_in * sizeof(double), c->stream));
         HIPX(hipMemsetAsync(c->hits.p, 0, n_in * sizeof(int32_t), c->stream));
-        HIPX(c->counter.reserve(16 * kMaxParts * sizeof(unsigned long long)));
+        HIPX(c->counter.reserve(kCounterBytes));
         const bool src = eq->source_kind != WOS_SRC_NONE;
         const bool scr = eq->kind == WOS_EQ_SCREENED || eq->kind == WOS_EQ_HELMHOLTZ;
         const bool adaptive = prm->target_se > 0.0;
@@ -2815,7 +2879,7 @@ int solve_graph(wos_ctx *c, const std::vector<Part> &parts, const wos_eq_desc *e
     HIPX(c->vals.reserve((size_t)(n * nw) * sizeof(double)));
     HIPX(c->steps.reserve((size_t)(n * nw) * sizeof(int32_t)));
     HIPX(reserve_terminal(c, ev, WOS_FP32_PERSISTENT, false, n * nw));
-    HIPX(c->counter.reserve(16 * kMaxParts * sizeof(unsigned long long)));
+    HIPX(c->counter.reserve(kCounterBytes));
     if (c->pid_all.bytes < (size_t)n * sizeof(int32_t)) {
         std::vector<int32_t> ids((size_t)n);
         for (int64_t k = 0; k < n; ++k) ids[(size_t)k] = (int32_t)k;
