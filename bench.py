@@ -260,10 +260,13 @@ def _fp32_vs_fp64(r32, r64, ins, floor=1e-300):
 
 
 def grid_workload(key, Context, capi, mesh, eq, grid, nw, eps, seed, steps, warmup, kernel,
-                  want_cpu, terminal=False, fp64=True, graph=False, floor=1e-300):
+                  want_cpu, terminal=False, fp64=True, graph=False, floor=1e-300, tree=False):
     """FP32 persistent and FP64 plain grid solves of one problem: grid pts/s
     at SE <= 1e-3 of each, the roofline of the FP32 walker, the host CPU
-    baseline; returns (entry, FP32 result)."""
+    baseline; returns (entry, FP32 result).  tree: also the FP32 walker
+    without the distance grid (every query searches the BVH, as the FP64
+    walker does), which splits FP32 / FP64 into the precision-and-launch
+    part and the distance-grid part."""
     V, F, B = mesh
     dim = F.shape[1]
     ctx = Context(0)
@@ -320,6 +323,20 @@ def grid_workload(key, Context, capi, mesh, eq, grid, nw, eps, seed, steps, warm
                                  "steps_per_s": s64["total_steps"] / t64},
                      fp32_over_fp64=v32 / v64,
                      fp32_vs_fp64=_fp32_vs_fp64(r32, r64, ins, floor))
+        if tree:
+            ctx.set_tuning(grid_cells=0)
+            ctx.solve_grid(*grid, eq, p32)
+            t0 = time.perf_counter()
+            rt = ctx.solve_grid(*grid, eq, p32)
+            tt = time.perf_counter() - t0
+            ctx.set_tuning()
+            vt = pts_per_s_at_se(ins.sum(), var_ref, rt["mean_steps"][ins], tt,
+                                 rt["summary"]["total_steps"])
+            entry["fp32_tree_only"] = {
+                "value": vt, "ms_per_solve": tt * 1e3,
+                "walk_kernel_ms": rt["summary"]["walk_kernel_seconds"] * 1e3,
+                "over_fp64_plain": vt / v64, "grid_speedup": v32 / vt,
+                "fp32_vs_fp64": _fp32_vs_fp64(rt, r64, ins, floor)}
         if want_cpu:
             inside_idx = np.nonzero(ins)[0]
             pts_all = _grid_points(s32, grid, dim)
@@ -353,7 +370,7 @@ def workload_icosphere(Context, meshgen, problems, capi, args, want_cpu):
     e, _ = grid_workload("icosphere", Context, capi, meshgen.icosphere(4),
                          problems.sphere_screened_constant(alpha=10.0, f=1.0), (32, 32, 32),
                          args.walks, 1e-4, 7, args.steps, args.warmup,
-                         "k_walk_fp32<3,1,true,false>", want_cpu, graph=True)
+                         "k_walk_fp32<3,1,true,false>", want_cpu, graph=True, tree=True)
     e["description"] = workload_icosphere.__doc__.strip()
     return e
 
@@ -740,6 +757,7 @@ def main():
         "vs_cpu": ico.get("vs_cpu"),
         "fp64_plain": ico["fp64_plain"],
         "fp32_over_fp64": ico["fp32_over_fp64"],
+        "fp32_tree_only": ico.get("fp32_tree_only"),
         "source_sha": source_sha(),
         "configs": configs,
     }

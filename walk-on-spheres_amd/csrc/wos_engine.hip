@@ -1576,18 +1576,37 @@ __global__ __launch_bounds__(kBlock) void k_welford(RoundView rv, const double *
     double mu = mean[p], M2 = m2[p], n = count[p], ts = steps_sum[p];
     int h = hits[p];
     unsigned nan_walks = 0u;
-    for (int j = 0; j < cnt; ++j) {
-        const size_t o = (size_t)(base + j);
-        const double x = vals[o];
-        const int32_t s = steps[o];
-        nan_walks += x != x ? 1u : 0u;
-        n = n + 1.0;
-        const double delta_before = x - mu;
-        mu += delta_before / n;
-        const double delta_after = x - mu;
-        M2 += delta_before * delta_after;
-        ts += (double)(s & 0x7fffffff);
-        h += (s < 0) ? 1 : 0;
+    // the point's walks in batches of kBatch: all loads of a batch are in
+    // flight together, then the updates run in walk order (one lane per
+    // point leaves too few wavefronts to hide one load latency per walk)
+    constexpr int kBatch = 16;
+    for (int j0 = 0; j0 < cnt; j0 += kBatch) {
+        const int nb = cnt - j0 < kBatch ? cnt - j0 : kBatch;
+        double xs[kBatch];
+        int32_t ss[kBatch];
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u) {
+            if (u < nb) {
+                const size_t o = (size_t)(base + j0 + u);
+                xs[u] = vals[o];
+                ss[u] = steps[o];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u) {
+            if (u < nb) {
+                const double x = xs[u];
+                const int32_t s = ss[u];
+                nan_walks += x != x ? 1u : 0u;
+                n = n + 1.0;
+                const double delta_before = x - mu;
+                mu += delta_before / n;
+                const double delta_after = x - mu;
+                M2 += delta_before * delta_after;
+                ts += (double)(s & 0x7fffffff);
+                h += (s < 0) ? 1 : 0;
+            }
+        }
     }
     mean[p] = mu; m2[p] = M2; count[p] = n; steps_sum[p] = ts; hits[p] = h;
     if (nan_walks) atomicAdd(failed, (unsigned long long)nan_walks);
