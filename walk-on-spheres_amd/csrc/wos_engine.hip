@@ -10,7 +10,9 @@
 //      ballot-driven refill) writing one FP64 value per walk into a
 //      [walk][point] staging buffer;
 //   4. k_welford: per point, the reference's sequential Welford update in walk
-//      order (include/wos/wos.hpp:169-180), continued across rounds.
+//      order (include/wos/wos.hpp:169-180), continued across rounds (FP64
+//      walkers); k_merge_wave: the same update on 64 lanes per point and a
+//      fixed-order pairwise combination (FP32 walker).
 // Results are bitwise reproducible: a walk's draws depend only on (point seed,
 // walk, step), and the merge order is fixed.
 #include <hip/hip_runtime.h>
@@ -1613,6 +1615,150 @@ __global__ __launch_bounds__(kBlock) void k_welford(RoundView rv, const double *
 }
 
 // --------------------------------------------------------------------------
+// kernel 3b: the FP32 walker's merge, one wavefront per point.  Lane l owns
+// the point's walks l, l + 64, l + 128, ... and runs the same Welford update
+// over them in walk order; the 64 lane states (count, mean, M2) are combined
+// pairwise by Chan, Golub and LeVeque's formula (1979) in a fixed butterfly
+// order (lane distance 32, 16, ..., 1; each pair combines its lower lane
+// first, so both partners hold the same state).  Where a point's walks come
+// in several rounds, the lane states persist between them (lane_mu, lane_m2:
+// 64 per point; a lane's count follows from the point's), so every split of
+// the walks into rounds, and every batch, gives the same bits; they equal
+// the sequential merge to rounding (the FP64 walkers keep the sequential
+// order, which defines their bits).  A point's serial chain is 1/64 as
+// long: the kernel no longer waits on one FP64 division per walk.
+// --------------------------------------------------------------------------
+struct Moments { double n, mu, M2; };
+// (n, mu, M2) of the union of two disjoint samples; a.n = 0 returns b exactly
+__device__ __forceinline__ Moments chan_combine(const Moments &a, const Moments &b) {
+    const double n = a.n + b.n;
+    if (n == 0.0) return a;
+    const double delta = b.mu - a.mu;
+    const double fb = b.n / n;
+    return Moments{n, a.mu + delta * fb, a.M2 + b.M2 + delta * delta * a.n * fb};
+}
+
+__global__ __launch_bounds__(kBlock) void k_merge_wave(RoundView rv, const double *vals,
+                                                       const int32_t *steps, double *mean,
+                                                       double *m2, double *count,
+                                                       double *steps_sum, int32_t *hits,
+                                                       unsigned long long *failed,
+                                                       double *lane_mu, double *lane_m2) {
+    const int k = (int)(blockIdx.x * (kBlock / 64) + threadIdx.x / 64);  // wave-uniform
+    const int lane = (int)(threadIdx.x & 63);
+    if (k >= (rv.n_dev ? *rv.n_dev : rv.n)) return;
+    const int p = rv.pid[k];
+    const int64_t base = rv.off ? rv.off[k] : (int64_t)k * rv.cw;
+    const int cnt = rv.off ? (int)(rv.off[k + 1] - rv.off[k]) : rv.cw;
+    const int wb = rv.off ? rv.wbeg[k] : rv.w0;  // walk index of the round's first walk
+    // the lane's state after the earlier rounds (walks 0 .. wb - 1)
+    const int64_t nl = wb > lane ? (int64_t)(wb - lane + 63) / 64 : 0;
+    const size_t ls = (size_t)p * 64 + (size_t)lane;
+    Moments s{(double)nl, 0.0, 0.0};
+    if (nl > 0 && lane_mu) { s.mu = lane_mu[ls]; s.M2 = lane_m2[ls]; }
+    double ts = 0.0;
+    int h = 0;
+    unsigned nan_walks = 0u;
+    for (int j = (lane - wb) & 63; j < cnt; j += 64) {
+        const double x = vals[base + j];
+        const int32_t st = steps[base + j];
+        nan_walks += x != x ? 1u : 0u;
+        s.n = s.n + 1.0;
+        const double delta_before = x - s.mu;
+        s.mu += delta_before / s.n;
+        s.M2 += delta_before * (x - s.mu);
+        ts += (double)(st & 0x7fffffff);
+        h += st < 0 ? 1 : 0;
+    }
+    if (lane_mu) { lane_mu[ls] = s.mu; lane_m2[ls] = s.M2; }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        const Moments t{__shfl_xor(s.n, o), __shfl_xor(s.mu, o), __shfl_xor(s.M2, o)};
+        const bool lower = (lane & o) == 0;
+        s = lower ? chan_combine(s, t) : chan_combine(t, s);
+        // integer-valued sums: exact in any order
+        ts += __shfl_xor(ts, o);
+        h += __shfl_xor(h, o);
+        nan_walks += __shfl_xor(nan_walks, o);
+    }
+    if (lane == 0) {
+        mean[p] = s.mu; m2[p] = s.M2; count[p] = s.n;
+        steps_sum[p] += ts;
+        hits[p] += h;
+        if (nan_walks) atomicAdd(failed, (unsigned long long)nan_walks);
+    }
+}
+
+// Merge of a round's walks: one wavefront per point for the FP32 walker,
+// the sequential merge for the FP64 walkers.
+// lane_mu / lane_m2: the FP32 merge's lane states (64 per point), needed
+// when a point's walks may come in more than one round, else nullptr.
+hipError_t launch_merge(hipStream_t stream, int precision, const RoundView &rv, int64_t n_max,
+                        double *vals, int32_t *steps, double *mean, double *m2, double *count,
+                        double *ssum, int32_t *hits, unsigned long long *failed,
+                        double *lane_mu = nullptr, double *lane_m2 = nullptr) {
+    if (precision == WOS_FP32_PERSISTENT) {
+        constexpr int kWaves = kBlock / 64;
+        hipLaunchKernelGGL(k_merge_wave, dim3((unsigned)((n_max + kWaves - 1) / kWaves)),
+                           dim3(kBlock), 0, stream, rv, vals, steps, mean, m2, count, ssum, hits,
+                           failed, lane_mu, lane_m2);
+    } else {
+        // one wavefront per block: a point's merge is a serial chain, so
+        // the kernel wants its few wavefronts on as many CUs as possible
+        hipLaunchKernelGGL(k_welford, dim3((unsigned)((n_max + 63) / 64)), dim3(64), 0, stream,
+                           rv, vals, steps, mean, m2, count, ssum, hits, failed);
+    }
+    return hipGetLastError();
+}
+
+// --------------------------------------------------------------------------
+// The results of a one-round grid solve in one block of device memory, so
+// that they reach the host in one transfer: the inside count and the
+// failed-walk count, then every per-point array (the inside points' arrays
+// hold n_in of their n slots).
+// --------------------------------------------------------------------------
+struct PackLayout {
+    size_t bval, mean, m2, cnt, ss, amb, hits, loc, total;
+    __host__ __device__ explicit PackLayout(int64_t n) {
+        const size_t m = (size_t)n;
+        size_t o = 16;
+        bval = o; o += 8 * m;
+        mean = o; o += 8 * m;
+        m2 = o; o += 8 * m;
+        cnt = o; o += 8 * m;
+        ss = o; o += 8 * m;
+        amb = o; o += 4 * m;
+        hits = o; o += 4 * m;
+        loc = o; o += m;
+        total = o;
+    }
+};
+
+__global__ __launch_bounds__(kBlock) void k_pack_results(
+    int64_t n, const int32_t *n_in_dev, const unsigned long long *failed, const uint8_t *loc,
+    const int32_t *amb, const double *bval, const double *mean, const double *m2,
+    const double *cnt, const double *ss, const int32_t *hits, unsigned char *pack) {
+    const PackLayout L(n);
+    const int64_t n_in = *n_in_dev;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) {
+        reinterpret_cast<int64_t *>(pack)[0] = n_in;
+        reinterpret_cast<unsigned long long *>(pack)[1] = *failed;
+    }
+    if (i >= n) return;
+    reinterpret_cast<double *>(pack + L.bval)[i] = bval[i];
+    reinterpret_cast<int32_t *>(pack + L.amb)[i] = amb[i];
+    pack[L.loc + (size_t)i] = loc[i];
+    if (i < n_in) {
+        reinterpret_cast<double *>(pack + L.mean)[i] = mean[i];
+        reinterpret_cast<double *>(pack + L.m2)[i] = m2[i];
+        reinterpret_cast<double *>(pack + L.cnt)[i] = cnt[i];
+        reinterpret_cast<double *>(pack + L.ss)[i] = ss[i];
+        reinterpret_cast<int32_t *>(pack + L.hits)[i] = hits[i];
+    }
+}
+
+// --------------------------------------------------------------------------
 // kernel 4 (adjoint): per point, sum the walks' parameter derivatives in walk
 // order; scatter coef x (weight x blend) to the exit primitive's vertices and
 // coef x weight to its boundary id.  coef = dL/du_p / N_p.
@@ -1698,6 +1844,21 @@ struct DevBuf {
     template <typename T> T *as() const { return static_cast<T *>(p); }
 };
 
+// page-locked host memory (hipHostMalloc), grown on demand: device-to-host
+// copies into it are single DMA transfers
+struct PinnedBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    ~PinnedBuf() { if (p) (void)hipHostFree(p); }
+    hipError_t reserve(size_t n) {
+        if (n <= bytes && p) return hipSuccess;
+        if (p) { (void)hipHostFree(p); p = nullptr; bytes = 0; }
+        hipError_t e = hipHostMalloc(&p, n ? n : 16, hipHostMallocDefault);
+        if (e == hipSuccess) bytes = n ? n : 16;
+        return e;
+    }
+};
+
 template <typename T>
 hipError_t upload(DevBuf &b, const std::vector<T> &v, hipStream_t s) {
     hipError_t e = b.reserve(v.size() * sizeof(T));
@@ -1766,6 +1927,9 @@ struct wos_ctx {
     DevBuf units;                     // FP32 query-unit counters
     DevBuf wstack;                    // FP32 walker traversal stacks (every lane of a full GPU)
     DevBuf failed;                    // walks with a NaN value (one counter)
+    DevBuf pack;                      // one-round grid solve results, packed (PackLayout)
+    DevBuf lane_st;                   // FP32 merge lane states of multi-round solves
+    PinnedBuf hpack;                  // their host copy
     std::vector<std::pair<TuneKey, wos_tuning>> tuned;  // autotuned shapes
     // graph replay: the captured solve, what it was captured for, its tolerances
     hipGraph_t graph = nullptr;
@@ -2471,10 +2635,9 @@ int classify_parts(wos_ctx *c, const std::vector<Part> &parts, const EqView &ev,
 // the host copies of the classification and the Welford state.
 int finish_solve(wos_ctx *c, const std::vector<Part> &parts,
                  const std::vector<double> &part_tol, int64_t n,
-                 const std::vector<uint8_t> &loc, const std::vector<int32_t> &amb,
-                 const std::vector<double> &bval, const std::vector<double> &h_mean,
-                 const std::vector<double> &h_m2, const std::vector<double> &h_cnt,
-                 const std::vector<double> &h_ss, const std::vector<int32_t> &h_hits,
+                 const uint8_t *loc, const int32_t *amb, const double *bval,
+                 const double *h_mean, const double *h_m2, const double *h_cnt,
+                 const double *h_ss, const int32_t *h_hits, unsigned long long failed_walks,
                  double classify_s, double walk_s, double total_walk_s, int blocks,
                  const wos_run_params *prm, wos_point_stats *out,
                  wos_run_summary *summary, wos_run_summary *part_summaries) {
@@ -2541,12 +2704,7 @@ int finish_solve(wos_ctx *c, const std::vector<Part> &parts,
     S.launch_blocks = blocks;
     S.launch_threads = prm->precision == WOS_FP32_PERSISTENT ? c->last_threads : kBlock;
     S.lds_cached_nodes = prm->precision == WOS_FP32_PERSISTENT ? c->last_cached : 0;
-    {
-        unsigned long long nf = 0;
-        HIPX(hipMemcpyAsync(&nf, c->failed.p, sizeof(nf), hipMemcpyDeviceToHost, c->stream));
-        HIPX(hipStreamSynchronize(c->stream));
-        S.failed_walks = (int64_t)nf;
-    }
+    S.failed_walks = (int64_t)failed_walks;
     S.grid_build_seconds = c->grid_build_pending_s;
     c->grid_build_pending_s = 0.0;
     S.grid_bytes = c->grid_valid ? c->grid_bytes : 0;
@@ -2661,6 +2819,15 @@ int solve_once(wos_ctx *c, const std::vector<Part> &parts, const wos_eq_desc *eq
             HIPX(hipMemsetAsync(c->grad_b.p, 0, WOS_MAX_BND_VALUES * sizeof(double), c->stream));
         }
         std::vector<int64_t> done(n_in, 0), want(n_in, std::min<int64_t>(prm->n_walks, cap));
+        // the FP32 merge keeps its lane states between rounds where a point's
+        // walks may come in more than one
+        double *lane_mu = nullptr, *lane_m2 = nullptr;
+        if (prm->precision == WOS_FP32_PERSISTENT &&
+            (adaptive || (int64_t)n_in * std::min<int64_t>(prm->n_walks, cap) > budget)) {
+            HIPX(c->lane_st.reserve((size_t)n_in * 128 * sizeof(double)));
+            lane_mu = c->lane_st.as<double>();
+            lane_m2 = lane_mu + (size_t)n_in * 64;
+        }
         std::vector<int32_t> pid, wbeg, kblk;
         std::vector<int64_t> off;
         std::vector<double> a_cnt(n_in), a_m2(n_in);
@@ -2754,14 +2921,10 @@ int solve_once(wos_ctx *c, const std::vector<Part> &parts, const wos_eq_desc *eq
             HIPX(dispatch_walk(c, dim, adjr != nullptr, ev, src, scr, prm->precision, rv,
                                prm->epsilon, prm->max_steps, adj, &blocks));
             HIPX(hipEventRecord(c->ev[3], c->stream));
-            // one wavefront per block: a point's merge is a serial chain, so
-            // the kernel wants its few wavefronts on as many CUs as possible
-            hipLaunchKernelGGL(k_welford, dim3((rv.n + 63) / 64), dim3(64), 0,
-                               c->stream, rv, c->vals.as<double>(), c->steps.as<int32_t>(),
-                               c->mean.as<double>(), c->m2.as<double>(), c->count.as<double>(),
-                               c->ssum.as<double>(), c->hits.as<int32_t>(),
-                               c->failed.as<unsigned long long>());
-            HIPX(hipGetLastError());
+            HIPX(launch_merge(c->stream, prm->precision, rv, rv.n, c->vals.as<double>(),
+                              c->steps.as<int32_t>(), c->mean.as<double>(), c->m2.as<double>(),
+                              c->count.as<double>(), c->ssum.as<double>(), c->hits.as<int32_t>(),
+                              c->failed.as<unsigned long long>(), lane_mu, lane_m2));
             if (adjr) {
                 hipLaunchKernelGGL(k_adj_reduce, dim3((rv.n + kBlock - 1) / kBlock), dim3(kBlock), 0,
                                    c->stream, rv, adj, c->view, c->coef.as<double>(),
@@ -2838,9 +3001,12 @@ int solve_once(wos_ctx *c, const std::vector<Part> &parts, const wos_eq_desc *eq
 
     // per-point result policy (runner.hpp:195-235) and the summary sums in
     // sweep order (runner.hpp:224-228), for the whole solve and per part
-    return finish_solve(c, parts, part_tol, n, loc, amb, bval, h_mean, h_m2, h_cnt, h_ss, h_hits,
-                        classify_s, walk_s, total_walk_s, blocks, prm, out, summary,
-                        part_summaries);
+    unsigned long long nf = 0;
+    HIPX(hipMemcpyAsync(&nf, c->failed.p, sizeof(nf), hipMemcpyDeviceToHost, c->stream));
+    HIPX(hipStreamSynchronize(c->stream));
+    return finish_solve(c, parts, part_tol, n, loc.data(), amb.data(), bval.data(), h_mean.data(),
+                        h_m2.data(), h_cnt.data(), h_ss.data(), h_hits.data(), nf, classify_s,
+                        walk_s, total_walk_s, blocks, prm, out, summary, part_summaries);
 }
 
 // ---------------------------------------------------------------------------
@@ -2904,6 +3070,9 @@ int solve_graph(wos_ctx *c, const std::vector<Part> &parts, const wos_eq_desc *e
         for (int64_t k = 0; k < n; ++k) ids[(size_t)k] = (int32_t)k;
         HIPX(upload(c->pid_all, ids, c->stream));
     }
+    const PackLayout PL(n);
+    HIPX(c->pack.reserve(PL.total));
+    HIPX(c->hpack.reserve(PL.total));
     HIPX(ensure_grid(c));
     HIPX(ensure_walk_stack(c));
     c->count_units = false;
@@ -2920,7 +3089,7 @@ int solve_graph(wos_ctx *c, const std::vector<Part> &parts, const wos_eq_desc *e
     for (const DevBuf *b : {&c->loc, &c->starts, &c->starts_f, &c->amb, &c->bvalue, &c->inside, &c->g_bcount,
                             &c->g_boff, &c->g_misc, &c->mean, &c->m2, &c->count, &c->ssum,
                             &c->hits, &c->vals, &c->steps, &c->counter, &c->pid_all, &c->vv,
-                            &c->failed, &c->wstack,
+                            &c->failed, &c->wstack, &c->pack,
                             &c->field, &c->term_pw, &c->term_slot})
         key_put(key, b->p);
     auto body = [&]() -> int {
@@ -2963,10 +3132,15 @@ int solve_graph(wos_ctx *c, const std::vector<Part> &parts, const wos_eq_desc *e
                                WOS_FP32_PERSISTENT, rv, prm->epsilon,
                                prm->max_steps, AdjOut{}, &c->graph_blocks));
             if (!capture) HIPX(hipEventRecord(c->ev[3], c->stream));
-            hipLaunchKernelGGL(k_welford, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, c->stream, rv,
-                               c->vals.as<double>(), c->steps.as<int32_t>(), c->mean.as<double>(),
+            HIPX(launch_merge(c->stream, WOS_FP32_PERSISTENT, rv, n, c->vals.as<double>(),
+                              c->steps.as<int32_t>(), c->mean.as<double>(), c->m2.as<double>(),
+                              c->count.as<double>(), c->ssum.as<double>(), c->hits.as<int32_t>(),
+                              c->failed.as<unsigned long long>()));
+            hipLaunchKernelGGL(k_pack_results, dim3((unsigned)nb), dim3(kBlock), 0, c->stream, n,
+                               n_in_dev, c->failed.as<unsigned long long>(), c->loc.as<uint8_t>(),
+                               c->amb.as<int32_t>(), c->bvalue.as<double>(), c->mean.as<double>(),
                                c->m2.as<double>(), c->count.as<double>(), c->ssum.as<double>(),
-                               c->hits.as<int32_t>(), c->failed.as<unsigned long long>());
+                               c->hits.as<int32_t>(), c->pack.as<unsigned char>());
             HIPX(hipGetLastError());
             return WOS_OK;
         };
@@ -3004,27 +3178,21 @@ int solve_graph(wos_ctx *c, const std::vector<Part> &parts, const wos_eq_desc *e
         walk_s = total_walk_s = elapsed(c->ev[0], c->ev[1]);
     }
 
-    int32_t n_in = 0;
-    HIPX(hipMemcpy(&n_in, n_in_dev, sizeof(int32_t), hipMemcpyDeviceToHost));
-    std::vector<uint8_t> loc((size_t)n);
-    std::vector<int32_t> amb((size_t)n), inside((size_t)n_in), h_hits((size_t)n_in);
-    std::vector<double> bval((size_t)n), h_mean((size_t)n_in), h_m2((size_t)n_in),
-        h_cnt((size_t)n_in), h_ss((size_t)n_in);
-    HIPX(hipMemcpyAsync(loc.data(), c->loc.p, (size_t)n, hipMemcpyDeviceToHost, c->stream));
-    HIPX(hipMemcpyAsync(amb.data(), c->amb.p, (size_t)n * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
-    HIPX(hipMemcpyAsync(bval.data(), c->bvalue.p, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    if (n_in > 0) {
-        HIPX(hipMemcpyAsync(h_mean.data(), c->mean.p, (size_t)n_in * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-        HIPX(hipMemcpyAsync(h_m2.data(), c->m2.p, (size_t)n_in * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-        HIPX(hipMemcpyAsync(h_cnt.data(), c->count.p, (size_t)n_in * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-        HIPX(hipMemcpyAsync(h_ss.data(), c->ssum.p, (size_t)n_in * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-        HIPX(hipMemcpyAsync(h_hits.data(), c->hits.p, (size_t)n_in * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
-    }
+    // every result in one transfer into page-locked memory
+    HIPX(hipMemcpyAsync(c->hpack.p, c->pack.p, PL.total, hipMemcpyDeviceToHost, c->stream));
     HIPX(hipStreamSynchronize(c->stream));
+    const unsigned char *P = static_cast<const unsigned char *>(c->hpack.p);
+    const unsigned long long n_failed = reinterpret_cast<const unsigned long long *>(P)[1];
     c->last_rounds = 1;
-    return finish_solve(c, parts, c->graph_tol, n, loc, amb, bval, h_mean, h_m2, h_cnt, h_ss,
-                        h_hits, classify_s, walk_s, total_walk_s, c->graph_blocks, prm, out,
-                        summary, part_summaries);
+    return finish_solve(c, parts, c->graph_tol, n, P + PL.loc,
+                        reinterpret_cast<const int32_t *>(P + PL.amb),
+                        reinterpret_cast<const double *>(P + PL.bval),
+                        reinterpret_cast<const double *>(P + PL.mean),
+                        reinterpret_cast<const double *>(P + PL.m2),
+                        reinterpret_cast<const double *>(P + PL.cnt),
+                        reinterpret_cast<const double *>(P + PL.ss),
+                        reinterpret_cast<const int32_t *>(P + PL.hits), n_failed, classify_s,
+                        walk_s, total_walk_s, c->graph_blocks, prm, out, summary, part_summaries);
 }
 
 // ---------------------------------------------------------------------------
