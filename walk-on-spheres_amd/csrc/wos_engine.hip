@@ -152,9 +152,9 @@ constexpr int kMisMaxAttempts = 1000000;
 // nothing; otherwise the term is weight G f / (beta G / mass + (1 - beta) q)
 // with q = exp(-((x-cx)/sigma)^2/2 - ((y-cy)/sigma)^2/2) / norm inside the
 // box, 0 outside (gaussian_heat_source.hpp:45-51, :109-113).
-template <typename T, typename RNG>
+template <typename T, typename RNG, bool SCR = true>
 __device__ double source_term_mis(const EqView &e, T px, T py, T R, T weight, RNG &rng) {
-    const bool screened = e.kind == WOS_EQ_SCREENED;
+    const bool screened = SCR && e.kind == WOS_EQ_SCREENED;
     const T alpha = (T)e.alpha, beta = (T)e.mis_beta;
     const T cx = (T)e.cx, cy = (T)e.cy, sigma = (T)e.mis_sigma;
     T sx, sy;
@@ -198,15 +198,17 @@ __device__ double source_term_mis(const EqView &e, T px, T py, T R, T weight, RN
 // value to add (already multiplied by the weight).  T is the geometry type;
 // SX selects the source extensions compiled in: 0 none, 1 the GREEN_MIS
 // branch, 2 the WOS_SRC_FIELD source.  Each goes only into the kernels
-// launched for it, so the other kernels keep their registers.
-template <int DIM, typename T, typename RNG, int SX = 0>
+// launched for it, so the other kernels keep their registers.  SCR = false
+// (the kernels for Laplace and Poisson) leaves the screened samplers out.
+template <int DIM, typename T, typename RNG, int SX = 0, bool SCR = true>
 __device__ __forceinline__ double source_term(const EqView &e, T px, T py, T pz, T R, T weight,
                                               RNG &rng) {
-    const bool screened = e.kind == WOS_EQ_SCREENED;
+    const bool screened = SCR && e.kind == WOS_EQ_SCREENED;
     const T alpha = (T)e.alpha;
     if (DIM == 2 && !eq_source_may_intersect<T>(e, px, py, R)) return 0.0;
     if constexpr (SX == 1 && DIM == 2)
-        if (e.source_mode == WOS_SOURCE_GREEN_MIS) return source_term_mis<T>(e, px, py, R, weight, rng);
+        if (e.source_mode == WOS_SOURCE_GREEN_MIS)
+            return source_term_mis<T, RNG, SCR>(e, px, py, R, weight, rng);
     constexpr bool FLD = SX == 2;
     T sx, sy, sz = (T)0;
     if (e.source_mode == WOS_SOURCE_GREEN) {
@@ -786,7 +788,7 @@ void k_walk_fp64(MeshView m, EqView e, RoundView rv,
             if constexpr (ADJ)
                 sample += source_term_adj<DIM, double, 1>(e, px, py, pz, R, weight, rng, D);
             else
-                sample += source_term<DIM, double, StepStreamD, SX>(e, px, py, pz, R, weight, rng);
+                sample += source_term<DIM, double, StepStreamD, SX, SCR>(e, px, py, pz, R, weight, rng);
         }
         if (SCR) {
             if constexpr (ADJ && DIM == 3)
@@ -884,7 +886,7 @@ void k_walk_ref(MeshView m, EqView e, RoundView rv, double eps, int max_steps, d
         double sample = 0.0, weight = 1.0;
         int steps = 0;
         while (R > eps && steps < max_steps) {
-            if (SRC) sample += source_term<DIM, double, XoRef, SX>(e, px, py, pz, R, weight, source);
+            if (SRC) sample += source_term<DIM, double, XoRef, SX, SCR>(e, px, py, pz, R, weight, source);
             if (SCR) weight *= step_weight<DIM, double>(e.kind, alpha, R);
             if (DIM == 2) {
                 const double theta = walk.unit() * 2 * kPi;
@@ -920,16 +922,18 @@ void k_walk_ref(MeshView m, EqView e, RoundView rv, double eps, int max_steps, d
 // shares its Philox block with the step direction).
 // SMODE: 1 uniform, 2 Green, 3 GREEN_MIS, 4 Green with WOS_SRC_FIELD,
 // 5 uniform with WOS_SRC_FIELD (the field cases 2D only).
-template <int DIM, int SMODE>
+template <int DIM, int SMODE, bool SCR = true>
 __device__ __forceinline__ double source_term_f(const EqView &e, float px, float py, float pz,
                                                 float R, float weight, StepStreamF &rng,
                                                 bool use_s3 = false, Scr3 s3 = Scr3{}) {
     constexpr bool FLD = SMODE == 5;
-    if (SMODE == 2) return source_term<DIM, float>(e, px, py, pz, R, weight, rng);
-    if constexpr (SMODE == 4) return source_term<DIM, float, StepStreamF, 2>(e, px, py, pz, R, weight, rng);
-    if constexpr (SMODE == 3 && DIM == 2) return source_term_mis<float>(e, px, py, R, weight, rng);
+    if (SMODE == 2) return source_term<DIM, float, StepStreamF, 0, SCR>(e, px, py, pz, R, weight, rng);
+    if constexpr (SMODE == 4)
+        return source_term<DIM, float, StepStreamF, 2, SCR>(e, px, py, pz, R, weight, rng);
+    if constexpr (SMODE == 3 && DIM == 2)
+        return source_term_mis<float, StepStreamF, SCR>(e, px, py, R, weight, rng);
     if (DIM == 2 && !eq_source_may_intersect<float>(e, px, py, R)) return 0.0;
-    const bool screened = e.kind == WOS_EQ_SCREENED;
+    const bool screened = SCR && e.kind == WOS_EQ_SCREENED;
     const float alpha = (float)e.alpha;
     float f, dist;
     if (e.source_kind == WOS_SRC_CONSTANT) {
@@ -1465,7 +1469,7 @@ void k_walk_fp32(MeshView m, EqView e, PersistArgs a, AdjOut adj) {
                 if constexpr (ADJ)
                     sample += source_term_adj<DIM, float, SMODE>(e, px, py, pz, R, weight, src, D);
                 else
-                    sample += source_term_f<DIM, SMODE>(e, px, py, pz, R, weight, src, scr3, s3);
+                    sample += source_term_f<DIM, SMODE, SCR>(e, px, py, pz, R, weight, src, scr3, s3);
             }
             if (SCR) {
                 if constexpr (ADJ && DIM == 3)
