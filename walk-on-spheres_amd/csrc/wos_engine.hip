@@ -1245,9 +1245,10 @@ struct PersistArgs {
     double *vals;
     int32_t *steps_out;
     unsigned long long *units;  // 7 query-unit counters, or nullptr
-    // forward walks with g != 0: the terminal projection is left to
-    // k_terminal, which reads the end point and weight (term_pw) and the
-    // nearest primitive's slot (term_slot, -1: value final) of every walk
+    // forward walks with g != 0: the terminal projection is left to the
+    // merge (terminal_value), which reads the end point and weight (term_pw)
+    // and the nearest primitive's slot (term_slot, -1: value final) of
+    // every walk
     float4 *term_pw;
     int32_t *term_slot;
 };
@@ -1303,7 +1304,7 @@ void k_walk_fp32(MeshView m, EqView e, PersistArgs a, AdjOut adj) {
     const float alpha = (float)e.alpha;
     WalkDer D{};
     // forward walks: the lane's walk has ended and its results are not yet
-    // stored (g = 0: the value is the source sum; else k_terminal adds
+    // stored (g = 0: the value is the source sum; else the merge adds
     // weight x g at the FP64 nearest point, one lane per walk)
     bool pending = false;
     auto store_walk = [&]() {
@@ -1547,22 +1548,22 @@ void k_walk_fp32(MeshView m, EqView e, PersistArgs a, AdjOut adj) {
 // the persistent walker only the few ending lanes of a wavefront would run
 // this FP64 code)
 // --------------------------------------------------------------------------
+// terminal value of an FP32 forward walk when g != 0: weight x g at the FP64
+// nearest point of the walk's nearest primitive `slot` (from the FP32 end
+// point; walks whose value is already final carry slot -1 and are skipped by
+// the caller).  Formed in the FP32 merge, one lane per walk, where the walk's
+// value is read anyway (in the persistent walker only the few ending lanes
+// of a wavefront would run this FP64 code).
+// --------------------------------------------------------------------------
 template <int DIM>
-__global__ __launch_bounds__(kBlock) void k_terminal(MeshView m, EqView e, RoundView rv,
-                                                     const float4 *pw, const int32_t *slots,
-                                                     double *vals) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t total = rv.total_dev ? *rv.total_dev : rv.total;
-    if (i >= total) return;
-    const int slot = slots[i];
-    if (slot < 0) return;
-    const float4 p = pw[i];
+__device__ __forceinline__ double terminal_value(const MeshView &m, const EqView &e, float4 p,
+                                                 int slot) {
     const int prim = m.slot_prim[slot];
     double nx, ny, nz = 0.0, s = 0.0, t = 0.0;
     if (DIM == 2) npq_seg_d(m.segd[slot], (double)p.x, (double)p.y, nx, ny, s);
     else eberly_d(m.trid[slot], (double)p.x, (double)p.y, (double)p.z, nx, ny, nz, s, t);
     const double g = eq_boundary(e, m, nx, ny, nz, prim, s, t);
-    vals[i] = vals[i] + (double)p.w * g;
+    return (double)p.w * g;
 }
 
 // --------------------------------------------------------------------------
@@ -1630,7 +1631,9 @@ __global__ __launch_bounds__(kBlock) void k_welford(RoundView rv, const double *
 // the walks into rounds, and every batch, gives the same bits; they equal
 // the sequential merge to rounding (the FP64 walkers keep the sequential
 // order, which defines their bits).  A point's serial chain is 1/64 as
-// long: the kernel no longer waits on one FP64 division per walk.
+// long: the kernel no longer waits on one FP64 division per walk.  TERM:
+// forward walks ending where g is not zero add their terminal value
+// (terminal_value) as they are merged.
 // --------------------------------------------------------------------------
 struct Moments { double n, mu, M2; };
 // (n, mu, M2) of the union of two disjoint samples; a.n = 0 returns b exactly
@@ -1642,12 +1645,16 @@ __device__ __forceinline__ Moments chan_combine(const Moments &a, const Moments 
     return Moments{n, a.mu + delta * fb, a.M2 + b.M2 + delta * delta * a.n * fb};
 }
 
+template <int DIM, bool TERM>
 __global__ __launch_bounds__(kBlock) void k_merge_wave(RoundView rv, const double *vals,
                                                        const int32_t *steps, double *mean,
                                                        double *m2, double *count,
                                                        double *steps_sum, int32_t *hits,
                                                        unsigned long long *failed,
-                                                       double *lane_mu, double *lane_m2) {
+                                                       double *lane_mu, double *lane_m2,
+                                                       MeshView m, EqView e,
+                                                       const float4 *term_pw,
+                                                       const int32_t *term_slot) {
     const int k = (int)(blockIdx.x * (kBlock / 64) + threadIdx.x / 64);  // wave-uniform
     const int lane = (int)(threadIdx.x & 63);
     if (k >= (rv.n_dev ? *rv.n_dev : rv.n)) return;
@@ -1664,8 +1671,12 @@ __global__ __launch_bounds__(kBlock) void k_merge_wave(RoundView rv, const doubl
     int h = 0;
     unsigned nan_walks = 0u;
     for (int j = (lane - wb) & 63; j < cnt; j += 64) {
-        const double x = vals[base + j];
+        double x = vals[base + j];
         const int32_t st = steps[base + j];
+        if constexpr (TERM) {
+            const int slot = term_slot[base + j];
+            if (slot >= 0) x = x + terminal_value<DIM>(m, e, term_pw[base + j], slot);
+        }
         nan_walks += x != x ? 1u : 0u;
         s.n = s.n + 1.0;
         const double delta_before = x - s.mu;
@@ -1694,18 +1705,26 @@ __global__ __launch_bounds__(kBlock) void k_merge_wave(RoundView rv, const doubl
 }
 
 // Merge of a round's walks: one wavefront per point for the FP32 walker,
-// the sequential merge for the FP64 walkers.
-// lane_mu / lane_m2: the FP32 merge's lane states (64 per point), needed
-// when a point's walks may come in more than one round, else nullptr.
+// the sequential merge for the FP64 walkers.  lane_mu / lane_m2: the FP32
+// merge's lane states (64 per point), needed when a point's walks may come
+// in more than one round, else nullptr.  term_slot (FP32 forward walks with
+// g != 0): the merge adds each walk's terminal value.
 hipError_t launch_merge(hipStream_t stream, int precision, const RoundView &rv, int64_t n_max,
                         double *vals, int32_t *steps, double *mean, double *m2, double *count,
                         double *ssum, int32_t *hits, unsigned long long *failed,
-                        double *lane_mu = nullptr, double *lane_m2 = nullptr) {
+                        double *lane_mu, double *lane_m2, int dim, const MeshView &m,
+                        const EqView &e, const float4 *term_pw, const int32_t *term_slot) {
     if (precision == WOS_FP32_PERSISTENT) {
         constexpr int kWaves = kBlock / 64;
-        hipLaunchKernelGGL(k_merge_wave, dim3((unsigned)((n_max + kWaves - 1) / kWaves)),
-                           dim3(kBlock), 0, stream, rv, vals, steps, mean, m2, count, ssum, hits,
-                           failed, lane_mu, lane_m2);
+        const dim3 grid((unsigned)((n_max + kWaves - 1) / kWaves));
+#define WOS_MERGE(D, T)                                                                     \
+    hipLaunchKernelGGL((k_merge_wave<D, T>), grid, dim3(kBlock), 0, stream, rv, vals, steps, \
+                       mean, m2, count, ssum, hits, failed, lane_mu, lane_m2, m, e, term_pw, \
+                       term_slot)
+        if (term_slot && dim == 2) WOS_MERGE(2, true);
+        else if (term_slot) WOS_MERGE(3, true);
+        else WOS_MERGE(3, false);
+#undef WOS_MERGE
     } else {
         // one wavefront per block: a point's merge is a serial chain, so
         // the kernel wants its few wavefronts on as many CUs as possible
@@ -2274,13 +2293,6 @@ hipError_t launch_fp32_w(wos_ctx *c, const EqView &ev, int mode, const RoundView
     else
         hipLaunchKernelGGL((k_walk_fp32<DIM, SMODE, SCR, ADJ, W>), dim3(blocks), dim3(threads), lds,
                            c->stream, view, ev, a, adj);
-    if (defer) {
-        e = hipGetLastError();
-        if (e != hipSuccess) return e;
-        hipLaunchKernelGGL(k_terminal<DIM>, dim3((unsigned)((rv.total + kBlock - 1) / kBlock)),
-                           dim3(kBlock), 0, c->stream, c->view, ev, rv, a.term_pw, a.term_slot,
-                           a.vals);
-    }
     *blocks_out = blocks;
     c->last_threads = threads;
     c->last_cached = n_cached;
@@ -2925,10 +2937,14 @@ int solve_once(wos_ctx *c, const std::vector<Part> &parts, const wos_eq_desc *eq
             HIPX(dispatch_walk(c, dim, adjr != nullptr, ev, src, scr, prm->precision, rv,
                                prm->epsilon, prm->max_steps, adj, &blocks));
             HIPX(hipEventRecord(c->ev[3], c->stream));
+            const bool term = prm->precision == WOS_FP32_PERSISTENT && !adjr &&
+                              ev.boundary_kind != WOS_BND_ZERO;
             HIPX(launch_merge(c->stream, prm->precision, rv, rv.n, c->vals.as<double>(),
                               c->steps.as<int32_t>(), c->mean.as<double>(), c->m2.as<double>(),
                               c->count.as<double>(), c->ssum.as<double>(), c->hits.as<int32_t>(),
-                              c->failed.as<unsigned long long>(), lane_mu, lane_m2));
+                              c->failed.as<unsigned long long>(), lane_mu, lane_m2, dim, c->view,
+                              ev, term ? c->term_pw.as<float4>() : nullptr,
+                              term ? c->term_slot.as<int32_t>() : nullptr));
             if (adjr) {
                 hipLaunchKernelGGL(k_adj_reduce, dim3((rv.n + kBlock - 1) / kBlock), dim3(kBlock), 0,
                                    c->stream, rv, adj, c->view, c->coef.as<double>(),
@@ -3136,10 +3152,13 @@ int solve_graph(wos_ctx *c, const std::vector<Part> &parts, const wos_eq_desc *e
                                WOS_FP32_PERSISTENT, rv, prm->epsilon,
                                prm->max_steps, AdjOut{}, &c->graph_blocks));
             if (!capture) HIPX(hipEventRecord(c->ev[3], c->stream));
+            const bool term = ev.boundary_kind != WOS_BND_ZERO;
             HIPX(launch_merge(c->stream, WOS_FP32_PERSISTENT, rv, n, c->vals.as<double>(),
                               c->steps.as<int32_t>(), c->mean.as<double>(), c->m2.as<double>(),
                               c->count.as<double>(), c->ssum.as<double>(), c->hits.as<int32_t>(),
-                              c->failed.as<unsigned long long>()));
+                              c->failed.as<unsigned long long>(), nullptr, nullptr, dim, c->view,
+                              ev, term ? c->term_pw.as<float4>() : nullptr,
+                              term ? c->term_slot.as<int32_t>() : nullptr));
             hipLaunchKernelGGL(k_pack_results, dim3((unsigned)nb), dim3(kBlock), 0, c->stream, n,
                                n_in_dev, c->failed.as<unsigned long long>(), c->loc.as<uint8_t>(),
                                c->amb.as<int32_t>(), c->bvalue.as<double>(), c->mean.as<double>(),
