@@ -1133,11 +1133,6 @@ __global__ void k_init_parts(unsigned long long *counter, int parts, int64_t tot
     counter[16 * p] = t * (uint64_t)p / (uint64_t)parts;
 }
 
-__global__ void k_zero_u32(uint32_t *p, int64_t n) {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x)
-        p[i] = 0u;
-}
 
 // Inside-point compaction on the device (graph replay, where the host does
 // not see the classification): per block of kBlock points its number of
@@ -1193,8 +1188,11 @@ __global__ __launch_bounds__(kBlock) void k_start_f(const StartInfo *starts, con
     sf[j] = f;
 }
 
+// The inside points in sweep order: inside[j] = index of the j-th inside
+// point, and (sf != nullptr) its FP32 start copy sf[j] (k_start_f's record).
 __global__ __launch_bounds__(kBlock) void k_inside_scatter(const uint8_t *loc, int64_t n,
-                                                           const int64_t *boff, int32_t *inside) {
+                                                           const int64_t *boff, int32_t *inside,
+                                                           const StartInfo *starts, StartF *sf) {
     __shared__ int scan[kBlock];
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int f = (i < n && loc[i] == 1) ? 1 : 0;
@@ -1206,7 +1204,38 @@ __global__ __launch_bounds__(kBlock) void k_inside_scatter(const uint8_t *loc, i
         scan[threadIdx.x] += v;
         __syncthreads();
     }
-    if (f) inside[boff[blockIdx.x] + scan[threadIdx.x] - 1] = (int32_t)i;
+    if (!f) return;
+    const int64_t j = boff[blockIdx.x] + scan[threadIdx.x] - 1;
+    inside[j] = (int32_t)i;
+    if (sf) {
+        const StartInfo &S = starts[i];
+        StartF r;
+        r.x = (float)S.x;
+        r.y = (float)S.y;
+        r.z = (float)S.z;
+        r.r = (float)S.r;
+        r.slot = S.slot;
+        r.domain = S.domain;
+        r.key0 = S.key0;
+        r.key1 = S.key1;
+        sf[j] = r;
+    }
+}
+
+// the per-point statistics of a solve and its failed-walk counter to zero,
+// in one launch (a kernel, not memset nodes, so that it can be captured)
+__global__ __launch_bounds__(kBlock) void k_zero_stats(double *mean, double *m2, double *count,
+                                                       double *ssum, int32_t *hits,
+                                                       unsigned long long *failed, int64_t n) {
+    const int64_t i0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i0 == 0) *failed = 0ull;
+    for (int64_t i = i0; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        mean[i] = 0.0;
+        m2[i] = 0.0;
+        count[i] = 0.0;
+        ssum[i] = 0.0;
+        hits[i] = 0;
+    }
 }
 
 // --------------------------------------------------------------------------
@@ -3122,17 +3151,12 @@ int solve_graph(wos_ctx *c, const std::vector<Part> &parts, const wos_eq_desc *e
                                total_dev, nw);
             hipLaunchKernelGGL(k_inside_scatter, dim3((unsigned)nb), dim3(kBlock), 0, c->stream,
                                c->loc.as<uint8_t>(), n, c->g_boff.as<int64_t>(),
-                               c->inside.as<int32_t>());
-            hipLaunchKernelGGL(k_start_f, dim3((unsigned)nb), dim3(kBlock), 0, c->stream,
-                               c->starts.as<StartInfo>(), c->inside.as<int32_t>(), n, n_in_dev,
+                               c->inside.as<int32_t>(), c->starts.as<StartInfo>(),
                                c->starts_f.as<StartF>());
-            for (DevBuf *b : {&c->mean, &c->m2, &c->count, &c->ssum})
-                hipLaunchKernelGGL(k_zero_u64, dim3(256), dim3(256), 0, c->stream,
-                                   b->as<unsigned long long>(), n);
-            hipLaunchKernelGGL(k_zero_u32, dim3(256), dim3(256), 0, c->stream,
-                               c->hits.as<uint32_t>(), n);
-            hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, c->stream,
-                               c->failed.as<unsigned long long>(), 1);
+            hipLaunchKernelGGL(k_zero_stats, dim3(256), dim3(kBlock), 0, c->stream,
+                               c->mean.as<double>(), c->m2.as<double>(), c->count.as<double>(),
+                               c->ssum.as<double>(), c->hits.as<int32_t>(),
+                               c->failed.as<unsigned long long>(), n);
             HIPX(hipGetLastError());
             RoundView rv{};
             rv.starts = c->starts.as<StartInfo>();
