@@ -250,35 +250,50 @@ __device__ __forceinline__ double box_d2_3(const NodeD3 &n, double px, double py
     return dx * dx + dy * dy + dz * dz;
 }
 
-// Closest point, reference traversal (src/bvh.cpp:295-387): pop, cull by
-// the popped box, test leaf primitives with strict <, push the farther
-// child first.  `stk` is this lane's LDS stack (stride = block size).
+// Closest point, reference traversal.  `stk` is this lane's LDS stack
+// (stride = block size).  The reference's search (src/bvh.cpp:295-387): pop
+// a node, skip it when
+// its box is no nearer than the best so far, test a leaf's primitives in
+// order, or push the far child and then the near one (near = smaller box
+// distance, ties to the left).  Here the near child is not pushed and
+// popped again: the search continues into it with the record it has just
+// loaded and the box distance it has just formed (nothing changes the best
+// distance in between, so the skip test is the same), which saves one
+// dependent load per internal node; the far child waits on the stack as in
+// the reference.  Same nodes, same order, same result.
 __device__ __forceinline__ NpqD npq_ref_2d(const MeshView &m, int root, double px, double py,
                                            int *stk, int stride) {
     NpqD r;
     r.d2 = INFINITY; r.px = r.py = r.pz = 0.0; r.s = r.t = 0.0; r.prim = -1; r.slot = -1;
-    int sp = 1;
-    stk[0] = root;
-    while (sp > 0) {
-        const int curr = stk[(--sp) * stride];
-        const NodeD2 n = m.nd2[curr];
-        if (box_d2_2(n, px, py) >= r.d2) continue;
-        if (n.count > 0) {
-            for (int i = 0; i < n.count; ++i) {
-                const int slot = n.idx + i;
-                double nx, ny, par;
-                const double d = npq_seg_d(m.segd[slot], px, py, nx, ny, par);
-                if (d < r.d2) {
-                    r.d2 = d; r.px = nx; r.py = ny; r.s = par; r.prim = m.slot_prim[slot]; r.slot = slot;
+    int sp = 0;
+    NodeD2 n = m.nd2[root];
+    int curr = root;
+    double nd = box_d2_2(n, px, py);
+    for (;;) {
+        if (nd < r.d2) {
+            if (n.count > 0) {
+                for (int i = 0; i < n.count; ++i) {
+                    const int slot = n.idx + i;
+                    double nx, ny, par;
+                    const double d = npq_seg_d(m.segd[slot], px, py, nx, ny, par);
+                    if (d < r.d2) {
+                        r.d2 = d; r.px = nx; r.py = ny; r.s = par; r.prim = m.slot_prim[slot]; r.slot = slot;
+                    }
                 }
+            } else {
+                const int l = curr + 1, rr = n.idx;
+                const NodeD2 nl = m.nd2[l], nr = m.nd2[rr];
+                const double ld = box_d2_2(nl, px, py);
+                const double rd = box_d2_2(nr, px, py);
+                if (ld <= rd) { stk[(sp++) * stride] = rr; curr = l; n = nl; nd = ld; }
+                else { stk[(sp++) * stride] = l; curr = rr; n = nr; nd = rd; }
+                continue;
             }
-        } else {
-            const int l = curr + 1, rr = n.idx;
-            const double ld = box_d2_2(m.nd2[l], px, py);
-            const double rd = box_d2_2(m.nd2[rr], px, py);
-            if (ld <= rd) { stk[(sp++) * stride] = rr; stk[(sp++) * stride] = l; }
-            else { stk[(sp++) * stride] = l; stk[(sp++) * stride] = rr; }
         }
+        if (sp == 0) break;
+        curr = stk[(--sp) * stride];
+        n = m.nd2[curr];
+        nd = box_d2_2(n, px, py);
     }
     return r;
 }
@@ -287,30 +302,37 @@ __device__ __forceinline__ NpqD npq_ref_3d(const MeshView &m, int root, double p
                                            double pz, int *stk, int stride) {
     NpqD r;
     r.d2 = INFINITY; r.px = r.py = r.pz = 0.0; r.s = r.t = 0.0; r.prim = -1; r.slot = -1;
-    int sp = 1;
-    stk[0] = root;
-    while (sp > 0) {
-        const int curr = stk[(--sp) * stride];
-        const NodeD3 n = m.nd3[curr];
-        if (box_d2_3(n, px, py, pz) >= r.d2) continue;
-        if (n.count > 0) {
-            for (int i = 0; i < n.count; ++i) {
-                const int slot = n.idx + i;
-                double nx, ny, nz, s, t;
-                const double d = eberly_d(m.trid[slot], px, py, pz, nx, ny, nz, s, t);
-                if (d < r.d2) {
-                    r.d2 = d; r.px = nx; r.py = ny; r.pz = nz; r.s = s; r.t = t;
-                    r.prim = m.slot_prim[slot];
-                    r.slot = slot;
+    int sp = 0;
+    NodeD3 n = m.nd3[root];
+    int curr = root;
+    double nd = box_d2_3(n, px, py, pz);
+    for (;;) {
+        if (nd < r.d2) {
+            if (n.count > 0) {
+                for (int i = 0; i < n.count; ++i) {
+                    const int slot = n.idx + i;
+                    double nx, ny, nz, s, t;
+                    const double d = eberly_d(m.trid[slot], px, py, pz, nx, ny, nz, s, t);
+                    if (d < r.d2) {
+                        r.d2 = d; r.px = nx; r.py = ny; r.pz = nz; r.s = s; r.t = t;
+                        r.prim = m.slot_prim[slot];
+                        r.slot = slot;
+                    }
                 }
+            } else {
+                const int l = curr + 1, rr = n.idx;
+                const NodeD3 nl = m.nd3[l], nr = m.nd3[rr];
+                const double ld = box_d2_3(nl, px, py, pz);
+                const double rd = box_d2_3(nr, px, py, pz);
+                if (ld <= rd) { stk[(sp++) * stride] = rr; curr = l; n = nl; nd = ld; }
+                else { stk[(sp++) * stride] = l; curr = rr; n = nr; nd = rd; }
+                continue;
             }
-        } else {
-            const int l = curr + 1, rr = n.idx;
-            const double ld = box_d2_3(m.nd3[l], px, py, pz);
-            const double rd = box_d2_3(m.nd3[rr], px, py, pz);
-            if (ld <= rd) { stk[(sp++) * stride] = rr; stk[(sp++) * stride] = l; }
-            else { stk[(sp++) * stride] = l; stk[(sp++) * stride] = rr; }
         }
+        if (sp == 0) break;
+        curr = stk[(--sp) * stride];
+        n = m.nd3[curr];
+        nd = box_d2_3(n, px, py, pz);
     }
     return r;
 }
