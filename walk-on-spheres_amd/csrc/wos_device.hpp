@@ -43,7 +43,8 @@ struct GridInfo {
 };
 struct alignas(16) GridCellF {
     uint32_t dcbits;   // FP32 bits of dc, low 7 bits = list length
-    uint32_t first;    // first list entry
+    uint32_t first;    // first list entry; bit 31 (kGridPlanar): the list's
+                       // triangles lie in one plane, kept in MeshView::planes
     int32_t slot0;     // leaf slot of list entry 0 (nearest to the centre), -1 if none
     float dmin1;       // list entry 1's distance from the centre, +inf if none
 };
@@ -53,6 +54,7 @@ struct GridEntry {
 };
 constexpr int kGridListMax = 127;
 constexpr uint32_t kGridLenMask = 127u;
+constexpr uint32_t kGridPlanar = 0x80000000u;
 
 // --------------------------------------------------------------------------
 // kernel-side views
@@ -88,6 +90,9 @@ struct MeshView {
     const GridInfo *grids;     // per domain
     const GridCellF *cells;
     const GridEntry *glist;    // cell lists
+    // per cell (3D grids): for a planar cell the plane (unit normal n, offset
+    // d = n . x on the plane) of its list's triangles
+    const float4 *planes;
     const DomainF *doms;       // per domain
 };
 
@@ -1186,7 +1191,13 @@ __device__ __forceinline__ T domain_record(const T *arr, int index) {
     return rec;
 }
 
-// Begin a query at p in `domain`.  With a distance grid: with cf < 1, a cell
+// Begin a query at p in `domain`.  With a distance grid: with cf < 1 and
+// PLANE, a planar cell (its list's triangles all in one plane) ends it at
+// once with lb = (n . p - d)^2 and best = +inf: the nearest primitive of
+// every point of a cell is on its list, and a triangle in the plane is no
+// nearer than the plane, so |n . p - d| is a lower bound of the distance,
+// and equal to it wherever p projects onto the face (a relaxed answer: a
+// walk that it would end searches again exactly).  With cf < 1, a cell
 // whose centre distance dc gives lb = dc - |p - c| and ub = dc' + |p - c|
 // (dc' = dc rounded up) with lb^2 >= cf ub^2 ends it at once with
 // best = ub^2, lb = lb^2; otherwise a cell that holds a primitive list turns
@@ -1195,7 +1206,7 @@ __device__ __forceinline__ T domain_record(const T *arr, int index) {
 // in q.best_slot (the previous step's nearest, -1 for none), which bounds it
 // from the first node on.  Returns the kQs* bits; kQsMore is clear when the
 // query is already complete.
-template <int DIM>
+template <int DIM, bool PLANE = false>
 __device__ __forceinline__ uint32_t query_begin(QueryF &q, const MeshView &m, int32_t root,
                                                 int domain, float cf, float px, float py,
                                                 float pz, TriCache tc = TriCache{nullptr, 0}) {
@@ -1216,6 +1227,16 @@ __device__ __forceinline__ uint32_t query_begin(QueryF &q, const MeshView &m, in
                 q.ckey = key;
             }
             const GridCellF C = q.cell;
+            if constexpr (PLANE && DIM == 3) {
+                if (cf < 1.0f && (C.first & kGridPlanar) != 0u) {
+                    const float4 P = m.planes[key - 1u];
+                    const float h = fabsf(__builtin_fmaf(
+                        P.x, px, __builtin_fmaf(P.y, py, __builtin_fmaf(P.z, pz, -P.w))));
+                    q.lb = h * h;
+                    q.best = INFINITY;
+                    return 0u;
+                }
+            }
             const float dx = px - (G.ox + ((float)ix + 0.5f) * G.h);
             const float dy = py - (G.oy + ((float)iy + 0.5f) * G.h);
             const float dz = DIM == 3 ? pz - (G.oz + ((float)iz + 0.5f) * G.h) : 0.0f;
@@ -1243,7 +1264,7 @@ __device__ __forceinline__ uint32_t query_begin(QueryF &q, const MeshView &m, in
                     q.best = leaf_d2f<DIM>(m, C.slot0, px, py, pz);
                 }
                 q.best_slot = C.slot0;
-                q.lslot = (int32_t)C.first + 1;
+                q.lslot = (int32_t)(C.first & ~kGridPlanar) + 1;
                 q.lcount = (int32_t)len - 1;
                 if (q.lcount > 0) {
                     const float l1 = fmaxf(C.dmin1 - t, 0.0f);

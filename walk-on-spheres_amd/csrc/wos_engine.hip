@@ -565,7 +565,9 @@ __global__ __launch_bounds__(kBlock) void k_grid_build(MeshView m, GridInfo G, i
                                                        double pad, int cap, double band,
                                                        int phase, GridCellF *cells,
                                                        int32_t *counts, double *rl2,
-                                                       const int64_t *offsets, GridEntry *glist) {
+                                                       const int64_t *offsets, GridEntry *glist,
+                                                       float4 *planes, double plane_tol,
+                                                       unsigned int *n_planar) {
     extern __shared__ int lds_g[];
     int *stk = lds_g + threadIdx.x;
     const int n = G.nx * G.ny * G.nz;
@@ -596,9 +598,36 @@ __global__ __launch_bounds__(kBlock) void k_grid_build(MeshView m, GridInfo G, i
         const int cnt = range_collect<DIM>(m, root64, cx, cy, cz, rl2[c], stk, blockDim.x,
                                            glist + off, cap);
         cells[c].dcbits |= (uint32_t)cnt;
-        cells[c].first = (uint32_t)off;
         cells[c].slot0 = glist[off].slot;
         cells[c].dmin1 = cnt > 1 ? glist[off + 1].dmin : INFINITY;
+        // planar cell: two or more listed triangles, every vertex of each
+        // within plane_tol of entry 0's plane (FP64).  (A one-triangle list
+        // is answered exactly by its entry 0, from the lane's LDS copy.)
+        bool planar = false;
+        if (DIM == 3 && planes && cnt >= 2) {
+            const TriD &T0 = m.trid[glist[off].slot];
+            double nx = T0.e1y * T0.e2z - T0.e1z * T0.e2y;
+            double ny = T0.e1z * T0.e2x - T0.e1x * T0.e2z;
+            double nz = T0.e1x * T0.e2y - T0.e1y * T0.e2x;
+            const double len = sqrt(nx * nx + ny * ny + nz * nz);
+            planar = len > 0.0;
+            if (planar) {
+                nx /= len; ny /= len; nz /= len;
+                const double d = nx * T0.v0x + ny * T0.v0y + nz * T0.v0z;
+                for (int k = 0; k < cnt && planar; ++k) {
+                    const TriD &T = m.trid[glist[off + k].slot];
+                    const double h0 = nx * T.v0x + ny * T.v0y + nz * T.v0z - d;
+                    const double h1 = h0 + nx * T.e1x + ny * T.e1y + nz * T.e1z;
+                    const double h2 = h0 + nx * T.e2x + ny * T.e2y + nz * T.e2z;
+                    planar = fabs(h0) <= plane_tol && fabs(h1) <= plane_tol && fabs(h2) <= plane_tol;
+                }
+                if (planar) {
+                    planes[c] = make_float4((float)nx, (float)ny, (float)nz, (float)d);
+                    atomicAdd(n_planar, 1u);
+                }
+            }
+        }
+        cells[c].first = (uint32_t)off | (planar ? kGridPlanar : 0u);
     }
 }
 
@@ -1287,7 +1316,9 @@ struct PersistArgs {
 // most 128 VGPRs; the 3D Green samplers would spill at 96).  Adjoint
 // replays carry the FP64 parameter derivatives and may use up to 256 VGPRs
 // rather than spill them.
-template <int DIM, int SMODE, bool SCR, bool ADJ, int W>
+// PL: the grid has planar cells (3D), whose relaxed queries end at the
+// plane distance (query_begin).
+template <int DIM, int SMODE, bool SCR, bool ADJ, int W, bool PL>
 __global__ __launch_bounds__(kBlock)
 __attribute__((amdgpu_waves_per_eu(ADJ ? 2 : (DIM == 3 && SMODE != 2 ? 5 : 4), 8)))
 void k_walk_fp32(MeshView m, EqView e, PersistArgs a, AdjOut adj) {
@@ -1525,7 +1556,7 @@ void k_walk_fp32(MeshView m, EqView e, PersistArgs a, AdjOut adj) {
             ++steps;
         }
         if (stepping || requery) {
-            qb = query_begin<DIM>(q, m, root, dom, requery ? 1.0f : a.cf, px, py, pz, tcp);
+            qb = query_begin<DIM, PL>(q, m, root, dom, requery ? 1.0f : a.cf, px, py, pz, tcp);
             searching = (qb & kQsMore) != 0u;
             began = true;
         }
@@ -1992,6 +2023,8 @@ struct wos_ctx {
     DevBuf g_bcount, g_boff, g_misc;
     bool count_units = false;
     DevBuf g_info, g_cells, g_list;   // distance grid
+    DevBuf g_planes;                  // its planar cells' planes (3D)
+    bool grid_planar = false;         // the grid has planar cells
     bool grid_valid = false;          // the grid below belongs to the current mesh
     int grid_built_cells = 0;         // the tuning's grid_cells it was built for
     int grid_built_cap = 0;           // its list cap
@@ -2051,6 +2084,7 @@ int64_t grid_cells_for(int tune_cells, int dim, int64_t n_prims) {
 hipError_t ensure_grid(wos_ctx *c) {
     if (c->tune.grid_cells == 0) {
         c->view.grids = nullptr; c->view.cells = nullptr; c->view.glist = nullptr;
+        c->view.planes = nullptr;
         return hipSuccess;
     }
     if (!c->grid_valid || c->grid_built_cells != c->tune.grid_cells ||
@@ -2090,9 +2124,14 @@ hipError_t ensure_grid(wos_ctx *c) {
             total += (int64_t)G.nx * G.ny * G.nz;
             if (total > (int64_t)1 << 30) return hipErrorOutOfMemory;
         }
-        DevBuf counts, rl2, offs;
+        DevBuf counts, rl2, offs, n_planar;
         hipError_t e;
+        if ((e = n_planar.reserve(sizeof(unsigned int))) != hipSuccess) return e;
+        if ((e = hipMemsetAsync(n_planar.p, 0, sizeof(unsigned int), c->stream)) != hipSuccess)
+            return e;
         if ((e = c->g_cells.reserve((size_t)total * sizeof(GridCellF))) != hipSuccess) return e;
+        if (dim == 3 && (e = c->g_planes.reserve((size_t)total * sizeof(float4))) != hipSuccess)
+            return e;
         if ((e = counts.reserve((size_t)total * sizeof(int32_t))) != hipSuccess) return e;
         if ((e = rl2.reserve((size_t)total * sizeof(double))) != hipSuccess) return e;
         const size_t lds = (size_t)kBlock * (c->L.depth + 2) * sizeof(int);
@@ -2134,25 +2173,36 @@ hipError_t ensure_grid(wos_ctx *c) {
                 const double band = r > 0.0 ? 1.05 * hd * (2.0 - r) / r + pad : INFINITY;
                 const int cap = c->tune.grid_list_max;
                 const int blocks = (n + kBlock - 1) / kBlock;
+                // coplanar within a quarter of an FP32 ulp of the domain's
+                // coordinates: the plane distance's own FP32 error (a few
+                // ulp) stays inside the walker's radius margin (32 ulp)
+                const double plane_tol = std::ldexp(1.0, -26) * D.coord_scale;
                 if (dim == 2)
                     hipLaunchKernelGGL(k_grid_build<2>, dim3(blocks), dim3(kBlock), lds, c->stream, v,
                                        G, D.node_off, pad, cap, band, phase, c->g_cells.as<GridCellF>(),
                                        counts.as<int32_t>(), rl2.as<double>(), offs.as<int64_t>(),
-                                       c->g_list.as<GridEntry>());
+                                       c->g_list.as<GridEntry>(), nullptr, 0.0, nullptr);
                 else
                     hipLaunchKernelGGL(k_grid_build<3>, dim3(blocks), dim3(kBlock), lds, c->stream, v,
                                        G, D.node_off, pad, cap, band, phase, c->g_cells.as<GridCellF>(),
                                        counts.as<int32_t>(), rl2.as<double>(), offs.as<int64_t>(),
-                                       c->g_list.as<GridEntry>());
+                                       c->g_list.as<GridEntry>(), c->g_planes.as<float4>(),
+                                       plane_tol, n_planar.as<unsigned int>());
                 if ((e = hipGetLastError()) != hipSuccess) return e;
             }
         }
+        unsigned int planar_cells = 0;
+        if ((e = hipMemcpyAsync(&planar_cells, n_planar.p, sizeof(planar_cells),
+                                hipMemcpyDeviceToHost, c->stream)) != hipSuccess)
+            return e;
         if ((e = upload(c->g_info, info, c->stream)) != hipSuccess) return e;
         if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) return e;
+        c->grid_planar = planar_cells > 0;
         c->grid_built_cells = c->tune.grid_cells;
         c->grid_built_cap = c->tune.grid_list_max;
         c->grid_built_relax = c->tune.relax;
         c->grid_bytes = total * (int64_t)sizeof(GridCellF) +
+                        (dim == 3 ? total * (int64_t)sizeof(float4) : 0) +
                         c->grid_list_len * (int64_t)sizeof(GridEntry) +
                         nd * (int64_t)sizeof(GridInfo);
         c->grid_build_pending_s =
@@ -2162,6 +2212,7 @@ hipError_t ensure_grid(wos_ctx *c) {
     c->view.grids = c->g_info.as<GridInfo>();
     c->view.cells = c->g_cells.as<GridCellF>();
     c->view.glist = c->g_list.as<GridEntry>();
+    c->view.planes = c->mesh.dim == 3 ? c->g_planes.as<float4>() : nullptr;
     return hipSuccess;
 }
 
@@ -2221,14 +2272,14 @@ void launch_fp64_sx(wos_ctx *c, const EqView &ev, const RoundView &rv, double ep
 
 // The FP32 persistent walker on the binary (W = 2) or 4-wide (W = 4) FP32
 // forest: launch shape from the occupancy, LDS layout, arguments.
-template <int DIM, int SMODE, int kM3, int kM4, int kM5, bool SCR, bool ADJ, int W>
+template <int DIM, int SMODE, int kM3, int kM4, int kM5, bool SCR, bool ADJ, int W, bool PL>
 hipError_t launch_fp32_w(wos_ctx *c, const EqView &ev, int mode, const RoundView &rv, double eps,
                          int max_steps, const AdjOut &adj, int *blocks_out) {
-    const void *kfn = mode == 3   ? (const void *)k_walk_fp32<DIM, kM3, SCR, ADJ, W>
-                      : mode == 4 ? (const void *)k_walk_fp32<DIM, kM4, SCR, ADJ, W>
-                      : mode == 5 ? (const void *)k_walk_fp32<DIM, kM5, SCR, ADJ, W>
-                      : mode == 2 ? (const void *)k_walk_fp32<DIM, 2, SCR, ADJ, W>
-                                  : (const void *)k_walk_fp32<DIM, SMODE, SCR, ADJ, W>;
+    const void *kfn = mode == 3   ? (const void *)k_walk_fp32<DIM, kM3, SCR, ADJ, W, PL>
+                      : mode == 4 ? (const void *)k_walk_fp32<DIM, kM4, SCR, ADJ, W, PL>
+                      : mode == 5 ? (const void *)k_walk_fp32<DIM, kM5, SCR, ADJ, W, PL>
+                      : mode == 2 ? (const void *)k_walk_fp32<DIM, 2, SCR, ADJ, W, PL>
+                                  : (const void *)k_walk_fp32<DIM, SMODE, SCR, ADJ, W, PL>;
     const int threads = c->tune.block_threads;
     // LDS per block beyond the node cache: one triangle record per thread
     // (3D) and the first klds entries of every thread's traversal stack
@@ -2308,19 +2359,19 @@ hipError_t launch_fp32_w(wos_ctx *c, const EqView &ev, int mode, const RoundView
     e = hipGetLastError();
     if (e != hipSuccess) return e;
     if (mode == 3)
-        hipLaunchKernelGGL((k_walk_fp32<DIM, kM3, SCR, ADJ, W>), dim3(blocks), dim3(threads), lds,
+        hipLaunchKernelGGL((k_walk_fp32<DIM, kM3, SCR, ADJ, W, PL>), dim3(blocks), dim3(threads), lds,
                            c->stream, view, ev, a, adj);
     else if (mode == 4)
-        hipLaunchKernelGGL((k_walk_fp32<DIM, kM4, SCR, ADJ, W>), dim3(blocks), dim3(threads), lds,
+        hipLaunchKernelGGL((k_walk_fp32<DIM, kM4, SCR, ADJ, W, PL>), dim3(blocks), dim3(threads), lds,
                            c->stream, view, ev, a, adj);
     else if (mode == 5)
-        hipLaunchKernelGGL((k_walk_fp32<DIM, kM5, SCR, ADJ, W>), dim3(blocks), dim3(threads), lds,
+        hipLaunchKernelGGL((k_walk_fp32<DIM, kM5, SCR, ADJ, W, PL>), dim3(blocks), dim3(threads), lds,
                            c->stream, view, ev, a, adj);
     else if (mode == 2)
-        hipLaunchKernelGGL((k_walk_fp32<DIM, 2, SCR, ADJ, W>), dim3(blocks), dim3(threads), lds,
+        hipLaunchKernelGGL((k_walk_fp32<DIM, 2, SCR, ADJ, W, PL>), dim3(blocks), dim3(threads), lds,
                            c->stream, view, ev, a, adj);
     else
-        hipLaunchKernelGGL((k_walk_fp32<DIM, SMODE, SCR, ADJ, W>), dim3(blocks), dim3(threads), lds,
+        hipLaunchKernelGGL((k_walk_fp32<DIM, SMODE, SCR, ADJ, W, PL>), dim3(blocks), dim3(threads), lds,
                            c->stream, view, ev, a, adj);
     *blocks_out = blocks;
     c->last_threads = threads;
@@ -2370,11 +2421,22 @@ hipError_t launch_walk(wos_ctx *c, const EqView &ev, int precision, const RoundV
     constexpr int kM3 = kExt ? 3 : SMODE, kM4 = kExt ? 4 : SMODE, kM5 = kExt ? 5 : SMODE;
     const bool green = ev.source_mode == WOS_SOURCE_GREEN && SRC;
     const int mode = sx == 1 ? 3 : sx == 2 ? (green ? 4 : 5) : green ? 2 : SMODE;
+    // the planar-cell query only where the grid has planar cells (3D): the
+    // other kernels do without its test
+    if constexpr (DIM == 3) {
+        if (c->grid_planar && c->view.planes) {
+            if (c->tune.bvh_width == 4)
+                return launch_fp32_w<DIM, SMODE, kM3, kM4, kM5, SCR, ADJ, 4, true>(
+                    c, ev, mode, rv, eps, max_steps, adj, blocks_out);
+            return launch_fp32_w<DIM, SMODE, kM3, kM4, kM5, SCR, ADJ, 2, true>(
+                c, ev, mode, rv, eps, max_steps, adj, blocks_out);
+        }
+    }
     if (c->tune.bvh_width == 4)
-        return launch_fp32_w<DIM, SMODE, kM3, kM4, kM5, SCR, ADJ, 4>(c, ev, mode, rv, eps, max_steps,
-                                                                      adj, blocks_out);
-    return launch_fp32_w<DIM, SMODE, kM3, kM4, kM5, SCR, ADJ, 2>(c, ev, mode, rv, eps, max_steps,
-                                                                  adj, blocks_out);
+        return launch_fp32_w<DIM, SMODE, kM3, kM4, kM5, SCR, ADJ, 4, false>(c, ev, mode, rv, eps,
+                                                                             max_steps, adj, blocks_out);
+    return launch_fp32_w<DIM, SMODE, kM3, kM4, kM5, SCR, ADJ, 2, false>(c, ev, mode, rv, eps,
+                                                                         max_steps, adj, blocks_out);
 }
 
 template <int DIM, bool ADJ>
