@@ -2907,10 +2907,21 @@ int solve_once(wos_ctx *c, const std::vector<Part> &parts, const wos_eq_desc *eq
         const bool scr = eq->kind == WOS_EQ_SCREENED || eq->kind == WOS_EQ_HELMHOLTZ;
         const bool adaptive = prm->target_se > 0.0;
         const int64_t cap = adaptive ? std::max<int64_t>(prm->max_walks, prm->n_walks) : prm->n_walks;
-        // staging: one FP64 value + one int32 per walk (plus the adjoint
-        // record: 9 doubles + one int32), at most 4 GiB a round
+        // staging: one FP64 value + one int32 per walk, at most 4 GiB a
+        // round; an adjoint replay adds 9 doubles + one int32 per walk and
+        // may take up to 24 GiB (a quarter of the free device memory at
+        // most), so that a solve's replay needs few rounds (each round ends
+        // with a walker tail).  Results do not depend on the split.
         const int64_t walk_bytes = adjr ? 12 + 76 : 12;
-        const int64_t budget = std::max<int64_t>((int64_t)(4LL << 30) / walk_bytes, 1);
+        int64_t round_bytes = 4LL << 30;
+        if (adjr) {
+            size_t free_b = 0, total_b = 0;
+            if (hipMemGetInfo(&free_b, &total_b) == hipSuccess)
+                round_bytes = std::max(round_bytes, std::min<int64_t>((int64_t)(free_b / 4),
+                                                                       24LL << 30));
+        }
+        const int64_t budget = std::max<int64_t>(
+            std::min<int64_t>(round_bytes / walk_bytes, (1LL << 31) - 1), 1);
         AdjOut adj{};
         const bool want_v = adjr && adjr->grad_vertex && eq->boundary_kind == WOS_BND_VERTEX;
         const bool want_b = adjr && adjr->grad_params && eq->boundary_kind == WOS_BND_ID_CONST;
