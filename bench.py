@@ -354,7 +354,7 @@ def workload_annulus(Context, meshgen, problems, capi, args, want_cpu):
     grid pts/s at SE <= 1e-3"""
     e, r32 = grid_workload("annulus", Context, capi, meshgen.annulus(128), problems.annulus_laplace(),
                            (64, 64, 1), 1024, 1e-3, 12345, args.steps, args.warmup,
-                           "k_walk_fp32<2,0,false,false,2>", want_cpu, terminal=True)
+                           "k_walk_fp32<2,0,false,false,2,false>", want_cpu, terminal=True)
     ins = r32["location"] == 1
     xs = np.linspace(-1, 1, 64)
     X, Y = np.meshgrid(xs, xs, indexing="ij")
@@ -370,7 +370,7 @@ def workload_icosphere(Context, meshgen, problems, capi, args, want_cpu):
     e, _ = grid_workload("icosphere", Context, capi, meshgen.icosphere(4),
                          problems.sphere_screened_constant(alpha=10.0, f=1.0), (32, 32, 32),
                          args.walks, 1e-4, 7, args.steps, args.warmup,
-                         "k_walk_fp32<3,1,true,false,2>", want_cpu, graph=True, tree=True)
+                         "k_walk_fp32<3,1,true,false,2,false>", want_cpu, graph=True, tree=True)
     e["description"] = workload_icosphere.__doc__.strip()
     return e
 
@@ -389,7 +389,7 @@ def workload_plate(Context, meshgen, problems, capi, args, want_cpu):
     # from x0: both estimates are 0 to 1e-160 and their SEs vanish)
     e, r32 = grid_workload("plate", Context, capi, mesh, eq, grid, 8192, 0.01, 5,
                            max(1, args.steps // 4), min(args.warmup, 1),
-                           "k_walk_fp32<3,1,true,false,2>", want_cpu, floor=1e-12)
+                           "k_walk_fp32<3,1,true,false,2,true>", want_cpu, floor=1e-12)
     ctx = Context(0)
     ctx.set_mesh(*mesh)
     p32 = Context.params(8192, 0.01, 5, precision=capi.WOS_FP32_PERSISTENT)
@@ -462,9 +462,9 @@ def workload_torus(Context, meshgen, problems, capi, args, want_cpu, walks=2048)
              adjoint_kernel_walks_per_s=n_walks / sa["walk_kernel_seconds"],
              grad_norm=float(np.linalg.norm(gv)),
              setup={"first_solve_ms": t_first * 1e3},
-             roofline=roofline("torus_adjoint", "k_walk_fp32<3,0,false,true,2>", sa["total_steps"],
+             roofline=roofline("torus_adjoint", "k_walk_fp32<3,0,false,true,2,false>", sa["total_steps"],
                                sa["walk_kernel_seconds"], units, 3, True),
-             forward_roofline=roofline("torus_forward", "k_walk_fp32<3,0,false,false,2>",
+             forward_roofline=roofline("torus_forward", "k_walk_fp32<3,0,false,false,2,false>",
                                        sf["total_steps"], sf["walk_kernel_seconds"], units, 3,
                                        True))
     # FP64 forward at the same walks
@@ -532,7 +532,7 @@ def workload_stars(Context, meshgen, problems, capi, args, want_cpu):
              walk_kernel_s=tot["walk_kernel_seconds"], failed_walks=tot["failed_walks"],
              setup={"grid_build_ms": first["grid_build_seconds"] * 1e3,
                     "grid_bytes": first["grid_bytes"], "first_solve_ms": t_setup * 1e3},
-             roofline=roofline("stars", "k_walk_fp32<2,2,false,false,2>", tot["total_steps"],
+             roofline=roofline("stars", "k_walk_fp32<2,2,false,false,2,false>", tot["total_steps"],
                                tot["walk_kernel_seconds"], units, 2, True),
              domains=per)
     if want_cpu:
