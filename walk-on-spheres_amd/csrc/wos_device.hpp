@@ -265,7 +265,7 @@ __device__ __forceinline__ double box_d2_3(const NodeD3 &n, double px, double py
 // loaded and the box distance it has just formed (nothing changes the best
 // distance in between, so the skip test is the same), which saves one
 // dependent load per internal node; the far child waits on the stack as in
-// the reference.  Same nodes, same order, same result.
+// the reference.  Same nodes, same order, same result (2D).
 __device__ __forceinline__ NpqD npq_ref_2d(const MeshView &m, int root, double px, double py,
                                            int *stk, int stride) {
     NpqD r;
@@ -303,41 +303,37 @@ __device__ __forceinline__ NpqD npq_ref_2d(const MeshView &m, int root, double p
     return r;
 }
 
+// In 3D the reference's loop as it stands: there the FP64 walker, which
+// spills registers, measured the continuation 8 % slower on the 200k-triangle
+// plate (5 % faster on the icosphere).
 __device__ __forceinline__ NpqD npq_ref_3d(const MeshView &m, int root, double px, double py,
                                            double pz, int *stk, int stride) {
     NpqD r;
     r.d2 = INFINITY; r.px = r.py = r.pz = 0.0; r.s = r.t = 0.0; r.prim = -1; r.slot = -1;
-    int sp = 0;
-    NodeD3 n = m.nd3[root];
-    int curr = root;
-    double nd = box_d2_3(n, px, py, pz);
-    for (;;) {
-        if (nd < r.d2) {
-            if (n.count > 0) {
-                for (int i = 0; i < n.count; ++i) {
-                    const int slot = n.idx + i;
-                    double nx, ny, nz, s, t;
-                    const double d = eberly_d(m.trid[slot], px, py, pz, nx, ny, nz, s, t);
-                    if (d < r.d2) {
-                        r.d2 = d; r.px = nx; r.py = ny; r.pz = nz; r.s = s; r.t = t;
-                        r.prim = m.slot_prim[slot];
-                        r.slot = slot;
-                    }
+    int sp = 1;
+    stk[0] = root;
+    while (sp > 0) {
+        const int curr = stk[(--sp) * stride];
+        const NodeD3 n = m.nd3[curr];
+        if (box_d2_3(n, px, py, pz) >= r.d2) continue;
+        if (n.count > 0) {
+            for (int i = 0; i < n.count; ++i) {
+                const int slot = n.idx + i;
+                double nx, ny, nz, s, t;
+                const double d = eberly_d(m.trid[slot], px, py, pz, nx, ny, nz, s, t);
+                if (d < r.d2) {
+                    r.d2 = d; r.px = nx; r.py = ny; r.pz = nz; r.s = s; r.t = t;
+                    r.prim = m.slot_prim[slot];
+                    r.slot = slot;
                 }
-            } else {
-                const int l = curr + 1, rr = n.idx;
-                const NodeD3 nl = m.nd3[l], nr = m.nd3[rr];
-                const double ld = box_d2_3(nl, px, py, pz);
-                const double rd = box_d2_3(nr, px, py, pz);
-                if (ld <= rd) { stk[(sp++) * stride] = rr; curr = l; n = nl; nd = ld; }
-                else { stk[(sp++) * stride] = l; curr = rr; n = nr; nd = rd; }
-                continue;
             }
+        } else {
+            const int l = curr + 1, rr = n.idx;
+            const double ld = box_d2_3(m.nd3[l], px, py, pz);
+            const double rd = box_d2_3(m.nd3[rr], px, py, pz);
+            if (ld <= rd) { stk[(sp++) * stride] = rr; stk[(sp++) * stride] = l; }
+            else { stk[(sp++) * stride] = l; stk[(sp++) * stride] = rr; }
         }
-        if (sp == 0) break;
-        curr = stk[(--sp) * stride];
-        n = m.nd3[curr];
-        nd = box_d2_3(n, px, py, pz);
     }
     return r;
 }
