@@ -411,7 +411,11 @@ int wos_solve_points_indexed(wos_ctx *ctx, const double *points, const int64_t *
  *       sampling depends on alpha (Green source mode, 2D).
  * grad_vertex: n_verts doubles (WOS_BND_VERTEX problems; zeros otherwise).
  * grad_params: WOS_N_GRAD doubles indexed by the WOS_GRAD_* offsets.
- * `out` receives the replayed forward statistics (out->mean required). */
+ * `out` receives the replayed forward statistics (out->mean required).
+ * grad_vertex and the WOS_GRAD_BND_VALUES entries are sums over walks formed
+ * with FP64 atomic adds, so their last bits can differ between runs (the
+ * order of the adds); `out` and the other grad_params entries (per-point
+ * sums added in point order) are bitwise reproducible. */
 enum wos_grad_index {
     WOS_GRAD_ALPHA = 0,
     WOS_GRAD_SRC_VALUE = 1,

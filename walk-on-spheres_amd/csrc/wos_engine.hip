@@ -760,6 +760,23 @@ __device__ __forceinline__ int round_point(const RoundView &r, int64_t i, uint32
     return k;
 }
 
+// kblk of an uneven round: for every block b of 64 items the k with
+// off[k] <= b * 64 < off[k + 1].  off (n + 1 entries, off[0] = 0) is strictly
+// increasing, since every point listed in a round has walks in it.
+__global__ __launch_bounds__(kBlock) void k_round_kblk(const int64_t *off, int n, int64_t nb,
+                                                       int32_t *kblk) {
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nb) return;
+    const int64_t item = b << kItemBlockShift;
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (off[mid] <= item) lo = mid;
+        else hi = mid - 1;
+    }
+    kblk[b] = lo;
+}
+
 __device__ __forceinline__ int round_inside_id(const RoundView &r, int k) {
     return r.pid_identity ? k : r.pid[k];
 }
@@ -2041,6 +2058,13 @@ struct wos_ctx {
         adj_scal, adj_vw, adj_prim;
     DevBuf field;                     // WOS_SRC_FIELD: the previous field
     DevBuf term_pw, term_slot;        // FP32 walks' end points and slots (g != 0)
+    // an uneven round's point tables (off, pid, wbeg) on the host: two, so
+    // that the next round's are written while the last upload may still be
+    // queued; each is rewritten only after its upload's event
+    PinnedBuf h_round[2];
+    hipEvent_t h_round_ev[2]{};
+    bool h_round_used[2]{};
+    std::vector<hipEvent_t> round_ev;  // walker / merge times of queued rounds (3 a round)
     int last_rounds = 0;
     MeshView view{};
     hipEvent_t ev[6]{};
@@ -2946,12 +2970,26 @@ int solve_once(wos_ctx *c, const std::vector<Part> &parts, const wos_eq_desc *eq
             lane_mu = c->lane_st.as<double>();
             lane_m2 = lane_mu + (size_t)n_in * 64;
         }
-        std::vector<int32_t> pid, wbeg, kblk;
-        std::vector<int64_t> off;
         std::vector<double> a_cnt(n_in), a_m2(n_in);
         std::vector<int32_t> all_ids(n_in);
         for (int k = 0; k < n_in; ++k) all_ids[k] = k;
         HIPX(upload(c->pid_all, all_ids, c->stream));
+        // Rounds are queued one after the other without waiting: the host
+        // plans the next round while the last one runs, and waits only where
+        // it needs their results (the SE test of an adaptive solve, the end).
+        int queued = 0;
+        auto settle = [&]() -> hipError_t {
+            if (queued == 0) return hipSuccess;
+            const hipError_t e = hipEventSynchronize(c->round_ev[3 * (queued - 1) + 2]);
+            if (e != hipSuccess) return e;
+            for (int r = 0; r < queued; ++r) {
+                walk_s += elapsed(c->round_ev[3 * r], c->round_ev[3 * r + 1]);
+                total_walk_s += elapsed(c->round_ev[3 * r], c->round_ev[3 * r + 2]);
+            }
+            queued = 0;
+            return hipSuccess;
+        };
+        int hslot = 0;
         for (int round = 0; round < 4096; ++round) {
             // ---- this round: uniform when every point needs the same walks
             int64_t remaining = 0;
@@ -2963,6 +3001,7 @@ int solve_once(wos_ctx *c, const std::vector<Part> &parts, const wos_eq_desc *eq
             if (remaining == 0) {
                 if (!adaptive) break;
                 // SE test on every point below the cap (wos_run statistics)
+                HIPX(settle());
                 HIPX(hipMemcpyAsync(a_cnt.data(), c->count.p, n_in * sizeof(double), hipMemcpyDeviceToHost, c->stream));
                 HIPX(hipMemcpyAsync(a_m2.data(), c->m2.p, n_in * sizeof(double), hipMemcpyDeviceToHost, c->stream));
                 HIPX(hipStreamSynchronize(c->stream));
@@ -2994,34 +3033,53 @@ int solve_once(wos_ctx *c, const std::vector<Part> &parts, const wos_eq_desc *eq
                 rv.total = cw * n_in;
                 for (int k = 0; k < n_in; ++k) { done[k] += cw; want[k] -= cw; }
             } else {
-                pid.clear(); wbeg.clear(); off.assign(1, 0);
+                // the round's points in sweep order until the budget is used
+                // up, their tables written to the pinned buffer whose last
+                // upload is done
+                PinnedBuf &hb = c->h_round[hslot];
+                if (c->h_round_used[hslot]) HIPX(hipEventSynchronize(c->h_round_ev[hslot]));
+                const size_t cap_n = (size_t)n_in;
+                HIPX(hb.reserve((cap_n + 1) * sizeof(int64_t) + 2 * cap_n * sizeof(int32_t)));
+                int64_t *h_off = static_cast<int64_t *>(hb.p);
+                int32_t *h_pid = reinterpret_cast<int32_t *>(h_off + cap_n + 1);
+                int32_t *h_wbeg = h_pid + cap_n;
+                int np = 0;
                 int64_t tot = 0;
+                h_off[0] = 0;
                 for (int k = 0; k < n_in && tot < budget; ++k) {
                     if (want[k] == 0) continue;
                     const int64_t cnt = std::min<int64_t>(want[k], budget - tot);
-                    pid.push_back(k);
-                    wbeg.push_back((int32_t)done[k]);
+                    h_pid[np] = k;
+                    h_wbeg[np] = (int32_t)done[k];
                     tot += cnt;
-                    off.push_back(tot);
+                    h_off[++np] = tot;
                     done[k] += cnt;
                     want[k] -= cnt;
                 }
+                HIPX(c->r_pid.reserve(cap_n * sizeof(int32_t)));
+                HIPX(c->r_wbeg.reserve(cap_n * sizeof(int32_t)));
+                HIPX(c->r_off.reserve((cap_n + 1) * sizeof(int64_t)));
+                HIPX(hipMemcpyAsync(c->r_pid.p, h_pid, (size_t)np * sizeof(int32_t),
+                                    hipMemcpyHostToDevice, c->stream));
+                HIPX(hipMemcpyAsync(c->r_wbeg.p, h_wbeg, (size_t)np * sizeof(int32_t),
+                                    hipMemcpyHostToDevice, c->stream));
+                HIPX(hipMemcpyAsync(c->r_off.p, h_off, (size_t)(np + 1) * sizeof(int64_t),
+                                    hipMemcpyHostToDevice, c->stream));
+                HIPX(hipEventRecord(c->h_round_ev[hslot], c->stream));
+                c->h_round_used[hslot] = true;
+                hslot ^= 1;
                 // the point holding the first item of every block of 64 items
-                kblk.assign((size_t)((tot + (1 << kItemBlockShift) - 1) >> kItemBlockShift), 0);
-                for (size_t b = 0, k = 0; b < kblk.size(); ++b) {
-                    const int64_t item = (int64_t)b << kItemBlockShift;
-                    while (off[k + 1] <= item) ++k;
-                    kblk[b] = (int32_t)k;
-                }
-                HIPX(upload(c->r_pid, pid, c->stream));
-                HIPX(upload(c->r_wbeg, wbeg, c->stream));
-                HIPX(upload(c->r_off, off, c->stream));
-                HIPX(upload(c->r_kblk, kblk, c->stream));
+                const int64_t nb = (tot + (1 << kItemBlockShift) - 1) >> kItemBlockShift;
+                HIPX(c->r_kblk.reserve((size_t)nb * sizeof(int32_t)));
+                hipLaunchKernelGGL(k_round_kblk, dim3((unsigned)((nb + kBlock - 1) / kBlock)),
+                                   dim3(kBlock), 0, c->stream, c->r_off.as<int64_t>(), np, nb,
+                                   c->r_kblk.as<int32_t>());
+                HIPX(hipGetLastError());
                 rv.pid = c->r_pid.as<int32_t>();
                 rv.wbeg = c->r_wbeg.as<int32_t>();
                 rv.off = c->r_off.as<int64_t>();
                 rv.kblk = c->r_kblk.as<int32_t>();
-                rv.n = (int)pid.size();
+                rv.n = np;
                 rv.total = tot;
             }
             HIPX(c->vals.reserve((size_t)rv.total * sizeof(double)));
@@ -3035,10 +3093,18 @@ int solve_once(wos_ctx *c, const std::vector<Part> &parts, const wos_eq_desc *eq
                 adj.vw = c->adj_vw.as<double>();
                 adj.prim = c->adj_prim.as<int32_t>();
             }
-            HIPX(hipEventRecord(c->ev[2], c->stream));
+            // at most 64 rounds queued (the timing events they hold)
+            if (queued == 64) HIPX(settle());
+            while ((int)c->round_ev.size() < 3 * (queued + 1)) {
+                hipEvent_t e = nullptr;
+                HIPX(hipEventCreate(&e));
+                c->round_ev.push_back(e);
+            }
+            hipEvent_t *const re = &c->round_ev[3 * queued];
+            HIPX(hipEventRecord(re[0], c->stream));
             HIPX(dispatch_walk(c, dim, adjr != nullptr, ev, src, scr, prm->precision, rv,
                                prm->epsilon, prm->max_steps, adj, &blocks));
-            HIPX(hipEventRecord(c->ev[3], c->stream));
+            HIPX(hipEventRecord(re[1], c->stream));
             const bool term = prm->precision == WOS_FP32_PERSISTENT && !adjr &&
                               ev.boundary_kind != WOS_BND_ZERO;
             HIPX(launch_merge(c->stream, prm->precision, rv, rv.n, c->vals.as<double>(),
@@ -3054,12 +3120,11 @@ int solve_once(wos_ctx *c, const std::vector<Part> &parts, const wos_eq_desc *eq
                                    want_b ? c->grad_b.as<double>() : nullptr);
                 HIPX(hipGetLastError());
             }
-            HIPX(hipEventRecord(c->ev[4], c->stream));
-            HIPX(hipEventSynchronize(c->ev[4]));
-            walk_s += elapsed(c->ev[2], c->ev[3]);
-            total_walk_s += elapsed(c->ev[2], c->ev[4]);
+            HIPX(hipEventRecord(re[2], c->stream));
+            ++queued;
             ++c->last_rounds;
         }
+        HIPX(settle());
         HIPX(hipMemcpyAsync(h_mean.data(), c->mean.p, n_in * sizeof(double), hipMemcpyDeviceToHost, c->stream));
         HIPX(hipMemcpyAsync(h_m2.data(), c->m2.p, n_in * sizeof(double), hipMemcpyDeviceToHost, c->stream));
         HIPX(hipMemcpyAsync(h_cnt.data(), c->count.p, n_in * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -3505,6 +3570,11 @@ wos_ctx *wos_ctx_create(int device) {
             fail(WOS_ERR_DEVICE, "hipEventCreate failed");
             return nullptr;
         }
+    for (auto &e : c->h_round_ev)
+        if (hipEventCreate(&e) != hipSuccess) {
+            fail(WOS_ERR_DEVICE, "hipEventCreate failed");
+            return nullptr;
+        }
     {
         const hipError_t e = scratch_address(device, &c->scratch_dev);
         if (e != hipSuccess) {
@@ -3522,6 +3592,8 @@ void wos_ctx_destroy(wos_ctx *c) {
     if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
     if (c->graph) (void)hipGraphDestroy(c->graph);
     for (auto &e : c->ev) if (e) (void)hipEventDestroy(e);
+    for (auto &e : c->h_round_ev) if (e) (void)hipEventDestroy(e);
+    for (auto &e : c->round_ev) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(c->stream);
     delete c;
 }
