@@ -2030,6 +2030,7 @@ struct wos_ctx {
     DevBuf pack;                      // one-round grid solve results, packed (PackLayout)
     DevBuf lane_st;                   // FP32 merge lane states of multi-round solves
     PinnedBuf hpack;                  // their host copy
+    PinnedBuf hcls, hstat;            // multi-round solves: classification and statistics staging
     std::vector<std::pair<TuneKey, wos_tuning>> tuned;  // autotuned shapes
     // graph replay: the captured solve, what it was captured for, its tolerances
     hipGraph_t graph = nullptr;
@@ -2077,6 +2078,22 @@ struct wos_ctx {
     int last_threads = 0;
     int last_cached = 0;
     void *scratch_dev = nullptr;      // this device's `scratch` array (no-allocation entry point)
+
+    wos_ctx() = default;
+    wos_ctx(const wos_ctx &) = delete;
+    wos_ctx &operator=(const wos_ctx &) = delete;
+    // the HIP objects this context made (a failed wos_ctx_create included);
+    // the device buffers are freed after this, as members
+    ~wos_ctx() {
+        (void)hipSetDevice(device);
+        if (stream) (void)hipStreamSynchronize(stream);
+        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        for (auto &e : ev) if (e) (void)hipEventDestroy(e);
+        for (auto &e : h_round_ev) if (e) (void)hipEventDestroy(e);
+        for (auto &e : round_ev) (void)hipEventDestroy(e);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
 };
 
 namespace {
@@ -2872,12 +2889,15 @@ int solve_once(wos_ctx *c, const std::vector<Part> &parts, const wos_eq_desc *eq
     if (rc != WOS_OK) return rc;
     HIPX(hipEventRecord(c->ev[1], c->stream));
 
-    std::vector<uint8_t> loc(n);
-    std::vector<int32_t> amb(n);
-    std::vector<double> bval(n);
-    HIPX(hipMemcpyAsync(loc.data(), c->loc.p, n, hipMemcpyDeviceToHost, c->stream));
-    HIPX(hipMemcpyAsync(amb.data(), c->amb.p, n * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
-    HIPX(hipMemcpyAsync(bval.data(), c->bvalue.p, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    // classification results, then the statistics, reach the host through
+    // page-locked staging (one DMA transfer each)
+    HIPX(c->hcls.reserve((size_t)n * (sizeof(double) + sizeof(int32_t) + 1)));
+    double *const bval = static_cast<double *>(c->hcls.p);
+    int32_t *const amb = reinterpret_cast<int32_t *>(bval + n);
+    uint8_t *const loc = reinterpret_cast<uint8_t *>(amb + n);
+    HIPX(hipMemcpyAsync(loc, c->loc.p, n, hipMemcpyDeviceToHost, c->stream));
+    HIPX(hipMemcpyAsync(amb, c->amb.p, n * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    HIPX(hipMemcpyAsync(bval, c->bvalue.p, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPX(hipStreamSynchronize(c->stream));
 
     std::vector<int32_t> inside;
@@ -2897,8 +2917,12 @@ int solve_once(wos_ctx *c, const std::vector<Part> &parts, const wos_eq_desc *eq
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, c->stream,
                        c->failed.as<unsigned long long>(), 1);
     HIPX(hipGetLastError());
-    std::vector<double> h_mean(n_in), h_m2(n_in), h_cnt(n_in), h_ss(n_in);
-    std::vector<int32_t> h_hits(n_in);
+    HIPX(c->hstat.reserve((size_t)n_in * (4 * sizeof(double) + sizeof(int32_t))));
+    double *const h_mean = static_cast<double *>(c->hstat.p);
+    double *const h_m2 = h_mean + n_in;
+    double *const h_cnt = h_m2 + n_in;
+    double *const h_ss = h_cnt + n_in;
+    int32_t *const h_hits = reinterpret_cast<int32_t *>(h_ss + n_in);
     double walk_s = 0.0, total_walk_s = 0.0;
     int blocks = 0;
     if (n_in > 0) {
@@ -2970,7 +2994,8 @@ int solve_once(wos_ctx *c, const std::vector<Part> &parts, const wos_eq_desc *eq
             lane_mu = c->lane_st.as<double>();
             lane_m2 = lane_mu + (size_t)n_in * 64;
         }
-        std::vector<double> a_cnt(n_in), a_m2(n_in);
+        // the SE test reads the counts and M2 into the statistics' staging
+        double *const a_cnt = h_cnt, *const a_m2 = h_m2;
         std::vector<int32_t> all_ids(n_in);
         for (int k = 0; k < n_in; ++k) all_ids[k] = k;
         HIPX(upload(c->pid_all, all_ids, c->stream));
@@ -3002,8 +3027,8 @@ int solve_once(wos_ctx *c, const std::vector<Part> &parts, const wos_eq_desc *eq
                 if (!adaptive) break;
                 // SE test on every point below the cap (wos_run statistics)
                 HIPX(settle());
-                HIPX(hipMemcpyAsync(a_cnt.data(), c->count.p, n_in * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-                HIPX(hipMemcpyAsync(a_m2.data(), c->m2.p, n_in * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+                HIPX(hipMemcpyAsync(a_cnt, c->count.p, n_in * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+                HIPX(hipMemcpyAsync(a_m2, c->m2.p, n_in * sizeof(double), hipMemcpyDeviceToHost, c->stream));
                 HIPX(hipStreamSynchronize(c->stream));
                 const double t2 = prm->target_se * prm->target_se;
                 for (int k = 0; k < n_in; ++k) {
@@ -3125,11 +3150,11 @@ int solve_once(wos_ctx *c, const std::vector<Part> &parts, const wos_eq_desc *eq
             ++c->last_rounds;
         }
         HIPX(settle());
-        HIPX(hipMemcpyAsync(h_mean.data(), c->mean.p, n_in * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-        HIPX(hipMemcpyAsync(h_m2.data(), c->m2.p, n_in * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-        HIPX(hipMemcpyAsync(h_cnt.data(), c->count.p, n_in * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-        HIPX(hipMemcpyAsync(h_ss.data(), c->ssum.p, n_in * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-        HIPX(hipMemcpyAsync(h_hits.data(), c->hits.p, n_in * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+        HIPX(hipMemcpyAsync(h_mean, c->mean.p, n_in * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPX(hipMemcpyAsync(h_m2, c->m2.p, n_in * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPX(hipMemcpyAsync(h_cnt, c->count.p, n_in * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPX(hipMemcpyAsync(h_ss, c->ssum.p, n_in * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPX(hipMemcpyAsync(h_hits, c->hits.p, n_in * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
         HIPX(hipStreamSynchronize(c->stream));
     }
     if (adjr) {
@@ -3191,8 +3216,8 @@ int solve_once(wos_ctx *c, const std::vector<Part> &parts, const wos_eq_desc *eq
     unsigned long long nf = 0;
     HIPX(hipMemcpyAsync(&nf, c->failed.p, sizeof(nf), hipMemcpyDeviceToHost, c->stream));
     HIPX(hipStreamSynchronize(c->stream));
-    return finish_solve(c, parts, part_tol, n, loc.data(), amb.data(), bval.data(), h_mean.data(),
-                        h_m2.data(), h_cnt.data(), h_ss.data(), h_hits.data(), nf, classify_s,
+    return finish_solve(c, parts, part_tol, n, loc, amb, bval, h_mean, h_m2, h_cnt, h_ss, h_hits,
+                        nf, classify_s,
                         walk_s, total_walk_s, blocks, prm, out, summary, part_summaries);
 }
 
@@ -3585,18 +3610,7 @@ wos_ctx *wos_ctx_create(int device) {
     return c.release();
 }
 
-void wos_ctx_destroy(wos_ctx *c) {
-    if (!c) return;
-    (void)hipSetDevice(c->device);
-    (void)hipStreamSynchronize(c->stream);
-    if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
-    if (c->graph) (void)hipGraphDestroy(c->graph);
-    for (auto &e : c->ev) if (e) (void)hipEventDestroy(e);
-    for (auto &e : c->h_round_ev) if (e) (void)hipEventDestroy(e);
-    for (auto &e : c->round_ev) (void)hipEventDestroy(e);
-    (void)hipStreamDestroy(c->stream);
-    delete c;
-}
+void wos_ctx_destroy(wos_ctx *c) { delete c; }
 
 int wos_ctx_set_domains(wos_ctx *c, int dim, int n_domains, const double *const *verts,
                         const int64_t *n_verts, const int32_t *const *prims,
