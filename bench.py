@@ -169,6 +169,20 @@ def roofline(key, kernel, steps, kernel_s, units=None, dim=3, terminal=False):
                                          ("q_queries", "q_grid_bound", "q_list_scans",
                                           "q_list_entries", "q_list_prims", "q_nodes",
                                           "q_leaf_prims")}}
+    # The bound is the limiter the counters show: HBM when the measured
+    # traffic takes a larger share of its peak than the issued VALU
+    # instructions take of theirs.  `achieved` then stays the algorithmic
+    # bytes (the records the queries must read) and the VALU figures move to
+    # `valu`; traffic far above the algorithmic bytes is cache-line waste.
+    if out["traffic"] and out["frac"] is not None and "hbm" in out:
+        traffic_frac = out["traffic"] / kernel_s / 1e9 / HBM_PEAK_GBS
+        out["traffic_frac"] = traffic_frac
+        if traffic_frac > out["frac"]:
+            valu = {k: out[k] for k in ("unit", "peak", "achieved", "frac", "valu_insts_per_step",
+                                        "lane_utilisation", "useful_lane_frac")}
+            h = out["hbm"]
+            out.update(bound="hbm", unit="GB/s", peak=HBM_PEAK_GBS, achieved=h["achieved_gbs"],
+                       frac=h["frac"], valu=valu)
     return out
 
 
