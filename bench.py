@@ -18,15 +18,17 @@ Grid pts/s at SE <= 1e-3 = #inside / sum_p n_p * t_walk,p with
 n_p = ceil(var_p / 1e-6) from the FP64 run (shared by every form) and
 t_walk,p = (T / total steps) * mean_steps_p of the form being measured.
 
-Roofline.  The walker kernels are bound by VALU instruction issue (no stage
-is a dense contraction, so MFMA does not apply).  achieved = VALU wave64
-instructions per walk step (SQ_INSTS_VALU of the same kernel on the same
-workload, profiles/walker_counters.json) x the steps of the live launch /
-its live duration (HIP events on the launch stream); peak = 256 CUs x
-2.4 GHz x 1 wave64 VALU instruction per CU per cycle (the issue rate behind
-the 78.6 TFLOP/s FP64 / 157.3 TFLOP/s packed-FP32 vector peaks).  The HBM
-figure (algorithmic bytes of the records the query algorithm must read,
-from the launch's own query-unit counts) is reported beside it.
+Roofline.  No stage is a dense contraction, so MFMA does not apply.  VALU
+issue: VALU wave64 instructions per walk step (SQ_INSTS_VALU of the same
+kernel on the same workload, profiles/walker_counters.json) x the steps of
+the live launch / its live duration (HIP events on the launch stream),
+against 256 CUs x 2.4 GHz x 1 wave64 VALU instruction per CU per cycle (the
+issue rate behind the 78.6 TFLOP/s FP64 / 157.3 TFLOP/s packed-FP32 vector
+peaks).  HBM: the algorithmic bytes of the records the query algorithm must
+read (from the launch's own query-unit counts) per second against 8 TB/s,
+with the counters' traffic (FETCH_SIZE + WRITE_SIZE, calibrated) beside it.
+`bound` is whichever the counters show binding: HBM where the traffic's
+share of 8 TB/s exceeds the VALU issue share (the torus), else VALU.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
        python bench.py --workload annulus|icosphere|stars|torus|plate|welding
@@ -137,9 +139,10 @@ def walker_bytes(s, dim, terminal=True):
 
 
 def roofline(key, kernel, steps, kernel_s, units=None, dim=3, terminal=False):
-    """VALU-issue roofline of the workload's dominant kernel, with the HBM
-    figure beside it.  `steps`/`kernel_s`: the live launch(es); `units`: the
-    summary of a count_units solve of the same problem."""
+    """Roofline of the workload's dominant kernel: VALU issue and HBM, the
+    bound the one the counters show binding.  `steps`/`kernel_s`: the live
+    launch(es); `units`: the summary of a count_units solve of the same
+    problem."""
     out = {"bound": "valu", "unit": "G VALU inst/s", "peak": VALU_PEAK_GINST, "kernel": kernel,
            "kernel_ms": kernel_s * 1e3, "steps": steps, "achieved": None, "frac": None,
            "traffic": None}
