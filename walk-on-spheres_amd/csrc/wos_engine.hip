@@ -6,7 +6,7 @@
 //      reference boundary / winding / ray-vote classification, the walk-stream
 //      key and g at the nearest point;
 //   2. host compaction of the inside points;
-//   3. k_walk_fp64 (one lane per walk) or k_walk_fp32 (persistent lanes,
+//   3. k_walk_fp64 (persistent lanes, one walk at a time) or k_walk_fp32 (persistent lanes,
 //      ballot-driven refill) writing one FP64 value per walk into a
 //      [walk][point] staging buffer;
 //   4. k_welford: per point, the reference's sequential Welford update in walk
@@ -797,78 +797,132 @@ void set_walks_per_point(RoundView &r, int cw) {
 }
 
 // --------------------------------------------------------------------------
-// kernel 2a: FP64 plain walker, one lane per (point, walk)
+// kernel 2a: FP64 plain walker, persistent lanes over (point, walk) items
 // --------------------------------------------------------------------------
+// The grid holds as many lanes as the GPU keeps resident.  Every trip of the
+// loop either takes one step of the lane's walk or, when the walk has ended,
+// stores it; lanes without a walk then take the next items of their
+// wavefront's pool (ranked by mbcnt; the pool takes kFp64Chunk items at a time
+// with one atomicAdd on `next`), so a lane whose walk ends early does not
+// wait for the longest walk of its wavefront.  A
+// walk's arithmetic is that of one lane per walk, so its value and steps do
+// not depend on the schedule.
 // At least 3 wavefronts per SIMD (at most 168 VGPRs) for the 3D forward
 // walks, which fit without spilling; 2D walks with a source term and adjoint
 // replays may use 256 rather than spill.
+constexpr int kFp64Chunk = 64;  // items a wavefront of k_walk_fp64 takes at a time
+
 template <int DIM, bool SRC, bool SCR, bool ADJ, int SX = 0>
 __global__ __launch_bounds__(kBlock)
 __attribute__((amdgpu_waves_per_eu(ADJ || DIM == 2 ? 2 : 3, 8)))
-void k_walk_fp64(MeshView m, EqView e, RoundView rv,
-                                                      double eps, int max_steps, double *vals,
-                                                      int32_t *steps_out, AdjOut adj) {
+void k_walk_fp64(MeshView m, EqView e, RoundView rv, double eps, int max_steps, double *vals,
+                 int32_t *steps_out, AdjOut adj, unsigned long long *next) {
     extern __shared__ int lds_i[];
     int *stk = lds_i + threadIdx.x;
     const int stride = blockDim.x;
-    const int64_t item = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (item >= rv.total) return;
-    uint32_t w;
-    // the start record is read field by field where needed (a register copy
-    // of all of it would stay live across the walk)
-    const StartInfo *S = &round_item(rv, item, w);
-    double px = S->x, py = S->y, pz = S->z, R = S->r;
-    const uint32_t k0 = S->key0, k1 = S->key1;
-    const int root = S->root64;
+    const int lane = __lane_id();
+    const double alpha = e.alpha;
+    // the walk held by the lane.  The start record is read field by field
+    // where needed (a register copy of all of it would stay live).
+    bool active = false;
+    // wave-uniform pool of items [pool_next, pool_end), kFp64Chunk at a time
+    // from `next`; exhausted once `next` has passed the last item
+    bool exhausted = false;
+    uint64_t pool_next = 0, pool_end = 0;
+    int64_t item = 0;
+    uint32_t w = 0;
+    const StartInfo *S = nullptr;
+    double px = 0.0, py = 0.0, pz = 0.0, R = 0.0;
+    uint32_t k0 = 0, k1 = 0;
+    int root = 0;
     NpqD q{};
     double sample = 0.0, weight = 1.0;
     int steps = 0;
-    const double alpha = e.alpha;
     WalkDer D{};
-    while (R > eps && steps < max_steps) {
-        // the step's counter stream: the source term draws from slot DIM-1
-        // on, then the direction takes slots 0 .. DIM-2 (block 0 is formed
-        // once when the source's draws stayed in it)
-        StepStreamD rng(w, (uint32_t)steps, k0, k1, DIM - 1);
-        if (SRC && e.source_kind != WOS_SRC_ZERO) {
-            if constexpr (ADJ)
-                sample += source_term_adj<DIM, double, 1>(e, px, py, pz, R, weight, rng, D);
-            else
-                sample += source_term<DIM, double, StepStreamD, SX, SCR>(e, px, py, pz, R, weight, rng);
+    for (;;) {
+        // ---- lanes without a walk take the next items
+        const uint64_t free_mask = __ballot(!active);
+        if (free_mask != 0 && !exhausted && pool_next >= pool_end) {
+            unsigned long long base = 0;
+            if (lane == 0) base = atomicAdd(next, (unsigned long long)kFp64Chunk);
+            base = __shfl(base, 0);
+            if (base >= (unsigned long long)rv.total) {
+                exhausted = true;
+            } else {
+                pool_next = base;
+                const uint64_t end = base + (uint64_t)kFp64Chunk;
+                pool_end = end < (uint64_t)rv.total ? end : (uint64_t)rv.total;
+            }
         }
-        if (SCR) {
-            if constexpr (ADJ && DIM == 3)
-                if (e.kind == WOS_EQ_SCREENED) D.L += dlog_w3_dalpha(alpha, R);
-            weight *= step_weight<DIM, double>(e.kind, alpha, R);
+        if (free_mask != 0 && pool_next < pool_end) {
+            if (!active) {
+                const uint32_t lo = (uint32_t)free_mask, hi = (uint32_t)(free_mask >> 32);
+                const int rank = __builtin_amdgcn_mbcnt_hi(hi, __builtin_amdgcn_mbcnt_lo(lo, 0));
+                item = (int64_t)(pool_next + (uint64_t)rank);
+                if ((uint64_t)item < pool_end) {
+                    S = &round_item(rv, item, w);
+                    px = S->x; py = S->y; pz = S->z; R = S->r;
+                    k0 = S->key0; k1 = S->key1;
+                    root = S->root64;
+                    q = NpqD{};
+                    sample = 0.0; weight = 1.0;
+                    steps = 0;
+                    if constexpr (ADJ) D = WalkDer{};
+                    active = true;
+                }
+            }
+            const uint64_t nfree = (uint64_t)__popcll(free_mask);
+            pool_next = pool_next + nfree < pool_end ? pool_next + nfree : pool_end;
         }
-        rng.seek(0);
-        const double du = rng.unit();
-        const double dv = DIM == 3 ? rng.unit() : 0.0;
-        if (DIM == 2) {
-            const double theta = du * 2 * kPi;
-            px = px + R * cos(theta);
-            py = py + R * sin(theta);
+        if (__ballot(active) == 0 && exhausted) break;
+        if (!active) continue;
+        if (R > eps && steps < max_steps) {
+            // the step's counter stream: the source term draws from slot
+            // DIM-1 on, then the direction takes slots 0 .. DIM-2 (block 0 is
+            // formed once when the source's draws stayed in it)
+            StepStreamD rng(w, (uint32_t)steps, k0, k1, DIM - 1);
+            if (SRC && e.source_kind != WOS_SRC_ZERO) {
+                if constexpr (ADJ)
+                    sample += source_term_adj<DIM, double, 1>(e, px, py, pz, R, weight, rng, D);
+                else
+                    sample += source_term<DIM, double, StepStreamD, SX, SCR>(e, px, py, pz, R, weight, rng);
+            }
+            if (SCR) {
+                if constexpr (ADJ && DIM == 3)
+                    if (e.kind == WOS_EQ_SCREENED) D.L += dlog_w3_dalpha(alpha, R);
+                weight *= step_weight<DIM, double>(e.kind, alpha, R);
+            }
+            rng.seek(0);
+            const double du = rng.unit();
+            const double dv = DIM == 3 ? rng.unit() : 0.0;
+            if (DIM == 2) {
+                const double theta = du * 2 * kPi;
+                px = px + R * cos(theta);
+                py = py + R * sin(theta);
+            } else {
+                const double u = du, v = dv;
+                const double z = 2 * u - 1, phi = 2 * kPi * v;
+                const double rr = sqrt(1 - z * z);
+                px = px + R * rr * cos(phi);
+                py = py + R * rr * sin(phi);
+                pz = pz + R * z;
+            }
+            ++steps;
+            q = DIM == 2 ? npq_ref_2d(m, root, px, py, stk, stride)
+                         : npq_ref_3d(m, root, px, py, pz, stk, stride);
+            R = sqrt(q.d2);
         } else {
-            const double u = du, v = dv;
-            const double z = 2 * u - 1, phi = 2 * kPi * v;
-            const double rr = sqrt(1 - z * z);
-            px = px + R * rr * cos(phi);
-            py = py + R * rr * sin(phi);
-            pz = pz + R * z;
+            const bool hit = R > eps;
+            const double g = steps == 0 ? S->g : eq_boundary(e, m, q.px, q.py, q.pz, q.prim, q.s, q.t);
+            vals[item] = sample + weight * g;
+            steps_out[item] = steps | (hit ? (int32_t)0x80000000 : 0);
+            if constexpr (ADJ) {
+                const int prim = steps == 0 ? S->prim : q.prim;
+                const double s = steps == 0 ? S->s : q.s, t = steps == 0 ? S->t : q.t;
+                store_adjoint(adj, item, D, weight, g, prim, s, DIM == 3 ? t : 0.0);
+            }
+            active = false;
         }
-        ++steps;
-        q = DIM == 2 ? npq_ref_2d(m, root, px, py, stk, stride)
-                     : npq_ref_3d(m, root, px, py, pz, stk, stride);
-        R = sqrt(q.d2);
-    }
-    const bool hit = R > eps;
-    const double g = steps == 0 ? S->g : eq_boundary(e, m, q.px, q.py, q.pz, q.prim, q.s, q.t);
-    vals[item] = sample + weight * g;
-    steps_out[item] = steps | (hit ? (int32_t)0x80000000 : 0);
-    if constexpr (ADJ) {
-        const int prim = steps == 0 ? S->prim : q.prim;
-        const double s = steps == 0 ? S->s : q.s, t = steps == 0 ? S->t : q.t;
-        store_adjoint(adj, item, D, weight, g, prim, s, DIM == 3 ? t : 0.0);
     }
 }
 
@@ -2304,11 +2358,21 @@ void launch_ref_sx(wos_ctx *c, const EqView &ev, const RoundView &rv, double eps
 }
 
 template <int DIM, bool SRC, bool SCR, bool ADJ, int SX>
-void launch_fp64_sx(wos_ctx *c, const EqView &ev, const RoundView &rv, double eps, int max_steps,
-                    const AdjOut &adj, int blocks, size_t lds) {
-    hipLaunchKernelGGL((k_walk_fp64<DIM, SRC, SCR, ADJ, SX>), dim3(blocks), dim3(kBlock), lds,
-                       c->stream, c->view, ev, rv, eps, max_steps, c->vals.as<double>(),
-                       c->steps.as<int32_t>(), adj);
+int launch_fp64_sx(wos_ctx *c, const EqView &ev, const RoundView &rv, double eps, int max_steps,
+                   const AdjOut &adj, size_t lds) {
+    // persistent: at most the blocks the GPU keeps resident
+    const auto kfn = k_walk_fp64<DIM, SRC, SCR, ADJ, SX>;
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, kBlock, lds) != hipSuccess ||
+        per_cu < 1)
+        per_cu = 1;
+    const int64_t need = (rv.total + kBlock - 1) / kBlock;
+    const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>(need, (int64_t)per_cu * c->n_cu));
+    unsigned long long *next = c->counter.as<unsigned long long>();
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, c->stream, next, 1);
+    hipLaunchKernelGGL(kfn, dim3(blocks), dim3(kBlock), lds, c->stream, c->view, ev, rv, eps,
+                       max_steps, c->vals.as<double>(), c->steps.as<int32_t>(), adj, next);
+    return blocks;
 }
 
 // The FP32 persistent walker on the binary (W = 2) or 4-wide (W = 4) FP32
@@ -2446,12 +2510,10 @@ hipError_t launch_walk(wos_ctx *c, const EqView &ev, int precision, const RoundV
         }
     }
     if (precision == WOS_FP64_PLAIN) {
-        const int blocks = (int)((rv.total + kBlock - 1) / kBlock);
         const size_t lds = (size_t)kBlock * (c->L.depth + 2) * sizeof(int);
-        if (sx == 1) launch_fp64_sx<DIM, SRC, SCR, ADJ, kMis>(c, ev, rv, eps, max_steps, adj, blocks, lds);
-        else if (sx == 2) launch_fp64_sx<DIM, SRC, SCR, ADJ, kFld>(c, ev, rv, eps, max_steps, adj, blocks, lds);
-        else launch_fp64_sx<DIM, SRC, SCR, ADJ, 0>(c, ev, rv, eps, max_steps, adj, blocks, lds);
-        *blocks_out = blocks;
+        *blocks_out = sx == 1   ? launch_fp64_sx<DIM, SRC, SCR, ADJ, kMis>(c, ev, rv, eps, max_steps, adj, lds)
+                      : sx == 2 ? launch_fp64_sx<DIM, SRC, SCR, ADJ, kFld>(c, ev, rv, eps, max_steps, adj, lds)
+                                : launch_fp64_sx<DIM, SRC, SCR, ADJ, 0>(c, ev, rv, eps, max_steps, adj, lds);
         return hipGetLastError();
     }
     hipError_t ge = ensure_grid(c);
