@@ -338,6 +338,88 @@ __device__ __forceinline__ NpqD npq_ref_3d(const MeshView &m, int root, double p
     return r;
 }
 
+// Resumable form of the reference's search (npq_ref_2d / npq_ref_3d): the
+// same pops, box tests against the best so far, pushes (far child first,
+// near child popped next) and primitive tests in slot order with the strict
+// update, one unit per call: a node popped and processed, or one primitive
+// of the leaf being scanned.  A lane's sequence of decisions is the
+// reference's, so its result is the reference's bit for bit, while the
+// lanes of a wavefront whose searches differ in length advance together.
+struct QueryD {
+    NpqD r;      // best so far
+    int sp;      // entries on the lane's LDS stack
+    int li, lc;  // leaf being scanned: next slot, primitives left
+};
+
+__device__ __forceinline__ void query_d_begin(QueryD &q, int root, int *stk) {
+    q.r.d2 = INFINITY; q.r.px = q.r.py = q.r.pz = 0.0; q.r.s = q.r.t = 0.0;
+    q.r.prim = -1; q.r.slot = -1;
+    q.sp = 1;
+    stk[0] = root;
+    q.li = 0;
+    q.lc = 0;
+}
+
+// One unit; returns whether the search goes on.
+template <int DIM>
+__device__ __forceinline__ bool query_d_step(QueryD &q, const MeshView &m, double px, double py,
+                                             double pz, int *stk, int stride) {
+    if (q.lc > 0) {
+        const int slot = q.li;
+        if constexpr (DIM == 3) {
+            double nx, ny, nz, s, t;
+            const double d = eberly_d(m.trid[slot], px, py, pz, nx, ny, nz, s, t);
+            if (d < q.r.d2) {
+                q.r.d2 = d; q.r.px = nx; q.r.py = ny; q.r.pz = nz; q.r.s = s; q.r.t = t;
+                q.r.prim = m.slot_prim[slot];
+                q.r.slot = slot;
+            }
+        } else {
+            double nx, ny, par;
+            const double d = npq_seg_d(m.segd[slot], px, py, nx, ny, par);
+            if (d < q.r.d2) {
+                q.r.d2 = d; q.r.px = nx; q.r.py = ny; q.r.s = par;
+                q.r.prim = m.slot_prim[slot];
+                q.r.slot = slot;
+            }
+        }
+        ++q.li;
+        --q.lc;
+        return q.lc > 0 || q.sp > 0;
+    }
+    const int curr = stk[(--q.sp) * stride];
+    int count, idx;
+    double bd;
+    if constexpr (DIM == 3) {
+        const NodeD3 n = m.nd3[curr];
+        bd = box_d2_3(n, px, py, pz);
+        count = n.count; idx = n.idx;
+    } else {
+        const NodeD2 n = m.nd2[curr];
+        bd = box_d2_2(n, px, py);
+        count = n.count; idx = n.idx;
+    }
+    if (bd < q.r.d2) {
+        if (count > 0) {
+            q.li = idx;
+            q.lc = count;
+        } else {
+            const int l = curr + 1, rr = idx;
+            double ld, rd;
+            if constexpr (DIM == 3) {
+                ld = box_d2_3(m.nd3[l], px, py, pz);
+                rd = box_d2_3(m.nd3[rr], px, py, pz);
+            } else {
+                ld = box_d2_2(m.nd2[l], px, py);
+                rd = box_d2_2(m.nd2[rr], px, py);
+            }
+            if (ld <= rd) { stk[(q.sp++) * stride] = rr; stk[(q.sp++) * stride] = l; }
+            else { stk[(q.sp++) * stride] = l; stk[(q.sp++) * stride] = rr; }
+        }
+    }
+    return q.lc > 0 || q.sp > 0;
+}
+
 // --------------------------------------------------------------------------
 // FP64 classification (src/inside.cpp)
 // --------------------------------------------------------------------------

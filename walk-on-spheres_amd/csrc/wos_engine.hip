@@ -797,50 +797,73 @@ void set_walks_per_point(RoundView &r, int cw) {
 }
 
 // --------------------------------------------------------------------------
-// kernel 2a: FP64 plain walker, persistent lanes over (point, walk) items
+// kernel 2a: FP64 plain walker, persistent lanes with resumable searches
 // --------------------------------------------------------------------------
-// The grid holds as many lanes as the GPU keeps resident.  Every trip of the
-// loop either takes one step of the lane's walk or, when the walk has ended,
-// stores it; lanes without a walk then take the next items of their
-// wavefront's pool (ranked by mbcnt; the pool takes kFp64Chunk items at a time
-// with one atomicAdd on `next`), so a lane whose walk ends early does not
-// wait for the longest walk of its wavefront.  A
-// walk's arithmetic is that of one lane per walk, so its value and steps do
-// not depend on the schedule.
+// The grid holds as many lanes as the GPU keeps resident; each holds one walk
+// at a time.  An iteration: the lanes whose search has finished end their
+// walk or take their step (the step's draws, source term, weight and move,
+// then the start of the next search); free lanes take the next walks of
+// their wavefront's pool (ranked by mbcnt; the pool takes kFp64Chunk items at
+// a time with one atomicAdd on `next`); then the searches advance by one unit
+// per pass (query_d_step) until min_ready lanes are ready or none searches.
+// A walk's draws and arithmetic, and each search's decisions, are those of
+// one lane per walk running the reference's loop, so a walk's value and
+// steps do not depend on the schedule.
 // At least 3 wavefronts per SIMD (at most 168 VGPRs) for the 3D forward
-// walks, which fit without spilling; 2D walks with a source term and adjoint
-// replays may use 256 rather than spill.
+// walks; 2D walks with a source term and adjoint replays may use 256 rather
+// than spill.
 constexpr int kFp64Chunk = 64;  // items a wavefront of k_walk_fp64 takes at a time
 
 template <int DIM, bool SRC, bool SCR, bool ADJ, int SX = 0>
 __global__ __launch_bounds__(kBlock)
 __attribute__((amdgpu_waves_per_eu(ADJ || DIM == 2 ? 2 : 3, 8)))
 void k_walk_fp64(MeshView m, EqView e, RoundView rv, double eps, int max_steps, double *vals,
-                 int32_t *steps_out, AdjOut adj, unsigned long long *next) {
+                 int32_t *steps_out, AdjOut adj, unsigned long long *next, int min_ready) {
     extern __shared__ int lds_i[];
     int *stk = lds_i + threadIdx.x;
     const int stride = blockDim.x;
     const int lane = __lane_id();
     const double alpha = e.alpha;
-    // the walk held by the lane.  The start record is read field by field
-    // where needed (a register copy of all of it would stay live).
-    bool active = false;
-    // wave-uniform pool of items [pool_next, pool_end), kFp64Chunk at a time
-    // from `next`; exhausted once `next` has passed the last item
+    bool active = false;     // the lane holds a walk
+    bool searching = false;  // its search is in progress
+    // wave-uniform pool of items [pool_next, pool_end); exhausted once `next`
+    // has passed the last item
     bool exhausted = false;
     uint64_t pool_next = 0, pool_end = 0;
     int64_t item = 0;
+    // the walk held by the lane.  The start record is read field by field
+    // where needed (a register copy of all of it would stay live).
     uint32_t w = 0;
     const StartInfo *S = nullptr;
     double px = 0.0, py = 0.0, pz = 0.0, R = 0.0;
     uint32_t k0 = 0, k1 = 0;
     int root = 0;
-    NpqD q{};
+    QueryD q{};
     double sample = 0.0, weight = 1.0;
     int steps = 0;
     WalkDer D{};
+    // the walk's end: its value (and adjoint record) to its staging slot
+    auto finish = [&]() {
+        const bool hit = R > eps;
+        const double g = steps == 0 ? S->g
+                                    : eq_boundary(e, m, q.r.px, q.r.py, q.r.pz, q.r.prim, q.r.s, q.r.t);
+        vals[item] = sample + weight * g;
+        steps_out[item] = steps | (hit ? (int32_t)0x80000000 : 0);
+        if constexpr (ADJ) {
+            const int prim = steps == 0 ? S->prim : q.r.prim;
+            const double s = steps == 0 ? S->s : q.r.s, t = steps == 0 ? S->t : q.r.t;
+            store_adjoint(adj, item, D, weight, g, prim, s, DIM == 3 ? t : 0.0);
+        }
+        active = false;
+    };
     for (;;) {
-        // ---- lanes without a walk take the next items
+        // ---- finished searches: end the walk or take a step
+        bool stepping = false;
+        if (active && !searching) {
+            if (R > eps && steps < max_steps) stepping = true;
+            else finish();
+        }
+        // ---- free lanes take the next walks
         const uint64_t free_mask = __ballot(!active);
         if (free_mask != 0 && !exhausted && pool_next >= pool_end) {
             unsigned long long base = 0;
@@ -864,19 +887,19 @@ void k_walk_fp64(MeshView m, EqView e, RoundView rv, double eps, int max_steps, 
                     px = S->x; py = S->y; pz = S->z; R = S->r;
                     k0 = S->key0; k1 = S->key1;
                     root = S->root64;
-                    q = NpqD{};
                     sample = 0.0; weight = 1.0;
                     steps = 0;
                     if constexpr (ADJ) D = WalkDer{};
                     active = true;
+                    if (R > eps && steps < max_steps) stepping = true;
+                    else finish();
                 }
             }
             const uint64_t nfree = (uint64_t)__popcll(free_mask);
             pool_next = pool_next + nfree < pool_end ? pool_next + nfree : pool_end;
         }
-        if (__ballot(active) == 0 && exhausted) break;
-        if (!active) continue;
-        if (R > eps && steps < max_steps) {
+        // ---- one walk step (include/wos/wos.hpp:68-127), then the next search
+        if (stepping) {
             // the step's counter stream: the source term draws from slot
             // DIM-1 on, then the direction takes slots 0 .. DIM-2 (block 0 is
             // formed once when the source's draws stayed in it)
@@ -908,20 +931,22 @@ void k_walk_fp64(MeshView m, EqView e, RoundView rv, double eps, int max_steps, 
                 pz = pz + R * z;
             }
             ++steps;
-            q = DIM == 2 ? npq_ref_2d(m, root, px, py, stk, stride)
-                         : npq_ref_3d(m, root, px, py, pz, stk, stride);
-            R = sqrt(q.d2);
-        } else {
-            const bool hit = R > eps;
-            const double g = steps == 0 ? S->g : eq_boundary(e, m, q.px, q.py, q.pz, q.prim, q.s, q.t);
-            vals[item] = sample + weight * g;
-            steps_out[item] = steps | (hit ? (int32_t)0x80000000 : 0);
-            if constexpr (ADJ) {
-                const int prim = steps == 0 ? S->prim : q.prim;
-                const double s = steps == 0 ? S->s : q.s, t = steps == 0 ? S->t : q.t;
-                store_adjoint(adj, item, D, weight, g, prim, s, DIM == 3 ? t : 0.0);
+            query_d_begin(q, root, stk);
+            searching = true;
+        }
+        if (__ballot(active) == 0) {
+            if (exhausted) break;
+            continue;
+        }
+        // ---- search units until enough lanes are ready to step
+        for (;;) {
+            if (searching) {
+                searching = query_d_step<DIM>(q, m, px, py, pz, stk, stride);
+                if (!searching) R = sqrt(q.r.d2);
             }
-            active = false;
+            if (__ballot(searching) == 0) break;
+            const uint64_t ready = __ballot(!searching && (active || !exhausted));
+            if (__popcll(ready) >= min_ready) break;
         }
     }
 }
@@ -2371,7 +2396,8 @@ int launch_fp64_sx(wos_ctx *c, const EqView &ev, const RoundView &rv, double eps
     unsigned long long *next = c->counter.as<unsigned long long>();
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, c->stream, next, 1);
     hipLaunchKernelGGL(kfn, dim3(blocks), dim3(kBlock), lds, c->stream, c->view, ev, rv, eps,
-                       max_steps, c->vals.as<double>(), c->steps.as<int32_t>(), adj, next);
+                       max_steps, c->vals.as<double>(), c->steps.as<int32_t>(), adj, next,
+                       c->tune.min_ready);
     return blocks;
 }
 
