@@ -813,6 +813,11 @@ void set_walks_per_point(RoundView &r, int cw) {
 // walks; 2D walks with a source term and adjoint replays may use 256 rather
 // than spill.
 constexpr int kFp64Chunk = 64;  // items a wavefront of k_walk_fp64 takes at a time
+// k_walk_fp64's ready lanes per step phase: its search units (an FP64 node
+// or primitive test) are dearer than the FP32 walker's, so it steps with
+// fewer lanes ready (walker ms at 8 / 16 / 24 / 40: icosphere 110 / 107 /
+// 109 / 115, torus 72 / 74 / 77 / 83, plate 385 / 370 / 379 / 399)
+constexpr int kFp64MinReady = 16;
 
 template <int DIM, bool SRC, bool SCR, bool ADJ, int SX = 0>
 __global__ __launch_bounds__(kBlock)
@@ -2397,7 +2402,7 @@ int launch_fp64_sx(wos_ctx *c, const EqView &ev, const RoundView &rv, double eps
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, c->stream, next, 1);
     hipLaunchKernelGGL(kfn, dim3(blocks), dim3(kBlock), lds, c->stream, c->view, ev, rv, eps,
                        max_steps, c->vals.as<double>(), c->steps.as<int32_t>(), adj, next,
-                       c->tune.min_ready);
+                       kFp64MinReady);
     return blocks;
 }
 
