@@ -201,11 +201,14 @@ def test_point_list_edge_cases(gpu_ctx, oracle):
     assert np.allclose(r["mean"][ins], o["mean"][ins], rtol=0, atol=1e-12)
 
 
-def test_fp32_bitwise_reproducible(gpu_ctx):
+@pytest.mark.parametrize("precision", [WOS_FP32_PERSISTENT, WOS_FP64_PLAIN])
+def test_bitwise_reproducible(gpu_ctx, precision):
+    # both walkers hand out walks to lanes dynamically (atomic counters), so
+    # two runs schedule them differently; a walk's result must not depend on it
     V, F, _ = M.icosphere(3)
     gpu_ctx.set_mesh(V, F)
     eq = Pr.sphere_screened_constant()
-    p = Context.params(128, 1e-4, 7, precision=WOS_FP32_PERSISTENT)
+    p = Context.params(128, 1e-4, 7, precision=precision)
     a = gpu_ctx.solve_grid(12, 12, 12, eq, p)
     b = gpu_ctx.solve_grid(12, 12, 12, eq, p)
     for k in ("mean", "variance", "mean_steps", "location"):
