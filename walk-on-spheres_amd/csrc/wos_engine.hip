@@ -957,7 +957,7 @@ void k_walk_fp64(MeshView m, EqView e, RoundView rv, double eps, int max_steps, 
 }
 
 // --------------------------------------------------------------------------
-// kernel 2c: FP64 walker on the reference's streams, one lane per point
+// kernel 2c: FP64 walker on the reference's streams, a lane per point
 // --------------------------------------------------------------------------
 // The reference draws a point's walks one after another from per-point
 // xoshiro256++ streams (include/wos/runner.hpp:208-221): 2D a walk stream and
@@ -989,33 +989,117 @@ __global__ __launch_bounds__(kBlock) void k_ref_seed(const StartInfo *starts, co
     o[4] = src.s0; o[5] = src.s1; o[6] = src.s2; o[7] = src.s3;
 }
 
+// A lane holds one point of the round and runs its walks in order; the
+// searches are resumable (query_d_step), and the walker loop is that of
+// k_walk_fp64: lanes whose search has finished end their walk (then start
+// the point's next walk, or save the point's streams and take the next
+// point of their wavefront's pool) or take their step together, and search
+// passes run until kFp64MinReady lanes are ready.
 template <int DIM, bool SRC, bool SCR, int SX = 0>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2, 8)))
 void k_walk_ref(MeshView m, EqView e, RoundView rv, double eps, int max_steps, double *vals,
-                int32_t *steps_out, uint64_t *state) {
+                int32_t *steps_out, uint64_t *state, unsigned long long *next, int min_ready) {
     extern __shared__ int lds_i[];
     int *stk = lds_i + threadIdx.x;
     const int stride = blockDim.x;
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= rv.n) return;
-    const int j = round_inside_id(rv, k);
-    const int64_t base = rv.off ? rv.off[k] : (int64_t)k * rv.cw;
-    const int cnt = rv.off ? (int)(rv.off[k + 1] - rv.off[k]) : rv.cw;
-    const StartInfo *S = &rv.starts[rv.inside[j]];
-    uint64_t *st = state + 8 * (int64_t)j;
+    const int lane = __lane_id();
+    const double alpha = e.alpha;
+    bool active = false;     // the lane holds a point
+    bool searching = false;  // its walk's search is in progress
+    bool exhausted = false;  // wave-uniform: every point is taken
+    uint64_t pool_next = 0, pool_end = 0;
+    // the point: its streams, its walks [base, base + cnt) of the round
+    uint64_t *st = nullptr;
     Xoshiro ws, ss;
-    ws.s0 = st[0]; ws.s1 = st[1]; ws.s2 = st[2]; ws.s3 = st[3];
-    ss.s0 = st[4]; ss.s1 = st[5]; ss.s2 = st[6]; ss.s3 = st[7];
     XoRef walk{&ws};
     XoRef source{DIM == 2 ? &ss : &ws};
-    const int root = S->root64;
-    const double alpha = e.alpha;
-    for (int i = 0; i < cnt; ++i) {
-        double px = S->x, py = S->y, pz = S->z, R = S->r;
-        NpqD q{};
-        double sample = 0.0, weight = 1.0;
-        int steps = 0;
-        while (R > eps && steps < max_steps) {
+    const StartInfo *S = nullptr;
+    int64_t base = 0;
+    int cnt = 0, i = 0, root = 0;
+    // the walk
+    double px = 0.0, py = 0.0, pz = 0.0, R = 0.0;
+    QueryD q{};
+    double sample = 0.0, weight = 1.0;
+    int steps = 0;
+    auto start_walk = [&]() {
+        px = S->x; py = S->y; pz = S->z; R = S->r;
+        sample = 0.0; weight = 1.0;
+        steps = 0;
+    };
+    auto store_walk = [&]() {
+        const bool hit = R > eps;
+        const double g = steps == 0 ? S->g
+                                    : eq_boundary(e, m, q.r.px, q.r.py, q.r.pz, q.r.prim, q.r.s, q.r.t);
+        vals[base + i] = sample + weight * g;
+        steps_out[base + i] = steps | (hit ? (int32_t)0x80000000 : 0);
+    };
+    auto save_streams = [&]() {
+        st[0] = ws.s0; st[1] = ws.s1; st[2] = ws.s2; st[3] = ws.s3;
+        st[4] = ss.s0; st[5] = ss.s1; st[6] = ss.s2; st[7] = ss.s3;
+        active = false;
+    };
+    for (;;) {
+        // ---- finished searches: end the walk or take a step
+        bool stepping = false;
+        if (active && !searching) {
+            if (R > eps && steps < max_steps) {
+                stepping = true;
+            } else {
+                store_walk();
+                if (++i < cnt) {
+                    start_walk();
+                    stepping = true;  // the point's start needs steps (checked when taken)
+                } else {
+                    save_streams();
+                }
+            }
+        }
+        // ---- free lanes take the next points
+        const uint64_t free_mask = __ballot(!active);
+        if (free_mask != 0 && !exhausted && pool_next >= pool_end) {
+            unsigned long long b0 = 0;
+            if (lane == 0) b0 = atomicAdd(next, (unsigned long long)kFp64Chunk);
+            b0 = __shfl(b0, 0);
+            if (b0 >= (unsigned long long)rv.n) {
+                exhausted = true;
+            } else {
+                pool_next = b0;
+                const uint64_t end = b0 + (uint64_t)kFp64Chunk;
+                pool_end = end < (uint64_t)rv.n ? end : (uint64_t)rv.n;
+            }
+        }
+        if (free_mask != 0 && pool_next < pool_end) {
+            if (!active) {
+                const uint32_t lo = (uint32_t)free_mask, hi = (uint32_t)(free_mask >> 32);
+                const int rank = __builtin_amdgcn_mbcnt_hi(hi, __builtin_amdgcn_mbcnt_lo(lo, 0));
+                const uint64_t kk = pool_next + (uint64_t)rank;
+                if (kk < pool_end) {
+                    const int k = (int)kk;
+                    const int j = round_inside_id(rv, k);
+                    base = rv.off ? rv.off[k] : (int64_t)k * rv.cw;
+                    cnt = rv.off ? (int)(rv.off[k + 1] - rv.off[k]) : rv.cw;
+                    S = &rv.starts[rv.inside[j]];
+                    root = S->root64;
+                    st = state + 8 * (int64_t)j;
+                    ws.s0 = st[0]; ws.s1 = st[1]; ws.s2 = st[2]; ws.s3 = st[3];
+                    ss.s0 = st[4]; ss.s1 = st[5]; ss.s2 = st[6]; ss.s3 = st[7];
+                    i = 0;
+                    start_walk();
+                    active = true;
+                    if (R > eps && steps < max_steps) {
+                        stepping = true;
+                    } else {
+                        // every walk of the point starts at its end: no draws
+                        for (; i < cnt; ++i) store_walk();
+                        save_streams();
+                    }
+                }
+            }
+            const uint64_t nfree = (uint64_t)__popcll(free_mask);
+            pool_next = pool_next + nfree < pool_end ? pool_next + nfree : pool_end;
+        }
+        // ---- one walk step in the reference's draw order, then the search
+        if (stepping) {
             if (SRC) sample += source_term<DIM, double, XoRef, SX, SCR>(e, px, py, pz, R, weight, source);
             if (SCR) weight *= step_weight<DIM, double>(e.kind, alpha, R);
             if (DIM == 2) {
@@ -1031,17 +1115,24 @@ void k_walk_ref(MeshView m, EqView e, RoundView rv, double eps, int max_steps, d
                 pz = pz + R * z;
             }
             ++steps;
-            q = DIM == 2 ? npq_ref_2d(m, root, px, py, stk, stride)
-                         : npq_ref_3d(m, root, px, py, pz, stk, stride);
-            R = sqrt(q.d2);
+            query_d_begin(q, root, stk);
+            searching = true;
         }
-        const bool hit = R > eps;
-        const double g = steps == 0 ? S->g : eq_boundary(e, m, q.px, q.py, q.pz, q.prim, q.s, q.t);
-        vals[base + i] = sample + weight * g;
-        steps_out[base + i] = steps | (hit ? (int32_t)0x80000000 : 0);
+        if (__ballot(active) == 0) {
+            if (exhausted) break;
+            continue;
+        }
+        // ---- search units until enough lanes are ready to step
+        for (;;) {
+            if (searching) {
+                searching = query_d_step<DIM>(q, m, px, py, pz, stk, stride);
+                if (!searching) R = sqrt(q.r.d2);
+            }
+            if (__ballot(searching) == 0) break;
+            const uint64_t ready = __ballot(!searching && (active || !exhausted));
+            if (__popcll(ready) >= min_ready) break;
+        }
     }
-    st[0] = ws.s0; st[1] = ws.s1; st[2] = ws.s2; st[3] = ws.s3;
-    st[4] = ss.s0; st[5] = ss.s1; st[6] = ss.s2; st[7] = ss.s3;
 }
 
 // FP32 source term.  Same draws as source_term (the step stream from slot
@@ -2380,11 +2471,22 @@ hipError_t reserve_terminal(wos_ctx *c, const EqView &ev, int precision, bool ad
 }
 
 template <int DIM, bool SRC, bool SCR, int SX>
-void launch_ref_sx(wos_ctx *c, const EqView &ev, const RoundView &rv, double eps, int max_steps,
-                   int blocks, size_t lds) {
-    hipLaunchKernelGGL((k_walk_ref<DIM, SRC, SCR, SX>), dim3(blocks), dim3(kBlock), lds, c->stream,
-                       c->view, ev, rv, eps, max_steps, c->vals.as<double>(),
-                       c->steps.as<int32_t>(), c->ref_state.as<uint64_t>());
+int launch_ref_sx(wos_ctx *c, const EqView &ev, const RoundView &rv, double eps, int max_steps,
+                  size_t lds) {
+    // persistent: at most the blocks the GPU keeps resident
+    const auto kfn = k_walk_ref<DIM, SRC, SCR, SX>;
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, kBlock, lds) != hipSuccess ||
+        per_cu < 1)
+        per_cu = 1;
+    const int64_t need = ((int64_t)rv.n + kBlock - 1) / kBlock;
+    const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>(need, (int64_t)per_cu * c->n_cu));
+    unsigned long long *next = c->counter.as<unsigned long long>();
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, c->stream, next, 1);
+    hipLaunchKernelGGL(kfn, dim3(blocks), dim3(kBlock), lds, c->stream, c->view, ev, rv, eps,
+                       max_steps, c->vals.as<double>(), c->steps.as<int32_t>(),
+                       c->ref_state.as<uint64_t>(), next, kFp64MinReady);
+    return blocks;
 }
 
 template <int DIM, bool SRC, bool SCR, bool ADJ, int SX>
@@ -2531,12 +2633,10 @@ hipError_t launch_walk(wos_ctx *c, const EqView &ev, int precision, const RoundV
         if constexpr (ADJ) {
             return hipErrorInvalidValue;
         } else {
-            const int blocks = (rv.n + kBlock - 1) / kBlock;
             const size_t lds = (size_t)kBlock * (c->L.depth + 2) * sizeof(int);
-            if (sx == 1) launch_ref_sx<DIM, SRC, SCR, kMis>(c, ev, rv, eps, max_steps, blocks, lds);
-            else if (sx == 2) launch_ref_sx<DIM, SRC, SCR, kFld>(c, ev, rv, eps, max_steps, blocks, lds);
-            else launch_ref_sx<DIM, SRC, SCR, 0>(c, ev, rv, eps, max_steps, blocks, lds);
-            *blocks_out = blocks;
+            *blocks_out = sx == 1   ? launch_ref_sx<DIM, SRC, SCR, kMis>(c, ev, rv, eps, max_steps, lds)
+                          : sx == 2 ? launch_ref_sx<DIM, SRC, SCR, kFld>(c, ev, rv, eps, max_steps, lds)
+                                    : launch_ref_sx<DIM, SRC, SCR, 0>(c, ev, rv, eps, max_steps, lds);
             return hipGetLastError();
         }
     }
