@@ -1091,10 +1091,6 @@ __device__ __forceinline__ float box_d2f2(float lox, float loy, float hix, float
     return __builtin_fmaf(dx, dx, dy * dy);
 }
 
-__device__ __forceinline__ float sq3(float x, float y, float z) {
-    return __builtin_fmaf(x, x, __builtin_fmaf(y, y, z * z));
-}
-
 // Squared distance point -> triangle in FP32 without branches.  The nearest
 // point is the plane projection when its barycentrics lie in the triangle,
 // otherwise the nearest of the three clamped edge projections; the minimum of

@@ -5,10 +5,12 @@
 //   1. k_classify: per point, FP64 start query (reference traversal), the
 //      reference boundary / winding / ray-vote classification, the walk-stream
 //      key and g at the nearest point;
-//   2. host compaction of the inside points;
-//   3. k_walk_fp64 (persistent lanes, one walk at a time) or k_walk_fp32 (persistent lanes,
-//      ballot-driven refill) writing one FP64 value per walk into a
-//      [walk][point] staging buffer;
+//   2. compaction of the inside points (on the device for a one-round grid
+//      solve);
+//   3. the walker: k_walk_fp32 (FP32 queries on the distance grid), k_walk_fp64
+//      (the reference's FP64 search, resumable) or k_walk_ref (the reference's
+//      per-point streams), all on persistent lanes refilled from a pool,
+//      writing one FP64 value per walk into a point-major staging buffer;
 //   4. k_welford: per point, the reference's sequential Welford update in walk
 //      order (include/wos/wos.hpp:169-180), continued across rounds (FP64
 //      walkers); k_merge_wave: the same update on 64 lanes per point and a
@@ -49,9 +51,6 @@ constexpr int kMaxParts = 64;  // largest xcd_parts
 // radius first (so an FP32 constant-source step needs one Philox block: the
 // direction's and the radius's slots share it), the reference draws the
 // direction first (src/wos.cpp:33-56).
-
-template <typename T> __device__ __forceinline__ T tsin(T x) { return sin(x); }
-template <typename T> __device__ __forceinline__ T tcos(T x) { return cos(x); }
 
 // include/wos/poisson.hpp:27-30, :46-50
 template <typename T, typename RNG>
@@ -1793,12 +1792,6 @@ void k_walk_fp32(MeshView m, EqView e, PersistArgs a, AdjOut adj) {
     }
 }
 
-// --------------------------------------------------------------------------
-// terminal values of the FP32 forward walks when g != 0: the FP64 nearest
-// point on the walk's nearest primitive (from the FP32 end point) and
-// value = sum of the source terms + weight x g there, one lane per walk (in
-// the persistent walker only the few ending lanes of a wavefront would run
-// this FP64 code)
 // --------------------------------------------------------------------------
 // terminal value of an FP32 forward walk when g != 0: weight x g at the FP64
 // nearest point of the walk's nearest primitive `slot` (from the FP32 end
