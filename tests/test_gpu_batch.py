@@ -55,3 +55,26 @@ def test_batch_adaptive(gpu_ctx):
     for r in res:
         ins = r["location"] == 1
         assert np.all((r["standard_error"][ins] <= 3e-3) | (r["walks"][ins] >= 4096))
+
+
+def test_fp64_adaptive_rounds_equal_fixed(gpu_ctx):
+    """FP64 plain over uneven rounds: adaptive rounds split a point's walks,
+    and a point that ends with W walks equals a fixed solve with W walks bit
+    for bit (a walk's draws depend on its index only; Welford in walk order)."""
+    V, F, B = M.icosphere(2)
+    gpu_ctx.set_mesh(V, F, B)
+    eq = Pr.sphere_screened_constant(10.0, 1.0, 0)
+    ad = gpu_ctx.solve_grid(6, 6, 6, eq, Context.params(16, 1e-3, 3, precision=WOS_FP64_PLAIN,
+                                                        target_se=2e-3, max_walks=256))
+    assert ad["summary"]["rounds"] >= 2
+    ins = ad["location"] == 1
+    counts = np.unique(ad["walks"][ins])
+    assert len(counts) >= 2
+    checked = 0
+    for w in counts[:4]:
+        fx = gpu_ctx.solve_grid(6, 6, 6, eq, Context.params(int(w), 1e-3, 3, precision=WOS_FP64_PLAIN))
+        sel = ins & (ad["walks"] == w)
+        for k in ("mean", "variance", "mean_steps"):
+            assert np.array_equal(ad[k][sel], fx[k][sel]), (w, k)
+        checked += int(sel.sum())
+    assert checked > 0
