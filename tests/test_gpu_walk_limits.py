@@ -8,7 +8,7 @@ import pytest
 
 from wos_amd import meshgen as M
 from wos_amd import problems as Pr
-from wos_amd.capi import WOS_FP32_PERSISTENT
+from wos_amd.capi import WOS_FP32_PERSISTENT, WOS_FP64_PLAIN, WOS_FP64_REFERENCE
 from wos_amd.engine import Context
 
 pytestmark = pytest.mark.gpu
@@ -35,10 +35,12 @@ def test_max_steps_ends_walks_2d_batch(gpu_ctx):
 
 
 @pytest.mark.timeout(120)
-def test_max_steps_ends_walks_3d(gpu_ctx):
+@pytest.mark.parametrize("precision", [WOS_FP32_PERSISTENT, WOS_FP64_PLAIN, WOS_FP64_REFERENCE])
+def test_max_steps_ends_walks_3d(gpu_ctx, precision):
+    # every walker: FP32, FP64 on counter streams, FP64 on the reference's streams
     V, F, _ = M.icosphere(4)
     gpu_ctx.set_mesh(V, F)
-    prm = Context.params(512, 1e-4, 17, max_steps=MAX_STEPS, precision=WOS_FP32_PERSISTENT)
+    prm = Context.params(512, 1e-4, 17, max_steps=MAX_STEPS, precision=precision)
     r = gpu_ctx.solve_grid(24, 24, 24, Pr.sphere_screened_constant(10.0, 1.0), prm)
     s = r["summary"]
     print("3D: walks", s["total_walks"], "steps", s["total_steps"])
