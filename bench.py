@@ -36,6 +36,8 @@ Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
            (the FP32 walker solves of workload X only, for counter runs)
        python bench.py --copy-calibration 1024   (one 1024 MiB fp32 read+write
            kernel, the scale for the HBM counters)
+       python bench.py --grid-sweep   (distance-grid size against build time,
+           memory and walker time, a markdown table)
 """
 import argparse
 import hashlib
@@ -715,6 +717,70 @@ def fp32_only(name, Context, meshgen, problems, capi, args):
 WORKLOADS = ("annulus", "icosphere", "stars", "torus", "plate")
 
 
+def grid_sweep(Context, meshgen, problems, capi):
+    """Distance-grid size against build time, memory and walker time: for each
+    workload and grid_cells setting (-1: the default sized from the primitive
+    count) a fresh context, one first solve (the grid build included) and the
+    best of two more solves.  FP32 walker, default relax; the stars with a
+    fixed 1024 walks.  Prints a markdown table."""
+    def setup(name, ctx):
+        if name in ("icosphere", "plate", "annulus"):
+            if name == "icosphere":
+                mesh, eq, grid, nw, eps, seed = (meshgen.icosphere(4), problems.sphere_screened_constant(),
+                                                 (32, 32, 32), 4096, 1e-4, 7)
+            elif name == "plate":
+                mesh, eq, grid, nw, eps, seed = (
+                    meshgen.plate_200k(), problems.plate_gaussian(0.05, 1.0, 3.0, (40.0, 30.0, 10.0)),
+                    (64, 32, 16), 8192, 0.01, 5)
+            else:
+                mesh, eq, grid, nw, eps, seed = (meshgen.annulus(128), problems.annulus_laplace(),
+                                                 (64, 64, 1), 1024, 1e-3, 12345)
+            ctx.set_mesh(*mesh)
+            p = Context.params(nw, eps, seed, precision=capi.WOS_FP32_PERSISTENT)
+            return lambda: ctx.solve_grid(*grid, eq, p)["summary"]
+        if name == "torus":
+            V, F, _ = meshgen.torus(1.0, 0.35, 200, 100)
+            ctx.set_mesh(V, F)
+            pts = torus_probes(10000, 3)
+            eq = problems.laplace_vertex_values(np.random.default_rng(3).normal(size=len(V)))
+            p = Context.params(2048, 1e-3, 3, precision=capi.WOS_FP32_PERSISTENT)
+            return lambda: ctx.solve_points(pts, eq, p)["summary"]
+        fam = meshgen.star_family(16, seed=2024, n_min=512, n_max=16384)
+        ctx.set_domains([f["mesh"] for f in fam])
+        p = Context.params(1024, 1e-3, 11, precision=capi.WOS_FP32_PERSISTENT)
+        return lambda: ctx.solve_grid_batch(256, 256, 1, problems.star_poisson_gaussian(1), p)[1]
+    settings = {"icosphere": (-1, 1 << 20, 1 << 22, 1 << 23, 1 << 25),
+                "plate": (-1, 1 << 21, 1 << 22, 1 << 23, 1 << 25),
+                "torus": (-1, 1 << 21, 1 << 22, 1 << 23),
+                "stars": (-1, 1 << 18, 1 << 20, 1 << 22),
+                "annulus": (-1, 1 << 16, 1 << 18, 1 << 20)}
+    lines = ["| workload | grid_cells | build ms | first solve ms | grid MB | solve ms | walker kernel ms | walk steps |",
+             "|---|---|---|---|---|---|---|---|"]
+    for name, cells in settings.items():
+        for g in cells:
+            log(f"grid sweep: {name} {g}")
+            ctx = Context(0)
+            run = setup(name, ctx)
+            ctx.set_tuning(grid_cells=g)
+            t0 = time.perf_counter()
+            first = run()
+            t_first = time.perf_counter() - t0
+            best = None
+            for _ in range(2):
+                t0 = time.perf_counter()
+                s = run()
+                t = time.perf_counter() - t0
+                if best is None or t < best[0]:
+                    best = (t, s)
+            t, s = best
+            label = "default" if g < 0 else f"2^{g.bit_length() - 1}"
+            lines.append(f"| {name} | {label} | {first['grid_build_seconds'] * 1e3:.1f} | "
+                         f"{t_first * 1e3:.1f} | {first['grid_bytes'] / 2 ** 20:.1f} | {t * 1e3:.2f} | "
+                         f"{s['walk_kernel_seconds'] * 1e3:.2f} | {s['total_steps']} |")
+            ctx.close()
+    print("\n".join(lines))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -724,6 +790,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--fp32-only", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB")
+    ap.add_argument("--grid-sweep", action="store_true",
+                    help="distance-grid size against build time, memory and walker time "
+                         "(markdown table)")
     ap.add_argument("--workload", default="all",
                     choices=("all",) + WORKLOADS + ("welding", "torus_forward", "torus_adjoint"),
                     help="all (default): the headline line with every workload under "
@@ -739,6 +808,10 @@ def main():
 
     from wos_amd import Context, meshgen, problems
     from wos_amd import capi
+    if args.grid_sweep:
+        print(f"Kernel sources: `source_sha` {source_sha()}.\n")
+        grid_sweep(Context, meshgen, problems, capi)
+        return
     if args.fp32_only:
         fp32_only("icosphere" if args.workload == "all" else args.workload, Context, meshgen,
                   problems, capi, args)
