@@ -146,10 +146,11 @@ typedef struct wos_eq_desc {
 
 /* ---- execution parameters ---------------------------------------------- */
 enum wos_precision {
-    WOS_FP64_PLAIN = 0,          /* one lane per walk, FP64 geometry           */
+    WOS_FP64_PLAIN = 0,          /* the reference's FP64 arithmetic and search on counter
+                                    streams (persistent lanes, one walk at a time)  */
     WOS_FP32_PERSISTENT = 1,     /* persistent lane-refilling walker, FP32 geometry, FP64 stats */
-    WOS_FP64_REFERENCE = 2       /* FP64 on the reference's own per-point streams: one lane per
-                                    point runs its walks in order on xoshiro256++ seeded as in
+    WOS_FP64_REFERENCE = 2       /* FP64 on the reference's own per-point streams: a lane holds
+                                    a point and runs its walks in order on xoshiro256++ seeded as in
                                     include/wos/runner.hpp:208-221 (2D: walk + source stream,
                                     3D: one shared stream), so every walk is the reference's
                                     walk (up to last-bit libm differences); no adjoint */
