@@ -418,9 +418,10 @@ __global__ __launch_bounds__(kBlock) void k_classify(MeshView m, EqView e, GridS
 // the least distance with the lowest j is the reference's own result.
 template <int S>
 __device__ __forceinline__ NpqD npq_ref_3d_sub(const MeshView &m, int root, double px, double py,
-                                               double pz, int j, int *stk, int stride) {
-    constexpr int D = S == 8 ? 3 : (S == 4 ? 2 : (S == 2 ? 1 : 0));
-    static_assert((1 << D) == S, "S must be 1, 2, 4 or 8");
+                                               double pz, int j, int *stk, int stride,
+                                               int budget = 0) {
+    constexpr int D = S == 64 ? 6 : S == 32 ? 5 : S == 16 ? 4 : S == 8 ? 3 : S == 4 ? 2 : S == 2 ? 1 : 0;
+    static_assert((1 << D) == S, "S must be a power of two up to 64");
     int curr = root;
     for (int l = 0; l < D; ++l) {
         const NodeD3 n = m.nd3[curr];
@@ -437,8 +438,19 @@ __device__ __forceinline__ NpqD npq_ref_3d_sub(const MeshView &m, int root, doub
         const bool far = ((j >> (D - 1 - l)) & 1) != 0;
         curr = far == left_near ? right : left;
     }
-    return npq_ref_3d(m, curr, px, py, pz, stk, stride);
+    return npq_ref_3d(m, curr, px, py, pz, stk, stride, budget);
 }
+
+// Points whose start query needs more than kClassifyBudget nodes in some
+// subtree of k_classify3 (points deep inside a closed surface, where nearly
+// every primitive is about as far) are queued for k_classify3_deep, which
+// searches each of them on a whole wavefront.
+constexpr int kClassifyBudget = 256;
+struct DeepQueue {
+    unsigned long long *count;
+    int64_t *point;    // the point's index in the launch
+    int32_t *rays;     // its first three casts (classify_tail's first_rays)
+};
 
 // 3D classification with its two searches spread over a block: 64 points
 // per block, wavefront j < S runs subtree j of their start queries and
@@ -453,7 +465,7 @@ __global__ __launch_bounds__((S + 3) * 64) void k_classify3(MeshView m, EqView e
                                                           uint64_t seed,
                                                           int max_ray_attempts, uint8_t *loc,
                                                           StartInfo *starts, int32_t *amb,
-                                                          double *bvalue) {
+                                                          double *bvalue, DeepQueue dq) {
     extern __shared__ int lds_i[];
     __shared__ NpqD s_npq[S * 64];
     __shared__ int s_ray[3 * 64];
@@ -467,7 +479,8 @@ __global__ __launch_bounds__((S + 3) * 64) void k_classify3(MeshView m, EqView e
     if (role < S) {
         NpqD q;
         q.d2 = INFINITY; q.px = q.py = q.pz = 0.0; q.s = q.t = 0.0; q.prim = -1; q.slot = -1;
-        if (i < n) q = npq_ref_3d_sub<S>(m, dom.root64, x, y, z, role, stk, stride);
+        if (i < n) q = npq_ref_3d_sub<S>(m, dom.root64, x, y, z, role, stk, stride,
+                                         dq.count ? kClassifyBudget : 0);
         s_npq[role * 64 + lane] = q;
     } else {
         int res = 0;
@@ -488,8 +501,10 @@ __global__ __launch_bounds__((S + 3) * 64) void k_classify3(MeshView m, EqView e
     __syncthreads();
     if (role != 0 || i >= n) return;
     NpqD q = s_npq[lane];
+    bool deep = q.slot == -2;
     for (int j = 1; j < S; ++j) {
         const NpqD o = s_npq[j * 64 + lane];
+        deep = deep || o.slot == -2;
         if (o.d2 < q.d2) q = o;
     }
     int first_rays = 0;
@@ -497,7 +512,51 @@ __global__ __launch_bounds__((S + 3) * 64) void k_classify3(MeshView m, EqView e
         const int res = s_ray[k * 64 + lane];
         first_rays |= ((res & 1) << k) | (((res >> 1) & 1) << (3 + k));
     }
+    if (deep) {
+        const unsigned long long slot = atomicAdd(dq.count, 1ull);
+        dq.point[slot] = i;
+        dq.rays[slot] = first_rays;
+        return;
+    }
     classify_tail<3>(m, e, dom, i, x, y, z, q, ps, first_rays, max_ray_attempts, stk, stride,
+                     loc, starts, amb, bvalue);
+}
+
+// The queued points of k_classify3, one wavefront each: lane j searches
+// subtree j of 64 (the split of npq_ref_3d_sub), the least distance with the
+// lowest j wins (the reference's result, as in k_classify3), and lane 0
+// finishes the point with the casts k_classify3 made.  The grid has a
+// wavefront for every point of the launch; those past the queue's length
+// return at once.
+__global__ __launch_bounds__(256) void k_classify3_deep(MeshView m, EqView e, GridSpec g,
+                                                        DomainView dom, const double *plist,
+                                                        const int64_t *sidx, uint64_t seed,
+                                                        int max_ray_attempts, uint8_t *loc,
+                                                        StartInfo *starts, int32_t *amb,
+                                                        double *bvalue, DeepQueue dq) {
+    extern __shared__ int lds_i[];
+    int *stk = lds_i + threadIdx.x;
+    const int stride = blockDim.x;
+    const int lane = (int)(threadIdx.x & 63u);
+    const unsigned long long w = (unsigned long long)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+    if (w >= *dq.count) return;
+    const int64_t i = dq.point[w];
+    double x = 0.0, y = 0.0, z = 0.0;
+    point_coords<3>(g, plist, i, x, y, z);
+    NpqD q = npq_ref_3d_sub<64>(m, dom.root64, x, y, z, lane, stk, stride);
+    // least distance, lowest subtree on ties
+    int rank = lane;
+    for (int off = 1; off < 64; off <<= 1) {
+        NpqD o;
+        o.d2 = __shfl_xor(q.d2, off); o.px = __shfl_xor(q.px, off); o.py = __shfl_xor(q.py, off);
+        o.pz = __shfl_xor(q.pz, off); o.s = __shfl_xor(q.s, off); o.t = __shfl_xor(q.t, off);
+        o.prim = __shfl_xor(q.prim, off); o.slot = __shfl_xor(q.slot, off);
+        const int orank = __shfl_xor(rank, off);
+        if (o.d2 < q.d2 || (o.d2 == q.d2 && orank < rank)) { q = o; rank = orank; }
+    }
+    if (lane != 0) return;
+    const uint64_t ps = point_seed(seed, sidx ? (uint64_t)sidx[i] : (uint64_t)i);
+    classify_tail<3>(m, e, dom, i, x, y, z, q, ps, dq.rays[w], max_ray_attempts, stk, stride,
                      loc, starts, amb, bvalue);
 }
 
@@ -2192,6 +2251,7 @@ struct wos_ctx {
     DevBuf nd2, nd3, segd, trid, slot_prim, prim_bid, prims, verts, nf2, nf3, segf, trif;
     DevBuf nw2, nw3, doms_w;          // the 4-wide FP32 forest and its domain roots
     DevBuf cq_out, cq_prim;
+    DevBuf deepq;                     // k_classify3's queue of points for k_classify3_deep
     DevBuf units;                     // FP32 query-unit counters
     DevBuf wstack;                    // FP32 walker traversal stacks (every lane of a full GPU)
     DevBuf failed;                    // walks with a NaN value (one counter)
@@ -2911,6 +2971,36 @@ float fp32_margin(const DomainInfo &D) {
     return (float)(32.0 * std::ldexp(1.0, -24) * D.coord_scale);
 }
 
+// k_classify3<S> (64 points a block), then k_classify3_deep over the points
+// it queued
+template <int S>
+hipError_t launch_classify3(wos_ctx *c, const EqView &ev, const GridSpec &g, const DomainView &dv,
+                            const double *plist, const int64_t *sidx, int64_t n, uint64_t seed,
+                            int mra) {
+    hipError_t e = c->deepq.reserve(sizeof(unsigned long long) +
+                                    (size_t)n * (sizeof(int64_t) + sizeof(int32_t)));
+    if (e != hipSuccess) return e;
+    unsigned char *base = c->deepq.as<unsigned char>();
+    DeepQueue dq{reinterpret_cast<unsigned long long *>(base),
+                 reinterpret_cast<int64_t *>(base + sizeof(unsigned long long)),
+                 reinterpret_cast<int32_t *>(base + sizeof(unsigned long long) +
+                                             (size_t)n * sizeof(int64_t))};
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, c->stream, dq.count, 1);
+    const int threads = (S + 3) * 64;
+    const int blocks = (int)((n + 63) / 64);
+    size_t lds = (size_t)threads * (c->L.depth + 2) * sizeof(int);
+    hipLaunchKernelGGL(k_classify3<S>, dim3(blocks), dim3(threads), lds, c->stream, c->view, ev, g,
+                       dv, plist, sidx, n, seed, mra, c->loc.as<uint8_t>(),
+                       c->starts.as<StartInfo>(), c->amb.as<int32_t>(), c->bvalue.as<double>(), dq);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    const int dblocks = (int)((n + 3) / 4);
+    lds = (size_t)256 * (c->L.depth + 2) * sizeof(int);
+    hipLaunchKernelGGL(k_classify3_deep, dim3(dblocks), dim3(256), lds, c->stream, c->view, ev, g,
+                       dv, plist, sidx, seed, mra, c->loc.as<uint8_t>(), c->starts.as<StartInfo>(),
+                       c->amb.as<int32_t>(), c->bvalue.as<double>(), dq);
+    return hipGetLastError();
+}
+
 int classify_parts(wos_ctx *c, const std::vector<Part> &parts, const EqView &ev,
                    const wos_run_params *prm, std::vector<double> &part_tol) {
     const int dim = c->mesh.dim;
@@ -2947,11 +3037,11 @@ int classify_parts(wos_ctx *c, const std::vector<Part> &parts, const EqView &ev,
                                (size_t)split * 64 * sizeof(NpqD) <= c->lds_per_block;
                 };
                 if (fits(8))
-                    le = launch_classify(c, k_classify3<8>, ev, g, dv, dp, ds, p.count, prm->seed,
-                                         prm->max_ray_attempts, 64, 11 * 64);
+                    le = launch_classify3<8>(c, ev, g, dv, dp, ds, p.count, prm->seed,
+                                             prm->max_ray_attempts);
                 else if (fits(4))
-                    le = launch_classify(c, k_classify3<4>, ev, g, dv, dp, ds, p.count, prm->seed,
-                                         prm->max_ray_attempts, 64, 7 * 64);
+                    le = launch_classify3<4>(c, ev, g, dv, dp, ds, p.count, prm->seed,
+                                             prm->max_ray_attempts);
                 else
                     le = launch_classify(c, k_classify<3>, ev, g, dv, dp, ds, p.count, prm->seed,
                                          prm->max_ray_attempts, kBlock, kBlock);
