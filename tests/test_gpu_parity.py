@@ -268,3 +268,19 @@ def test_empty_outside_and_single_axis_inputs(gpu_ctx, oracle):
     o = om3.solve_grid(eq3, 1, 5, 3, 32, 1e-4, seed=9, rng_mode=1, n_threads=1)
     assert np.array_equal(g["location"], o["location"])
     assert np.allclose(g["mean"], o["mean"], rtol=0, atol=1e-12, equal_nan=True)
+
+
+def test_deep_points_match_oracle(gpu_ctx, oracle):
+    """Points near the centre of the icosphere are about equally far from
+    every triangle, so their start queries run past the subtree budget of the
+    first classification pass and are searched again on a whole wavefront
+    (64 subtrees): codes and FP64 means equal the oracle's."""
+    V, F, B = M.icosphere(4)
+    gpu_ctx.set_mesh(V, F, B)
+    eq = Pr.sphere_screened_constant(10.0, 1.0)
+    pts = np.random.default_rng(4).uniform(-0.25, 0.25, size=(64, 3))
+    o = oracle.mesh(V, F, B).solve_points(eq, pts, 16, 1e-3, seed=5, rng_mode=1, n_threads=1)
+    r = gpu_ctx.solve_points(pts, eq, Context.params(16, 1e-3, 5))
+    assert np.array_equal(r["location"], o["location"])
+    assert np.all(r["location"] == 1)
+    assert np.allclose(r["mean"], o["mean"], rtol=0, atol=1e-12)
