@@ -237,6 +237,32 @@ def test_classification_on_shallow_trees(gpu_ctx, oracle, shape):
     assert np.array_equal(g["ambiguous_ray_retries"], o["ambiguous"])
 
 
+@pytest.mark.parametrize("shape", ["triangle", "rectangle", "octagon", "star"])
+def test_2d_classification_ties_and_shallow_trees(gpu_ctx, oracle, shape):
+    """2D meshes whose BVH has leaves above the split start query's depth
+    (3, 4 and 8 segments) and grid points at equal distance from two or more
+    segments (corners, diagonals, the centre).  Alternating boundary ids make
+    a boundary point's value depend on which segment the search picks:
+    location codes, boundary values and FP64 means equal the oracle's."""
+    if shape == "triangle":
+        V = np.array([[-1.0, -0.5], [1.0, -0.5], [0.0, 1.0]])
+    elif shape == "rectangle":
+        V = np.array([[-1.0, -0.5], [1.0, -0.5], [1.0, 0.5], [-1.0, 0.5]])
+    elif shape == "octagon":
+        V = M.polygon_loop(8, lambda t: np.full_like(t, 1.0))
+    else:
+        V = M.polygon_loop(96, lambda t: 0.6 + 0.3 * np.cos(5 * t))
+    n = len(V)
+    F = np.stack([np.arange(n), (np.arange(n) + 1) % n], axis=1).astype(np.int32)
+    B = (np.arange(n) % 2).astype(np.int32)
+    mesh = (V.astype(np.float64), F, B)
+    om = oracle.mesh(*mesh)
+    eq = Pr.laplace_builtin(2)
+    g, o = _run_both(gpu_ctx, om, mesh, eq, (9, 9, 1), 64, 1e-3, 23)
+    _compare_fp64(g, o, shape, exact_frac=0.9)
+    assert (g["location"] == 2).any(), "the grid should hold boundary points"
+
+
 def test_empty_outside_and_single_axis_inputs(gpu_ctx, oracle):
     """An empty point list is refused (the reference's grids have at least
     one point, src/main.cpp:233); a list with no inside point returns its

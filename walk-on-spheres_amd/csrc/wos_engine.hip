@@ -320,7 +320,8 @@ __device__ __forceinline__ void point_coords(const GridSpec &g, const double *pl
 // Location code, walk keys and start record of point i from its start query
 // q (src/inside.cpp:199-260, runner.hpp:182-221).  first_rays (3D): -1, or
 // the first three ray casts made beforehand, bit k the hit parity of ray k
-// and bit 3 + k set when ray k was ambiguous.
+// and bit 3 + k set when ray k was ambiguous.  first_rays (2D): -1, or the
+// inside flag (winding number != 0) found beforehand.
 template <int DIM>
 __device__ __forceinline__ void classify_tail(const MeshView &m, const EqView &e,
                                               const DomainView &dom, int64_t i, double x,
@@ -334,7 +335,7 @@ __device__ __forceinline__ void classify_tail(const MeshView &m, const EqView &e
     if (r <= dom.tol) {
         code = 2;
     } else if (DIM == 2) {
-        code = winding_number_d(m, dom.s0, dom.s1, x, y) != 0 ? 1 : 0;
+        code = first_rays >= 0 ? first_rays : (winding_number_d(m, dom.s0, dom.s1, x, y) != 0 ? 1 : 0);
     } else {
         // src/inside.cpp:228-260 with the inside stream of runner.hpp:182-183;
         // when none of the first three casts was ambiguous they decide
@@ -441,11 +442,39 @@ __device__ __forceinline__ NpqD npq_ref_3d_sub(const MeshView &m, int root, doub
     return npq_ref_3d(m, curr, px, py, pz, stk, stride, budget);
 }
 
+// npq_ref_3d_sub for the 2D tree: the same near/far descent over the top D
+// levels (near = the child with the smaller box distance, ties to the left,
+// as npq_ref_2d), then the reference's search inside subtree j.
+template <int S>
+__device__ __forceinline__ NpqD npq_ref_2d_sub(const MeshView &m, int root, double px, double py,
+                                               int j, int *stk, int stride) {
+    constexpr int D = S == 64 ? 6 : S == 32 ? 5 : S == 16 ? 4 : S == 8 ? 3 : S == 4 ? 2 : S == 2 ? 1 : 0;
+    static_assert((1 << D) == S, "S must be a power of two up to 64");
+    int curr = root;
+    for (int l = 0; l < D; ++l) {
+        const NodeD2 n = m.nd2[curr];
+        if (n.count > 0) {
+            if ((j & ((1 << (D - l)) - 1)) != 0) {
+                NpqD r;
+                r.d2 = INFINITY; r.px = r.py = r.pz = 0.0; r.s = r.t = 0.0; r.prim = -1; r.slot = -1;
+                return r;
+            }
+            break;
+        }
+        const int left = curr + 1, right = n.idx;
+        const bool left_near = box_d2_2(m.nd2[left], px, py) <= box_d2_2(m.nd2[right], px, py);
+        const bool far = ((j >> (D - 1 - l)) & 1) != 0;
+        curr = far == left_near ? right : left;
+    }
+    return npq_ref_2d(m, curr, px, py, stk, stride);
+}
+
 // Points whose start query needs more than kClassifyBudget nodes in some
 // subtree of k_classify3 (points deep inside a closed surface, where nearly
 // every primitive is about as far) are queued for k_classify3_deep, which
 // searches each of them on a whole wavefront.
 constexpr int kClassifyBudget = 256;
+constexpr int kClassify2Split = 16;  // lanes a point in k_classify2s
 struct DeepQueue {
     unsigned long long *count;
     int64_t *point;    // the point's index in the launch
@@ -557,6 +586,54 @@ __global__ __launch_bounds__(256) void k_classify3_deep(MeshView m, EqView e, Gr
     if (lane != 0) return;
     const uint64_t ps = point_seed(seed, sidx ? (uint64_t)sidx[i] : (uint64_t)i);
     classify_tail<3>(m, e, dom, i, x, y, z, q, ps, dq.rays[w], max_ray_attempts, stk, stride,
+                     loc, starts, amb, bvalue);
+}
+
+// 2D classification on S lanes a point: lane j searches subtree j of the
+// start query (npq_ref_2d_sub) and sums the winding number's crossings of
+// the segments s0 + j, s0 + j + S, ...; the least distance with the lowest
+// j is the reference's search result (as in k_classify3_deep) and the
+// integer crossing counts add up exactly to winding_number_d's.  Lane 0
+// finishes the point.
+template <int S>
+__global__ __launch_bounds__(256) void k_classify2s(MeshView m, EqView e, GridSpec g,
+                                                    DomainView dom, const double *plist,
+                                                    const int64_t *sidx, int64_t n, uint64_t seed,
+                                                    int max_ray_attempts, uint8_t *loc,
+                                                    StartInfo *starts, int32_t *amb,
+                                                    double *bvalue) {
+    extern __shared__ int lds_i[];
+    int *stk = lds_i + threadIdx.x;
+    const int stride = blockDim.x;
+    const int j = (int)(threadIdx.x % S);
+    const int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / S;
+    if (i >= n) return;  // whole groups of S lanes
+    double x = 0.0, y = 0.0, z = 0.0;
+    point_coords<2>(g, plist, i, x, y, z);
+    NpqD q = npq_ref_2d_sub<S>(m, dom.root64, x, y, j, stk, stride);
+    int wn = 0;
+    for (int sg = dom.s0 + j; sg < dom.s1; sg += S) {
+        const SegD sd = m.segd[sg];
+        const double area = (sd.s1x - sd.s0x) * (y - sd.s0y) - (x - sd.s0x) * (sd.s1y - sd.s0y);
+        if (sd.s0y <= y) {
+            if (sd.s1y > y && area > 0) wn++;
+        } else {
+            if (sd.s1y <= y && area < 0) wn--;
+        }
+    }
+    int rank = j;
+    for (int off = 1; off < S; off <<= 1) {
+        NpqD o;
+        o.d2 = __shfl_xor(q.d2, off); o.px = __shfl_xor(q.px, off); o.py = __shfl_xor(q.py, off);
+        o.pz = __shfl_xor(q.pz, off); o.s = __shfl_xor(q.s, off); o.t = __shfl_xor(q.t, off);
+        o.prim = __shfl_xor(q.prim, off); o.slot = __shfl_xor(q.slot, off);
+        const int orank = __shfl_xor(rank, off);
+        wn += __shfl_xor(wn, off);
+        if (o.d2 < q.d2 || (o.d2 == q.d2 && orank < rank)) { q = o; rank = orank; }
+    }
+    if (j != 0) return;
+    const uint64_t ps = point_seed(seed, sidx ? (uint64_t)sidx[i] : (uint64_t)i);
+    classify_tail<2>(m, e, dom, i, x, y, z, q, ps, wn != 0 ? 1 : 0, max_ray_attempts, stk, stride,
                      loc, starts, amb, bvalue);
 }
 
@@ -3024,8 +3101,14 @@ int classify_parts(wos_ctx *c, const std::vector<Part> &parts, const EqView &ev,
             const int64_t *ds = p.seed_index ? c->sidx.as<int64_t>() + poff : nullptr;
             hipError_t le;
             if (dim == 2) {
-                le = launch_classify(c, k_classify<2>, ev, g, dv, dp, ds, p.count, prm->seed,
-                                     prm->max_ray_attempts, kBlock, kBlock);
+                // S lanes a point (k_classify2s) where 256 traversal stacks fit the LDS
+                if ((size_t)256 * (c->L.depth + 2) * sizeof(int) <= c->lds_per_block)
+                    le = launch_classify(c, k_classify2s<kClassify2Split>, ev, g, dv, dp, ds,
+                                         p.count, prm->seed, prm->max_ray_attempts,
+                                         256 / kClassify2Split, 256);
+                else
+                    le = launch_classify(c, k_classify<2>, ev, g, dv, dp, ds, p.count, prm->seed,
+                                         prm->max_ray_attempts, kBlock, kBlock);
             } else {
                 // the start query on 8 (or 4) subtrees where the block's
                 // traversal stacks and result slots fit the LDS, else one
